@@ -1,0 +1,207 @@
+/*
+ * warpstate_amd — C ABI of the MI355X (gfx950) scan engine.
+ *
+ * This is the drop-in boundary for warpstate's GPU scan path. The reference
+ * crate exposes the path through Rust types; every entry point below names the
+ * reference interface it replaces (paths relative to the reference checkout).
+ *
+ * Conventions
+ *  - All offsets are u32 byte offsets, as in the reference (`Match` is
+ *    {u32 pattern_id, u32 start, u32 end}, end exclusive;
+ *    core/src/dfa/mod.rs:416-419). One scan covers at most 4 GiB - 16 bytes.
+ *  - Output lists are sorted by (start, pattern_id, end)
+ *    (core/src/cpu/mod.rs:67-78); batch output by
+ *    (source_id, start, pattern_id, end) (core/src/batch.rs:159-165).
+ *  - The caller owns inputs and outputs; the library owns device tables,
+ *    workspaces and streams. Every call is blocking on its context's stream.
+ *  - A context serialises calls made on it; use one context per host thread
+ *    for concurrent scans (each context has its own HIP stream).
+ *  - Errors: every entry point returns a ws_status; ws_last_error() returns
+ *    the calling thread's last message. Results never silently change
+ *    semantics: a full output buffer is WS_ERR_MATCH_BUFFER_OVERFLOW with the
+ *    required count in *count (the reference falls back to a CPU rescan,
+ *    core/src/gpu/dispatch.rs:243-273; this library does not).
+ */
+#ifndef WARPSTATE_AMD_H
+#define WARPSTATE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WS_ABI_VERSION 1u
+
+/* Error variants of core/src/error.rs:6-91, one code each. */
+typedef enum ws_status {
+  WS_OK = 0,
+  WS_ERR_EMPTY_PATTERN_SET = 1,
+  WS_ERR_EMPTY_PATTERN = 2,
+  WS_ERR_NO_GPU_ADAPTER = 3,
+  WS_ERR_GPU_DEVICE = 4,
+  WS_ERR_BUFFER_MAP_FAILED = 5,
+  WS_ERR_PATTERN_COMPILATION_FAILED = 6,
+  WS_ERR_DFA_OUT_OF_BOUNDS = 7,
+  WS_ERR_PATTERN_TOO_LARGE = 8,
+  WS_ERR_PATTERN_SET_TOO_LARGE = 9,
+  WS_ERR_INPUT_TOO_LARGE = 10,
+  WS_ERR_MATCH_BUFFER_OVERFLOW = 11,
+  WS_ERR_PATHOLOGICAL_REGEX = 12,
+  WS_ERR_INVALID_ARGUMENT = 13
+} ws_status;
+
+typedef struct ws_ctx ws_ctx;
+typedef struct ws_patterns ws_patterns;
+
+/* matchkit::Match (core/src/dfa/mod.rs:416-419). */
+typedef struct ws_match {
+  uint32_t pattern_id;
+  uint32_t start;
+  uint32_t end;
+} ws_match;
+
+/* batch::TaggedMatch (core/src/batch.rs:22-27); offsets relative to the item. */
+typedef struct ws_tagged_match {
+  uint64_t source_id;
+  uint32_t pattern_id;
+  uint32_t start;
+  uint32_t end;
+  uint32_t reserved;
+} ws_tagged_match;
+
+/* batch::ScanItem (core/src/batch.rs:13-18). */
+typedef struct ws_item {
+  uint64_t id;
+  const uint8_t* data;
+  uint64_t len;
+} ws_item;
+
+enum { WS_PATTERN_LITERAL = 0, WS_PATTERN_REGEX = 1 };
+
+/* One PatternSetBuilder::literal / ::regex call (core/src/pattern/compiler.rs:13-93). */
+typedef struct ws_pattern {
+  uint32_t kind;        /* WS_PATTERN_LITERAL or WS_PATTERN_REGEX */
+  const uint8_t* bytes; /* literal bytes, or regex source text */
+  uint64_t len;
+  const char* name;     /* optional, may be NULL */
+} ws_pattern;
+
+/* PatternSetBuilder flags (core/src/pattern/compiler.rs:73-84). */
+enum { WS_FLAG_CASE_INSENSITIVE = 1u };
+
+/* Match semantics.
+ * WS_MODE_OVERLAPPING: the GPU contract — every occurrence of every literal
+ *   (PatternSet::scan_overlapping, core/src/cpu/scan.rs:235-247,300-349) plus
+ *   leftmost-first non-overlapping regex matches per pattern.
+ * WS_MODE_STANDARD: PatternSet::scan (core/src/pattern/mod.rs:112-140): the
+ *   literal reduction that ScanStrategy::select picks
+ *   (core/src/specialize.rs:101-179, core/src/hash_scan.rs:321-363) plus the
+ *   same regex matches. */
+enum { WS_MODE_OVERLAPPING = 0, WS_MODE_STANDARD = 1 };
+
+/* Where a buffer lives. */
+enum { WS_MEM_HOST = 0, WS_MEM_DEVICE = 1 };
+
+/* AutoMatcherConfig subset (core/src/config.rs:4-61). Zero fields take defaults:
+ * max_matches 1,000,000 (config.rs:35), chunk_size 128 MiB, chunk_overlap 4096. */
+typedef struct ws_config {
+  uint64_t max_matches;
+  uint64_t chunk_size;
+  uint64_t chunk_overlap;
+  uint32_t flags;
+  uint32_t reserved;
+} ws_config;
+
+/* Kernel choice. The autotuner and the learned router produce these. */
+enum {
+  WS_VARIANT_AUTO = 0,
+  WS_VARIANT_HASH = 1,       /* rolling-hash prefilter, LDS bloom + L2 buckets */
+  WS_VARIANT_DFA_GLOBAL = 2, /* dense factor DFA, table served from L2/HBM */
+  WS_VARIANT_DFA_LDS = 3     /* dense factor DFA, table resident in LDS */
+};
+
+typedef struct ws_tuning {
+  uint32_t variant;       /* WS_VARIANT_* */
+  uint32_t bytes_per_lane;/* 16 or 32 (hash); 32..256 (DFA); 0 = default */
+  uint32_t blocks_per_cu; /* persistent grid = CUs * blocks_per_cu; 0 = default */
+  uint32_t reserved;
+} ws_tuning;
+
+/* Compiled-set facts (router features; core/src/pattern/ir.rs:6-33). */
+typedef struct ws_patterns_info {
+  uint32_t pattern_count;
+  uint32_t literal_count;
+  uint32_t regex_count;
+  uint32_t factor_count;
+  uint32_t min_factor_len;
+  uint32_t max_factor_len;
+  uint32_t min_literal_len;
+  uint32_t max_pattern_len;   /* longest match any pattern can produce; 0xFFFFFFFF if unbounded */
+  uint32_t dfa_states;        /* factor DFA states */
+  uint32_t dfa_classes;       /* factor DFA byte classes */
+  uint32_t regex_dfa_states;  /* sum over regex anchored DFAs */
+  uint32_t hash_window;       /* K of the rolling-hash prefilter */
+  uint32_t variants_supported;/* bit (1 << WS_VARIANT_*) */
+  uint32_t reserved;
+} ws_patterns_info;
+
+/* Per-call timings measured with HIP events on the context stream (ms). */
+typedef struct ws_timings {
+  float prefilter_ms;  /* the prefilter+verify kernel */
+  float regex_ms;      /* anchored walks + chain resolution */
+  float order_ms;      /* sort + pack */
+  float total_ms;      /* whole call, first launch to last */
+  uint64_t candidates; /* regex candidate starts */
+  uint64_t raw_matches;/* matches before ordering */
+} ws_timings;
+
+/* ---- context: GpuMatcher::new / with_config (core/src/gpu/mod.rs:95-220) ---- */
+ws_status ws_ctx_create(int device_ordinal, const ws_config* cfg, ws_ctx** out);
+void ws_ctx_destroy(ws_ctx* ctx);
+/* The context's HIP stream (hipStream_t) for callers that enqueue around it. */
+void* ws_ctx_stream(ws_ctx* ctx);
+/* Number of compute units of the context's device. */
+uint32_t ws_ctx_cu_count(ws_ctx* ctx);
+/* Enable per-call HIP-event timing (ws_ctx_last_timings). */
+void ws_ctx_set_profiling(ws_ctx* ctx, int on);
+ws_status ws_ctx_last_timings(ws_ctx* ctx, ws_timings* out);
+const char* ws_last_error(void);
+uint32_t ws_abi_version(void);
+
+/* ---- patterns: PatternSetBuilder::build (core/src/pattern/compiler.rs:97-340) ---- */
+ws_status ws_patterns_compile(ws_ctx* ctx, const ws_pattern* patterns, size_t count,
+                              uint32_t flags, ws_patterns** out);
+void ws_patterns_free(ws_patterns* patterns);
+ws_status ws_patterns_info_get(const ws_patterns* patterns, ws_patterns_info* out);
+
+/* ---- scan: GpuMatcher::scan (core/src/gpu/mod.rs:223-297) + dispatch::scan_literal_chunk
+ *      (core/src/gpu/dispatch.rs:57-301) + dispatch::scan_regex_chunk (dispatch.rs:331-380).
+ * data may be host or device memory (data_mem); out likewise (out_mem).
+ * On WS_ERR_MATCH_BUFFER_OVERFLOW *count holds the required capacity. */
+ws_status ws_scan(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t* data, uint64_t len,
+                  uint32_t data_mem, uint32_t mode, const ws_tuning* tuning, ws_match* out,
+                  uint64_t cap, uint32_t out_mem, uint64_t* count);
+
+/* ---- batch: batch::scan_batch / scan_batch_gpu / scan_batch_cpu (core/src/batch.rs:170-363).
+ * Every item is scanned on its own (no match crosses items). Item data is host memory
+ * when data_mem is WS_MEM_HOST, else device pointers. One persistent launch. */
+ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* patterns, const ws_item* items,
+                        size_t n_items, uint32_t data_mem, uint32_t mode,
+                        const ws_tuning* tuning, ws_tagged_match* out, uint64_t cap,
+                        uint64_t* count);
+
+/* ---- stream: StreamPipeline::scan (core/src/pipeline.rs:86-153).
+ * Scans in chunks of chunk_size bytes; each chunk reads `overlap` extra bytes and keeps
+ * matches that start inside it. Regex non-overlap chains carry across chunks, so the
+ * result equals one ws_scan over the whole input when overlap >= the longest match. */
+ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t* data,
+                         uint64_t len, uint32_t data_mem, uint64_t chunk_size, uint64_t overlap,
+                         uint32_t mode, const ws_tuning* tuning, ws_match* out, uint64_t cap,
+                         uint32_t out_mem, uint64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WARPSTATE_AMD_H */

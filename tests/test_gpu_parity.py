@@ -1,0 +1,177 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bit-exact (pattern_id, start, end) lists on seeded inputs, for every kernel
+variant, plus the reference's own known-answer cases run on the device.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from warpstate_amd import (GpuMatcher, PatternSet, ScanItem, StreamPipeline, Tuning, VARIANT_DFA_GLOBAL,
+                           VARIANT_DFA_LDS, VARIANT_HASH, WarpstateError, scan_batch)
+from warpstate_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def build(pats, ci=False):
+    b = PatternSet.builder().case_insensitive(ci)
+    for kind, p in pats:
+        (b.literal if kind == 0 else b.regex)(p)
+    return b.build()
+
+
+def as_list(arr):
+    return [(int(p), int(s), int(e)) for p, s, e in zip(arr["pattern_id"], arr["start"], arr["end"])]
+
+
+def variants_for(m):
+    v = m.info()["variants_supported"]
+    return [x for x in (VARIANT_HASH, VARIANT_DFA_GLOBAL, VARIANT_DFA_LDS) if v & (1 << x)]
+
+
+def check_all_variants(pats, data, mode=0, ci=False):
+    ps = build(pats, ci)
+    m = GpuMatcher(ps)
+    exp = Oracle(pats, ci).scan_list(data, mode)
+    for v in variants_for(m):
+        print(f"variant={v} n={len(data)} pats={len(pats)} mode={mode}", flush=True)
+        got = as_list(m.scan_array(data, mode="standard" if mode else "overlapping", tuning=Tuning(variant=v)))
+        assert got == exp, f"variant {v}: {len(got)} vs {len(exp)} matches"
+    return exp
+
+
+def test_literal_config_small():
+    pats, data = W.literal_workload(size=1 << 20, seed=0)
+    exp = check_all_variants([(0, p) for p in pats], data.tobytes())
+    assert len(exp) > 5
+
+
+def test_kmer_small():
+    pats = W.kmer_set(10_000, seed=1)
+    data = W.kmer_input(2 << 20, pats, seed=1)
+    exp = check_all_variants([(0, p) for p in pats], data.tobytes())
+    assert len(exp) > 100
+
+
+def test_regex_grammar_small():
+    srcs, data, _ = W.regex_workload(size=1 << 20, n=200, seed=2)
+    data = W.regex_input(1 << 20, W.regex_set(200, 2), seed=2, plant_gap=2048)
+    exp = check_all_variants([(1, s) for s in srcs], data.tobytes())
+    assert len(exp) > 100
+
+
+def test_mixed_set_standard_and_overlapping():
+    pats, _, specs = W.mixed_set(70, 30, seed=3)
+    pieces = [p for _, p in pats[:70]] + [s.sample(np.random.default_rng(9)) for s in specs]
+    data = W.printable_input(1 << 19, 7)
+    W.plant(data, pieces, 512, 8)
+    check_all_variants(pats, data.tobytes(), mode=0)
+    check_all_variants(pats, data.tobytes(), mode=1)
+
+
+@pytest.mark.parametrize("npat,lmax", [(4, 8), (64, 16), (6000, 14)])
+def test_standard_mode_strategies(npat, lmax):
+    # 4 short literals -> MultiMemchr; 64 -> LeftmostLongest; 6000 -> HashScanner.
+    alpha = np.frombuffer(b"ab", dtype=np.uint8)
+    lits = W.random_literals(npat, 2 if npat < 100 else 10, lmax, alpha, seed=npat)
+    data = alpha[np.random.default_rng(npat).integers(0, 2, size=200_000)].tobytes()
+    o = Oracle([(0, l) for l in lits])
+    assert o.strategy == {4: 2, 64: 1, 6000: 3}[npat]
+    check_all_variants([(0, l) for l in lits], data, mode=1)
+
+
+def test_reference_kats_on_device():
+    # core/tests/correctness_guarantees.rs:15-41 overlapping abc / bc
+    ps = PatternSet.builder().literal("abc").literal("bc").build()
+    assert GpuMatcher(ps).scan(b"abc") == [(0, 0, 3), (1, 1, 3)]
+    # correctness_guarantees.rs:257-299 duplicates, both ids in overlapping mode; one in scan
+    ps = PatternSet.builder().literal("DUPE").literal("DUPE").build()
+    assert GpuMatcher(ps).scan(b"xxDUPEyy") == [(0, 2, 6), (1, 2, 6)]
+    assert ps.scan(b"xxDUPEyy") == [(0, 2, 6)]
+    # all-null pattern, 3 overlapping matches (correctness_guarantees.rs:~230)
+    ps = PatternSet.builder().literal_bytes(b"\x00\x00\x00").build()
+    assert [m.start for m in ps.scan_overlapping(b"\x00" * 5)] == [0, 1, 2]
+    # core/src/cpu/mod.rs:351-364: "aa" in "aaaa" non-overlapping
+    ps = PatternSet.builder().literal("aa").build()
+    assert ps.scan(b"aaaa") == [(0, 0, 2), (0, 2, 4)]
+    # tests/unit/harness_regex.rs:65-68 and :77-80
+    ps = PatternSet.builder().regex("[0-9]+").build()
+    assert [(m.start, m.end) for m in ps.scan(b"abc 123 def 456")] == [(4, 7), (12, 15)]
+    ps = PatternSet.builder().regex("foo|bar").build()
+    assert [(m.start, m.end) for m in ps.scan(b"foo baz bar")] == [(0, 3), (8, 11)]
+    # harness_regex.rs:28-31 case-insensitive group
+    ps = PatternSet.builder().regex("(?i:hello)").build()
+    assert len(ps.scan(b"Hello hello HELLO hElLo")) == 4
+    # correctness_guarantees.rs:213-221 raw byte regex
+    ps = PatternSet.builder().regex(r"\xFF").build()
+    assert [(m.start, m.end) for m in ps.scan(bytes([0, 0xFF, 0]))] == [(1, 2)]
+
+
+def test_edges_empty_short_tail():
+    ps = PatternSet.builder().literal("needle").regex("ne+dle[0-9]{2}").build()
+    m = GpuMatcher(ps)
+    assert m.scan(b"") == []
+    assert m.scan(b"ne") == []
+    assert m.scan(b"needle") == [(0, 0, 6)]
+    data = b"x" * 4093 + b"needle77"  # crosses a 4 KiB work unit
+    exp = Oracle([(0, "needle"), (1, "ne+dle[0-9]{2}")]).scan_list(data)
+    assert m.scan(data) == exp
+
+
+def test_case_insensitive_literals():
+    pats = [(0, b"Needle"), (0, b"HAYstack"), (0, b"ab")]
+    data = b"needle NEEDLE haystack HayStack AB ab aB xx" * 50
+    check_all_variants(pats, data, mode=0, ci=True)
+    check_all_variants(pats, data, mode=1, ci=True)
+
+
+def test_stream_pipeline_equals_whole():
+    specs = W.regex_set(50, seed=11)
+    data = W.regex_input(1 << 20, specs, seed=11, plant_gap=1024).tobytes()
+    pats = [(1, s.source) for s in specs] + [(0, b"abc"), (0, b"zz_")]
+    ps = build(pats)
+    m = GpuMatcher(ps)
+    whole = m.scan(data)
+    overlap = max(s.max_len for s in specs)
+    for chunk in (4096, 65536, 300_001):
+        got = StreamPipeline(m, overlap).scan(data, chunk_size=chunk)
+        assert got == whole, f"chunk {chunk}"
+    assert whole == [tuple(x) for x in Oracle(pats).scan_list(data)]
+
+
+def test_stream_boundary_kat():
+    # tests/unit/mod.rs:11-54: "legendary" over a 1024-byte chunk edge
+    ps = PatternSet.builder().regex("legendary").build()
+    data = bytearray(b"." * 2048)
+    data[1020:1029] = b"legendary"
+    got = StreamPipeline(GpuMatcher(ps), 16).scan(bytes(data), chunk_size=1024)
+    assert got == [(0, 1020, 1029)]
+
+
+def test_batch_per_item_exact():
+    pats, _, specs = W.mixed_set(70, 30, seed=3)
+    rng = np.random.default_rng(5)
+    pieces = [p for _, p in pats[:70]] + [s.sample(rng) for s in specs]
+    ids, offs, sizes, buf = W.batch_items(500, seed=4, pieces=pieces, plant_gap=256)
+    items = [ScanItem(int(i), buf[o:o + s].tobytes()) for i, o, s in zip(ids, offs, sizes)]
+    m = GpuMatcher(build(pats))
+    got = scan_batch(m, items)
+    o = Oracle(pats)
+    exp = []
+    for it in items:
+        exp += [(it.id, x) for x in o.scan_list(it.data)]
+    assert [(t.source_id, tuple(t.matched)) for t in got] == exp
+    # tests/unit/gpu_batch.rs:6-65: a match split across two items is not reported
+    m2 = GpuMatcher(PatternSet.builder().literal("needle").build())
+    got2 = scan_batch(m2, [ScanItem(1, b"xxnee"), ScanItem(2, b"dlexx"), ScanItem(3, b"needle")])
+    assert [(t.source_id, tuple(t.matched)) for t in got2] == [(3, (0, 0, 6))]
+
+
+def test_overflow_is_an_error_not_a_fallback():
+    from warpstate_amd import AutoMatcherConfig, Context
+    ctx = Context(0, AutoMatcherConfig(max_matches=100))
+    ps = PatternSet.builder().literal("a").build()
+    with pytest.raises(WarpstateError) as e:
+        GpuMatcher(ps, context=ctx).scan(b"a" * 1000)
+    assert e.value.variant == "MatchBufferOverflow"

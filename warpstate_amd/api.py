@@ -1,0 +1,394 @@
+"""Python mirror of the reference crate's scan API over the C ABI.
+
+Names and argument meanings follow the reference (paths relative to its
+checkout): PatternSet / PatternSetBuilder (core/src/pattern/), GpuMatcher —
+documented there as WarpStateer — (core/src/gpu/mod.rs), AutoMatcherConfig
+(core/src/config.rs), StreamPipeline (core/src/pipeline.rs) and the batch
+free functions (core/src/batch.rs). Every scan runs on the GPU through
+libwarpstate_amd.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from dataclasses import dataclass, field
+from typing import Iterable, List, NamedTuple, Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+
+
+class Match(NamedTuple):
+    """matchkit::Match: {pattern_id, start, end}, end exclusive."""
+
+    pattern_id: int
+    start: int
+    end: int
+
+
+class ScanItem(NamedTuple):
+    """batch::ScanItem (core/src/batch.rs:13-18)."""
+
+    id: int
+    data: bytes
+
+
+class TaggedMatch(NamedTuple):
+    """batch::TaggedMatch (core/src/batch.rs:22-27)."""
+
+    source_id: int
+    matched: Match
+
+
+def _as_bytes(x) -> bytes:
+    if isinstance(x, str):
+        return x.encode()
+    return bytes(x)
+
+
+@dataclass
+class AutoMatcherConfig:
+    """Subset of core/src/config.rs:39-173 that the GPU path honours."""
+
+    max_matches: int = 1_000_000
+    chunk_size: int = 128 * 1024 * 1024
+    chunk_overlap: int = 4096
+    gpu_threshold: int = 64 * 1024
+
+    def with_max_matches(self, n: int) -> "AutoMatcherConfig":
+        self.max_matches = n
+        return self
+
+    def with_chunk_size(self, n: int) -> "AutoMatcherConfig":
+        self.chunk_size = n
+        return self
+
+    def with_chunk_overlap(self, n: int) -> "AutoMatcherConfig":
+        self.chunk_overlap = n
+        return self
+
+    def to_c(self) -> N.WsConfig:
+        return N.WsConfig(self.max_matches, self.chunk_size, self.chunk_overlap, 0, 0)
+
+
+@dataclass
+class Tuning:
+    """Kernel choice: variant (hash / dfa_global / dfa_lds) and launch shape."""
+
+    variant: int = N.VARIANT_AUTO
+    bytes_per_lane: int = 0
+    blocks_per_cu: int = 0
+
+    def to_c(self) -> N.WsTuning:
+        return N.WsTuning(self.variant, self.bytes_per_lane, self.blocks_per_cu, 0)
+
+
+class Context:
+    """One device context: a HIP stream plus workspaces (thread-safe, serialised)."""
+
+    def __init__(self, device: int = 0, config: Optional[AutoMatcherConfig] = None):
+        lib = N.load()
+        self.config = config or AutoMatcherConfig()
+        h = ctypes.c_void_p()
+        cfg = self.config.to_c()
+        N.check(lib.ws_ctx_create(device, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            N.load().ws_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return N.load().ws_ctx_stream(self.handle) or 0
+
+    @property
+    def cu_count(self) -> int:
+        return int(N.load().ws_ctx_cu_count(self.handle))
+
+    def set_profiling(self, on: bool) -> None:
+        N.load().ws_ctx_set_profiling(self.handle, 1 if on else 0)
+
+    def last_timings(self) -> N.WsTimings:
+        t = N.WsTimings()
+        N.check(N.load().ws_ctx_last_timings(self.handle, ctypes.byref(t)))
+        return t
+
+
+_default_ctx = {}
+_default_lock = threading.Lock()
+
+
+def default_context(device: int = 0) -> Context:
+    with _default_lock:
+        if device not in _default_ctx:
+            _default_ctx[device] = Context(device)
+        return _default_ctx[device]
+
+
+class PatternSetBuilder:
+    """PatternSetBuilder (core/src/pattern/compiler.rs:13-93)."""
+
+    def __init__(self):
+        self._patterns: List[tuple] = []
+        self._ci = False
+
+    def literal(self, s) -> "PatternSetBuilder":
+        self._patterns.append((0, _as_bytes(s), None))
+        return self
+
+    def literal_bytes(self, b) -> "PatternSetBuilder":
+        return self.literal(b)
+
+    def named_literal(self, name: str, s) -> "PatternSetBuilder":
+        self._patterns.append((0, _as_bytes(s), name))
+        return self
+
+    def regex(self, s) -> "PatternSetBuilder":
+        self._patterns.append((1, _as_bytes(s), None))
+        return self
+
+    def named_regex(self, name: str, s) -> "PatternSetBuilder":
+        self._patterns.append((1, _as_bytes(s), name))
+        return self
+
+    def case_insensitive(self, yes: bool = True) -> "PatternSetBuilder":
+        self._ci = yes
+        return self
+
+    def build(self) -> "PatternSet":
+        ps = PatternSet(list(self._patterns), self._ci)
+        ps._compiled(default_context())  # surface compile errors at build time
+        return ps
+
+
+class PatternSet:
+    """A compiled pattern set. `scan` has PatternSet::scan semantics
+    (core/src/pattern/mod.rs:112-140) and `scan_overlapping` has
+    PatternSet::scan_overlapping semantics (mod.rs:189-219); both run on the GPU."""
+
+    def __init__(self, patterns: List[tuple], case_insensitive: bool = False):
+        self.patterns = patterns
+        self.case_insensitive = case_insensitive
+        self._handles = {}
+        self._lock = threading.Lock()
+
+    @staticmethod
+    def builder() -> PatternSetBuilder:
+        return PatternSetBuilder()
+
+    def __len__(self) -> int:
+        return len(self.patterns)
+
+    def _compiled(self, ctx: Context):
+        key = id(ctx)
+        with self._lock:
+            h = self._handles.get(key)
+            if h is None:
+                lib = N.load()
+                bufs = [ctypes.create_string_buffer(b, len(b)) if b else None for _, b, _ in self.patterns]
+                arr = (N.WsPattern * len(self.patterns))()
+                for i, ((kind, b, name), buf) in enumerate(zip(self.patterns, bufs)):
+                    arr[i].kind = kind
+                    arr[i].bytes = ctypes.cast(buf, ctypes.c_void_p) if buf is not None else None
+                    arr[i].len = len(b)
+                    arr[i].name = name.encode() if name else None
+                h = ctypes.c_void_p()
+                flags = 1 if self.case_insensitive else 0
+                N.check(lib.ws_patterns_compile(ctx.handle, arr, len(self.patterns), flags, ctypes.byref(h)))
+                self._handles[key] = (h, ctx)
+                return h
+            return h[0]
+
+    def info(self, ctx: Optional[Context] = None) -> dict:
+        h = self._compiled(ctx or default_context())
+        i = N.WsPatternsInfo()
+        N.check(N.load().ws_patterns_info_get(h, ctypes.byref(i)))
+        return {name: getattr(i, name) for name, _ in N.WsPatternsInfo._fields_}
+
+    def max_pattern_len(self) -> int:
+        return int(self.info()["max_pattern_len"])
+
+    def scan(self, data) -> List[Match]:
+        return GpuMatcher(self).scan(data, mode="standard")
+
+    def scan_overlapping(self, data) -> List[Match]:
+        return GpuMatcher(self).scan(data, mode="overlapping")
+
+    def scan_count(self, data) -> int:
+        return len(self.scan(data))
+
+    def __del__(self):
+        try:
+            lib = N.load()
+            for h, _ctx in self._handles.values():
+                lib.ws_patterns_free(h)
+        except Exception:
+            pass
+
+
+def _mode_code(mode: str) -> int:
+    if mode in ("overlapping", N.MODE_OVERLAPPING):
+        return N.MODE_OVERLAPPING
+    if mode in ("standard", N.MODE_STANDARD):
+        return N.MODE_STANDARD
+    raise ValueError(f"unknown mode {mode!r}")
+
+
+def matches_to_list(arr: np.ndarray) -> List[Match]:
+    return [Match(int(a), int(b), int(c)) for a, b, c in zip(arr["pattern_id"], arr["start"], arr["end"])]
+
+
+class GpuMatcher:
+    """GpuMatcher (core/src/gpu/mod.rs:95-306); the reference README calls it
+    WarpStateer. `scan` returns the GPU contract: overlapping literal matches
+    plus leftmost-first regex matches, sorted by (start, pattern_id, end)."""
+
+    def __init__(self, patterns: PatternSet, config: Optional[AutoMatcherConfig] = None,
+                 context: Optional[Context] = None, device: int = 0):
+        if context is None:
+            context = default_context(device) if config is None else Context(device, config)
+        self.ctx = context
+        self.patterns = patterns
+        self.config = context.config
+        self.handle = patterns._compiled(context)
+
+    def max_block_size(self) -> int:
+        return self.config.chunk_size
+
+    def scan_array(self, data, mode="overlapping", tuning: Optional[Tuning] = None,
+                   capacity: Optional[int] = None) -> np.ndarray:
+        """Host input → structured numpy array (MATCH_DTYPE)."""
+        lib = N.load()
+        buf = np.frombuffer(_as_bytes(data) if not isinstance(data, np.ndarray) else data.tobytes(), dtype=np.uint8)
+        cap = capacity if capacity is not None else max(1024, min(self.config.max_matches, len(buf) // 8 + 1024))
+        t = tuning.to_c() if tuning else None
+        while True:
+            out = np.zeros(cap, dtype=N.MATCH_DTYPE)
+            count = ctypes.c_uint64(0)
+            st = lib.ws_scan(self.ctx.handle, self.handle, buf.ctypes.data if len(buf) else None, len(buf),
+                             N.MEM_HOST, _mode_code(mode), ctypes.byref(t) if t else None,
+                             out.ctypes.data, cap, N.MEM_HOST, ctypes.byref(count))
+            if st == 11 and capacity is None and count.value > cap and count.value <= self.config.max_matches:
+                cap = int(count.value)
+                continue
+            N.check(st, int(count.value))
+            return out[: count.value]
+
+    def scan(self, data, mode="overlapping", tuning: Optional[Tuning] = None) -> List[Match]:
+        return matches_to_list(self.scan_array(data, mode, tuning))
+
+    def scan_device(self, data_ptr: int, length: int, out_ptr: int, capacity: int, mode="overlapping",
+                    tuning: Optional[Tuning] = None, out_on_device: bool = True) -> int:
+        """Device-resident input (and output) — the HBM-resident fast path."""
+        lib = N.load()
+        count = ctypes.c_uint64(0)
+        t = tuning.to_c() if tuning else None
+        st = lib.ws_scan(self.ctx.handle, self.handle, ctypes.c_void_p(data_ptr), length, N.MEM_DEVICE,
+                         _mode_code(mode), ctypes.byref(t) if t else None, ctypes.c_void_p(out_ptr), capacity,
+                         N.MEM_DEVICE if out_on_device else N.MEM_HOST, ctypes.byref(count))
+        N.check(st, int(count.value))
+        return int(count.value)
+
+    def info(self) -> dict:
+        return self.patterns.info(self.ctx)
+
+
+WarpStateer = GpuMatcher
+
+
+class StreamPipeline:
+    """StreamPipeline (core/src/pipeline.rs:16-153): chunked scanning with an
+    overlap window. Unlike the reference's dedup stitching, regex chains carry
+    across chunks, so the result equals a whole-input scan."""
+
+    def __init__(self, backend: GpuMatcher, overlap_window: int):
+        self.backend = backend
+        self.overlap_window = overlap_window
+
+    def scan_array(self, data, mode="overlapping", tuning: Optional[Tuning] = None,
+                   chunk_size: Optional[int] = None) -> np.ndarray:
+        lib = N.load()
+        buf = np.frombuffer(_as_bytes(data), dtype=np.uint8)
+        chunk = chunk_size or self.backend.max_block_size()
+        cap = max(1024, min(self.backend.config.max_matches, len(buf) // 8 + 1024))
+        t = tuning.to_c() if tuning else None
+        while True:
+            out = np.zeros(cap, dtype=N.MATCH_DTYPE)
+            count = ctypes.c_uint64(0)
+            st = lib.ws_stream_scan(self.backend.ctx.handle, self.backend.handle,
+                                    buf.ctypes.data if len(buf) else None, len(buf), N.MEM_HOST, chunk,
+                                    self.overlap_window, _mode_code(mode), ctypes.byref(t) if t else None,
+                                    out.ctypes.data, cap, N.MEM_HOST, ctypes.byref(count))
+            if st == 11 and count.value > cap and count.value <= self.backend.config.max_matches:
+                cap = int(count.value)
+                continue
+            N.check(st, int(count.value))
+            return out[: count.value]
+
+    def scan(self, data, chunk_size: Optional[int] = None) -> List[Match]:
+        return matches_to_list(self.scan_array(data, chunk_size=chunk_size))
+
+
+def _batch_array(matcher: GpuMatcher, items: Sequence[ScanItem], mode: str,
+                 tuning: Optional[Tuning] = None) -> np.ndarray:
+    lib = N.load()
+    keep = []
+    arr = (N.WsItem * len(items))()
+    total = 0
+    for i, it in enumerate(items):
+        b = np.frombuffer(_as_bytes(it.data), dtype=np.uint8)
+        keep.append(b)
+        arr[i].id = int(it.id)
+        arr[i].data = b.ctypes.data if len(b) else None
+        arr[i].len = len(b)
+        total += len(b)
+    cap = max(1024, min(matcher.config.max_matches, total // 8 + 1024))
+    t = tuning.to_c() if tuning else None
+    while True:
+        out = np.zeros(cap, dtype=N.TAGGED_DTYPE)
+        count = ctypes.c_uint64(0)
+        st = lib.ws_scan_batch(matcher.ctx.handle, matcher.handle, arr, len(items), N.MEM_HOST,
+                               _mode_code(mode), ctypes.byref(t) if t else None, out.ctypes.data, cap,
+                               ctypes.byref(count))
+        if st == 11 and count.value > cap and count.value <= matcher.config.max_matches:
+            cap = int(count.value)
+            continue
+        N.check(st, int(count.value))
+        return out[: count.value]
+
+
+def _tagged_list(arr: np.ndarray) -> List[TaggedMatch]:
+    return [TaggedMatch(int(s), Match(int(p), int(a), int(b)))
+            for s, p, a, b in zip(arr["source_id"], arr["pattern_id"], arr["start"], arr["end"])]
+
+
+def scan_batch_gpu(matcher: GpuMatcher, items: Iterable[ScanItem]) -> List[TaggedMatch]:
+    """batch::scan_batch_gpu (core/src/batch.rs:336-363), per-item exact:
+    overlapping literals + leftmost-first regexes, no cross-item matches."""
+    return _tagged_list(_batch_array(matcher, list(items), "overlapping"))
+
+
+def scan_batch(matcher: GpuMatcher, items: Iterable[ScanItem]) -> List[TaggedMatch]:
+    """batch::scan_batch (core/src/batch.rs:314-320)."""
+    return scan_batch_gpu(matcher, items)
+
+
+def coalesce(items: Sequence[ScanItem]):
+    """batch::coalesce (core/src/batch.rs:65-98): (entries, buffer)."""
+    entries, buf, off = [], bytearray(), 0
+    for it in items:
+        d = _as_bytes(it.data)
+        entries.append((it.id, off, len(d)))
+        buf += d
+        off += len(d)
+    return entries, bytes(buf)

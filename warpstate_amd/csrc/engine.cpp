@@ -1,0 +1,939 @@
+// Host runtime and C ABI of the scan engine (GpuMatcher / StreamPipeline /
+// scan_batch equivalents of the reference, core/src/gpu/mod.rs,
+// core/src/pipeline.rs, core/src/batch.rs).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/warpstate.h"
+#include "kernels.hpp"
+#include "patterns.hpp"
+
+namespace ws {
+
+// Launch wrappers defined in kernels.hip.
+size_t hash_shmem_bytes(uint32_t nwords);
+hipError_t launch_hash(uint32_t K, bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c,
+                       const HashParams& h);
+uint32_t ac_unit_bytes();
+size_t ac_shmem_bytes(bool lds_table, uint32_t states, uint32_t classes);
+hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCommon& c,
+                     const AcParams& a);
+hipError_t launch_walk(uint32_t ncand_host, hipStream_t st, const WalkParams& w);
+hipError_t launch_order_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint32_t pbits,
+                             uint64_t* keys, uint32_t* vals, hipStream_t st);
+hipError_t launch_chain_keys(const Rec* raw, uint32_t n, uint32_t sbits, uint32_t rbits,
+                             uint64_t* keys, uint32_t* vals, hipStream_t st);
+hipError_t launch_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
+                              uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all, uint8_t* head,
+                              hipStream_t st);
+hipError_t launch_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
+                             uint32_t gshift, const uint8_t* head, const uint32_t* carried,
+                             uint32_t seg_off, const uint32_t* rx_pid, uint32_t ties, Rec* out,
+                             uint32_t out_cap, uint32_t* out_count, uint32_t* carried_out,
+                             hipStream_t st);
+hipError_t launch_lit_keys(const Rec* recs, uint32_t n, uint32_t lbits, uint32_t pbits, uint32_t maxlen,
+                           uint32_t len_desc, uint64_t* keys, uint32_t* vals, hipStream_t st);
+hipError_t launch_pid_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint64_t* keys, uint32_t* vals,
+                           hipStream_t st);
+hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t pbits,
+                               uint32_t start_off, void* out, hipStream_t st);
+hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t sbits,
+                              uint32_t pbits, const uint64_t* ids, void* out, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
+hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
+                      const uint32_t* vin, uint32_t* vout, uint32_t n, uint32_t bits,
+                      hipStream_t st);
+
+namespace {
+
+thread_local std::string g_err;
+
+ws_status fail(ws_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+uint32_t bits_for(uint64_t v) { return v == 0 ? 1u : 64u - (uint32_t)__builtin_clzll(v); }
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+}  // namespace
+}  // namespace ws
+
+using namespace ws;
+
+struct ws_ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  uint32_t cus = 0;
+  ws_config cfg{};
+  std::mutex mu;
+  bool profiling = false;
+  ws_timings last{};
+  hipEvent_t ev[8] = {};
+  DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
+      scratch, carried, ids;
+  uint32_t* hcount = nullptr;  // pinned host mirror of the counters
+};
+
+struct ws_patterns {
+  int dev = 0;
+  PatternsHost h;
+  DevBuf blob;
+  // Device pointers into `blob`.
+  const uint8_t* fbytes = nullptr;
+  const uint32_t *foff = nullptr, *flen = nullptr, *fown = nullptr, *lit_pid = nullptr;
+  const uint32_t *bloom = nullptr, *bstart = nullptr, *bent = nullptr, *fhash = nullptr;
+  const uint32_t *ac_table32 = nullptr, *ac_out_start = nullptr, *ac_out_list = nullptr,
+                 *ac_len = nullptr;
+  const uint16_t* ac_table16 = nullptr;
+  const uint8_t* ac_cmap = nullptr;
+  const uint32_t *rx_tables = nullptr, *rx_toff = nullptr, *rx_ncls = nullptr,
+                 *rx_pid = nullptr, *rx_maxlen = nullptr;
+  const uint8_t* rx_cmaps = nullptr;
+  uint32_t rx_states_total = 0;
+  bool ac_lds_ok = false;
+};
+
+namespace {
+
+#define WS_HIP(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess)                                                               \
+      return fail(WS_ERR_GPU_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+constexpr size_t kAcLdsBudget = 96 * 1024;
+
+// Builds one device blob holding every table of the compiled set.
+ws_status upload_patterns(ws_patterns* p) {
+  const PatternsHost& h = p->h;
+  std::vector<uint8_t> blob;
+  auto put = [&](const void* src, size_t bytes) -> size_t {
+    size_t off = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(off + std::max<size_t>(bytes, 4));
+    if (bytes) std::memcpy(blob.data() + off, src, bytes);
+    return off;
+  };
+  std::string fb;
+  std::vector<uint32_t> foff, flen, fown, aclen;
+  for (const auto& f : h.fac) {
+    foff.push_back((uint32_t)fb.size());
+    flen.push_back((uint32_t)f.bytes.size());
+    fown.push_back((f.kind << 31) | f.owner);
+    aclen.push_back(f.ac_len);
+    for (char ch : f.bytes) {
+      uint8_t b = (uint8_t)ch;
+      if (h.ci && b >= 'A' && b <= 'Z') b += 32;
+      fb.push_back((char)b);
+    }
+  }
+  size_t o_fbytes = put(fb.data(), fb.size());
+  size_t o_foff = put(foff.data(), foff.size() * 4);
+  size_t o_flen = put(flen.data(), flen.size() * 4);
+  size_t o_fown = put(fown.data(), fown.size() * 4);
+  size_t o_aclen = put(aclen.data(), aclen.size() * 4);
+  size_t o_litpid = put(h.lit_pid.data(), h.lit_pid.size() * 4);
+  size_t o_bloom = put(h.bloom.data(), h.bloom.size() * 4);
+  size_t o_bstart = put(h.bstart.data(), h.bstart.size() * 4);
+  size_t o_bent = put(h.bent.data(), h.bent.size() * 4);
+  size_t o_fhash = put(h.fhash.data(), h.fhash.size() * 4);
+  size_t o_ac32 = put(h.ac_table.data(), h.ac_table.size() * 4);
+  std::vector<uint16_t> t16;
+  p->ac_lds_ok = h.ac_ok && h.ac_states <= 65535 &&
+                 (size_t)h.ac_states * h.ac_classes * 2 <= kAcLdsBudget;
+  if (p->ac_lds_ok) {
+    t16.resize(h.ac_table.size());
+    for (size_t k = 0; k < t16.size(); ++k) t16[k] = (uint16_t)h.ac_table[k];
+  }
+  size_t o_ac16 = put(t16.data(), t16.size() * 2);
+  size_t o_accmap = put(h.ac_cmap.data(), 256);
+  size_t o_acos = put(h.ac_out_start.data(), h.ac_out_start.size() * 4);
+  size_t o_acol = put(h.ac_out_list.data(), h.ac_out_list.size() * 4);
+  std::vector<uint32_t> rxt, toff, ncls, rxmax;
+  std::vector<uint8_t> cm;
+  p->rx_states_total = 0;
+  for (size_t r = 0; r < h.rx.size(); ++r) {
+    toff.push_back((uint32_t)rxt.size());
+    ncls.push_back(h.rx[r].nclass);
+    rxt.insert(rxt.end(), h.rx[r].table.begin(), h.rx[r].table.end());
+    cm.insert(cm.end(), h.rx[r].cmap.begin(), h.rx[r].cmap.end());
+    long mx = h.rx_maxlen[r];
+    rxmax.push_back(mx < 0 ? 0xFFFFFFFFu : (uint32_t)mx);
+    p->rx_states_total += h.rx[r].nstates;
+  }
+  size_t o_rxt = put(rxt.data(), rxt.size() * 4);
+  size_t o_toff = put(toff.data(), toff.size() * 4);
+  size_t o_ncls = put(ncls.data(), ncls.size() * 4);
+  size_t o_cm = put(cm.data(), cm.size());
+  size_t o_rxpid = put(h.rx_pid.data(), h.rx_pid.size() * 4);
+  size_t o_rxmax = put(rxmax.data(), rxmax.size() * 4);
+
+  WS_HIP(p->blob.ensure(blob.size()));
+  WS_HIP(hipMemcpy(p->blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  uint8_t* b = p->blob.as<uint8_t>();
+  p->fbytes = b + o_fbytes;
+  p->foff = reinterpret_cast<const uint32_t*>(b + o_foff);
+  p->flen = reinterpret_cast<const uint32_t*>(b + o_flen);
+  p->fown = reinterpret_cast<const uint32_t*>(b + o_fown);
+  p->ac_len = reinterpret_cast<const uint32_t*>(b + o_aclen);
+  p->lit_pid = reinterpret_cast<const uint32_t*>(b + o_litpid);
+  p->bloom = reinterpret_cast<const uint32_t*>(b + o_bloom);
+  p->bstart = reinterpret_cast<const uint32_t*>(b + o_bstart);
+  p->bent = reinterpret_cast<const uint32_t*>(b + o_bent);
+  p->fhash = reinterpret_cast<const uint32_t*>(b + o_fhash);
+  p->ac_table32 = reinterpret_cast<const uint32_t*>(b + o_ac32);
+  p->ac_table16 = reinterpret_cast<const uint16_t*>(b + o_ac16);
+  p->ac_cmap = b + o_accmap;
+  p->ac_out_start = reinterpret_cast<const uint32_t*>(b + o_acos);
+  p->ac_out_list = reinterpret_cast<const uint32_t*>(b + o_acol);
+  p->rx_tables = reinterpret_cast<const uint32_t*>(b + o_rxt);
+  p->rx_toff = reinterpret_cast<const uint32_t*>(b + o_toff);
+  p->rx_ncls = reinterpret_cast<const uint32_t*>(b + o_ncls);
+  p->rx_cmaps = b + o_cm;
+  p->rx_pid = reinterpret_cast<const uint32_t*>(b + o_rxpid);
+  p->rx_maxlen = reinterpret_cast<const uint32_t*>(b + o_rxmax);
+  return WS_OK;
+}
+
+uint32_t supported_variants(const ws_patterns* p) {
+  uint32_t v = 1u << WS_VARIANT_HASH;
+  if (p->h.ac_ok) v |= 1u << WS_VARIANT_DFA_GLOBAL;
+  if (p->ac_lds_ok) v |= 1u << WS_VARIANT_DFA_LDS;
+  return v;
+}
+
+uint32_t pick_variant(const ws_patterns* p, const ws_tuning* t) {
+  uint32_t v = t ? t->variant : WS_VARIANT_AUTO;
+  if (v != WS_VARIANT_AUTO) return v;
+  // Without a tuned choice: the hash prefilter unless the window is tiny and a
+  // small factor DFA fits in LDS.
+  if (p->h.K <= 2 && p->ac_lds_ok) return WS_VARIANT_DFA_LDS;
+  return WS_VARIANT_HASH;
+}
+
+// One scan over segments already laid out in device memory.
+struct Plan {
+  const uint8_t* base = nullptr;   // 16-aligned
+  uint64_t load_end = 0;
+  std::vector<uint2> items;        // {lo, hi}
+  bool batch = false;              // explicit units per item
+  uint32_t emit_end = 0xFFFFFFFFu;
+  bool window_truncated = false;
+  uint32_t seg_off = 0;            // absolute offset of item 0 (stream chunks)
+  bool use_carried = false;
+  uint64_t max_seg_len = 0;
+};
+
+struct Result {
+  uint32_t count = 0;  // records in ctx->k1/v1 (sorted keys/values)
+  uint32_t sbits = 0, pbits = 0;
+};
+
+ws_status sort_into(ws_ctx* ctx, uint32_t n, uint32_t bits, const uint64_t* kin, uint64_t* kout,
+                    const uint32_t* vin, uint32_t* vout) {
+  size_t need = 0;
+  WS_HIP(sort_pairs(nullptr, &need, kin, kout, vin, vout, n, bits, ctx->stream));
+  WS_HIP(ctx->tmp.ensure(need));
+  size_t have = ctx->tmp.cap;
+  WS_HIP(sort_pairs(ctx->tmp.p, &have, kin, kout, vin, vout, n, bits, ctx->stream));
+  return WS_OK;
+}
+
+ws_status read_counters(ws_ctx* ctx) {
+  WS_HIP(hipMemcpyAsync(ctx->hcount, ctx->counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  WS_HIP(hipStreamSynchronize(ctx->stream));
+  return WS_OK;
+}
+
+// One greedy pass over records sorted by `keys/vals` (records read from
+// ctx->scratch, kept ones appended to ctx->recs from index 0).
+ws_status greedy_pass(ws_ctx* ctx, uint32_t n, uint32_t gshift, uint32_t maxlen, uint32_t ties,
+                      uint32_t* kept) {
+  const hipStream_t st = ctx->stream;
+  WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->scratch.as<Rec>(), n,
+                            gshift, nullptr, maxlen, ctx->head.as<uint8_t>(), st));
+  WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 1, 0, st));
+  WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->scratch.as<Rec>(), n,
+                           gshift, ctx->head.as<uint8_t>(), nullptr, 0, nullptr, ties,
+                           ctx->recs.as<Rec>(), n, ctx->counters.as<uint32_t>(), nullptr, st));
+  ws_status s = read_counters(ctx);
+  if (s != WS_OK) return s;
+  *kept = ctx->hcount[0];
+  return WS_OK;
+}
+
+// Non-overlapping literal semantics of PatternSet::scan, derived exactly from
+// the overlapping match list in ctx->recs[0, nlit):
+//  * LeftmostLongest (Aho-Corasick find_iter, core/src/cpu/scan.rs:250-293):
+//    order by (start, longest, lowest id), keep greedily.
+//  * MultiMemchr (core/src/specialize.rs:313-373): each needle's own
+//    non-overlapping occurrences, then the same greedy order.
+//  * HashScanner (core/src/hash_scan.rs:321-363): at the cursor every id of the
+//    shortest matching length, then jump to its end.
+ws_status reduce_literals(ws_ctx* ctx, uint32_t strategy, uint32_t nlit, uint32_t sbits,
+                          uint32_t pbits, uint32_t maxlit) {
+  const hipStream_t st = ctx->stream;
+  const uint32_t lbits = bits_for(maxlit);
+  if (sbits + lbits + pbits > 64)
+    return fail(WS_ERR_INPUT_TOO_LARGE, "reduction key does not fit 64 bits");
+  uint32_t n = nlit;
+  if (strategy == STRAT_MULTI_MEMCHR) {
+    WS_HIP(hipMemcpyAsync(ctx->scratch.p, ctx->recs.p, (size_t)n * sizeof(Rec), hipMemcpyDeviceToDevice, st));
+    WS_HIP(launch_pid_keys(ctx->scratch.as<Rec>(), n, sbits, ctx->k0.as<uint64_t>(), ctx->v0.as<uint32_t>(), st));
+    ws_status s = sort_into(ctx, n, sbits + pbits, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
+                            ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>());
+    if (s != WS_OK) return s;
+    s = greedy_pass(ctx, n, sbits, maxlit, 0, &n);
+    if (s != WS_OK) return s;
+  }
+  WS_HIP(hipMemcpyAsync(ctx->scratch.p, ctx->recs.p, (size_t)n * sizeof(Rec), hipMemcpyDeviceToDevice, st));
+  const uint32_t desc = strategy == STRAT_HASH_SHORTEST ? 0u : 1u;
+  WS_HIP(launch_lit_keys(ctx->scratch.as<Rec>(), n, lbits, pbits, maxlit, desc, ctx->k0.as<uint64_t>(),
+                         ctx->v0.as<uint32_t>(), st));
+  ws_status s = sort_into(ctx, n, sbits + lbits + pbits, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
+                          ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>());
+  if (s != WS_OK) return s;
+  s = greedy_pass(ctx, n, 64, maxlit, strategy == STRAT_HASH_SHORTEST ? 1u : 0u, &n);
+  if (s != WS_OK) return s;
+  ctx->hcount[0] = n;
+  return WS_OK;
+}
+
+ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t mode,
+                   const ws_tuning* tuning, Result* res) {
+  const PatternsHost& h = p->h;
+  const hipStream_t st = ctx->stream;
+  const uint32_t variant = pick_variant(p, tuning);
+  if (!(supported_variants(p) & (1u << variant)))
+    return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
+  const uint32_t unit = variant == WS_VARIANT_HASH ? kHashUnit : ac_unit_bytes();
+  std::vector<uint2> units;
+  uint32_t nunits;
+  if (plan.batch) {
+    for (uint32_t i = 0; i < plan.items.size(); ++i) {
+      uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
+      for (uint64_t t = lo; t < hi; t += unit) units.push_back(make_uint2((uint32_t)t, i));
+    }
+    nunits = (uint32_t)units.size();
+  } else {
+    nunits = (uint32_t)((plan.items[0].y + unit - 1) / unit);
+  }
+  WS_HIP(ctx->items.ensure(plan.items.size() * sizeof(uint2)));
+  WS_HIP(hipMemcpyAsync(ctx->items.p, plan.items.data(), plan.items.size() * sizeof(uint2),
+                        hipMemcpyHostToDevice, st));
+  if (plan.batch && !units.empty()) {
+    WS_HIP(ctx->units.ensure(units.size() * sizeof(uint2)));
+    WS_HIP(hipMemcpyAsync(ctx->units.p, units.data(), units.size() * sizeof(uint2),
+                          hipMemcpyHostToDevice, st));
+  }
+  const uint32_t max_matches = (uint32_t)std::min<uint64_t>(ctx->cfg.max_matches, 0xFFFFFFF0ull);
+  uint32_t lit_cap = (uint32_t)std::min<uint64_t>(ctx->recs.cap / sizeof(Rec), max_matches);
+  uint32_t cand_cap = (uint32_t)(ctx->cands.cap / sizeof(Rec));
+  if (lit_cap < 4096) {
+    WS_HIP(ctx->recs.ensure(std::min<uint64_t>(max_matches, 1u << 16) * sizeof(Rec)));
+    lit_cap = (uint32_t)std::min<uint64_t>(ctx->recs.cap / sizeof(Rec), max_matches);
+  }
+  if (cand_cap < 4096) {
+    WS_HIP(ctx->cands.ensure((1u << 16) * sizeof(Rec)));
+    cand_cap = (uint32_t)(ctx->cands.cap / sizeof(Rec));
+  }
+  const uint32_t per_cu = tuning && tuning->blocks_per_cu ? tuning->blocks_per_cu : 4;
+  const uint32_t grid = std::max(1u, std::min(nunits, ctx->cus * per_cu));
+
+  if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[0], st));
+  for (int attempt = 0;; ++attempt) {
+    WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 4, 0, st));
+    ScanCommon c{};
+    c.base = plan.base;
+    c.load_end = plan.load_end;
+    c.units = plan.batch ? ctx->units.as<uint2>() : nullptr;
+    c.nunits = nunits;
+    c.items = ctx->items.as<uint2>();
+    c.emit_end = plan.emit_end;
+    c.ci = h.ci ? 1u : 0u;
+    c.fbytes = p->fbytes;
+    c.foff = p->foff;
+    c.flen = p->flen;
+    c.fown = p->fown;
+    c.lit_pid = p->lit_pid;
+    c.lit_out = ctx->recs.as<Rec>();
+    c.lit_cap = lit_cap;
+    c.cand_out = ctx->cands.as<Rec>();
+    c.cand_cap = cand_cap;
+    c.counters = ctx->counters.as<uint32_t>();
+    if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[1], st));
+    if (nunits > 0) {
+      if (variant == WS_VARIANT_HASH) {
+        HashParams hp{p->bloom, (uint32_t)h.bloom.size(), h.wshift, h.wbits, h.B, h.NBK, h.nbits,
+                      p->bstart, p->bent, p->fhash};
+        WS_HIP(launch_hash(h.K, h.ci, grid, st, c, hp));
+      } else {
+        AcParams ap{p->ac_table32, p->ac_table16, p->ac_cmap, p->ac_out_start, p->ac_out_list,
+                    p->ac_len, h.ac_states, h.ac_classes, h.ac_first_out, h.ac_maxlen};
+        WS_HIP(launch_ac(variant == WS_VARIANT_DFA_LDS, grid, st, c, ap));
+      }
+    }
+    if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[2], st));
+    ws_status s = read_counters(ctx);
+    if (s != WS_OK) return s;
+    const uint32_t nlit = ctx->hcount[0], ncand = ctx->hcount[1];
+    if (nlit > max_matches) {
+      res->count = nlit;
+      return fail(WS_ERR_MATCH_BUFFER_OVERFLOW,
+                  "too many matches (" + std::to_string(nlit) + " exceeds " +
+                      std::to_string(max_matches) + ")");
+    }
+    bool retry = false;
+    if (nlit > lit_cap) {
+      WS_HIP(ctx->recs.ensure((size_t)nlit * 2 * sizeof(Rec) > (size_t)max_matches * sizeof(Rec) * 2
+                                  ? (size_t)max_matches * 2 * sizeof(Rec)
+                                  : (size_t)nlit * 2 * sizeof(Rec)));
+      lit_cap = (uint32_t)std::min<uint64_t>(ctx->recs.cap / sizeof(Rec), max_matches);
+      retry = true;
+    }
+    if (ncand > cand_cap) {
+      WS_HIP(ctx->cands.ensure((size_t)ncand * 5 / 4 * sizeof(Rec)));
+      cand_cap = (uint32_t)(ctx->cands.cap / sizeof(Rec));
+      retry = true;
+    }
+    if (!retry) break;
+    if (attempt >= 2) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
+  }
+  uint32_t nlit = ctx->hcount[0];
+  const uint32_t ncand = ctx->hcount[1];
+  res->sbits = bits_for(plan.max_seg_len);
+  res->pbits = bits_for(h.npat ? h.npat - 1 : 0);
+  const uint32_t ibits = plan.batch ? bits_for(plan.items.size() - 1) : 0;
+  if (res->sbits + res->pbits + ibits > 64)
+    return fail(WS_ERR_INPUT_TOO_LARGE, "batch key does not fit 64 bits (items x length x patterns)");
+
+  // Literal reductions for PatternSet::scan semantics.
+  if (mode == WS_MODE_STANDARD && nlit > 0 && h.strategy != STRAT_NONE) {
+    if (plan.batch)
+      return fail(WS_ERR_INVALID_ARGUMENT, "standard mode is not available for batch scans yet");
+    const size_t need = (size_t)nlit;
+    WS_HIP(ctx->k0.ensure(need * 8));
+    WS_HIP(ctx->k1.ensure(need * 8));
+    WS_HIP(ctx->v0.ensure(need * 4));
+    WS_HIP(ctx->v1.ensure(need * 4));
+    WS_HIP(ctx->head.ensure(need));
+    WS_HIP(ctx->scratch.ensure(need * sizeof(Rec)));
+    uint32_t maxlit = 0;
+    for (auto& l : h.lits) maxlit = std::max<uint32_t>(maxlit, (uint32_t)l.size());
+    ws_status rs = reduce_literals(ctx, h.strategy, nlit, res->sbits, res->pbits, maxlit);
+    if (rs != WS_OK) return rs;
+    nlit = ctx->hcount[0];
+  }
+
+  // Regex candidates: anchored walks, then per-(item, regex) chains.
+  if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[3], st));
+  ctx->last.candidates = ncand;
+  if (ncand > 0) {
+    WS_HIP(ctx->raw.ensure((size_t)ncand * sizeof(Rec)));
+    WalkParams w{};
+    w.base = plan.base;
+    w.items = ctx->items.as<uint2>();
+    w.cands = ctx->cands.as<Rec>();
+    w.ncand = ctx->counters.as<uint32_t>() + 1;
+    w.cand_cap = cand_cap;
+    w.tables = p->rx_tables;
+    w.toff = p->rx_toff;
+    w.ncls = p->rx_ncls;
+    w.cmaps = p->rx_cmaps;
+    w.rx_pid = p->rx_pid;
+    w.raw = ctx->raw.as<Rec>();
+    w.raw_cap = ncand;
+    w.counters = ctx->counters.as<uint32_t>();
+    w.window_truncated = plan.window_truncated ? 1u : 0u;
+    WS_HIP(launch_walk(ncand, st, w));
+    ws_status s = read_counters(ctx);
+    if (s != WS_OK) return s;
+    if (ctx->hcount[3] & kFlagTruncatedWalk)
+      return fail(WS_ERR_INPUT_TOO_LARGE,
+                  "a regex match may extend past the chunk overlap; raise the overlap to the "
+                  "longest possible match");
+    const uint32_t nraw = ctx->hcount[2];
+    if (nraw > 0) {
+      const uint32_t rbits = bits_for(h.rx.size() - 1);
+      const uint32_t cbits = res->sbits + rbits + ibits;
+      if (cbits > 64) return fail(WS_ERR_INPUT_TOO_LARGE, "chain key does not fit 64 bits");
+      WS_HIP(ctx->k0.ensure((size_t)nraw * 8));
+      WS_HIP(ctx->k1.ensure((size_t)nraw * 8));
+      WS_HIP(ctx->v0.ensure((size_t)nraw * 4));
+      WS_HIP(ctx->v1.ensure((size_t)nraw * 4));
+      WS_HIP(ctx->head.ensure(nraw));
+      WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), nraw, res->sbits, rbits, ctx->k0.as<uint64_t>(),
+                               ctx->v0.as<uint32_t>(), st));
+      ws_status s2 = sort_into(ctx, nraw, cbits, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
+                               ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>());
+      if (s2 != WS_OK) return s2;
+      WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(),
+                                nraw, res->sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st));
+      // The chain appends kept matches after the literal matches.
+      if ((size_t)nlit + nraw > ctx->recs.cap / sizeof(Rec)) {
+        // Grow while keeping the literal matches.
+        DevBuf bigger;
+        WS_HIP(bigger.ensure(((size_t)nlit + nraw) * sizeof(Rec)));
+        WS_HIP(hipMemcpyAsync(bigger.p, ctx->recs.p, (size_t)nlit * sizeof(Rec),
+                              hipMemcpyDeviceToDevice, st));
+        WS_HIP(hipStreamSynchronize(st));
+        ctx->recs.release();
+        ctx->recs = bigger;
+      }
+      WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 1, nlit, st));
+      WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(),
+                               nraw, res->sbits, ctx->head.as<uint8_t>(),
+                               plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr,
+                               plan.seg_off, p->rx_pid, 0u, ctx->recs.as<Rec>(),
+                               (uint32_t)(ctx->recs.cap / sizeof(Rec)), ctx->counters.as<uint32_t>(),
+                               plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, st));
+      s = read_counters(ctx);
+      if (s != WS_OK) return s;
+      nlit = ctx->hcount[0];
+    }
+  }
+  if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[4], st));
+  const uint32_t total = nlit;
+  ctx->last.raw_matches = total;
+  if (total > max_matches) {
+    res->count = total;
+    return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(total) +
+                                                  " exceeds " + std::to_string(max_matches) + ")");
+  }
+  // Order by (item, start, pattern_id).
+  if (total > 0) {
+    WS_HIP(ctx->k0.ensure((size_t)total * 8));
+    WS_HIP(ctx->k1.ensure((size_t)total * 8));
+    WS_HIP(ctx->v0.ensure((size_t)total * 4));
+    WS_HIP(ctx->v1.ensure((size_t)total * 4));
+    WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), total, res->sbits, res->pbits,
+                             ctx->k0.as<uint64_t>(), ctx->v0.as<uint32_t>(), st));
+    ws_status s = sort_into(ctx, total, res->sbits + res->pbits + ibits, ctx->k0.as<uint64_t>(),
+                            ctx->k1.as<uint64_t>(), ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>());
+    if (s != WS_OK) return s;
+  }
+  res->count = total;
+  return WS_OK;
+}
+
+void finish_timings(ws_ctx* ctx) {
+  if (!ctx->profiling) return;
+  (void)hipEventRecord(ctx->ev[5], ctx->stream);
+  (void)hipEventSynchronize(ctx->ev[5]);
+  float a = 0, b = 0, c2 = 0, d = 0;
+  (void)hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]);
+  (void)hipEventElapsedTime(&b, ctx->ev[3], ctx->ev[4]);
+  (void)hipEventElapsedTime(&c2, ctx->ev[4], ctx->ev[5]);
+  (void)hipEventElapsedTime(&d, ctx->ev[0], ctx->ev[5]);
+  ctx->last.prefilter_ms = a;
+  ctx->last.regex_ms = b;
+  ctx->last.order_ms = c2;
+  ctx->last.total_ms = d;
+}
+
+bool is_device_ptr(const void* p) {
+  hipPointerAttribute_t attr{};
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice;
+}
+
+// Lays out one contiguous input (host or device) as a single-segment plan.
+ws_status plan_single(ws_ctx* ctx, const uint8_t* data, uint64_t len, uint32_t data_mem, Plan* plan) {
+  if (len > 0xFFFFFFFFull - 16)
+    return fail(WS_ERR_INPUT_TOO_LARGE, "scan input is too large (" + std::to_string(len) + " bytes)");
+  const uint8_t* dptr = data;
+  if (data_mem == WS_MEM_HOST) {
+    WS_HIP(ctx->input.ensure(((len + 15) & ~15ull) + 16));
+    WS_HIP(hipMemcpyAsync(ctx->input.p, data, len, hipMemcpyHostToDevice, ctx->stream));
+    dptr = ctx->input.as<uint8_t>();
+  }
+  const uintptr_t a = reinterpret_cast<uintptr_t>(dptr);
+  plan->base = reinterpret_cast<const uint8_t*>(a & ~(uintptr_t)15);
+  const uint32_t shift = (uint32_t)(a & 15);
+  plan->items = {make_uint2(shift, (uint32_t)(shift + len))};
+  plan->load_end = (shift + len + 15) & ~15ull;
+  plan->max_seg_len = len;
+  plan->batch = false;
+  return WS_OK;
+}
+
+ws_status pack_out(ws_ctx* ctx, const Result& r, uint32_t start_off, ws_match* out, uint64_t cap,
+                   uint32_t out_mem, uint64_t written) {
+  uint32_t n = r.count;
+  if (written >= cap) return WS_OK;
+  if (written + n > cap) n = (uint32_t)(cap - written);
+  if (n == 0) return WS_OK;
+  if (out_mem == WS_MEM_DEVICE) {
+    WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.pbits, start_off,
+                               out + written, ctx->stream));
+    WS_HIP(hipStreamSynchronize(ctx->stream));
+  } else {
+    WS_HIP(ctx->outbuf.ensure((size_t)n * sizeof(ws_match)));
+    WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.pbits, start_off,
+                               ctx->outbuf.p, ctx->stream));
+    WS_HIP(hipMemcpyAsync(out + written, ctx->outbuf.p, (size_t)n * sizeof(ws_match),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    WS_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return WS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t ws_abi_version(void) { return WS_ABI_VERSION; }
+
+const char* ws_last_error(void) { return g_err.c_str(); }
+
+ws_status ws_ctx_create(int device_ordinal, const ws_config* cfg, ws_ctx** out) {
+  if (!out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return fail(WS_ERR_NO_GPU_ADAPTER, "no GPU adapter available");
+  }
+  if (device_ordinal < 0 || device_ordinal >= n)
+    return fail(WS_ERR_NO_GPU_ADAPTER, "device ordinal out of range");
+  ws_ctx* ctx = new ws_ctx();
+  ctx->dev = device_ordinal;
+  if (cfg) ctx->cfg = *cfg;
+  if (!ctx->cfg.max_matches) ctx->cfg.max_matches = 1000000;
+  if (!ctx->cfg.chunk_size) ctx->cfg.chunk_size = 128ull << 20;
+  if (!ctx->cfg.chunk_overlap) ctx->cfg.chunk_overlap = 4096;
+  auto bail = [&](const char* what, hipError_t e) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    delete ctx;
+    return fail(WS_ERR_GPU_DEVICE, m);
+  };
+  hipError_t e = hipSetDevice(device_ordinal);
+  if (e != hipSuccess) return bail("hipSetDevice", e);
+  hipDeviceProp_t prop{};
+  e = hipGetDeviceProperties(&prop, device_ordinal);
+  if (e != hipSuccess) return bail("hipGetDeviceProperties", e);
+  ctx->cus = (uint32_t)prop.multiProcessorCount;
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail("hipStreamCreate", e);
+  for (auto& ev : ctx->ev) {
+    e = hipEventCreate(&ev);
+    if (e != hipSuccess) return bail("hipEventCreate", e);
+  }
+  e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hcount), 16 * sizeof(uint32_t), hipHostMallocDefault);
+  if (e != hipSuccess) return bail("hipHostMalloc", e);
+  e = ctx->counters.ensure(16 * sizeof(uint32_t));
+  if (e != hipSuccess) return bail("hipMalloc", e);
+  *out = ctx;
+  return WS_OK;
+}
+
+void ws_ctx_destroy(ws_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->dev);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (DevBuf* b : {&ctx->recs, &ctx->cands, &ctx->raw, &ctx->k0, &ctx->v0, &ctx->k1, &ctx->v1,
+                    &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
+                    &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids})
+    b->release();
+  for (auto& ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (ctx->hcount) (void)hipHostFree(ctx->hcount);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+void* ws_ctx_stream(ws_ctx* ctx) { return ctx ? reinterpret_cast<void*>(ctx->stream) : nullptr; }
+
+uint32_t ws_ctx_cu_count(ws_ctx* ctx) { return ctx ? ctx->cus : 0; }
+
+void ws_ctx_set_profiling(ws_ctx* ctx, int on) {
+  if (ctx) ctx->profiling = on != 0;
+}
+
+ws_status ws_ctx_last_timings(ws_ctx* ctx, ws_timings* out) {
+  if (!ctx || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  *out = ctx->last;
+  return WS_OK;
+}
+
+ws_status ws_patterns_compile(ws_ctx* ctx, const ws_pattern* pats, size_t count, uint32_t flags,
+                              ws_patterns** out) {
+  if (!ctx || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (count == 0) return fail(WS_ERR_EMPTY_PATTERN_SET, "pattern set is empty. Fix: add at least one pattern");
+  if (!pats) return fail(WS_ERR_INVALID_ARGUMENT, "patterns is null");
+  std::vector<PatternSrc> src(count);
+  for (size_t i = 0; i < count; ++i) {
+    src[i].kind = pats[i].kind;
+    if (pats[i].len && !pats[i].bytes) return fail(WS_ERR_INVALID_ARGUMENT, "pattern bytes are null");
+    src[i].bytes.assign(reinterpret_cast<const char*>(pats[i].bytes), pats[i].len);
+  }
+  ws_patterns* p = new ws_patterns();
+  try {
+    p->h = compile_patterns(src, (flags & WS_FLAG_CASE_INSENSITIVE) != 0);
+  } catch (const CompileError& e) {
+    delete p;
+    return fail((ws_status)e.status, e.msg);
+  } catch (const std::bad_alloc&) {
+    delete p;
+    return fail(WS_ERR_PATTERN_SET_TOO_LARGE, "out of host memory compiling the pattern set");
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  p->dev = ctx->dev;
+  if (hipSetDevice(ctx->dev) != hipSuccess) {
+    delete p;
+    return fail(WS_ERR_GPU_DEVICE, "hipSetDevice failed");
+  }
+  ws_status s = upload_patterns(p);
+  if (s != WS_OK) {
+    p->blob.release();
+    delete p;
+    return s;
+  }
+  *out = p;
+  return WS_OK;
+}
+
+void ws_patterns_free(ws_patterns* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->dev);
+  (void)hipDeviceSynchronize();
+  p->blob.release();
+  delete p;
+}
+
+ws_status ws_patterns_info_get(const ws_patterns* p, ws_patterns_info* out) {
+  if (!p || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  const PatternsHost& h = p->h;
+  ws_patterns_info i{};
+  i.pattern_count = h.npat;
+  i.literal_count = (uint32_t)h.lits.size();
+  i.regex_count = (uint32_t)h.rx.size();
+  i.factor_count = (uint32_t)h.fac.size();
+  i.min_factor_len = h.min_flen;
+  i.max_factor_len = h.max_flen;
+  i.min_literal_len = h.min_lit_len;
+  i.max_pattern_len = h.max_match_len;
+  i.dfa_states = h.ac_states;
+  i.dfa_classes = h.ac_classes;
+  i.regex_dfa_states = p->rx_states_total;
+  i.hash_window = h.K;
+  i.variants_supported = supported_variants(p);
+  *out = i;
+  return WS_OK;
+}
+
+ws_status ws_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64_t len,
+                  uint32_t data_mem, uint32_t mode, const ws_tuning* tuning, ws_match* out,
+                  uint64_t cap, uint32_t out_mem, uint64_t* count) {
+  if (!ctx || !p || !count) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (len && !data) return fail(WS_ERR_INVALID_ARGUMENT, "data is null");
+  if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
+  if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");
+  *count = 0;
+  if (len == 0) return WS_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  WS_HIP(hipSetDevice(ctx->dev));
+  ctx->last = ws_timings{};
+  Plan plan;
+  ws_status s = plan_single(ctx, data, len, data_mem, &plan);
+  if (s != WS_OK) return s;
+  Result r;
+  s = run_plan(ctx, p, plan, mode, tuning, &r);
+  if (s != WS_OK) {
+    if (s == WS_ERR_MATCH_BUFFER_OVERFLOW) *count = r.count;
+    return s;
+  }
+  *count = r.count;
+  s = pack_out(ctx, r, 0, out, cap, out_mem, 0);
+  finish_timings(ctx);
+  if (s != WS_OK) return s;
+  if (r.count > cap)
+    return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(r.count) +
+                                                  " exceeds " + std::to_string(cap) + ")");
+  return WS_OK;
+}
+
+ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64_t len,
+                         uint32_t data_mem, uint64_t chunk_size, uint64_t overlap, uint32_t mode,
+                         const ws_tuning* tuning, ws_match* out, uint64_t cap, uint32_t out_mem,
+                         uint64_t* count) {
+  if (!ctx || !p || !count) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (len && !data) return fail(WS_ERR_INVALID_ARGUMENT, "data is null");
+  if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
+  if (chunk_size == 0) return fail(WS_ERR_INVALID_ARGUMENT, "chunk_size must be positive");
+  if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");
+  if (mode == WS_MODE_STANDARD && !p->h.lits.empty() && p->h.strategy != STRAT_NONE &&
+      len > chunk_size)
+    return fail(WS_ERR_INVALID_ARGUMENT,
+                "standard-mode literal reductions are not chunk-stitched; use overlapping mode");
+  *count = 0;
+  if (len == 0) return WS_OK;
+  if (len > 0xFFFFFFFFull - 16) return fail(WS_ERR_INPUT_TOO_LARGE, "stream input is too large");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  WS_HIP(hipSetDevice(ctx->dev));
+  ctx->last = ws_timings{};
+  const uint32_t nrx = (uint32_t)p->h.rx.size();
+  if (nrx) {
+    WS_HIP(ctx->carried.ensure(nrx * sizeof(uint32_t)));
+    WS_HIP(launch_fill_u32(ctx->carried.as<uint32_t>(), nrx, 0, ctx->stream));
+  }
+  uint64_t total = 0;
+  for (uint64_t a = 0; a < len; a += chunk_size) {
+    const uint64_t b = std::min(len, a + chunk_size);
+    const uint64_t e = std::min(len, b + overlap);
+    Plan plan;
+    ws_status s = plan_single(ctx, data + a, e - a, data_mem, &plan);
+    if (s != WS_OK) return s;
+    plan.emit_end = (uint32_t)(b - a);
+    plan.window_truncated = e < len;
+    plan.seg_off = (uint32_t)a;
+    plan.use_carried = nrx > 0;
+    Result r;
+    s = run_plan(ctx, p, plan, mode, tuning, &r);
+    if (s != WS_OK) {
+      if (s == WS_ERR_MATCH_BUFFER_OVERFLOW) *count = total + r.count;
+      return s;
+    }
+    s = pack_out(ctx, r, (uint32_t)a, out, cap, out_mem, total);
+    if (s != WS_OK) return s;
+    total += r.count;
+  }
+  finish_timings(ctx);
+  *count = total;
+  if (total > cap)
+    return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(total) +
+                                                  " exceeds " + std::to_string(cap) + ")");
+  return WS_OK;
+}
+
+ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items, size_t n_items,
+                        uint32_t data_mem, uint32_t mode, const ws_tuning* tuning,
+                        ws_tagged_match* out, uint64_t cap, uint64_t* count) {
+  if (!ctx || !p || !count) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (n_items && !items) return fail(WS_ERR_INVALID_ARGUMENT, "items is null");
+  if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
+  if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");
+  *count = 0;
+  if (n_items == 0) return WS_OK;
+  if (n_items > 0x7FFFFFFFull) return fail(WS_ERR_INPUT_TOO_LARGE, "too many batch items");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  WS_HIP(hipSetDevice(ctx->dev));
+  ctx->last = ws_timings{};
+  // Items are ordered by source id so the ordering key carries the id rank
+  // (batch output order is (source_id, start, pattern_id, end)).
+  std::vector<uint32_t> order(n_items);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t x, uint32_t y) { return items[x].id < items[y].id; });
+  Plan plan;
+  plan.batch = true;
+  std::vector<uint64_t> ids;
+  ids.reserve(n_items);
+  uint64_t max_len = 0;
+  if (data_mem == WS_MEM_HOST) {
+    // Coalesce into the device input buffer, each item 16-byte aligned.
+    std::vector<uint64_t> off(n_items);
+    uint64_t tot = 0;
+    for (size_t k = 0; k < n_items; ++k) {
+      const ws_item& it = items[order[k]];
+      if (it.len && !it.data) return fail(WS_ERR_INVALID_ARGUMENT, "item data is null");
+      off[k] = tot;
+      tot += (it.len + 15) & ~15ull;
+    }
+    if (tot > 0xFFFFFFFFull - 16)
+      return fail(WS_ERR_INPUT_TOO_LARGE, "coalesced batch exceeds 4 GiB");
+    std::vector<uint8_t> host(tot, 0);
+    for (size_t k = 0; k < n_items; ++k) {
+      const ws_item& it = items[order[k]];
+      if (it.len) std::memcpy(host.data() + off[k], it.data, it.len);
+      plan.items.push_back(make_uint2((uint32_t)off[k], (uint32_t)(off[k] + it.len)));
+      ids.push_back(it.id);
+      max_len = std::max<uint64_t>(max_len, it.len);
+    }
+    WS_HIP(ctx->input.ensure(tot + 16));
+    WS_HIP(hipMemcpyAsync(ctx->input.p, host.data(), tot, hipMemcpyHostToDevice, ctx->stream));
+    WS_HIP(hipStreamSynchronize(ctx->stream));
+    plan.base = ctx->input.as<uint8_t>();
+    plan.load_end = tot;
+  } else {
+    uintptr_t lo = UINTPTR_MAX, hi = 0;
+    for (size_t k = 0; k < n_items; ++k) {
+      const ws_item& it = items[k];
+      if (!it.len) continue;
+      if (!it.data) return fail(WS_ERR_INVALID_ARGUMENT, "item data is null");
+      lo = std::min(lo, reinterpret_cast<uintptr_t>(it.data));
+      hi = std::max(hi, reinterpret_cast<uintptr_t>(it.data) + (uintptr_t)it.len);
+    }
+    if (lo == UINTPTR_MAX) lo = hi = 0;
+    const uintptr_t base = lo & ~(uintptr_t)15;
+    if (hi - base > 0xFFFFFFFFull - 16)
+      return fail(WS_ERR_INPUT_TOO_LARGE, "device batch items span more than 4 GiB");
+    for (size_t k = 0; k < n_items; ++k) {
+      const ws_item& it = items[order[k]];
+      uint64_t o = it.len ? reinterpret_cast<uintptr_t>(it.data) - base : 0;
+      plan.items.push_back(make_uint2((uint32_t)o, (uint32_t)(o + it.len)));
+      ids.push_back(it.id);
+      max_len = std::max<uint64_t>(max_len, it.len);
+    }
+    plan.base = reinterpret_cast<const uint8_t*>(base);
+    plan.load_end = (hi - base + 15) & ~15ull;
+  }
+  plan.max_seg_len = max_len;
+  Result r;
+  ws_status s = run_plan(ctx, p, plan, mode, tuning, &r);
+  if (s != WS_OK) {
+    if (s == WS_ERR_MATCH_BUFFER_OVERFLOW) *count = r.count;
+    return s;
+  }
+  *count = r.count;
+  uint32_t n = (uint32_t)std::min<uint64_t>(r.count, cap);
+  if (n) {
+    WS_HIP(ctx->ids.ensure(ids.size() * 8));
+    WS_HIP(hipMemcpyAsync(ctx->ids.p, ids.data(), ids.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    WS_HIP(ctx->outbuf.ensure((size_t)n * sizeof(ws_tagged_match)));
+    WS_HIP(launch_pack_tagged(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.sbits, r.pbits,
+                              ctx->ids.as<uint64_t>(), ctx->outbuf.p, ctx->stream));
+    WS_HIP(hipMemcpyAsync(out, ctx->outbuf.p, (size_t)n * sizeof(ws_tagged_match),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    WS_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  finish_timings(ctx);
+  if (r.count > cap)
+    return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(r.count) +
+                                                  " exceeds " + std::to_string(cap) + ")");
+  return WS_OK;
+}
+
+}  // extern "C"

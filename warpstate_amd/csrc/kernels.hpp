@@ -1,0 +1,77 @@
+// Device-side parameter blocks shared by the kernels and the host runtime.
+#pragma once
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace ws {
+
+constexpr int kTPB = 256;                 // threads per block for the scan kernels
+constexpr uint32_t kHashUnit = kTPB * 16; // bytes per hash-kernel work unit
+constexpr uint32_t kRecCap = 256;         // per-block LDS staging records (each list)
+
+// Match / candidate record. Match: {pattern_id, start, end, item}.
+// Regex candidate: {regex index, start, 0, item}. Offsets are relative to the
+// item (segment) start.
+using Rec = uint4;
+
+struct ScanCommon {
+  const uint8_t* base;  // 16-byte aligned base of the scanned buffer
+  uint64_t load_end;    // bytes [0, load_end) may be loaded; multiple of 16
+  const uint2* units;   // {tile_base, item} per work unit, or null (one segment)
+  uint32_t nunits;
+  const uint2* items;   // {lo, hi} segment bounds in base coordinates
+  uint32_t emit_end;    // only starts (relative to lo) below this are emitted
+  uint32_t ci;          // ASCII case folding
+  // Factors: verification strings and owners.
+  const uint8_t* fbytes;
+  const uint32_t* foff;
+  const uint32_t* flen;
+  const uint32_t* fown;     // kind << 31 | owner
+  const uint32_t* lit_pid;  // literal index -> pattern id
+  // Outputs.
+  Rec* lit_out;
+  uint32_t lit_cap;
+  Rec* cand_out;
+  uint32_t cand_cap;
+  uint32_t* counters;  // [0] matches, [1] candidates, [2] raw regex matches, [3] flags
+};
+
+struct HashParams {
+  const uint32_t* bloom;
+  uint32_t nwords, wshift, wbits, B, NBK, nbits;
+  const uint32_t* bstart;
+  const uint32_t* bent;
+  const uint32_t* fhash;
+};
+
+struct AcParams {
+  const uint32_t* table32;
+  const uint16_t* table16;
+  const uint8_t* cmap;
+  const uint32_t* out_start;
+  const uint32_t* out_list;
+  const uint32_t* aclen;  // per factor: length the automaton recognises
+  uint32_t states, classes, first_out, maxlen;
+};
+
+struct WalkParams {
+  const uint8_t* base;
+  const uint2* items;
+  const Rec* cands;
+  const uint32_t* ncand;
+  uint32_t cand_cap;
+  const uint32_t* tables;
+  const uint32_t* toff;
+  const uint32_t* ncls;
+  const uint8_t* cmaps;
+  const uint32_t* rx_pid;
+  Rec* raw;
+  uint32_t raw_cap;
+  uint32_t* counters;
+  uint32_t window_truncated;  // segment ends before the true input end
+};
+
+enum : uint32_t { kFlagTruncatedWalk = 1u };
+
+}  // namespace ws

@@ -1,0 +1,273 @@
+// Pattern-set compilation: literals, regex DFAs, prefilter factors, the
+// rolling-hash tables and the dense factor automaton.
+#include "patterns.hpp"
+
+#include <algorithm>
+#include <deque>
+#include <map>
+
+#include "../../include/warpstate.h"
+
+namespace ws {
+namespace {
+
+constexpr uint32_t kAcMaxLen = 12;        // factor DFA recognises literal prefixes up to this
+constexpr uint32_t kAcMaxStates = 1u << 22;
+constexpr size_t kNfaMaxStates = 200000;  // per regex
+constexpr size_t kDfaMaxStates = 50000;   // per regex
+constexpr int kFactorMaxLen = 8;
+constexpr int kFactorFanout = 4;
+constexpr int kFactorMaxCount = 256;
+
+inline uint8_t fold(uint8_t b, bool ci) { return (ci && b >= 'A' && b <= 'Z') ? b + 32 : b; }
+
+uint32_t ceil_log2(uint64_t x) {
+  uint32_t r = 0;
+  while ((1ull << r) < x) ++r;
+  return r;
+}
+
+void build_hash(PatternsHost& ps) {
+  const uint32_t F = (uint32_t)ps.fac.size();
+  static const uint32_t kAllowed[] = {1, 2, 3, 4, 6, 8, 12, 16};
+  uint32_t K = 1;
+  for (uint32_t k : kAllowed) if (k <= ps.min_flen) K = k;
+  ps.K = K;
+  ps.B = K <= 2 ? 256u : 0x3779B1u;
+  uint64_t bk = 1;
+  for (uint32_t t = 0; t < K; ++t) bk = (bk * ps.B) & 0xFFFFFFu;
+  ps.NBK = (uint32_t)((0x1000000u - bk) & 0xFFFFFFu);
+  if (K <= 2) {
+    ps.wbits = 8 * K - 5;
+    ps.wshift = 5;
+  } else {
+    uint32_t lb = std::min(18u, std::max(10u, ceil_log2(F) + 4));
+    ps.wbits = lb - 5;
+    ps.wshift = 24 - ps.wbits;
+  }
+  ps.bloom.assign(1u << ps.wbits, 0);
+  ps.nbits = std::max(1u, ceil_log2(2ull * F));
+  const uint32_t nb = 1u << ps.nbits;
+  ps.fhash.resize(F);
+  std::vector<uint32_t> bucket(F), cnt(nb + 1, 0);
+  for (uint32_t f = 0; f < F; ++f) {
+    uint32_t h = rolling_hash((const uint8_t*)ps.fac[f].bytes.data(), K, ps.B, ps.ci);
+    ps.fhash[f] = h;
+    uint32_t w = (h >> ps.wshift) & ((1u << ps.wbits) - 1);
+    ps.bloom[w] |= 1u << (h & 31);
+    bucket[f] = (uint32_t)((h * 0x9E3779B1u) >> (32 - ps.nbits));
+    cnt[bucket[f] + 1]++;
+  }
+  for (uint32_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+  ps.bstart = cnt;
+  ps.bent.resize(F);
+  std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
+  for (uint32_t f = 0; f < F; ++f) ps.bent[fill[bucket[f]]++] = f;
+  ps.hash_ok = true;
+}
+
+void build_ac(PatternsHost& ps) {
+  // Trie over the factor strings (folded when case-insensitive).
+  std::vector<std::map<uint8_t, uint32_t>> kids(1);
+  std::vector<std::vector<uint32_t>> own(1);
+  std::vector<uint8_t> used(256, 0);
+  ps.ac_maxlen = 0;
+  for (uint32_t f = 0; f < ps.fac.size(); ++f) {
+    const Factor& fc = ps.fac[f];
+    uint32_t node = 0;
+    for (uint32_t t = 0; t < fc.ac_len; ++t) {
+      uint8_t b = fold((uint8_t)fc.bytes[t], ps.ci);
+      used[b] = 1;
+      auto it = kids[node].find(b);
+      if (it == kids[node].end()) {
+        if (kids.size() >= kAcMaxStates) { ps.ac_ok = false; return; }
+        uint32_t nn = (uint32_t)kids.size();
+        kids.emplace_back();
+        own.emplace_back();
+        kids[node][b] = nn;
+        node = nn;
+      } else {
+        node = it->second;
+      }
+    }
+    own[node].push_back(f);
+    ps.ac_maxlen = std::max(ps.ac_maxlen, fc.ac_len);
+  }
+  const uint32_t S = (uint32_t)kids.size();
+  // Byte classes: every used byte is its own class; the rest share class 0.
+  uint32_t C = 1;
+  std::array<uint8_t, 256> cmap{};
+  std::vector<int> rep(1, -1);
+  for (int b = 0; b < 256; ++b)
+    if (used[b]) { cmap[b] = (uint8_t)C++; rep.push_back(b); }
+  if (C > 256) { ps.ac_ok = false; return; }
+  if (ps.ci)
+    for (int b = 'A'; b <= 'Z'; ++b) cmap[b] = cmap[b + 32];
+  // BFS: failure links, dense goto, inherited outputs.
+  std::vector<uint32_t> fail(S, 0), order;
+  std::vector<uint32_t> delta((size_t)S * C, 0);
+  std::vector<std::vector<uint32_t>> out(S);
+  std::deque<uint32_t> q;
+  order.push_back(0);
+  out[0] = own[0];
+  for (uint32_t c = 1; c < C; ++c) {
+    auto it = kids[0].find((uint8_t)rep[c]);
+    if (it != kids[0].end()) {
+      delta[c] = it->second;
+      fail[it->second] = 0;
+      q.push_back(it->second);
+    }
+  }
+  while (!q.empty()) {
+    uint32_t u = q.front();
+    q.pop_front();
+    order.push_back(u);
+    out[u] = own[u];
+    for (uint32_t f : out[fail[u]]) out[u].push_back(f);
+    for (uint32_t c = 0; c < C; ++c) {
+      uint32_t via_fail = delta[(size_t)fail[u] * C + c];
+      if (c == 0) { delta[(size_t)u * C] = via_fail; continue; }
+      auto it = kids[u].find((uint8_t)rep[c]);
+      if (it != kids[u].end()) {
+        uint32_t v = it->second;
+        fail[v] = via_fail;
+        delta[(size_t)u * C + c] = v;
+        q.push_back(v);
+      } else {
+        delta[(size_t)u * C + c] = via_fail;
+      }
+    }
+  }
+  // Renumber: states with outputs last so the kernel tests `s >= first_out`.
+  std::vector<uint32_t> newid(S);
+  uint32_t next = 0;
+  for (uint32_t u : order) if (out[u].empty()) newid[u] = next++;
+  ps.ac_first_out = next;
+  for (uint32_t u : order) if (!out[u].empty()) newid[u] = next++;
+  ps.ac_states = S;
+  ps.ac_classes = C;
+  ps.ac_cmap = cmap;
+  ps.ac_table.assign((size_t)S * C, 0);
+  for (uint32_t u = 0; u < S; ++u)
+    for (uint32_t c = 0; c < C; ++c)
+      ps.ac_table[(size_t)newid[u] * C + c] = newid[delta[(size_t)u * C + c]];
+  std::vector<uint32_t> by_new(S);
+  for (uint32_t u = 0; u < S; ++u) by_new[newid[u]] = u;
+  ps.ac_out_start.assign(S - ps.ac_first_out + 1, 0);
+  ps.ac_out_list.clear();
+  for (uint32_t s = ps.ac_first_out; s < S; ++s) {
+    ps.ac_out_start[s - ps.ac_first_out] = (uint32_t)ps.ac_out_list.size();
+    auto& o = out[by_new[s]];
+    std::vector<uint32_t> sorted(o);
+    std::sort(sorted.begin(), sorted.end());
+    for (uint32_t f : sorted) ps.ac_out_list.push_back(f);
+  }
+  ps.ac_out_start[S - ps.ac_first_out] = (uint32_t)ps.ac_out_list.size();
+  ps.ac_ok = newid[0] == 0;  // the root must be state 0 and carry no output
+}
+
+}  // namespace
+
+uint32_t rolling_hash(const uint8_t* p, uint32_t K, uint32_t B, bool ci) {
+  uint32_t h = 0;
+  for (uint32_t t = 0; t < K; ++t) h = (h * B + fold(p[t], ci)) & 0xFFFFFFu;
+  return h;
+}
+
+PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_insensitive) {
+  if (src.empty()) throw CompileError{WS_ERR_EMPTY_PATTERN_SET, "pattern set is empty"};
+  if (src.size() > 0x7FFFFFFFull)
+    throw CompileError{WS_ERR_PATTERN_SET_TOO_LARGE, "too many patterns"};
+  PatternsHost ps;
+  ps.npat = (uint32_t)src.size();
+  ps.ci = case_insensitive;
+  bool unbounded = false;
+  uint64_t max_len = 0;
+  for (uint32_t i = 0; i < ps.npat; ++i) {
+    const PatternSrc& p = src[i];
+    if (p.bytes.empty())
+      throw CompileError{WS_ERR_EMPTY_PATTERN, "pattern " + std::to_string(i) + " is empty"};
+    if (p.bytes.size() > 0xFFFFFFFFull)
+      throw CompileError{WS_ERR_PATTERN_TOO_LARGE, "pattern " + std::to_string(i) + " too large"};
+    if (p.kind == WS_PATTERN_LITERAL) {
+      ps.lits.push_back(p.bytes);
+      ps.lit_pid.push_back(i);
+      max_len = std::max<uint64_t>(max_len, p.bytes.size());
+      continue;
+    }
+    if (p.kind != WS_PATTERN_REGEX)
+      throw CompileError{WS_ERR_INVALID_ARGUMENT, "unknown pattern kind"};
+    Ast ast;
+    try {
+      ast = parse_regex(p.bytes, false);
+    } catch (const ParseError& e) {
+      throw CompileError{WS_ERR_PATTERN_COMPILATION_FAILED,
+                         "invalid regex at index " + std::to_string(i) + ": " + e.msg};
+    }
+    if (is_pathological(ast))
+      throw CompileError{WS_ERR_PATHOLOGICAL_REGEX,
+                         "regex pattern " + std::to_string(i) + " contains nested repetitions"};
+    std::string lit;
+    if (extract_literal(ast, &lit) && !lit.empty()) {
+      ps.lits.push_back(lit);
+      ps.lit_pid.push_back(i);
+      max_len = std::max<uint64_t>(max_len, lit.size());
+      continue;
+    }
+    long mn, mx;
+    length_bounds(ast, &mn, &mx);
+    if (mn == 0)
+      throw CompileError{WS_ERR_PATTERN_COMPILATION_FAILED,
+                         "regex " + std::to_string(i) +
+                             " can match the empty string; not supported on the GPU path"};
+    try {
+      Nfa nfa = build_nfa(ast, kNfaMaxStates);
+      AnchoredDfa dfa = build_anchored_dfa(nfa, kDfaMaxStates);
+      std::vector<std::string> fs = extract_factors(nfa, kFactorMaxLen, kFactorFanout, kFactorMaxCount);
+      for (auto& s : fs)
+        if (s.empty())
+          throw ParseError{"no selective prefix (first byte matches too many values)"};
+      uint32_t r = (uint32_t)ps.rx.size();
+      ps.rx.push_back(std::move(dfa));
+      ps.rx_pid.push_back(i);
+      ps.rx_maxlen.push_back(mx);
+      if (mx < 0) unbounded = true;
+      else max_len = std::max<uint64_t>(max_len, (uint64_t)mx);
+      for (auto& s : fs) ps.fac.push_back({s, 1u, r, (uint32_t)s.size()});
+    } catch (const ParseError& e) {
+      throw CompileError{WS_ERR_PATTERN_COMPILATION_FAILED,
+                         "regex " + std::to_string(i) + ": " + e.msg};
+    }
+  }
+  // Literal factors first (verification string = the whole literal).
+  std::vector<Factor> fac;
+  for (uint32_t l = 0; l < ps.lits.size(); ++l)
+    fac.push_back({ps.lits[l], 0u, l, std::min<uint32_t>((uint32_t)ps.lits[l].size(), kAcMaxLen)});
+  for (auto& f : ps.fac) fac.push_back(f);
+  ps.fac = std::move(fac);
+  ps.min_flen = 0xFFFFFFFFu;
+  ps.max_flen = 0;
+  for (auto& f : ps.fac) {
+    ps.min_flen = std::min<uint32_t>(ps.min_flen, (uint32_t)f.bytes.size());
+    ps.max_flen = std::max<uint32_t>(ps.max_flen, (uint32_t)f.bytes.size());
+  }
+  ps.min_lit_len = 0;
+  for (auto& l : ps.lits)
+    ps.min_lit_len = ps.min_lit_len == 0 ? (uint32_t)l.size() : std::min<uint32_t>(ps.min_lit_len, (uint32_t)l.size());
+  ps.max_match_len = unbounded ? 0xFFFFFFFFu : (uint32_t)std::min<uint64_t>(max_len, 0xFFFFFFFEull);
+
+  const size_t nl = ps.lits.size(), nr = ps.rx.size();
+  if (nl == 0) ps.strategy = STRAT_NONE;
+  else if (ps.ci || nr > 0) ps.strategy = STRAT_LEFTMOST_LONGEST;
+  else if (nl >= 2 && nl <= 8 &&
+           std::all_of(ps.lits.begin(), ps.lits.end(), [](const std::string& s) { return s.size() <= 16; }))
+    ps.strategy = STRAT_MULTI_MEMCHR;
+  else if (nl > 5000) ps.strategy = STRAT_HASH_SHORTEST;
+  else ps.strategy = STRAT_LEFTMOST_LONGEST;
+
+  build_hash(ps);
+  build_ac(ps);
+  return ps;
+}
+
+}  // namespace ws

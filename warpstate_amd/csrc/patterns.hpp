@@ -1,0 +1,74 @@
+// Host-side compiled pattern set: what PatternSetBuilder::build produces in
+// the reference (core/src/pattern/compiler.rs:97-340), laid out for the GPU.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "regex.hpp"
+
+namespace ws {
+
+// Literal reduction used by WS_MODE_STANDARD, as ScanStrategy::select picks it
+// (core/src/specialize.rs:101-179, core/src/cpu/mod.rs:225-230).
+enum Strategy : uint32_t {
+  STRAT_NONE = 0,        // no literals
+  STRAT_LEFTMOST_LONGEST = 1,  // Aho-Corasick LeftmostLongest (also single literal)
+  STRAT_MULTI_MEMCHR = 2,      // 2..8 literals <= 16 bytes, no regex, case-sensitive
+  STRAT_HASH_SHORTEST = 3,     // > 5000 literals, no regex, case-sensitive
+};
+
+// A prefilter factor: every match of its owner starts with `bytes`.
+// Literal factors are the whole literal; regex factors come from
+// extract_factors(). `ac_len` is the length the factor DFA recognises
+// (literals are truncated to keep the automaton small).
+struct Factor {
+  std::string bytes;
+  uint32_t kind;   // 0 = literal (owner = literal index), 1 = regex (owner = regex index)
+  uint32_t owner;
+  uint32_t ac_len;
+};
+
+struct PatternsHost {
+  uint32_t npat = 0;
+  bool ci = false;
+  std::vector<std::string> lits;
+  std::vector<uint32_t> lit_pid;
+  std::vector<uint32_t> rx_pid;
+  std::vector<AnchoredDfa> rx;
+  std::vector<long> rx_maxlen;  // -1 = unbounded
+  std::vector<Factor> fac;
+  uint32_t min_flen = 0, max_flen = 0, min_lit_len = 0;
+  uint32_t max_match_len = 0;  // 0xFFFFFFFF when some regex is unbounded
+  uint32_t strategy = STRAT_NONE;
+
+  // Rolling-hash prefilter.
+  bool hash_ok = false;
+  uint32_t K = 0, B = 0, NBK = 0, wshift = 0, wbits = 0, nbits = 0;
+  std::vector<uint32_t> bloom, bstart, bent, fhash;
+
+  // Factor DFA (dense Aho-Corasick automaton over byte classes).
+  bool ac_ok = false;
+  uint32_t ac_states = 0, ac_classes = 0, ac_first_out = 0, ac_maxlen = 0;
+  std::array<uint8_t, 256> ac_cmap{};
+  std::vector<uint32_t> ac_table, ac_out_start, ac_out_list;
+};
+
+struct CompileError {
+  int status;  // ws_status
+  std::string msg;
+};
+
+struct PatternSrc {
+  uint32_t kind;
+  std::string bytes;
+};
+
+// Throws CompileError.
+PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_insensitive);
+
+// Rolling hash of the first K bytes (mod 2^24), shared by host and kernel.
+uint32_t rolling_hash(const uint8_t* p, uint32_t K, uint32_t B, bool ci);
+
+}  // namespace ws
