@@ -174,6 +174,10 @@ uint32_t ws_abi_version(void);
 ws_status ws_patterns_compile(ws_ctx* ctx, const ws_pattern* patterns, size_t count,
                               uint32_t flags, ws_patterns** out);
 void ws_patterns_free(ws_patterns* patterns);
+/* Host-only compile (no device needed): validates the set exactly like
+ * ws_patterns_compile and reports its facts; the learned router's features. */
+ws_status ws_patterns_analyze(const ws_pattern* patterns, size_t count, uint32_t flags,
+                              ws_patterns_info* out);
 ws_status ws_patterns_info_get(const ws_patterns* patterns, ws_patterns_info* out);
 
 /* ---- scan: GpuMatcher::scan (core/src/gpu/mod.rs:223-297) + dispatch::scan_literal_chunk

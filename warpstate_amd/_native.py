@@ -117,6 +117,7 @@ SIGNATURES = {
     "ws_patterns_compile": (c_int, [c_void_p, POINTER(WsPattern), c_size_t, c_uint32, POINTER(c_void_p)]),
     "ws_patterns_free": (None, [c_void_p]),
     "ws_patterns_info_get": (c_int, [c_void_p, POINTER(WsPatternsInfo)]),
+    "ws_patterns_analyze": (c_int, [POINTER(WsPattern), c_size_t, c_uint32, POINTER(WsPatternsInfo)]),
     "ws_scan": (
         c_int,
         [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, POINTER(WsTuning), c_void_p, c_uint64,

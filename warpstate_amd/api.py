@@ -167,7 +167,7 @@ class PatternSetBuilder:
 
     def build(self) -> "PatternSet":
         ps = PatternSet(list(self._patterns), self._ci)
-        ps._compiled(default_context())  # surface compile errors at build time
+        ps.analyze()  # host-side compile: errors surface at build time, no GPU needed
         return ps
 
 
@@ -195,19 +195,31 @@ class PatternSet:
             h = self._handles.get(key)
             if h is None:
                 lib = N.load()
-                bufs = [ctypes.create_string_buffer(b, len(b)) if b else None for _, b, _ in self.patterns]
-                arr = (N.WsPattern * len(self.patterns))()
-                for i, ((kind, b, name), buf) in enumerate(zip(self.patterns, bufs)):
-                    arr[i].kind = kind
-                    arr[i].bytes = ctypes.cast(buf, ctypes.c_void_p) if buf is not None else None
-                    arr[i].len = len(b)
-                    arr[i].name = name.encode() if name else None
+                arr, _bufs = self._c_patterns()
                 h = ctypes.c_void_p()
                 flags = 1 if self.case_insensitive else 0
                 N.check(lib.ws_patterns_compile(ctx.handle, arr, len(self.patterns), flags, ctypes.byref(h)))
                 self._handles[key] = (h, ctx)
                 return h
             return h[0]
+
+    def _c_patterns(self):
+        bufs = [ctypes.create_string_buffer(b, len(b)) if b else None for _, b, _ in self.patterns]
+        arr = (N.WsPattern * len(self.patterns))()
+        for i, ((kind, b, name), buf) in enumerate(zip(self.patterns, bufs)):
+            arr[i].kind = kind
+            arr[i].bytes = ctypes.cast(buf, ctypes.c_void_p) if buf is not None else None
+            arr[i].len = len(b)
+            arr[i].name = name.encode() if name else None
+        return arr, bufs
+
+    def analyze(self) -> dict:
+        """Host-only compile (ws_patterns_analyze): validation + set facts."""
+        arr, _bufs = self._c_patterns()
+        i = N.WsPatternsInfo()
+        N.check(N.load().ws_patterns_analyze(arr, len(self.patterns), 1 if self.case_insensitive else 0,
+                                             ctypes.byref(i)))
+        return {name: getattr(i, name) for name, _ in N.WsPatternsInfo._fields_}
 
     def info(self, ctx: Optional[Context] = None) -> dict:
         h = self._compiled(ctx or default_context())
@@ -216,7 +228,7 @@ class PatternSet:
         return {name: getattr(i, name) for name, _ in N.WsPatternsInfo._fields_}
 
     def max_pattern_len(self) -> int:
-        return int(self.info()["max_pattern_len"])
+        return int(self.analyze()["max_pattern_len"])
 
     def scan(self, data) -> List[Match]:
         return GpuMatcher(self).scan(data, mode="standard")

@@ -733,9 +733,8 @@ void ws_patterns_free(ws_patterns* p) {
   delete p;
 }
 
-ws_status ws_patterns_info_get(const ws_patterns* p, ws_patterns_info* out) {
-  if (!p || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
-  const PatternsHost& h = p->h;
+static void fill_info(const PatternsHost& h, uint32_t rx_states, uint32_t variants,
+                      ws_patterns_info* out) {
   ws_patterns_info i{};
   i.pattern_count = h.npat;
   i.literal_count = (uint32_t)h.lits.size();
@@ -747,10 +746,43 @@ ws_status ws_patterns_info_get(const ws_patterns* p, ws_patterns_info* out) {
   i.max_pattern_len = h.max_match_len;
   i.dfa_states = h.ac_states;
   i.dfa_classes = h.ac_classes;
-  i.regex_dfa_states = p->rx_states_total;
+  i.regex_dfa_states = rx_states;
   i.hash_window = h.K;
-  i.variants_supported = supported_variants(p);
+  i.variants_supported = variants;
   *out = i;
+}
+
+ws_status ws_patterns_analyze(const ws_pattern* pats, size_t count, uint32_t flags,
+                              ws_patterns_info* out) {
+  if (!out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (count == 0) return fail(WS_ERR_EMPTY_PATTERN_SET, "pattern set is empty. Fix: add at least one pattern");
+  if (!pats) return fail(WS_ERR_INVALID_ARGUMENT, "patterns is null");
+  std::vector<PatternSrc> src(count);
+  for (size_t i = 0; i < count; ++i) {
+    if (pats[i].len && !pats[i].bytes) return fail(WS_ERR_INVALID_ARGUMENT, "pattern bytes are null");
+    src[i].kind = pats[i].kind;
+    src[i].bytes.assign(reinterpret_cast<const char*>(pats[i].bytes), pats[i].len);
+  }
+  try {
+    PatternsHost h = compile_patterns(src, (flags & WS_FLAG_CASE_INSENSITIVE) != 0);
+    uint32_t rs = 0;
+    for (auto& d : h.rx) rs += d.nstates;
+    uint32_t v = 1u << WS_VARIANT_HASH;
+    if (h.ac_ok) v |= 1u << WS_VARIANT_DFA_GLOBAL;
+    if (h.ac_ok && h.ac_states <= 65535 && (size_t)h.ac_states * h.ac_classes * 2 <= kAcLdsBudget)
+      v |= 1u << WS_VARIANT_DFA_LDS;
+    fill_info(h, rs, v, out);
+  } catch (const CompileError& e) {
+    return fail((ws_status)e.status, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(WS_ERR_PATTERN_SET_TOO_LARGE, "out of host memory compiling the pattern set");
+  }
+  return WS_OK;
+}
+
+ws_status ws_patterns_info_get(const ws_patterns* p, ws_patterns_info* out) {
+  if (!p || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  fill_info(p->h, p->rx_states_total, supported_variants(p), out);
   return WS_OK;
 }
 

@@ -25,6 +25,14 @@ __device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
   return __umul24(a, b) + c;
 }
 
+// Same, pinned to a single v_mad_u32_u24 with a uniform multiplier (the
+// compiler otherwise splits chained multiply-adds into mul + mul + add3).
+__device__ __forceinline__ uint32_t mad24_s(uint32_t a, uint32_t b_uniform, uint32_t c) {
+  uint32_t r;
+  asm volatile("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b_uniform), "v"(c));
+  return r;
+}
+
 __device__ __forceinline__ uint8_t fold_byte(uint8_t b) {
   return (b >= 'A' && b <= 'Z') ? (uint8_t)(b + 32) : b;
 }
@@ -166,11 +174,10 @@ __global__ __launch_bounds__(kTPB) void k_hash_scan(ScanCommon c, HashParams h) 
       for (int t = 0; t < K; ++t) H = mad24(H, h.B, byte_of(d, t));
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        if (i > 0)
-          H = mad24(H, h.B,
-                                      mad24(byte_of(d, i - 1), h.NBK, byte_of(d, i + K - 1)));
-        uint32_t w = bloom[__builtin_amdgcn_ubfe(H, h.wshift, h.wbits)];
-        mask |= ((w >> (H & 31u)) & 1u) << i;
+        if (i > 0) H = mad24_s(H, h.B, mad24_s(byte_of(d, i - 1), h.NBK, byte_of(d, i + K - 1)));
+        const uint32_t w = bloom[__builtin_amdgcn_ubfe(H, h.wshift, h.wbits)];
+        // v_bfe_u32 reads its offset from H[4:0].
+        mask |= __builtin_amdgcn_ubfe(w, H, 1) << i;
       }
       // Positions q are valid when lo <= q, q + K <= hi and q - lo < emit_end.
       uint64_t qmin = p0 > lo ? p0 : lo;
