@@ -149,10 +149,13 @@ typedef struct ws_patterns_info {
 
 /* Per-call timings measured with HIP events on the context stream (ms). */
 typedef struct ws_timings {
-  float prefilter_ms;  /* the prefilter+verify kernel */
+  float prefilter_ms;  /* the streaming prefilter kernel */
+  float verify_ms;     /* exact verification of prefilter hits (hash variant) */
   float regex_ms;      /* anchored walks + chain resolution */
   float order_ms;      /* sort + pack */
   float total_ms;      /* whole call, first launch to last */
+  float reserved;
+  uint64_t hits;       /* prefilter hits (hash variant) */
   uint64_t candidates; /* regex candidate starts */
   uint64_t raw_matches;/* matches before ordering */
 } ws_timings;

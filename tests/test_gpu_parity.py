@@ -25,6 +25,18 @@ def as_list(arr):
     return [(int(p), int(s), int(e)) for p, s, e in zip(arr["pattern_id"], arr["start"], arr["end"])]
 
 
+def assert_same(got, exp, what=""):
+    """Compact mismatch report (pytest's full list diff is too slow for big lists)."""
+    if got == exp:
+        return
+    sg, se = set(got), set(exp)
+    first = next((i for i, (a, b) in enumerate(zip(got, exp)) if a != b), min(len(got), len(exp)))
+    raise AssertionError(
+        f"{what}: {len(got)} vs {len(exp)} matches; first difference at {first}: "
+        f"got {got[first:first + 3]} expected {exp[first:first + 3]}; "
+        f"only-gpu {sorted(sg - se)[:5]} only-oracle {sorted(se - sg)[:5]}")
+
+
 def variants_for(m):
     v = m.info()["variants_supported"]
     return [x for x in (VARIANT_HASH, VARIANT_DFA_GLOBAL, VARIANT_DFA_LDS) if v & (1 << x)]
@@ -37,7 +49,7 @@ def check_all_variants(pats, data, mode=0, ci=False):
     for v in variants_for(m):
         print(f"variant={v} n={len(data)} pats={len(pats)} mode={mode}", flush=True)
         got = as_list(m.scan_array(data, mode="standard" if mode else "overlapping", tuning=Tuning(variant=v)))
-        assert got == exp, f"variant {v}: {len(got)} vs {len(exp)} matches"
+        assert_same(got, exp, f"variant {v}")
     return exp
 
 
@@ -136,7 +148,7 @@ def test_stream_pipeline_equals_whole():
     overlap = max(s.max_len for s in specs)
     for chunk in (4096, 65536, 300_001):
         got = StreamPipeline(m, overlap).scan(data, chunk_size=chunk)
-        assert got == whole, f"chunk {chunk}"
+        assert_same(got, whole, f"chunk {chunk}")
     assert whole == [tuple(x) for x in Oracle(pats).scan_list(data)]
 
 
@@ -161,7 +173,7 @@ def test_batch_per_item_exact():
     exp = []
     for it in items:
         exp += [(it.id, x) for x in o.scan_list(it.data)]
-    assert [(t.source_id, tuple(t.matched)) for t in got] == exp
+    assert_same([(t.source_id, tuple(t.matched)) for t in got], exp, "batch")
     # tests/unit/gpu_batch.rs:6-65: a match split across two items is not reported
     m2 = GpuMatcher(PatternSet.builder().literal("needle").build())
     got2 = scan_batch(m2, [ScanItem(1, b"xxnee"), ScanItem(2, b"dlexx"), ScanItem(3, b"needle")])

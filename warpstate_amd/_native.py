@@ -96,9 +96,12 @@ class WsPatternsInfo(Structure):
 class WsTimings(Structure):
     _fields_ = [
         ("prefilter_ms", c_float),
+        ("verify_ms", c_float),
         ("regex_ms", c_float),
         ("order_ms", c_float),
         ("total_ms", c_float),
+        ("reserved", c_float),
+        ("hits", c_uint64),
         ("candidates", c_uint64),
         ("raw_matches", c_uint64),
     ]

@@ -19,7 +19,7 @@ namespace ws {
 // Launch wrappers defined in kernels.hip.
 size_t hash_shmem_bytes(uint32_t nwords);
 hipError_t launch_hash(uint32_t K, bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c,
-                       const HashParams& h);
+                       const HashParams& h, const HitSlices& hs, bool verify_only);
 uint32_t ac_unit_bytes();
 size_t ac_shmem_bytes(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCommon& c,
@@ -97,8 +97,9 @@ struct ws_ctx {
   ws_timings last{};
   hipEvent_t ev[8] = {};
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids;
+      scratch, carried, ids, hits, slice_counts;
   uint32_t* hcount = nullptr;  // pinned host mirror of the counters
+  uint32_t last_hits = 0;      // filter hits of the last hash-variant scan
 };
 
 struct ws_patterns {
@@ -107,6 +108,8 @@ struct ws_patterns {
   DevBuf blob;
   // Device pointers into `blob`.
   const uint8_t* fbytes = nullptr;
+  const uint4* fmeta = nullptr;
+  const uint2* bent2 = nullptr;
   const uint32_t *foff = nullptr, *flen = nullptr, *fown = nullptr, *lit_pid = nullptr;
   const uint32_t *bloom = nullptr, *bstart = nullptr, *bent = nullptr, *fhash = nullptr;
   const uint32_t *ac_table32 = nullptr, *ac_out_start = nullptr, *ac_out_list = nullptr,
@@ -144,6 +147,7 @@ ws_status upload_patterns(ws_patterns* p) {
   std::string fb;
   std::vector<uint32_t> foff, flen, fown, aclen;
   for (const auto& f : h.fac) {
+    while (fb.size() % 4) fb.push_back(0);  // dword-aligned factor bytes
     foff.push_back((uint32_t)fb.size());
     flen.push_back((uint32_t)f.bytes.size());
     fown.push_back((f.kind << 31) | f.owner);
@@ -154,6 +158,22 @@ ws_status upload_patterns(ws_patterns* p) {
       fb.push_back((char)b);
     }
   }
+  fb.append(8, '\0');
+  std::vector<uint32_t> fmeta;
+  for (size_t f = 0; f < h.fac.size(); ++f) {
+    const auto& fc = h.fac[f];
+    fmeta.push_back(foff[f]);
+    fmeta.push_back(flen[f]);
+    fmeta.push_back(fown[f]);
+    fmeta.push_back(fc.kind ? fc.owner : h.lit_pid[fc.owner]);
+  }
+  std::vector<uint32_t> bent2;
+  for (uint32_t e = 0; e < h.bent.size(); ++e) {
+    bent2.push_back(h.fhash[h.bent[e]]);
+    bent2.push_back(h.bent[e]);
+  }
+  size_t o_fmeta = put(fmeta.data(), fmeta.size() * 4);
+  size_t o_bent2 = put(bent2.data(), bent2.size() * 4);
   size_t o_fbytes = put(fb.data(), fb.size());
   size_t o_foff = put(foff.data(), foff.size() * 4);
   size_t o_flen = put(flen.data(), flen.size() * 4);
@@ -199,6 +219,8 @@ ws_status upload_patterns(ws_patterns* p) {
   WS_HIP(hipMemcpy(p->blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
   uint8_t* b = p->blob.as<uint8_t>();
   p->fbytes = b + o_fbytes;
+  p->fmeta = reinterpret_cast<const uint4*>(b + o_fmeta);
+  p->bent2 = reinterpret_cast<const uint2*>(b + o_bent2);
   p->foff = reinterpret_cast<const uint32_t*>(b + o_foff);
   p->flen = reinterpret_cast<const uint32_t*>(b + o_flen);
   p->fown = reinterpret_cast<const uint32_t*>(b + o_fown);
@@ -267,7 +289,7 @@ ws_status sort_into(ws_ctx* ctx, uint32_t n, uint32_t bits, const uint64_t* kin,
 }
 
 ws_status read_counters(ws_ctx* ctx) {
-  WS_HIP(hipMemcpyAsync(ctx->hcount, ctx->counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+  WS_HIP(hipMemcpyAsync(ctx->hcount, ctx->counters.p, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost,
                         ctx->stream));
   WS_HIP(hipStreamSynchronize(ctx->stream));
   return WS_OK;
@@ -334,7 +356,15 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   const uint32_t variant = pick_variant(p, tuning);
   if (!(supported_variants(p) & (1u << variant)))
     return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
-  const uint32_t unit = variant == WS_VARIANT_HASH ? kHashUnit : ac_unit_bytes();
+  // Hash filter: 16 or 32 positions per lane (32 halves per-lane overhead;
+  // 16 suits batches of small items).
+  uint32_t lane_bytes = tuning && tuning->bytes_per_lane ? tuning->bytes_per_lane : 0;
+  if (lane_bytes != 16 && lane_bytes != 32) {
+    uint64_t span = 0;
+    for (auto& it : plan.items) span += it.y - it.x;
+    lane_bytes = (plan.batch && span / plan.items.size() < 64 * 1024) ? 16 : 32;
+  }
+  const uint32_t unit = variant == WS_VARIANT_HASH ? kTPB * lane_bytes : ac_unit_bytes();
   std::vector<uint2> units;
   uint32_t nunits;
   if (plan.batch) {
@@ -367,10 +397,24 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   }
   const uint32_t per_cu = tuning && tuning->blocks_per_cu ? tuning->blocks_per_cu : 4;
   const uint32_t grid = std::max(1u, std::min(nunits, ctx->cus * per_cu));
-
+  // Hash variant: one 1024-lane filter block per CU (its LDS holds the bloom
+  // filter); each block owns a slice of the hit buffer, plus an overflow area.
+  const uint32_t fgrid = std::max(1u, std::min((nunits + 7) / 8, ctx->cus * (per_cu > 4 ? 2u : 1u)));
+  uint64_t hit_total = (uint64_t)(ctx->hits.cap / sizeof(uint2));
+  if (variant == WS_VARIANT_HASH) {
+    uint64_t span = 0;
+    for (auto& it : plan.items) span += it.y - it.x;
+    const uint64_t want = std::min<uint64_t>(std::max<uint64_t>(1u << 20, span / 64), 0x3FFFFFFFull);
+    if (hit_total < want) {
+      WS_HIP(ctx->hits.ensure(want * sizeof(uint2)));
+      hit_total = ctx->hits.cap / sizeof(uint2);
+    }
+    WS_HIP(ctx->slice_counts.ensure((size_t)fgrid * 4));
+  }
+  uint32_t hit_overflow_cap = 0;
   if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[0], st));
   for (int attempt = 0;; ++attempt) {
-    WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 4, 0, st));
+    WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 8, 0, st));
     ScanCommon c{};
     c.base = plan.base;
     c.load_end = plan.load_end;
@@ -390,21 +434,41 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     c.cand_cap = cand_cap;
     c.counters = ctx->counters.as<uint32_t>();
     if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[1], st));
+    if (nunits == 0 && ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
     if (nunits > 0) {
       if (variant == WS_VARIANT_HASH) {
-        HashParams hp{p->bloom, (uint32_t)h.bloom.size(), h.wshift, h.wbits, h.B, h.NBK, h.nbits,
-                      p->bstart, p->bent, p->fhash};
-        WS_HIP(launch_hash(h.K, h.ci, grid, st, c, hp));
+        HashParams hp{p->bloom, (uint32_t)h.bloom.size(), h.wshift, h.wbits, h.B, h.NBK, h.nbits, h.s2,
+                      lane_bytes, p->bstart, p->bent2, p->fmeta};
+        HitSlices hs{};
+        hs.overflow_cap = (uint32_t)std::min<uint64_t>(hit_total / 4, 0xFFFFFFF0ull);
+        hs.slice_cap = (uint32_t)((hit_total - hs.overflow_cap) / fgrid);
+        hs.hits = ctx->hits.as<uint2>();
+        hs.overflow = hs.hits + (size_t)hs.slice_cap * fgrid;
+        hs.slice_count = ctx->slice_counts.as<uint32_t>();
+        hs.overflow_count = ctx->counters.as<uint32_t>() + 4;
+        WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, false));
+        if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
+        WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, true));
+        hit_overflow_cap = hs.overflow_cap;
       } else {
         AcParams ap{p->ac_table32, p->ac_table16, p->ac_cmap, p->ac_out_start, p->ac_out_list,
                     p->ac_len, h.ac_states, h.ac_classes, h.ac_first_out, h.ac_maxlen};
         WS_HIP(launch_ac(variant == WS_VARIANT_DFA_LDS, grid, st, c, ap));
+        if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
       }
     }
     if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[2], st));
     ws_status s = read_counters(ctx);
     if (s != WS_OK) return s;
-    const uint32_t nlit = ctx->hcount[0], ncand = ctx->hcount[1];
+    const uint32_t nlit = ctx->hcount[0], ncand = ctx->hcount[1], nover = ctx->hcount[4];
+    ctx->last_hits = nover;
+    if (variant == WS_VARIANT_HASH && nover > hit_overflow_cap) {
+      // Slices and the overflow area were not enough: grow and rescan.
+      WS_HIP(ctx->hits.ensure(hit_total * 2 * sizeof(uint2) + (size_t)nover * 8 * sizeof(uint2)));
+      hit_total = ctx->hits.cap / sizeof(uint2);
+      if (attempt >= 2) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
+      continue;
+    }
     if (nlit > max_matches) {
       res->count = nlit;
       return fail(WS_ERR_MATCH_BUFFER_OVERFLOW,
@@ -426,6 +490,13 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     }
     if (!retry) break;
     if (attempt >= 2) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
+  }
+  if (ctx->profiling && variant == WS_VARIANT_HASH) {
+    std::vector<uint32_t> sc(fgrid);
+    WS_HIP(hipMemcpy(sc.data(), ctx->slice_counts.p, fgrid * 4, hipMemcpyDeviceToHost));
+    uint64_t tot = ctx->last_hits;
+    for (uint32_t v : sc) tot += v;
+    ctx->last_hits = (uint32_t)std::min<uint64_t>(tot, 0xFFFFFFFFull);
   }
   uint32_t nlit = ctx->hcount[0];
   const uint32_t ncand = ctx->hcount[1];
@@ -548,8 +619,11 @@ void finish_timings(ws_ctx* ctx) {
   if (!ctx->profiling) return;
   (void)hipEventRecord(ctx->ev[5], ctx->stream);
   (void)hipEventSynchronize(ctx->ev[5]);
-  float a = 0, b = 0, c2 = 0, d = 0;
-  (void)hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]);
+  float a = 0, b = 0, c2 = 0, d = 0, v = 0;
+  (void)hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[6]);
+  (void)hipEventElapsedTime(&v, ctx->ev[6], ctx->ev[2]);
+  ctx->last.verify_ms = v;
+  ctx->last.hits = ctx->last_hits;
   (void)hipEventElapsedTime(&b, ctx->ev[3], ctx->ev[4]);
   (void)hipEventElapsedTime(&c2, ctx->ev[4], ctx->ev[5]);
   (void)hipEventElapsedTime(&d, ctx->ev[0], ctx->ev[5]);
@@ -664,7 +738,7 @@ void ws_ctx_destroy(ws_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->recs, &ctx->cands, &ctx->raw, &ctx->k0, &ctx->v0, &ctx->k1, &ctx->v1,
                     &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
-                    &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids})
+                    &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids, &ctx->hits, &ctx->slice_counts})
     b->release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);

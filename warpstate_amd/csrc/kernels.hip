@@ -1,8 +1,9 @@
 // HIP kernels of the scan path, written for gfx950 (CDNA4, wave64).
 //
-//  k_hash_scan   rolling-hash prefilter over 16 positions per lane, bloom
-//                filter in LDS, bucket chains in L2, exact verify, wave
-//                prefix-sum compaction into an LDS staging buffer.
+//  k_hash_filter rolling-hash prefilter over 16 positions per lane with the
+//                bloom filter in LDS; hit positions are wave prefix-sum
+//                compacted into the block's private slice of the hit list.
+//  k_hash_verify one lane per hit: bucket chain (L2) and exact verification.
 //  k_ac_scan     dense factor-DFA (Aho-Corasick over byte classes) traversal,
 //                one chunk per lane, table in LDS or L2.
 //  k_rx_walk     anchored leftmost-first DFA walk from each regex candidate.
@@ -123,119 +124,251 @@ __device__ __forceinline__ uint32_t byte_of(const uint32_t* d, int j) {
 // multiply-adds per position. A set bit in the LDS bloom filter sends the
 // position to the bucket chain (L2) and exact byte verification.
 // ---------------------------------------------------------------------------
-template <int K, bool CI>
-__global__ __launch_bounds__(kTPB) void k_hash_scan(ScanCommon c, HashParams h) {
+// Filter bits of the PL positions [p0, p0 + PL) whose window (PL + 16 bytes)
+// is in d. Per position: one byte extract, two 24-bit multiply-adds (rolling
+// hash), one LDS read of the bloom word and a two-bit test.
+template <int K, bool CI, int PL>
+__device__ __forceinline__ uint32_t filter_lane(const HashParams& h, const uint32_t* bloom, uint32_t* d) {
+  if (CI) {
+#pragma unroll
+    for (int j = 0; j < PL / 4 + 4; ++j) d[j] = fold_word(d[j]);
+  }
+  uint32_t H = byte_of(d, 0), mask = 0;
+#pragma unroll
+  for (int t = 1; t < K; ++t) H = mad24_s(H, h.B, byte_of(d, t));
+#pragma unroll
+  for (int i = 0; i < PL; ++i) {
+    if (i > 0) H = mad24_s(H, h.B, mad24_s(byte_of(d, i - 1), h.NBK, byte_of(d, i + K - 1)));
+    // Blocked bloom: one LDS word, two bits (H[4:0] and H[s2+4:s2]); v_bfe
+    // reads its offset from the low five bits of the offset operand.
+    const uint32_t w = bloom[__builtin_amdgcn_ubfe(H, h.wshift, h.wbits)];
+    const uint32_t hit = __builtin_amdgcn_ubfe(w, H, 1) & __builtin_amdgcn_ubfe(w, H >> h.s2, 1);
+    mask = (mask << 1) | hit;  // position i lands in bit PL - 1 - i
+  }
+  return __builtin_bitreverse32(mask) >> (32 - PL);
+}
+
+// Valid positions q of [p0, p0 + PL): lo <= q, q + K <= hi, q - lo < emit_end.
+template <int PL>
+__device__ __forceinline__ uint32_t valid_lane(uint32_t p0, uint32_t lo, uint32_t hi, uint32_t K,
+                                               uint32_t emit_end) {
+  if (hi < K) return 0;
+  uint32_t qmin = p0 > lo ? p0 : lo;
+  uint32_t qmax = hi - K + 1;
+  if (emit_end < qmax - lo) qmax = lo + emit_end;
+  if (qmax > p0 + PL) qmax = p0 + PL;
+  if (qmax <= qmin) return 0;
+  const uint32_t a0 = qmin - p0, a1 = qmax - p0;
+  const uint32_t top = a1 >= 32 ? 0xFFFFFFFFu : ((1u << a1) - 1u);
+  return top & ~((1u << a0) - 1u);
+}
+
+constexpr int kUnitsPerIter = 2;                  // work units a lane loads before computing
+constexpr int kSubBlocks = kFilterTPB / kTPB;      // 256-lane sub-blocks, one unit each
+
+// Streaming rolling-hash filter, one 1024-lane block per CU so the bloom
+// filter can use up to 128 KiB of LDS. Each 256-lane sub-block owns one work
+// unit (256 * PL bytes) at a time; per iteration a lane loads the (PL + 16)-
+// byte windows of kUnitsPerIter units with coalesced 16-byte loads, then
+// hashes PL positions per unit. Tiles wholly inside their segment skip all
+// bounds work. Hit positions are compacted per wave (prefix sum + one LDS
+// reservation) into the block's private slice of the hit buffer, so the loop
+// has no global atomics and no block barriers; a slice that fills up spills
+// into a shared overflow area.
+template <int K, bool CI, int PL>
+__global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashParams h, HitSlices hs) {
   extern __shared__ uint32_t smem[];
   uint32_t* bloom = smem;
-  Rec* s_lit = reinterpret_cast<Rec*>(smem + h.nwords);
-  Rec* s_cand = s_lit + kRecCap;
-  __shared__ uint32_t s_n[2];
-  __shared__ uint32_t s_base[4];
+  constexpr int NW = PL / 4 + 4;  // window dwords
+  constexpr uint32_t kUnit = kTPB * PL;
+  const uint32_t sub = threadIdx.x / kTPB, ltid = threadIdx.x % kTPB;
+  __shared__ uint32_t s_n;
+  uint2* slice = hs.hits + (size_t)blockIdx.x * hs.slice_cap;
 
-  for (uint32_t k = threadIdx.x; k < h.nwords; k += kTPB) bloom[k] = h.bloom[k];
-  if (threadIdx.x == 0) { s_n[0] = 0; s_n[1] = 0; }
+  for (uint32_t k = threadIdx.x; k < h.nwords; k += kFilterTPB) bloom[k] = h.bloom[k];
+  if (threadIdx.x == 0) s_n = 0;
   __syncthreads();
 
-  for (uint32_t u = blockIdx.x; u < c.nunits; u += gridDim.x) {
-    uint64_t tile_base;
-    uint32_t item;
-    if (c.units) {
-      uint2 uu = c.units[u];
-      tile_base = uu.x;
-      item = uu.y;
-    } else {
-      tile_base = (uint64_t)u * kHashUnit;
-      item = 0;
-    }
-    const uint2 seg = c.items[item];
-    const uint64_t lo = seg.x, hi = seg.y;
-    const uint64_t p0 = tile_base + 16ull * threadIdx.x;
-    uint32_t mask = 0;
-    if (p0 < hi && p0 + 16 > lo && hi >= (uint64_t)K) {
-      uint32_t d[8];
-      uint64_t lend = (hi + 15) & ~15ull;
-      if (lend > c.load_end) lend = c.load_end;
-      if (p0 + 32 <= lend) {
-        uint4 a = *reinterpret_cast<const uint4*>(c.base + p0);
-        uint4 b = *reinterpret_cast<const uint4*>(c.base + p0 + 16);
-        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
-        d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
-      } else {
-        uint4 a = *reinterpret_cast<const uint4*>(c.base + p0);
-        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
-        d[4] = d[5] = d[6] = d[7] = 0;
-      }
-      if (CI) {
+  const uint32_t load_end = (uint32_t)c.load_end;
+  constexpr uint32_t kPerGroup = kSubBlocks * kUnitsPerIter;
+  const uint32_t groups = (c.nunits + kPerGroup - 1) / kPerGroup;
+  for (uint32_t g = blockIdx.x; g < groups; g += gridDim.x) {
+    uint32_t d[kUnitsPerIter][NW];
+    uint32_t p0[kUnitsPerIter], lo[kUnitsPerIter], hi[kUnitsPerIter], item[kUnitsPerIter];
+    bool inner[kUnitsPerIter];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) d[j] = fold_word(d[j]);
-      }
-      uint32_t H = 0;
-#pragma unroll
-      for (int t = 0; t < K; ++t) H = mad24(H, h.B, byte_of(d, t));
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        if (i > 0) H = mad24_s(H, h.B, mad24_s(byte_of(d, i - 1), h.NBK, byte_of(d, i + K - 1)));
-        const uint32_t w = bloom[__builtin_amdgcn_ubfe(H, h.wshift, h.wbits)];
-        // v_bfe_u32 reads its offset from H[4:0].
-        mask |= __builtin_amdgcn_ubfe(w, H, 1) << i;
-      }
-      // Positions q are valid when lo <= q, q + K <= hi and q - lo < emit_end.
-      uint64_t qmin = p0 > lo ? p0 : lo;
-      uint64_t qmax = hi - K + 1;
-      uint64_t em = lo + c.emit_end;
-      if (em < qmax) qmax = em;
-      if (qmax > p0 + 16) qmax = p0 + 16;
-      uint32_t valid = 0;
-      if (qmax > qmin) {
-        uint32_t a0 = (uint32_t)(qmin - p0), a1 = (uint32_t)(qmax - p0);
-        valid = ((a1 >= 32 ? 0xFFFFFFFFu : ((1u << a1) - 1u))) & ~((1u << a0) - 1u);
-      }
-      mask &= valid;
-    }
-    // Count pass over the (rare) filter hits, then one reservation per wave.
-    uint32_t nl = 0, nc = 0;
-    for (uint32_t m = mask; m; m &= m - 1) {
-      const uint64_t q = p0 + __builtin_ctz(m);
-      uint32_t H = 0;
-      for (int t = 0; t < K; ++t) {
-        uint8_t b = c.base[q + t];
-        if (CI) b = fold_byte(b);
-        H = (H * h.B + b) & 0xFFFFFFu;
-      }
-      const uint32_t bk = (H * 0x9E3779B1u) >> (32 - h.nbits);
-      for (uint32_t e = h.bstart[bk]; e < h.bstart[bk + 1]; ++e) {
-        const uint32_t f = h.bent[e];
-        if (h.fhash[f] != H || !verify_factor(c, q, (uint32_t)hi, f)) continue;
-        if (c.fown[f] >> 31) ++nc; else ++nl;
-      }
-    }
-    if (__any((nl | nc) != 0u)) {
-      uint32_t lslot = wave_reserve(nl, &s_n[0]);
-      uint32_t cslot = wave_reserve(nc, &s_n[1]);
-      for (uint32_t m = (nl | nc) ? mask : 0u; m; m &= m - 1) {
-        const uint64_t q = p0 + __builtin_ctz(m);
-        uint32_t H = 0;
-        for (int t = 0; t < K; ++t) {
-          uint8_t b = c.base[q + t];
-          if (CI) b = fold_byte(b);
-          H = (H * h.B + b) & 0xFFFFFFu;
+    for (int j = 0; j < kUnitsPerIter; ++j) {
+      const uint32_t u = g * kPerGroup + sub * kUnitsPerIter + j;
+      uint32_t tile_base = 0;
+      item[j] = 0;
+      if (u < c.nunits) {
+        if (c.units) {
+          const uint2 uu = c.units[u];
+          tile_base = uu.x;
+          item[j] = uu.y;
+        } else {
+          tile_base = u * kUnit;
         }
-        const uint32_t bk = (H * 0x9E3779B1u) >> (32 - h.nbits);
-        for (uint32_t e = h.bstart[bk]; e < h.bstart[bk + 1]; ++e) {
-          const uint32_t f = h.bent[e];
-          if (h.fhash[f] != H || !verify_factor(c, q, (uint32_t)hi, f)) continue;
-          const uint32_t own = c.fown[f];
-          const uint32_t start = (uint32_t)(q - lo);
-          if (own >> 31) {
-            stage_rec(make_uint4(own & 0x7FFFFFFFu, start, 0u, item), cslot++, s_cand,
-                      &c.counters[1], c.cand_out, c.cand_cap);
+      }
+      const uint2 seg = c.items[item[j]];
+      lo[j] = seg.x;
+      hi[j] = u < c.nunits ? seg.y : 0;
+      p0[j] = tile_base + PL * ltid;
+      // Inner tile: every window byte lies inside the segment and every
+      // position may start a match (the common case of a large scan).
+      inner[j] = tile_base >= lo[j] && tile_base + kUnit + 16 <= hi[j] &&
+                 tile_base + kUnit - lo[j] + 0u <= c.emit_end;
+      uint32_t lend = (hi[j] + 15) & ~15u;
+      if (lend > load_end) lend = load_end;
+#pragma unroll
+      for (int v = 0; v < NW / 4; ++v) {
+        uint4 x = make_uint4(0, 0, 0, 0);
+        const uint32_t a = p0[j] + 16 * v;
+        if (inner[j] || (a + 16 <= lend && a + 16 > (lo[j] & ~15u))) x = *reinterpret_cast<const uint4*>(c.base + a);
+        d[j][4 * v] = x.x; d[j][4 * v + 1] = x.y; d[j][4 * v + 2] = x.z; d[j][4 * v + 3] = x.w;
+      }
+    }
+    uint32_t mask[kUnitsPerIter];
+    uint32_t nh = 0;
+#pragma unroll
+    for (int j = 0; j < kUnitsPerIter; ++j) {
+      mask[j] = 0;
+      if (inner[j] || (p0[j] < hi[j] && p0[j] + PL > lo[j])) {
+        mask[j] = filter_lane<K, CI, PL>(h, bloom, d[j]);
+        if (!inner[j]) mask[j] &= valid_lane<PL>(p0[j], lo[j], hi[j], K, c.emit_end);
+      }
+      nh += __builtin_popcount(mask[j]);
+    }
+    if (__any(nh != 0u)) {
+      uint32_t slot = wave_reserve(nh, &s_n);
+#pragma unroll
+      for (int j = 0; j < kUnitsPerIter; ++j) {
+        for (uint32_t m = mask[j]; m; m &= m - 1, ++slot) {
+          const uint2 rec = make_uint2(p0[j] + __builtin_ctz(m), item[j]);
+          if (slot < hs.slice_cap) {
+            slice[slot] = rec;
           } else {
-            stage_rec(make_uint4(c.lit_pid[own], start, start + c.flen[f], item), lslot++, s_lit,
-                      &c.counters[0], c.lit_out, c.lit_cap);
+            const uint32_t gi = atomicAdd(hs.overflow_count, 1u);
+            if (gi < hs.overflow_cap) hs.overflow[gi] = rec;
           }
         }
       }
     }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) hs.slice_count[blockIdx.x] = s_n;
+}
+
+// Bytes [q, q + 32) of the segment (zero past lend), from two or three
+// aligned 16-byte loads.
+template <bool CI>
+__device__ __forceinline__ void load32_at(const uint8_t* base, uint64_t q, uint64_t lend, uint32_t* w) {
+  const uint64_t a = q & ~15ull;
+  uint32_t r[12];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    uint4 x = make_uint4(0, 0, 0, 0);
+    if (a + 16 * v + 16 <= lend) x = *reinterpret_cast<const uint4*>(base + a + 16 * v);
+    r[4 * v] = x.x; r[4 * v + 1] = x.y; r[4 * v + 2] = x.z; r[4 * v + 3] = x.w;
+  }
+  const uint32_t sh = (uint32_t)(q & 15);
+  const uint32_t dw = sh >> 2, bs = (sh & 3) * 8;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    // dynamic dword select from a small register array (compiles to v_cndmask chains)
+    uint32_t lo = r[0], hi = r[1];
+#pragma unroll
+    for (int t = 1; t < 4; ++t)
+      if (dw == (uint32_t)t) { lo = r[k + t]; hi = r[k + t + 1]; }
+    if (dw == 0) { lo = r[k]; hi = r[k + 1]; }
+    const uint32_t v = bs ? ((lo >> bs) | (hi << (32 - bs))) : lo;
+    w[k] = CI ? fold_word(v) : v;
+  }
+}
+
+template <int K, bool CI>
+__device__ __forceinline__ void verify_hit(const ScanCommon& c, const HashParams& h, uint2 hr, uint32_t* s_n,
+                                           Rec* s_lit, Rec* s_cand) {
+  const uint64_t q = hr.x;
+  const uint32_t item = hr.y;
+  const uint2 seg = c.items[item];
+  uint64_t lend = ((uint64_t)seg.y + 15) & ~15ull;
+  if (lend > c.load_end) lend = c.load_end;
+  uint32_t w[8];
+  load32_at<CI>(c.base, q, lend, w);
+  uint32_t H = 0;
+#pragma unroll
+  for (int t = 0; t < K; ++t) H = (H * h.B + ((w[t >> 2] >> ((t & 3) * 8)) & 0xFFu)) & 0xFFFFFFu;
+  const uint32_t bk = (H * 0x9E3779B1u) >> (32 - h.nbits);
+  const uint32_t e1 = h.bstart[bk + 1];
+  for (uint32_t e = h.bstart[bk]; e < e1; ++e) {
+    const uint2 ent = h.bent2[e];  // {hash, factor}
+    if (ent.x != H) continue;
+    const uint4 m = h.fmeta[ent.y];  // {offset, length, owner word, output id}
+    if (q + m.y > seg.y) continue;
+    // Compare in dwords: the first 32 bytes from registers, the rest from memory.
+    const uint32_t* pat = reinterpret_cast<const uint32_t*>(c.fbytes + m.x);
+    bool ok = true;
+    const uint32_t nw = (m.y + 3) >> 2;
+    for (uint32_t k = 0; k < nw && ok; ++k) {
+      uint32_t dv;
+      if (k < 8) {
+        dv = w[0];
+#pragma unroll
+        for (int t = 1; t < 8; ++t) if (k == (uint32_t)t) dv = w[t];
+      } else {
+        dv = 0;
+        for (int t = 0; t < 4; ++t) {
+          const uint64_t p = q + 4 * k + t;
+          uint8_t bb = p < seg.y ? c.base[p] : 0;
+          if (CI) bb = fold_byte(bb);
+          dv |= (uint32_t)bb << (8 * t);
+        }
+      }
+      const uint32_t rem = m.y - 4 * k;
+      const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
+      ok = ((dv ^ pat[k]) & msk) == 0;
+    }
+    if (!ok) continue;
+    const uint32_t start = (uint32_t)(q - seg.x);
+    if (m.z >> 31) {
+      stage_rec(make_uint4(m.w, start, 0u, item), atomicAdd(&s_n[1], 1u), s_cand, &c.counters[1], c.cand_out,
+                c.cand_cap);
+    } else {
+      stage_rec(make_uint4(m.w, start, start + m.y, item), atomicAdd(&s_n[0], 1u), s_lit, &c.counters[0],
+                c.lit_out, c.lit_cap);
+    }
+  }
+}
+
+// Exact check of every filter hit: rolling hash of the K-byte window, its
+// bucket chain (L2), byte verification of each candidate factor. Filter
+// slice b is shared by kVerifySplit blocks; then all blocks share the
+// overflow area. Literal matches and regex candidate starts go to block
+// staging buffers flushed with one global atomic each.
+constexpr uint32_t kVerifySplit = 8;
+
+template <int K, bool CI>
+__global__ __launch_bounds__(kTPB) void k_hash_verify(ScanCommon c, HashParams h, HitSlices hs) {
+  __shared__ Rec s_lit[kRecCap];
+  __shared__ Rec s_cand[kRecCap];
+  __shared__ uint32_t s_n[2];
+  __shared__ uint32_t s_base[4];
+  if (threadIdx.x == 0) { s_n[0] = 0; s_n[1] = 0; }
+  __syncthreads();
+  const uint32_t b = blockIdx.x / kVerifySplit, part = blockIdx.x % kVerifySplit;
+  const uint2* slice = hs.hits + (size_t)b * hs.slice_cap;
+  const uint32_t n = min(hs.slice_count[b], hs.slice_cap);
+  for (uint32_t base = part * kTPB; base < n; base += kVerifySplit * kTPB) {
+    const uint32_t i = base + threadIdx.x;
+    if (i < n) verify_hit<K, CI>(c, h, slice[i], s_n, s_lit, s_cand);
+    __syncthreads();
+    if (s_n[0] > kRecCap / 2 || s_n[1] > kRecCap / 2) flush_block(s_n, s_lit, s_cand, s_base, c);
+  }
+  const uint32_t no = min(*hs.overflow_count, hs.overflow_cap);
+  for (uint32_t base = blockIdx.x * kTPB; base < no; base += gridDim.x * kTPB) {
+    const uint32_t i = base + threadIdx.x;
+    if (i < no) verify_hit<K, CI>(c, h, hs.overflow[i], s_n, s_lit, s_cand);
     __syncthreads();
     if (s_n[0] > kRecCap / 2 || s_n[1] > kRecCap / 2) flush_block(s_n, s_lit, s_cand, s_base, c);
   }
@@ -344,7 +477,7 @@ __global__ __launch_bounds__(kTPB) void k_rx_walk(WalkParams w) {
   const uint32_t n = min(*w.ncand, w.cand_cap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const Rec cd = w.cands[i];
-    const uint32_t rx = cd.x, item = cd.z;
+    const uint32_t rx = cd.x, item = cd.w;
     const uint2 seg = w.items[item];
     const uint32_t* T = w.tables + w.toff[rx];
     const uint32_t C = w.ncls[rx];
@@ -517,35 +650,43 @@ static hipError_t allow_lds(F* fn, size_t bytes) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <int K>
-static hipError_t launch_hash_k(bool ci, uint32_t grid, size_t shmem, hipStream_t st,
-                                const ScanCommon& c, const HashParams& h) {
-  if (ci) {
-    hipError_t e = allow_lds(k_hash_scan<K, true>, shmem);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_hash_scan<K, true>), dim3(grid), dim3(kTPB), shmem, st, c, h);
-  } else {
-    hipError_t e = allow_lds(k_hash_scan<K, false>, shmem);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_hash_scan<K, false>), dim3(grid), dim3(kTPB), shmem, st, c, h);
+size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u; }
+
+template <int K, bool CI>
+static hipError_t launch_hash_kc(uint32_t grid, hipStream_t st, const ScanCommon& c, const HashParams& h,
+                                 const HitSlices& hs, bool verify_only) {
+  const size_t shmem = hash_shmem_bytes(h.nwords);
+  if (!verify_only) {
+    if (h.lane_bytes == 32) {
+      hipError_t e = allow_lds(k_hash_filter<K, CI, 32>, shmem);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_hash_filter<K, CI, 32>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, hs);
+    } else {
+      hipError_t e = allow_lds(k_hash_filter<K, CI, 16>, shmem);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_hash_filter<K, CI, 16>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, hs);
+    }
+    return hipGetLastError();
   }
+  hipLaunchKernelGGL((k_hash_verify<K, CI>), dim3(grid * kVerifySplit), dim3(kTPB), 0, st, c, h, hs);
   return hipGetLastError();
 }
 
-size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u + 2u * kRecCap * sizeof(Rec); }
+template <int K>
+static hipError_t launch_hash_k(bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c, const HashParams& h,
+                                const HitSlices& hs, bool verify_only) {
+  return ci ? launch_hash_kc<K, true>(grid, st, c, h, hs, verify_only)
+            : launch_hash_kc<K, false>(grid, st, c, h, hs, verify_only);
+}
 
+// verify_only = false launches the streaming filter, true the verify pass;
+// both use `grid` blocks (verify block b checks filter slice b).
 hipError_t launch_hash(uint32_t K, bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c,
-                       const HashParams& h) {
-  const size_t shmem = hash_shmem_bytes(h.nwords);
+                       const HashParams& h, const HitSlices& hs, bool verify_only) {
   switch (K) {
-    case 1: return launch_hash_k<1>(ci, grid, shmem, st, c, h);
-    case 2: return launch_hash_k<2>(ci, grid, shmem, st, c, h);
-    case 3: return launch_hash_k<3>(ci, grid, shmem, st, c, h);
-    case 4: return launch_hash_k<4>(ci, grid, shmem, st, c, h);
-    case 6: return launch_hash_k<6>(ci, grid, shmem, st, c, h);
-    case 8: return launch_hash_k<8>(ci, grid, shmem, st, c, h);
-    case 12: return launch_hash_k<12>(ci, grid, shmem, st, c, h);
-    case 16: return launch_hash_k<16>(ci, grid, shmem, st, c, h);
+#define WS_K(k) case k: return launch_hash_k<k>(ci, grid, st, c, h, hs, verify_only);
+    WS_K(1) WS_K(2) WS_K(3) WS_K(4) WS_K(6) WS_K(8) WS_K(12) WS_K(16)
+#undef WS_K
     default: return hipErrorInvalidValue;
   }
 }
@@ -653,17 +794,6 @@ hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_
 }
 
 // Explicit instantiations used by the host runtime.
-#define WS_HASH_INST(K)                                                          \
-  template __global__ void k_hash_scan<K, false>(ScanCommon, HashParams);        \
-  template __global__ void k_hash_scan<K, true>(ScanCommon, HashParams);
-WS_HASH_INST(1)
-WS_HASH_INST(2)
-WS_HASH_INST(3)
-WS_HASH_INST(4)
-WS_HASH_INST(6)
-WS_HASH_INST(8)
-WS_HASH_INST(12)
-WS_HASH_INST(16)
 template __global__ void k_ac_scan<true, 64>(ScanCommon, AcParams);
 template __global__ void k_ac_scan<false, 64>(ScanCommon, AcParams);
 

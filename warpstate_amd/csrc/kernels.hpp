@@ -7,8 +7,9 @@
 namespace ws {
 
 constexpr int kTPB = 256;                 // threads per block for the scan kernels
-constexpr uint32_t kHashUnit = kTPB * 16; // bytes per hash-kernel work unit
+constexpr uint32_t kHashUnit = kTPB * 16; // bytes per hash-kernel work unit at 16 positions per lane
 constexpr uint32_t kRecCap = 256;         // per-block LDS staging records (each list)
+constexpr int kFilterTPB = 1024;          // threads per block of the streaming hash filter
 
 // Match / candidate record. Match: {pattern_id, start, end, item}.
 // Regex candidate: {regex index, start, 0, item}. Offsets are relative to the
@@ -40,9 +41,22 @@ struct ScanCommon {
 struct HashParams {
   const uint32_t* bloom;
   uint32_t nwords, wshift, wbits, B, NBK, nbits;
+  uint32_t s2;  // shift of the second bloom bit (0 = single-bit exact filter)
+  uint32_t lane_bytes;  // positions per lane per work unit: 16 or 32
   const uint32_t* bstart;
-  const uint32_t* bent;
-  const uint32_t* fhash;
+  const uint2* bent2;   // bucket entries {hash, factor}
+  const uint4* fmeta;   // per factor {byte offset (4-aligned), length, kind << 31 | owner, output id}
+};
+
+// Filter hits {position, item}: one private slice per filter block plus a
+// shared overflow area.
+struct HitSlices {
+  uint2* hits;
+  uint32_t slice_cap;
+  uint32_t* slice_count;  // per block
+  uint2* overflow;
+  uint32_t overflow_cap;
+  uint32_t* overflow_count;
 };
 
 struct AcParams {

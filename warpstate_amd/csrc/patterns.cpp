@@ -38,12 +38,17 @@ void build_hash(PatternsHost& ps) {
   for (uint32_t t = 0; t < K; ++t) bk = (bk * ps.B) & 0xFFFFFFu;
   ps.NBK = (uint32_t)((0x1000000u - bk) & 0xFFFFFFu);
   if (K <= 2) {
+    // Exact filter over the raw 1- or 2-byte key: one bit per key value.
     ps.wbits = 8 * K - 5;
     ps.wshift = 5;
+    ps.s2 = 0;
   } else {
-    uint32_t lb = std::min(18u, std::max(10u, ceil_log2(F) + 4));
+    // Blocked bloom filter: the word comes from H's top bits, two bits inside
+    // it from H[4:0] and H[9:5]; about 256 bits per factor, 1 KiB .. 128 KiB.
+    uint32_t lb = std::min(20u, std::max(13u, ceil_log2(F) + 8));
     ps.wbits = lb - 5;
     ps.wshift = 24 - ps.wbits;
+    ps.s2 = 5;
   }
   ps.bloom.assign(1u << ps.wbits, 0);
   ps.nbits = std::max(1u, ceil_log2(2ull * F));
@@ -55,6 +60,7 @@ void build_hash(PatternsHost& ps) {
     ps.fhash[f] = h;
     uint32_t w = (h >> ps.wshift) & ((1u << ps.wbits) - 1);
     ps.bloom[w] |= 1u << (h & 31);
+    ps.bloom[w] |= 1u << ((h >> ps.s2) & 31);
     bucket[f] = (uint32_t)((h * 0x9E3779B1u) >> (32 - ps.nbits));
     cnt[bucket[f] + 1]++;
   }

@@ -45,7 +45,7 @@ struct PatternsHost {
 
   // Rolling-hash prefilter.
   bool hash_ok = false;
-  uint32_t K = 0, B = 0, NBK = 0, wshift = 0, wbits = 0, nbits = 0;
+  uint32_t K = 0, B = 0, NBK = 0, wshift = 0, wbits = 0, nbits = 0, s2 = 0;
   std::vector<uint32_t> bloom, bstart, bent, fhash;
 
   // Factor DFA (dense Aho-Corasick automaton over byte classes).
