@@ -19,7 +19,11 @@ namespace ws {
 // Launch wrappers defined in kernels.hip.
 size_t hash_shmem_bytes(uint32_t nwords);
 hipError_t launch_hash(uint32_t K, bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c,
-                       const HashParams& h, const HitSlices& hs, bool verify_only);
+                       const HashParams& h, const HitSlices& hs, const SliceOut& lit, const SliceOut& cand,
+                       bool verify_only);
+uint32_t verify_blocks(uint32_t filter_grid);
+hipError_t launch_slice_compact(const SliceOut& so, uint32_t n, uint32_t* offsets, uint32_t* total, Rec* out,
+                                uint32_t out_cap, hipStream_t st);
 uint32_t ac_unit_bytes();
 size_t ac_shmem_bytes(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCommon& c,
@@ -97,7 +101,7 @@ struct ws_ctx {
   ws_timings last{};
   hipEvent_t ev[8] = {};
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids, hits, slice_counts;
+      scratch, carried, ids, hits, slice_counts, lslices, cslices, scounts;
   uint32_t* hcount = nullptr;  // pinned host mirror of the counters
   uint32_t last_hits = 0;      // filter hits of the last hash-variant scan
 };
@@ -411,7 +415,16 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     }
     WS_HIP(ctx->slice_counts.ensure((size_t)fgrid * 4));
   }
-  uint32_t hit_overflow_cap = 0;
+  // Verify output slices (literal matches, regex candidates), one per verify block.
+  const uint32_t nvb = verify_blocks(fgrid);
+  uint64_t lslice_total = ctx->lslices.cap / sizeof(Rec), cslice_total = ctx->cslices.cap / sizeof(Rec);
+  if (variant == WS_VARIANT_HASH) {
+    const uint64_t want = (uint64_t)nvb * 256 + (1u << 16);
+    if (lslice_total < want) { WS_HIP(ctx->lslices.ensure(want * sizeof(Rec))); lslice_total = want; }
+    if (cslice_total < want) { WS_HIP(ctx->cslices.ensure(want * sizeof(Rec))); cslice_total = want; }
+    WS_HIP(ctx->scounts.ensure(((size_t)nvb * 2 + 4) * 4 * 2));
+  }
+  uint32_t hit_overflow_cap = 0, lit_ovf_cap = 0, cand_ovf_cap = 0;
   if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[0], st));
   for (int attempt = 0;; ++attempt) {
     WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 8, 0, st));
@@ -446,10 +459,29 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
         hs.overflow = hs.hits + (size_t)hs.slice_cap * fgrid;
         hs.slice_count = ctx->slice_counts.as<uint32_t>();
         hs.overflow_count = ctx->counters.as<uint32_t>() + 4;
-        WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, false));
+        auto make_slices = [&](DevBuf& buf, uint64_t total, uint32_t* counts, uint32_t* ovf_count) {
+          SliceOut so{};
+          so.ovf_cap = (uint32_t)std::min<uint64_t>(total / 4, 0xFFFFFFF0ull);
+          so.cap = (uint32_t)((total - so.ovf_cap) / nvb);
+          so.buf = buf.as<Rec>();
+          so.ovf = so.buf + (size_t)so.cap * nvb;
+          so.counts = counts;
+          so.ovf_count = ovf_count;
+          return so;
+        };
+        uint32_t* sc = ctx->scounts.as<uint32_t>();
+        SliceOut lso = make_slices(ctx->lslices, lslice_total, sc, ctx->counters.as<uint32_t>() + 5);
+        SliceOut cso = make_slices(ctx->cslices, cslice_total, sc + nvb, ctx->counters.as<uint32_t>() + 6);
+        WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, false));
         if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
-        WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, true));
+        WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, true));
+        uint32_t* offs = sc + 2 * nvb;
+        WS_HIP(launch_slice_compact(lso, nvb, offs, ctx->counters.as<uint32_t>() + 0, c.lit_out, c.lit_cap, st));
+        WS_HIP(launch_slice_compact(cso, nvb, offs + nvb + 2, ctx->counters.as<uint32_t>() + 1, c.cand_out,
+                                    c.cand_cap, st));
         hit_overflow_cap = hs.overflow_cap;
+        lit_ovf_cap = lso.ovf_cap;
+        cand_ovf_cap = cso.ovf_cap;
       } else {
         AcParams ap{p->ac_table32, p->ac_table16, p->ac_cmap, p->ac_out_start, p->ac_out_list,
                     p->ac_len, h.ac_states, h.ac_classes, h.ac_first_out, h.ac_maxlen};
@@ -467,6 +499,18 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       WS_HIP(ctx->hits.ensure(hit_total * 2 * sizeof(uint2) + (size_t)nover * 8 * sizeof(uint2)));
       hit_total = ctx->hits.cap / sizeof(uint2);
       if (attempt >= 2) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
+      continue;
+    }
+    if (variant == WS_VARIANT_HASH && (ctx->hcount[5] > lit_ovf_cap || ctx->hcount[6] > cand_ovf_cap)) {
+      if (ctx->hcount[5] > lit_ovf_cap) {
+        lslice_total = lslice_total * 2 + (uint64_t)ctx->hcount[5] * 8;
+        WS_HIP(ctx->lslices.ensure(lslice_total * sizeof(Rec)));
+      }
+      if (ctx->hcount[6] > cand_ovf_cap) {
+        cslice_total = cslice_total * 2 + (uint64_t)ctx->hcount[6] * 8;
+        WS_HIP(ctx->cslices.ensure(cslice_total * sizeof(Rec)));
+      }
+      if (attempt >= 3) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
       continue;
     }
     if (nlit > max_matches) {
@@ -738,7 +782,8 @@ void ws_ctx_destroy(ws_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->recs, &ctx->cands, &ctx->raw, &ctx->k0, &ctx->v0, &ctx->k1, &ctx->v1,
                     &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
-                    &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids, &ctx->hits, &ctx->slice_counts})
+                    &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids, &ctx->hits, &ctx->slice_counts, &ctx->lslices, &ctx->cslices,
+                    &ctx->scounts})
     b->release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);

@@ -286,9 +286,18 @@ __device__ __forceinline__ void load32_at(const uint8_t* base, uint64_t q, uint6
   }
 }
 
+__device__ __forceinline__ void slice_put(const SliceOut& so, uint32_t slice, uint32_t slot, Rec r) {
+  if (slot < so.cap) {
+    so.buf[(size_t)slice * so.cap + slot] = r;
+  } else {
+    const uint32_t g = atomicAdd(so.ovf_count, 1u);
+    if (g < so.ovf_cap) so.ovf[g] = r;
+  }
+}
+
 template <int K, bool CI>
 __device__ __forceinline__ void verify_hit(const ScanCommon& c, const HashParams& h, uint2 hr, uint32_t* s_n,
-                                           Rec* s_lit, Rec* s_cand) {
+                                           const SliceOut& lit, const SliceOut& cand) {
   const uint64_t q = hr.x;
   const uint32_t item = hr.y;
   const uint2 seg = c.items[item];
@@ -331,13 +340,10 @@ __device__ __forceinline__ void verify_hit(const ScanCommon& c, const HashParams
     }
     if (!ok) continue;
     const uint32_t start = (uint32_t)(q - seg.x);
-    if (m.z >> 31) {
-      stage_rec(make_uint4(m.w, start, 0u, item), atomicAdd(&s_n[1], 1u), s_cand, &c.counters[1], c.cand_out,
-                c.cand_cap);
-    } else {
-      stage_rec(make_uint4(m.w, start, start + m.y, item), atomicAdd(&s_n[0], 1u), s_lit, &c.counters[0],
-                c.lit_out, c.lit_cap);
-    }
+    if (m.z >> 31)
+      slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), make_uint4(m.w, start, 0u, item));
+    else
+      slice_put(lit, blockIdx.x, atomicAdd(&s_n[0], 1u), make_uint4(m.w, start, start + m.y, item));
   }
 }
 
@@ -349,30 +355,68 @@ __device__ __forceinline__ void verify_hit(const ScanCommon& c, const HashParams
 constexpr uint32_t kVerifySplit = 8;
 
 template <int K, bool CI>
-__global__ __launch_bounds__(kTPB) void k_hash_verify(ScanCommon c, HashParams h, HitSlices hs) {
-  __shared__ Rec s_lit[kRecCap];
-  __shared__ Rec s_cand[kRecCap];
+__global__ __launch_bounds__(kTPB) void k_hash_verify(ScanCommon c, HashParams h, HitSlices hs, SliceOut lit,
+                                                       SliceOut cand) {
   __shared__ uint32_t s_n[2];
-  __shared__ uint32_t s_base[4];
   if (threadIdx.x == 0) { s_n[0] = 0; s_n[1] = 0; }
   __syncthreads();
   const uint32_t b = blockIdx.x / kVerifySplit, part = blockIdx.x % kVerifySplit;
   const uint2* slice = hs.hits + (size_t)b * hs.slice_cap;
   const uint32_t n = min(hs.slice_count[b], hs.slice_cap);
-  for (uint32_t base = part * kTPB; base < n; base += kVerifySplit * kTPB) {
-    const uint32_t i = base + threadIdx.x;
-    if (i < n) verify_hit<K, CI>(c, h, slice[i], s_n, s_lit, s_cand);
-    __syncthreads();
-    if (s_n[0] > kRecCap / 2 || s_n[1] > kRecCap / 2) flush_block(s_n, s_lit, s_cand, s_base, c);
-  }
+  for (uint32_t i = part * kTPB + threadIdx.x; i < n; i += kVerifySplit * kTPB)
+    verify_hit<K, CI>(c, h, slice[i], s_n, lit, cand);
   const uint32_t no = min(*hs.overflow_count, hs.overflow_cap);
-  for (uint32_t base = blockIdx.x * kTPB; base < no; base += gridDim.x * kTPB) {
-    const uint32_t i = base + threadIdx.x;
-    if (i < no) verify_hit<K, CI>(c, h, hs.overflow[i], s_n, s_lit, s_cand);
-    __syncthreads();
-    if (s_n[0] > kRecCap / 2 || s_n[1] > kRecCap / 2) flush_block(s_n, s_lit, s_cand, s_base, c);
+  for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < no; i += gridDim.x * kTPB)
+    verify_hit<K, CI>(c, h, hs.overflow[i], s_n, lit, cand);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    lit.counts[blockIdx.x] = s_n[0];
+    cand.counts[blockIdx.x] = s_n[1];
   }
-  flush_block(s_n, s_lit, s_cand, s_base, c);
+}
+
+// Exclusive scan of min(count, cap) over the slices (one 1024-lane block);
+// offsets[n] and *total receive the sum including the overflow area.
+__global__ __launch_bounds__(1024) void k_slice_offsets(const uint32_t* counts, uint32_t n, uint32_t cap,
+                                                         const uint32_t* ovf_count, uint32_t ovf_cap,
+                                                         uint32_t* offsets, uint32_t* total) {
+  __shared__ uint32_t part[1024];
+  const uint32_t per = (n + 1023) / 1024;
+  const uint32_t b0 = threadIdx.x * per;
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < per && b0 + k < n; ++k) sum += min(counts[b0 + k], cap);
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t d = 1; d < 1024; d <<= 1) {
+    uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - sum;
+  for (uint32_t k = 0; k < per && b0 + k < n; ++k) {
+    offsets[b0 + k] = run;
+    run += min(counts[b0 + k], cap);
+  }
+  if (threadIdx.x == 1023) {
+    offsets[n] = part[1023];
+    *total = part[1023] + min(*ovf_count, ovf_cap);
+  }
+}
+
+// Copies slice b (block b) and the overflow area into one contiguous list.
+__global__ __launch_bounds__(kTPB) void k_slice_gather(SliceOut so, uint32_t n, const uint32_t* offsets,
+                                                        Rec* out, uint32_t out_cap) {
+  if (blockIdx.x < n) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t m = min(so.counts[b], so.cap), o = offsets[b];
+    for (uint32_t k = threadIdx.x; k < m; k += kTPB)
+      if (o + k < out_cap) out[o + k] = so.buf[(size_t)b * so.cap + k];
+  } else {
+    const uint32_t no = min(*so.ovf_count, so.ovf_cap), o = offsets[n];
+    for (uint32_t k = (blockIdx.x - n) * kTPB + threadIdx.x; k < no; k += (gridDim.x - n) * kTPB)
+      if (o + k < out_cap) out[o + k] = so.ovf[k];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -654,7 +698,8 @@ size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u; }
 
 template <int K, bool CI>
 static hipError_t launch_hash_kc(uint32_t grid, hipStream_t st, const ScanCommon& c, const HashParams& h,
-                                 const HitSlices& hs, bool verify_only) {
+                                 const HitSlices& hs, const SliceOut& lit, const SliceOut& cand,
+                                 bool verify_only) {
   const size_t shmem = hash_shmem_bytes(h.nwords);
   if (!verify_only) {
     if (h.lane_bytes == 32) {
@@ -668,23 +713,24 @@ static hipError_t launch_hash_kc(uint32_t grid, hipStream_t st, const ScanCommon
     }
     return hipGetLastError();
   }
-  hipLaunchKernelGGL((k_hash_verify<K, CI>), dim3(grid * kVerifySplit), dim3(kTPB), 0, st, c, h, hs);
+  hipLaunchKernelGGL((k_hash_verify<K, CI>), dim3(grid * kVerifySplit), dim3(kTPB), 0, st, c, h, hs, lit, cand);
   return hipGetLastError();
 }
 
 template <int K>
 static hipError_t launch_hash_k(bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c, const HashParams& h,
-                                const HitSlices& hs, bool verify_only) {
-  return ci ? launch_hash_kc<K, true>(grid, st, c, h, hs, verify_only)
-            : launch_hash_kc<K, false>(grid, st, c, h, hs, verify_only);
+                                const HitSlices& hs, const SliceOut& lit, const SliceOut& cand, bool verify_only) {
+  return ci ? launch_hash_kc<K, true>(grid, st, c, h, hs, lit, cand, verify_only)
+            : launch_hash_kc<K, false>(grid, st, c, h, hs, lit, cand, verify_only);
 }
 
 // verify_only = false launches the streaming filter, true the verify pass;
 // both use `grid` blocks (verify block b checks filter slice b).
 hipError_t launch_hash(uint32_t K, bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c,
-                       const HashParams& h, const HitSlices& hs, bool verify_only) {
+                       const HashParams& h, const HitSlices& hs, const SliceOut& lit, const SliceOut& cand,
+                       bool verify_only) {
   switch (K) {
-#define WS_K(k) case k: return launch_hash_k<k>(ci, grid, st, c, h, hs, verify_only);
+#define WS_K(k) case k: return launch_hash_k<k>(ci, grid, st, c, h, hs, lit, cand, verify_only);
     WS_K(1) WS_K(2) WS_K(3) WS_K(4) WS_K(6) WS_K(8) WS_K(12) WS_K(16)
 #undef WS_K
     default: return hipErrorInvalidValue;
@@ -778,6 +824,20 @@ hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32
                               uint32_t pbits, const uint64_t* ids, void* out, hipStream_t st) {
   hipLaunchKernelGGL(k_pack_tagged, dim3(grid_for(n)), dim3(kTPB), 0, st, keys, vals, n, sbits, pbits, ids,
                      reinterpret_cast<TaggedOut*>(out));
+  return hipGetLastError();
+}
+
+uint32_t verify_blocks(uint32_t filter_grid) { return filter_grid * kVerifySplit; }
+
+// Compacts per-block slices (and the overflow area) into `out`; the total
+// lands in *total.
+hipError_t launch_slice_compact(const SliceOut& so, uint32_t n, uint32_t* offsets, uint32_t* total, Rec* out,
+                                uint32_t out_cap, hipStream_t st) {
+  hipLaunchKernelGGL(k_slice_offsets, dim3(1), dim3(1024), 0, st, so.counts, n, so.cap, so.ovf_count, so.ovf_cap,
+                     offsets, total);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_slice_gather, dim3(n + 64), dim3(kTPB), 0, st, so, n, offsets, out, out_cap);
   return hipGetLastError();
 }
 

@@ -59,6 +59,17 @@ struct HitSlices {
   uint32_t* overflow_count;
 };
 
+// Per-block output slices (no shared counter in the hot path) plus a shared
+// overflow area; k_slice_offsets + k_slice_gather compact them afterwards.
+struct SliceOut {
+  Rec* buf;
+  uint32_t cap;        // records per slice
+  uint32_t* counts;    // per slice
+  Rec* ovf;
+  uint32_t ovf_cap;
+  uint32_t* ovf_count;
+};
+
 struct AcParams {
   const uint32_t* table32;
   const uint16_t* table16;
