@@ -42,10 +42,9 @@ def main() -> int:
         print("bench.py measures one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
         return 2
 
-    import torch
-
     from warpstate_amd import GpuMatcher, PatternSet, Tuning, workloads as W
     from warpstate_amd import _native as N
+    from warpstate_amd.device import DeviceBuffer, synchronize
 
     variant = {"auto": 0, "hash": 1, "dfa_global": 2, "dfa_lds": 3}[args.variant]
     t_gen = time.time()
@@ -58,20 +57,20 @@ def main() -> int:
     ps = b.build()
     m = GpuMatcher(ps)
     info = m.info()
-    dev = torch.from_numpy(data).to("cuda")
+    dev = DeviceBuffer.from_array(data)
     cap = 8 << 20
-    out = torch.empty(cap * 3, dtype=torch.int32, device="cuda")
+    out = DeviceBuffer(cap * 12)
     tuning = Tuning(variant=variant, blocks_per_cu=args.blocks_per_cu)
     ctx = m.ctx
     ctx.set_profiling(True)
     n = len(data)
 
     def step():
-        return m.scan_device(dev.data_ptr(), n, out.data_ptr(), cap, tuning=tuning)
+        return m.scan_device(dev.ptr, n, out.ptr, cap, tuning=tuning)
 
     for _ in range(args.warmup):
         count = step()
-    torch.cuda.synchronize()
+    synchronize()
     kern_ms, order_ms, ver_ms, hits = [], [], [], 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -81,7 +80,7 @@ def main() -> int:
         order_ms.append(t.order_ms)
         ver_ms.append(t.verify_ms)
         hits = t.hits
-    torch.cuda.synchronize()
+    synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
     value = n / (ms_per_step * 1e-3) / 1e9

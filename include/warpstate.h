@@ -173,7 +173,9 @@ ws_status ws_ctx_last_timings(ws_ctx* ctx, ws_timings* out);
 const char* ws_last_error(void);
 uint32_t ws_abi_version(void);
 
-/* ---- patterns: PatternSetBuilder::build (core/src/pattern/compiler.rs:97-340) ---- */
+/* ---- patterns: PatternSetBuilder::build (core/src/pattern/compiler.rs:97-340) ----
+ * The compiled set is uploaded to the context's device and is read-only from
+ * then on: any context on that device may scan with it, concurrently. */
 ws_status ws_patterns_compile(ws_ctx* ctx, const ws_pattern* patterns, size_t count,
                               uint32_t flags, ws_patterns** out);
 void ws_patterns_free(ws_patterns* patterns);
@@ -207,6 +209,52 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t
                          uint64_t len, uint32_t data_mem, uint64_t chunk_size, uint64_t overlap,
                          uint32_t mode, const ws_tuning* tuning, ws_match* out, uint64_t cap,
                          uint32_t out_mem, uint64_t* count);
+
+/* ---- kernel routing: AutoMatcher (core/src/router.rs:205-291, core/src/config.rs:4-35).
+ * The reference routes on one input-size threshold. Here an exhaustive autotuner
+ * times every kernel configuration, and a small learned cost model (FP32 MLP,
+ * trained on the GPU from the autotuner's timings) predicts the fastest one.
+ * Routing never changes results: every configuration returns the same matches. */
+#define WS_ROUTER_FEATURES 7
+#define WS_ROUTER_CONFIGS 8
+
+typedef struct ws_router ws_router;
+
+/* Configuration `index` (< WS_ROUTER_CONFIGS) of the routing table. */
+ws_status ws_config_tuning(uint32_t index, ws_tuning* out);
+/* Router features of a request: log2 bytes, log2 patterns, log2 factor-DFA states,
+ * byte classes, literal fraction, min literal length, log2(1 + matches per MiB);
+ * out[7] is padding (0). */
+ws_status ws_router_features(const ws_patterns_info* info, uint64_t bytes, float matches_per_mib,
+                             float out[8]);
+/* Exhaustive autotune: times each supported configuration `repeats` times on
+ * the request (device or host data) and writes the best time per configuration
+ * (ms; 0 where unsupported), mask[i] = 1 for supported ones, and the fastest index. */
+ws_status ws_autotune(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t* data, uint64_t len,
+                      uint32_t data_mem, uint32_t mode, uint32_t repeats, float* times_ms,
+                      float* mask, uint32_t* best);
+
+ws_status ws_router_create(int device_ordinal, uint32_t seed, ws_router** out);
+void ws_router_destroy(ws_router* router);
+const char* ws_router_last_error(void);
+uint32_t ws_router_param_count(void);
+ws_status ws_router_get_params(ws_router* router, float* out, uint32_t n);
+ws_status ws_router_set_params(ws_router* router, const float* in, uint32_t n);
+ws_status ws_router_set_normalization(ws_router* router, const float* xmean, const float* xstd,
+                                      float ymean, float ystd);
+ws_status ws_router_get_normalization(ws_router* router, float* xmean, float* xstd, float* ymean,
+                                      float* ystd);
+/* Trains on raw features x[n][8] and log-ms targets y[n][8] (mask[n][8] = 1 where measured). */
+ws_status ws_router_train(ws_router* router, const float* x, const float* y, const float* mask,
+                          uint32_t n, uint32_t epochs, uint32_t batch, float lr, uint32_t seed,
+                          float* final_loss);
+/* Predicted ms per configuration (optional) and the fastest allowed configuration. */
+ws_status ws_router_predict(ws_router* router, const float* x, uint32_t n, const float* mask,
+                            float* pred_ms, uint32_t* choice);
+/* Raw outputs / loss gradient on standardized inputs (for checking against a CPU reference). */
+ws_status ws_router_forward(ws_router* router, const float* x, uint32_t n, float* out);
+ws_status ws_router_gradient(ws_router* router, const float* x, const float* y, const float* mask,
+                             uint32_t n, float scale, float* grad_out);
 
 #ifdef __cplusplus
 }

@@ -131,6 +131,23 @@ SIGNATURES = {
         [c_void_p, c_void_p, POINTER(WsItem), c_size_t, c_uint32, c_uint32, POINTER(WsTuning), c_void_p,
          c_uint64, POINTER(c_uint64)],
     ),
+    "ws_config_tuning": (c_int, [c_uint32, POINTER(WsTuning)]),
+    "ws_router_features": (c_int, [POINTER(WsPatternsInfo), c_uint64, c_float, c_void_p]),
+    "ws_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p,
+                            POINTER(c_uint32)]),
+    "ws_router_create": (c_int, [c_int, c_uint32, POINTER(c_void_p)]),
+    "ws_router_destroy": (None, [c_void_p]),
+    "ws_router_last_error": (c_char_p, []),
+    "ws_router_param_count": (c_uint32, []),
+    "ws_router_get_params": (c_int, [c_void_p, c_void_p, c_uint32]),
+    "ws_router_set_params": (c_int, [c_void_p, c_void_p, c_uint32]),
+    "ws_router_set_normalization": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_float]),
+    "ws_router_get_normalization": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(c_float), POINTER(c_float)]),
+    "ws_router_train": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_uint32, c_float,
+                                c_uint32, POINTER(c_float)]),
+    "ws_router_predict": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
+    "ws_router_forward": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p]),
+    "ws_router_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_float, c_void_p]),
     "ws_stream_scan": (
         c_int,
         [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint64, c_uint64, c_uint32, POINTER(WsTuning),
@@ -174,3 +191,9 @@ def check(status: int, count: int | None = None) -> None:
     if status != 0:
         msg = load().ws_last_error()
         raise WarpstateError(status, msg.decode(errors="replace") if msg else "", count)
+
+
+def check_router(status: int) -> None:
+    if status != 0:
+        msg = load().ws_router_last_error()
+        raise WarpstateError(status, msg.decode(errors="replace") if msg else "")

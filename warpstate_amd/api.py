@@ -190,7 +190,9 @@ class PatternSet:
         return len(self.patterns)
 
     def _compiled(self, ctx: Context):
-        key = id(ctx)
+        # A compiled set is read-only device data: every context on the same
+        # device shares one copy.
+        key = ctx.device
         with self._lock:
             h = self._handles.get(key)
             if h is None:
@@ -310,6 +312,24 @@ class GpuMatcher:
                          N.MEM_DEVICE if out_on_device else N.MEM_HOST, ctypes.byref(count))
         N.check(st, int(count.value))
         return int(count.value)
+
+    def scan_device_array(self, data_ptr: int, length: int, mode="overlapping", tuning: Optional[Tuning] = None,
+                          capacity: Optional[int] = None) -> np.ndarray:
+        """Device-resident input, match list returned to the host (MATCH_DTYPE)."""
+        lib = N.load()
+        cap = capacity if capacity is not None else max(1024, min(self.config.max_matches, length // 64 + 4096))
+        t = tuning.to_c() if tuning else None
+        while True:
+            out = np.empty(cap, dtype=N.MATCH_DTYPE)
+            count = ctypes.c_uint64(0)
+            st = lib.ws_scan(self.ctx.handle, self.handle, ctypes.c_void_p(data_ptr), length, N.MEM_DEVICE,
+                             _mode_code(mode), ctypes.byref(t) if t else None, out.ctypes.data, cap, N.MEM_HOST,
+                             ctypes.byref(count))
+            if st == 11 and capacity is None and count.value > cap and count.value <= self.config.max_matches:
+                cap = int(count.value)
+                continue
+            N.check(st, int(count.value))
+            return out[: count.value]
 
     def info(self) -> dict:
         return self.patterns.info(self.ctx)

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -372,7 +374,14 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   std::vector<uint2> units;
   uint32_t nunits;
   if (plan.batch) {
-    for (uint32_t i = 0; i < plan.items.size(); ++i) {
+    // Work-queue order: largest items first (longest-processing-time first),
+    // each item's tiles contiguous.
+    std::vector<uint32_t> by_size(plan.items.size());
+    std::iota(by_size.begin(), by_size.end(), 0u);
+    std::stable_sort(by_size.begin(), by_size.end(), [&](uint32_t x, uint32_t y) {
+      return plan.items[x].y - plan.items[x].x > plan.items[y].y - plan.items[y].x;
+    });
+    for (uint32_t i : by_size) {
       uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
       for (uint64_t t = lo; t < hi; t += unit) units.push_back(make_uint2((uint32_t)t, i));
     }
@@ -446,6 +455,9 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     c.cand_out = ctx->cands.as<Rec>();
     c.cand_cap = cand_cap;
     c.counters = ctx->counters.as<uint32_t>();
+    // Batches pull uneven work from a device queue (zeroed with the
+    // counters); a single input's equal tiles use a static stride.
+    c.queue = plan.batch ? ctx->counters.as<uint32_t>() + 7 : nullptr;
     if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[1], st));
     if (nunits == 0 && ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
     if (nunits > 0) {
@@ -909,6 +921,7 @@ ws_status ws_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64
                   uint32_t data_mem, uint32_t mode, const ws_tuning* tuning, ws_match* out,
                   uint64_t cap, uint32_t out_mem, uint64_t* count) {
   if (!ctx || !p || !count) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (p->dev != ctx->dev) return fail(WS_ERR_INVALID_ARGUMENT, "pattern set was compiled for another device");
   if (len && !data) return fail(WS_ERR_INVALID_ARGUMENT, "data is null");
   if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
   if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");
@@ -941,6 +954,7 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
                          const ws_tuning* tuning, ws_match* out, uint64_t cap, uint32_t out_mem,
                          uint64_t* count) {
   if (!ctx || !p || !count) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (p->dev != ctx->dev) return fail(WS_ERR_INVALID_ARGUMENT, "pattern set was compiled for another device");
   if (len && !data) return fail(WS_ERR_INVALID_ARGUMENT, "data is null");
   if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
   if (chunk_size == 0) return fail(WS_ERR_INVALID_ARGUMENT, "chunk_size must be positive");
@@ -989,10 +1003,78 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
   return WS_OK;
 }
 
+static const ws_tuning kConfigTable[WS_ROUTER_CONFIGS] = {
+    {WS_VARIANT_HASH, 16, 0, 0},       {WS_VARIANT_HASH, 32, 0, 0},
+    {WS_VARIANT_DFA_GLOBAL, 0, 2, 0},  {WS_VARIANT_DFA_GLOBAL, 0, 4, 0},
+    {WS_VARIANT_DFA_GLOBAL, 0, 8, 0},  {WS_VARIANT_DFA_LDS, 0, 2, 0},
+    {WS_VARIANT_DFA_LDS, 0, 4, 0},     {WS_VARIANT_DFA_LDS, 0, 8, 0},
+};
+
+ws_status ws_config_tuning(uint32_t index, ws_tuning* out) {
+  if (!out || index >= WS_ROUTER_CONFIGS) return fail(WS_ERR_INVALID_ARGUMENT, "bad configuration index");
+  *out = kConfigTable[index];
+  return WS_OK;
+}
+
+ws_status ws_router_features(const ws_patterns_info* info, uint64_t bytes, float matches_per_mib,
+                             float out[8]) {
+  if (!info || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  out[0] = std::log2((float)bytes + 1.f);
+  out[1] = std::log2((float)std::max(1u, info->pattern_count));
+  out[2] = std::log2((float)info->dfa_states + 1.f);
+  out[3] = (float)info->dfa_classes;
+  out[4] = info->pattern_count ? (float)info->literal_count / (float)info->pattern_count : 0.f;
+  out[5] = (float)info->min_literal_len;
+  out[6] = std::log2(1.f + std::max(0.f, matches_per_mib));
+  out[7] = 0.f;
+  return WS_OK;
+}
+
+ws_status ws_autotune(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64_t len,
+                      uint32_t data_mem, uint32_t mode, uint32_t repeats, float* times_ms, float* mask,
+                      uint32_t* best) {
+  if (!ctx || !p || !times_ms || !mask || !best || !data || len == 0)
+    return fail(WS_ERR_INVALID_ARGUMENT, "bad arguments");
+  if (p->dev != ctx->dev) return fail(WS_ERR_INVALID_ARGUMENT, "pattern set was compiled for another device");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  WS_HIP(hipSetDevice(ctx->dev));
+  const uint32_t sup = supported_variants(p);
+  int bi = -1;
+  for (uint32_t i = 0; i < WS_ROUTER_CONFIGS; ++i) {
+    times_ms[i] = 0.f;
+    mask[i] = (sup >> kConfigTable[i].variant) & 1u ? 1.f : 0.f;
+    if (mask[i] == 0.f) continue;
+    float tbest = 0.f;
+    for (uint32_t r = 0; r < std::max(1u, repeats); ++r) {
+      Plan plan;
+      ws_status s = plan_single(ctx, data, len, data_mem, &plan);
+      if (s != WS_OK) return s;
+      WS_HIP(hipStreamSynchronize(ctx->stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      Result res;
+      s = run_plan(ctx, p, plan, mode, &kConfigTable[i], &res);
+      if (s != WS_OK) return s;
+      if (res.count) {
+        WS_HIP(ctx->outbuf.ensure((size_t)res.count * sizeof(ws_match)));
+        WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), res.count, res.pbits, 0,
+                                   ctx->outbuf.p, ctx->stream));
+      }
+      WS_HIP(hipStreamSynchronize(ctx->stream));
+      const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      if (r == 0 || ms < tbest) tbest = ms;
+    }
+    times_ms[i] = tbest;
+    if (bi < 0 || tbest < times_ms[bi]) bi = (int)i;
+  }
+  *best = bi < 0 ? 0u : (uint32_t)bi;
+  return WS_OK;
+}
+
 ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items, size_t n_items,
                         uint32_t data_mem, uint32_t mode, const ws_tuning* tuning,
                         ws_tagged_match* out, uint64_t cap, uint64_t* count) {
   if (!ctx || !p || !count) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (p->dev != ctx->dev) return fail(WS_ERR_INVALID_ARGUMENT, "pattern set was compiled for another device");
   if (n_items && !items) return fail(WS_ERR_INVALID_ARGUMENT, "items is null");
   if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
   if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");

@@ -114,6 +114,23 @@ __device__ void flush_block(uint32_t* s_n, Rec* s_lit, Rec* s_cand, uint32_t* s_
   __syncthreads();
 }
 
+// Next work group of a persistent block: pulled from the device work queue
+// when there is one, else a static stride. Thread 0 claims one group ahead
+// (`ahead` lives in its registers), so the atomic's round trip overlaps the
+// current group's loads instead of stalling the block at the barrier.
+// Must be called by every thread of the block.
+__device__ __forceinline__ uint32_t next_group(const ScanCommon& c, uint32_t cur, bool first, uint32_t* s_slot,
+                                               uint32_t& ahead) {
+  if (!c.queue) return first ? blockIdx.x : cur + gridDim.x;
+  if (!first) __syncthreads();
+  if (threadIdx.x == 0) {
+    *s_slot = first ? atomicAdd(c.queue, 1u) : ahead;
+    ahead = atomicAdd(c.queue, 1u);
+  }
+  __syncthreads();
+  return *s_slot;
+}
+
 __device__ __forceinline__ uint32_t byte_of(const uint32_t* d, int j) {
   return (d[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
 }
@@ -192,7 +209,10 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
   const uint32_t load_end = (uint32_t)c.load_end;
   constexpr uint32_t kPerGroup = kSubBlocks * kUnitsPerIter;
   const uint32_t groups = (c.nunits + kPerGroup - 1) / kPerGroup;
-  for (uint32_t g = blockIdx.x; g < groups; g += gridDim.x) {
+  __shared__ uint32_t s_slot;
+  uint32_t ahead = 0;
+  for (uint32_t g = next_group(c, 0, true, &s_slot, ahead); g < groups;
+       g = next_group(c, g, false, &s_slot, ahead)) {
     uint32_t d[kUnitsPerIter][NW];
     uint32_t p0[kUnitsPerIter], lo[kUnitsPerIter], hi[kUnitsPerIter], item[kUnitsPerIter];
     bool inner[kUnitsPerIter];
@@ -445,7 +465,10 @@ __global__ __launch_bounds__(kTPB) void k_ac_scan(ScanCommon c, AcParams a) {
   if (threadIdx.x == 0) { s_n[0] = 0; s_n[1] = 0; }
   __syncthreads();
 
-  for (uint32_t u = blockIdx.x; u < c.nunits; u += gridDim.x) {
+  __shared__ uint32_t s_slot;
+  uint32_t ahead = 0;
+  for (uint32_t u = next_group(c, 0, true, &s_slot, ahead); u < c.nunits;
+       u = next_group(c, u, false, &s_slot, ahead)) {
     uint64_t tile_base;
     uint32_t item;
     if (c.units) {

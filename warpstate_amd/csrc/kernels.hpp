@@ -36,6 +36,7 @@ struct ScanCommon {
   Rec* cand_out;
   uint32_t cand_cap;
   uint32_t* counters;  // [0] matches, [1] candidates, [2] raw regex matches, [3] flags
+  uint32_t* queue;     // device work queue head (zeroed per launch), or null: static schedule
 };
 
 struct HashParams {
