@@ -1,19 +1,30 @@
 #!/usr/bin/env python3
 """Scan throughput benchmark on one MI355X.
 
-Workload: the k-mer literal scan — 10,000 ACGT literals (lengths 12-32),
-1 GiB of ACGT input with 0.5% 'N' bytes and one planted k-mer per 4 KiB.
-A step is one whole ws_scan call over the device-resident input: prefilter +
-verify kernel, ordering (sort) and packing of the sorted match list into a
-device buffer. `value` is input bytes / step time (GB/s).
+Default workload (the headline): the k-mer literal scan — 10,000 ACGT
+literals (lengths 12-32), 1 GiB of ACGT input with 0.5% 'N' bytes and one
+planted k-mer per 4 KiB. A step is one whole ws_scan call over the
+device-resident input: prefilter + verify kernels, ordering (sort) and
+packing of the sorted match list into a device buffer. `value` is input bytes
+per step time (GB/s).
 
-The roofline object covers the dominant kernel (the rolling-hash prefilter),
-timed with HIP events on the library's stream; `cpu_baseline` is the C oracle
-(overlapping literal scan, one core) on a bounded sample of the same input.
+Other workloads (--workload): `literal` (64 literals, 16 MiB printable; a
+rotating pool of 32 input copies so every step reads HBM, not the 256 MiB
+MALL), `regex` (200 grammar regexes, 1 GiB, StreamPipeline with 64 MiB
+chunks and overlap = longest match), `batch` (100,000 log-normal items,
+~1 GiB, one scan_batch call over device-resident items).
+
+`roofline` covers the dominant kernel (the rolling-hash prefilter,
+k_hash_filter), timed with HIP events on the library's stream inside the timed
+region. `cpu_baseline` is the C oracle on the box's host cores (byte-range
+split over threads for the literal workloads). With the default workload the
+learned-router study also runs (harvest, train, regret, continuous-batching
+request loop) and its results ride in the same JSON line.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -26,87 +37,226 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "scan GB/s on 1 MI355X, % of HBM peak; learned-router regret vs autotune"
 
 
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def profile_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, calibrated against a copy kernel)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.txt")
+    try:
+        with open(path) as f:
+            m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def build(patterns):
+    from warpstate_amd import PatternSet
+
+    b = PatternSet.builder()
+    for kind, p in patterns:
+        (b.literal if kind == 0 else b.regex)(p)
+    return b.build()
+
+
+def cpu_rate(patterns, data: np.ndarray, threads: int, mode: int = 0):
+    """One oracle pass over `data`; returns (GB/s, seconds)."""
+    from oracle.oracle import MATCH_DTYPE, Oracle
+
+    o = Oracle(patterns)
+    out = np.zeros(max(1 << 16, len(data) // 256), dtype=MATCH_DTYPE)
+    c0 = time.perf_counter()
+    n = o.scan_into(data, out, mode, threads)
+    dt = time.perf_counter() - c0
+    if n > len(out):  # the count pass is complete; the timing stands
+        pass
+    return len(data) / dt / 1e9, dt
+
+
+class Workload:
+    def __init__(self, args):
+        from warpstate_amd import AutoMatcherConfig, Context, GpuMatcher, StreamPipeline, Tuning
+        from warpstate_amd import workloads as W
+        from warpstate_amd.device import DeviceBuffer
+
+        self.args = args
+        self.name = args.workload
+        variant = {"auto": 0, "hash": 1, "dfa_global": 2, "dfa_lds": 3}[args.variant]
+        self.tuning = Tuning(variant=variant, blocks_per_cu=args.blocks_per_cu)
+        t0 = time.time()
+        self.copies = 1
+        if self.name == "kmer":
+            lits, data = W.kmer_workload(size=args.size or (1 << 30), n=10_000, seed=1)
+            self.patterns = [(0, p) for p in lits]
+            self.config = {"workload": "kmer-literal-scan", "patterns": len(lits), "pattern_len": "12-32 ACGT",
+                           "input_bytes": len(data), "n_frac": 0.005, "plant_every": 4096, "mode": "overlapping"}
+        elif self.name == "literal":
+            lits, data = W.literal_workload(size=args.size or (16 << 20), seed=0)
+            self.patterns = [(0, p) for p in lits]
+            self.copies = 32
+            self.config = {"workload": "literal-scan", "patterns": len(lits), "pattern_len": "4-16 [a-z_]",
+                           "input_bytes": len(data), "plant_every": 65536, "mode": "overlapping",
+                           "input_copies": self.copies}
+        elif self.name == "regex":
+            srcs, data, maxlen = W.regex_workload(size=args.size or (1 << 30), n=200, seed=2)
+            self.patterns = [(1, s.encode()) for s in srcs]
+            self.maxlen = maxlen
+            self.config = {"workload": "regex-stream-scan", "patterns": len(srcs), "input_bytes": len(data),
+                           "chunk": 64 << 20, "overlap": maxlen, "plant_every": 32768, "mode": "overlapping"}
+        elif self.name == "batch":
+            pats, lits, specs = W.mixed_set(700, 300, seed=3)
+            pieces = lits[:256] + [s.sample(np.random.default_rng(5)) for s in specs[:256]]
+            ids, offs, sizes, data = W.batch_items(100_000, seed=4, pieces=pieces)
+            self.patterns = pats
+            self.ids, self.offs, self.sizes = ids, offs, sizes
+            self.config = {"workload": "uneven-batch-scan", "patterns": len(pats), "items": len(ids),
+                           "input_bytes": int(sizes.sum()), "item_median": 4096, "sigma": 1.5,
+                           "mode": "overlapping"}
+        else:
+            raise SystemExit(f"unknown workload {self.name}")
+        self.data = data
+        log(f"generated {len(self.patterns)} patterns, {len(data)} bytes in {time.time() - t0:.1f}s")
+        self.ps = build(self.patterns)
+        self.ctx = Context(0, AutoMatcherConfig())
+        self.m = GpuMatcher(self.ps, context=self.ctx)
+        self.dev = [DeviceBuffer.from_array(data)]
+        for _ in range(self.copies - 1):
+            b = DeviceBuffer(len(data))
+            b.copy_from(self.dev[0])
+            self.dev.append(b)
+        self.cap = 16 << 20
+        self.out = DeviceBuffer(self.cap * 12)
+        self.n = len(data) if self.name != "batch" else int(self.sizes.sum())
+        self.k = 0
+        if self.name == "regex":
+            self.pipe = StreamPipeline(self.m, self.maxlen)
+        if self.name == "batch":
+            self.ptrs = (self.dev[0].ptr + self.offs).astype(np.uint64)
+
+    def step(self) -> int:
+        from warpstate_amd import scan_batch_device
+
+        buf = self.dev[self.k % self.copies]
+        self.k += 1
+        if self.name in ("kmer", "literal"):
+            return self.m.scan_device(buf.ptr, self.n, self.out.ptr, self.cap, tuning=self.tuning)
+        if self.name == "regex":
+            return self.pipe.scan_device(buf.ptr, self.n, self.out.ptr, self.cap, tuning=self.tuning,
+                                         chunk_size=64 << 20)
+        return len(scan_batch_device(self.m, self.ids, self.ptrs, self.sizes, tuning=self.tuning))
+
+    def cpu_baseline(self):
+        threads = cpu_threads()
+        if self.name in ("kmer", "literal"):
+            sample = self.data if self.name == "literal" else self.data[: self.args.cpu_sample or len(self.data)]
+            rate_mt, dt_mt = cpu_rate(self.patterns, sample, threads)
+            one = sample[: min(len(sample), 64 << 20)]
+            rate_1, dt_1 = cpu_rate(self.patterns, one, 1)
+            return {"value": round(rate_mt, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+                    "sample": f"{len(sample) >> 20} MiB of the same input, overlapping literal scan, C oracle "
+                              f"split over {threads} threads by byte range ({dt_mt:.1f} s)",
+                    "single_core_value": round(rate_1, 4),
+                    "single_core_sample": f"first {len(one) >> 20} MiB, one thread ({dt_1:.1f} s)"}
+        # Regex / mixed sets: the oracle's regex engine is single-threaded.
+        size = self.args.cpu_sample or (8 << 20)
+        sample = self.data[:size]
+        rate_1, dt_1 = cpu_rate(self.patterns, sample, 1)
+        return {"value": round(rate_1, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+                "sample": f"first {len(sample) >> 20} MiB of the same input as one scan, overlapping literals + "
+                          f"leftmost-first regexes, C oracle, one thread ({dt_1:.1f} s)"}
+
+
+def copy_calibration(args) -> int:
+    """1 GiB device-to-device copies, for calibrating FETCH_SIZE / WRITE_SIZE."""
+    from warpstate_amd.device import DeviceBuffer, synchronize
+
+    n = args.size or (1 << 30)
+    a, b = DeviceBuffer(n), DeviceBuffer(n)
+    a.zero()
+    for _ in range(args.warmup):
+        b.copy_from(a)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_from(a)
+    synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"copy_bytes": n, "ms_per_copy": round(dt * 1e3, 4),
+                      "gbs_read_plus_write": round(2 * n / dt / 1e9, 1)}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--size", type=int, default=1 << 30, help="input bytes")
-    ap.add_argument("--patterns", type=int, default=10_000)
+    ap.add_argument("--workload", default="kmer", choices=["kmer", "literal", "regex", "batch"])
+    ap.add_argument("--size", type=int, default=0, help="input bytes (default: the workload's own)")
     ap.add_argument("--variant", default="auto", choices=["auto", "hash", "dfa_global", "dfa_lds"])
     ap.add_argument("--blocks-per-cu", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=32 << 20, help="bytes for the CPU baseline")
+    ap.add_argument("--cpu-sample", type=int, default=0, help="bytes for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--router", dest="router", action="store_true", default=None,
+                    help="run the learned-router study (default: on for the kmer workload)")
+    ap.add_argument("--no-router", dest="router", action="store_false")
+    ap.add_argument("--requests", type=int, default=50_000, help="requests in the serving loop")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration(args)
 
-    from warpstate_amd import GpuMatcher, PatternSet, Tuning, workloads as W
-    from warpstate_amd import _native as N
-    from warpstate_amd.device import DeviceBuffer, synchronize
+    from warpstate_amd.device import synchronize
 
-    variant = {"auto": 0, "hash": 1, "dfa_global": 2, "dfa_lds": 3}[args.variant]
-    t_gen = time.time()
-    pats, data = W.kmer_workload(size=args.size, n=args.patterns, seed=1)
-    print(f"# generated {len(pats)} patterns, {len(data)} bytes in {time.time() - t_gen:.1f}s", file=sys.stderr,
-          flush=True)
-    b = PatternSet.builder()
-    for p in pats:
-        b.literal(p)
-    ps = b.build()
-    m = GpuMatcher(ps)
-    info = m.info()
-    dev = DeviceBuffer.from_array(data)
-    cap = 8 << 20
-    out = DeviceBuffer(cap * 12)
-    tuning = Tuning(variant=variant, blocks_per_cu=args.blocks_per_cu)
-    ctx = m.ctx
-    ctx.set_profiling(True)
-    n = len(data)
-
-    def step():
-        return m.scan_device(dev.ptr, n, out.ptr, cap, tuning=tuning)
-
+    w = Workload(args)
+    w.ctx.set_profiling(True)
     for _ in range(args.warmup):
-        count = step()
+        count = w.step()
     synchronize()
-    kern_ms, order_ms, ver_ms, hits = [], [], [], 0
+    kern_ms, ver_ms, order_ms, rx_ms, hits = [], [], [], [], 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        count = step()
-        t = ctx.last_timings()
+        count = w.step()
+        t = w.ctx.last_timings()
         kern_ms.append(t.prefilter_ms)
-        order_ms.append(t.order_ms)
         ver_ms.append(t.verify_ms)
+        order_ms.append(t.order_ms)
+        rx_ms.append(t.regex_ms)
         hits = t.hits
     synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
-    value = n / (ms_per_step * 1e-3) / 1e9
+    value = w.n / (ms_per_step * 1e-3) / 1e9
     k_ms = float(np.mean(kern_ms))
-    # The streaming prefilter reads every input byte once; its algorithmic
-    # traffic per launch is N (match records are written by the verify pass).
-    alg_bytes = n
-    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-    print(f"# matches={count} hits={hits} kernel_ms={k_ms:.4f} verify_ms={np.mean(ver_ms):.4f} "
-          f"order_ms={np.mean(order_ms):.4f} "
-          f"step_ms={ms_per_step:.4f} variant={info['variants_supported']} K={info['hash_window']}",
-          file=sys.stderr, flush=True)
+    # k_hash_filter reads every input byte once and writes only candidate
+    # hits (the match records come from the verify kernel): N bytes a launch.
+    alg_bytes = w.n
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    info = w.m.info()
+    log(f"matches={count} hits={hits} kernel_ms={k_ms:.4f} verify_ms={np.mean(ver_ms):.4f} "
+        f"regex_ms={np.mean(rx_ms):.4f} order_ms={np.mean(order_ms):.4f} step_ms={ms_per_step:.4f} "
+        f"variants={info['variants_supported']} K={info['hash_window']}")
 
-    cpu = None
-    if not args.no_cpu:
-        from oracle.oracle import Oracle
-
-        sample = data[: args.cpu_sample]
-        o = Oracle([(0, p) for p in pats])
-        c0 = time.perf_counter()
-        o.scan(sample, 0)
-        cdt = time.perf_counter() - c0
-        cpu = {"value": round(len(sample) / cdt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-               "sample": f"first {len(sample) >> 20} MiB of the same input, overlapping literal scan "
-                         f"(C oracle, {cdt:.1f} s)"}
-
+    cpu = None if args.no_cpu else w.cpu_baseline()
+    if cpu:
+        log(f"cpu baseline {cpu}")
+    traffic = profile_traffic("k_hash_filter", w.name)
+    config = dict(w.config)
+    config.update({"variant": args.variant, "matches": int(count), "parallelism": "single-gpu"})
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -120,15 +270,25 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": "kmer-literal-scan", "patterns": len(pats), "pattern_len": "12-32 ACGT",
-                   "input_bytes": n, "n_frac": 0.005, "plant_every": 4096, "mode": "overlapping",
-                   "variant": args.variant, "matches": int(count), "parallelism": "single-gpu"},
+        "config": config,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_hash_filter", "kernel_ms": round(k_ms, 4)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_hash_filter", "kernel_ms": round(k_ms, 4),
+                     "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
         "pct_hbm_peak_whole_step": round(100 * value / HBM_PEAK_GBS, 2),
+        "step_breakdown_ms": {"prefilter": round(k_ms, 4), "verify": round(float(np.mean(ver_ms)), 4),
+                              "regex": round(float(np.mean(rx_ms)), 4),
+                              "order": round(float(np.mean(order_ms)), 4)},
     }
+    run_router = args.router if args.router is not None else (w.name == "kmer")
+    if run_router:
+        from warpstate_amd.router import run_study
+
+        del w
+        t1 = time.time()
+        study = run_study(n_requests=args.requests, log=lambda m: log(f"{time.time() - t1:.1f}s {m}"))
+        line.update(study)
     print(json.dumps(line), flush=True)
     return 0
 

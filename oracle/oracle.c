@@ -9,6 +9,8 @@
  * crate reports (core/src/regex_engine.rs:7-17, core/src/dfa/scan.rs:433-457).
  * It shares no code with the product's DFA construction.
  */
+#include <pthread.h>
+
 #include "oracle.h"
 
 #include <stdio.h>
@@ -1061,5 +1063,59 @@ uint64_t or_scan(const or_set* s, const uint8_t* d, uint64_t len, int mode, or_m
   uint64_t n = all.n;
   for (uint64_t k = 0; k < n && k < cap; ++k) out[k] = all.m[k];
   free(all.m);
+  return n;
+}
+
+/* Byte-range split of the overlapping literal scan over `threads` POSIX
+ * threads: shard k scans [lo, hi + max_len - 1) and keeps starts in [lo, hi),
+ * the halo rule of build_scan_shards (core/src/multi_gpu.rs:170-200). Shards
+ * are start-ordered, so their sorted lists concatenate in canonical order.
+ * Literal-only sets; returns UINT64_MAX for a set with regexes. */
+typedef struct {
+  const or_set* s;
+  const uint8_t* d;
+  uint64_t len, lo, hi;
+  mlist out;
+} shard_job;
+
+static void* shard_run(void* arg) {
+  shard_job* j = (shard_job*)arg;
+  uint64_t end = j->hi + (j->s->max_lit ? j->s->max_lit - 1 : 0);
+  if (end > j->len) end = j->len;
+  mlist tmp = {0};
+  lits_overlapping(j->s, j->d + j->lo, end - j->lo, &tmp);
+  for (uint64_t k = 0; k < tmp.n; ++k)
+    if (tmp.m[k].start < j->hi - j->lo)
+      ml_push(&j->out, tmp.m[k].pattern_id, tmp.m[k].start + j->lo, tmp.m[k].end + j->lo);
+  free(tmp.m);
+  if (j->out.n) qsort(j->out.m, j->out.n, sizeof(or_match), cmp_canon);
+  return NULL;
+}
+
+uint64_t or_scan_overlapping_mt(const or_set* s, const uint8_t* d, uint64_t len, uint32_t threads,
+                                or_match* out, uint64_t cap) {
+  if (s->nrx) return UINT64_MAX;
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  shard_job* jobs = (shard_job*)calloc(threads, sizeof(shard_job));
+  pthread_t* tid = (pthread_t*)calloc(threads, sizeof(pthread_t));
+  const uint64_t step = (len + threads - 1) / threads;
+  for (uint32_t t = 0; t < threads; ++t) {
+    jobs[t].s = s;
+    jobs[t].d = d;
+    jobs[t].len = len;
+    jobs[t].lo = step * t < len ? step * t : len;
+    jobs[t].hi = step * (t + 1) < len ? step * (t + 1) : len;
+    pthread_create(&tid[t], NULL, shard_run, &jobs[t]);
+  }
+  uint64_t n = 0;
+  for (uint32_t t = 0; t < threads; ++t) {
+    pthread_join(tid[t], NULL);
+    for (uint64_t k = 0; k < jobs[t].out.n; ++k, ++n)
+      if (n < cap) out[n] = jobs[t].out.m[k];
+    free(jobs[t].out.m);
+  }
+  free(jobs);
+  free(tid);
   return n;
 }

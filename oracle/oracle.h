@@ -60,6 +60,12 @@ uint32_t or_strategy(const or_set* s);
 uint64_t or_scan(const or_set* s, const uint8_t* data, uint64_t len, int mode, or_match* out,
                  uint64_t cap);
 
+/* scan_overlapping split over `threads` threads by byte range with a
+ * (max_len - 1) halo (core/src/multi_gpu.rs:170-200); literal-only sets,
+ * UINT64_MAX otherwise. Same result as or_scan(mode 0). */
+uint64_t or_scan_overlapping_mt(const or_set* s, const uint8_t* data, uint64_t len, uint32_t threads,
+                                or_match* out, uint64_t cap);
+
 /* Leftmost-first find_iter of one regex (core/src/dfa/scan.rs:433-457). */
 uint64_t or_regex_find_iter(const or_set* s, uint32_t regex_index, const uint8_t* data,
                             uint64_t len, or_match* out, uint64_t cap);

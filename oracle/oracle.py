@@ -52,6 +52,8 @@ def lib():
             getattr(L, name).argtypes = [c_void_p]
         L.or_scan.restype = c_uint64
         L.or_scan.argtypes = [c_void_p, c_void_p, c_uint64, c_int, c_void_p, c_uint64]
+        L.or_scan_overlapping_mt.restype = c_uint64
+        L.or_scan_overlapping_mt.argtypes = [c_void_p, c_void_p, c_uint64, c_uint32, c_void_p, c_uint64]
         L.or_regex_find_iter.restype = c_uint64
         L.or_regex_find_iter.argtypes = [c_void_p, c_uint32, c_void_p, c_uint64, c_void_p, c_uint64]
         L.or_regex_match_at.restype = c_int64
@@ -131,6 +133,19 @@ class Oracle:
         out = np.zeros(max(int(n), 1), dtype=MATCH_DTYPE)
         lib().or_scan(self.h, buf.ctypes.data, len(buf), mode, out.ctypes.data, n)
         return out[: int(n)]
+
+    def scan_into(self, data, out: np.ndarray, mode: int = 0, threads: int = 1) -> int:
+        """One pass into a caller buffer (MATCH_DTYPE); returns the total count.
+        threads > 1 splits an overlapping literal-only scan by byte range."""
+        buf = np.frombuffer(_b(data) if not isinstance(data, np.ndarray) else data, dtype=np.uint8)
+        if threads > 1:
+            if mode != 0:
+                raise ValueError("the threaded oracle scan is overlapping-only")
+            n = lib().or_scan_overlapping_mt(self.h, buf.ctypes.data, len(buf), threads, out.ctypes.data, len(out))
+            if n == 2 ** 64 - 1:
+                raise ValueError("the threaded oracle scan is literal-only")
+            return int(n)
+        return int(lib().or_scan(self.h, buf.ctypes.data, len(buf), mode, out.ctypes.data, len(out)))
 
     def scan_list(self, data, mode: int = 0):
         a = self.scan(data, mode)

@@ -180,6 +180,48 @@ def test_batch_per_item_exact():
     assert [(t.source_id, tuple(t.matched)) for t in got2] == [(3, (0, 0, 6))]
 
 
+def test_batch_device_items_unaligned():
+    # Items back to back in one device allocation (arbitrary alignment),
+    # ids out of order, some empty: TaggedMatch order is (source_id, start,
+    # pattern_id, end) and every item is scanned on its own.
+    from warpstate_amd import scan_batch_device
+    from warpstate_amd.device import DeviceBuffer
+
+    pats, _, specs = W.mixed_set(70, 30, seed=8)
+    rng = np.random.default_rng(9)
+    pieces = [p for _, p in pats[:70]] + [s.sample(rng) for s in specs]
+    ids, offs, sizes, buf = W.batch_items(700, seed=10, pieces=pieces, plant_gap=300)
+    sizes = sizes.copy()
+    sizes[::37] = 0
+    ids = rng.permutation(np.arange(700, dtype=np.uint64) * 3 + 11)
+    dev = DeviceBuffer.from_array(buf)
+    m = GpuMatcher(build(pats))
+    got = scan_batch_device(m, ids, (dev.ptr + offs).astype(np.uint64), sizes)
+    o = Oracle(pats)
+    exp = []
+    for i in np.argsort(ids, kind="stable"):
+        chunk = buf[offs[i]:offs[i] + sizes[i]].tobytes()
+        exp += [(int(ids[i]),) + x for x in o.scan_list(chunk)]
+    got_l = [(int(a), int(b), int(c), int(d)) for a, b, c, d in
+             zip(got["source_id"], got["pattern_id"], got["start"], got["end"])]
+    assert len(exp) > 200
+    assert_same(got_l, exp, "device batch")
+
+
+def test_stream_scan_device_input_equals_whole():
+    from warpstate_amd.device import DeviceBuffer
+
+    srcs, data, maxlen = W.regex_workload(size=3 << 20, n=40, seed=14)
+    m = GpuMatcher(build([(1, s.encode()) for s in srcs]))
+    whole = m.scan_array(data.tobytes())
+    dev = DeviceBuffer.from_array(data)
+    out = np.zeros(len(whole) + 16, dtype=whole.dtype)
+    n = StreamPipeline(m, maxlen).scan_device(dev.ptr, len(data), out.ctypes.data, len(out),
+                                              chunk_size=256 << 10, out_on_device=False)
+    assert n == len(whole) and len(whole) > 50
+    assert np.array_equal(out[:n], whole)
+
+
 def test_overflow_is_an_error_not_a_fallback():
     from warpstate_amd import AutoMatcherConfig, Context
     ctx = Context(0, AutoMatcherConfig(max_matches=100))

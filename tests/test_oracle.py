@@ -192,3 +192,21 @@ def test_literal_regex_becomes_literal():
     assert o.scan_list(b"abcabc", 0) == [(0, 0, 3), (0, 3, 6)]
     o = Oracle([(1, "aa")])
     assert o.scan_list(b"aaa", 0) == [(0, 0, 2), (0, 1, 3)]
+
+
+def test_threaded_overlapping_scan_equals_single_thread():
+    # The byte-range split with a (max_len - 1) halo must reproduce the
+    # one-thread overlapping scan exactly, including matches that straddle
+    # shard edges (short input, many shards).
+    import numpy as np
+    from oracle.oracle import MATCH_DTYPE, Oracle
+
+    lits = [b"abcab", b"bca", b"ab", b"cabcabc", b"a"]
+    data = (b"abcabcabcab" * 200)[:1999]
+    o = Oracle([(0, l) for l in lits])
+    ref = o.scan(data, 0)
+    for threads in (1, 2, 3, 7, 64):
+        out = np.zeros(len(ref) + 8, dtype=MATCH_DTYPE)
+        n = o.scan_into(data, out, 0, threads)
+        assert n == len(ref)
+        assert np.array_equal(out[:n], ref), threads

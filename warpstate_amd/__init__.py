@@ -31,6 +31,7 @@ from .api import (  # noqa: F401
     default_context,
     matches_to_list,
     scan_batch,
+    scan_batch_device,
     scan_batch_gpu,
 )
 

@@ -367,6 +367,21 @@ class StreamPipeline:
             N.check(st, int(count.value))
             return out[: count.value]
 
+    def scan_device(self, data_ptr: int, length: int, out_ptr: int, capacity: int, mode="overlapping",
+                    tuning: Optional[Tuning] = None, chunk_size: Optional[int] = None,
+                    out_on_device: bool = True) -> int:
+        """Device-resident input; the match list goes to out_ptr (device or host)."""
+        lib = N.load()
+        chunk = chunk_size or self.backend.max_block_size()
+        count = ctypes.c_uint64(0)
+        t = tuning.to_c() if tuning else None
+        st = lib.ws_stream_scan(self.backend.ctx.handle, self.backend.handle, ctypes.c_void_p(data_ptr), length,
+                                N.MEM_DEVICE, chunk, self.overlap_window, _mode_code(mode),
+                                ctypes.byref(t) if t else None, ctypes.c_void_p(out_ptr), capacity,
+                                N.MEM_DEVICE if out_on_device else N.MEM_HOST, ctypes.byref(count))
+        N.check(st, int(count.value))
+        return int(count.value)
+
     def scan(self, data, chunk_size: Optional[int] = None) -> List[Match]:
         return matches_to_list(self.scan_array(data, chunk_size=chunk_size))
 
@@ -393,6 +408,34 @@ def _batch_array(matcher: GpuMatcher, items: Sequence[ScanItem], mode: str,
                                _mode_code(mode), ctypes.byref(t) if t else None, out.ctypes.data, cap,
                                ctypes.byref(count))
         if st == 11 and count.value > cap and count.value <= matcher.config.max_matches:
+            cap = int(count.value)
+            continue
+        N.check(st, int(count.value))
+        return out[: count.value]
+
+
+def scan_batch_device(matcher: GpuMatcher, ids: np.ndarray, ptrs: np.ndarray, lens: np.ndarray,
+                      mode: str = "overlapping", tuning: Optional[Tuning] = None,
+                      capacity: Optional[int] = None) -> np.ndarray:
+    """scan_batch over items already in device memory (one allocation span of
+    at most 4 GiB): ids u64, ptrs device addresses, lens bytes. Returns
+    TAGGED_DTYPE records in (source_id, start, pattern_id, end) order."""
+    lib = N.load()
+    n = len(ids)
+    arr = (N.WsItem * n)()
+    view = np.frombuffer(arr, dtype=np.dtype([("id", "<u8"), ("data", "<u8"), ("len", "<u8")]), count=n)
+    view["id"] = ids
+    view["data"] = ptrs
+    view["len"] = lens
+    total = int(np.sum(lens))
+    cap = capacity if capacity is not None else max(1024, min(matcher.config.max_matches, total // 64 + 4096))
+    t = tuning.to_c() if tuning else None
+    while True:
+        out = np.empty(cap, dtype=N.TAGGED_DTYPE)
+        count = ctypes.c_uint64(0)
+        st = lib.ws_scan_batch(matcher.ctx.handle, matcher.handle, arr, n, N.MEM_DEVICE, _mode_code(mode),
+                               ctypes.byref(t) if t else None, out.ctypes.data, cap, ctypes.byref(count))
+        if st == 11 and capacity is None and count.value > cap and count.value <= matcher.config.max_matches:
             cap = int(count.value)
             continue
         N.check(st, int(count.value))

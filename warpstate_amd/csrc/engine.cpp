@@ -295,7 +295,7 @@ ws_status sort_into(ws_ctx* ctx, uint32_t n, uint32_t bits, const uint64_t* kin,
 }
 
 ws_status read_counters(ws_ctx* ctx) {
-  WS_HIP(hipMemcpyAsync(ctx->hcount, ctx->counters.p, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+  WS_HIP(hipMemcpyAsync(ctx->hcount, ctx->counters.p, 9 * sizeof(uint32_t), hipMemcpyDeviceToHost,
                         ctx->stream));
   WS_HIP(hipStreamSynchronize(ctx->stream));
   return WS_OK;
@@ -436,7 +436,7 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   uint32_t hit_overflow_cap = 0, lit_ovf_cap = 0, cand_ovf_cap = 0;
   if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[0], st));
   for (int attempt = 0;; ++attempt) {
-    WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 8, 0, st));
+    WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 9, 0, st));
     ScanCommon c{};
     c.base = plan.base;
     c.load_end = plan.load_end;
@@ -547,13 +547,8 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     if (!retry) break;
     if (attempt >= 2) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
   }
-  if (ctx->profiling && variant == WS_VARIANT_HASH) {
-    std::vector<uint32_t> sc(fgrid);
-    WS_HIP(hipMemcpy(sc.data(), ctx->slice_counts.p, fgrid * 4, hipMemcpyDeviceToHost));
-    uint64_t tot = ctx->last_hits;
-    for (uint32_t v : sc) tot += v;
-    ctx->last_hits = (uint32_t)std::min<uint64_t>(tot, 0xFFFFFFFFull);
-  }
+  if (variant == WS_VARIANT_HASH)
+    ctx->last_hits = (uint32_t)std::min<uint64_t>((uint64_t)ctx->last_hits + ctx->hcount[8], 0xFFFFFFFFull);
   uint32_t nlit = ctx->hcount[0];
   const uint32_t ncand = ctx->hcount[1];
   res->sbits = bits_for(plan.max_seg_len);
@@ -671,13 +666,24 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   return WS_OK;
 }
 
-void finish_timings(ws_ctx* ctx) {
+// Reads the last run's event intervals into ctx->last; with `accumulate`
+// they are added (a stream scan sums its chunks).
+void finish_timings(ws_ctx* ctx, bool accumulate = false) {
   if (!ctx->profiling) return;
   (void)hipEventRecord(ctx->ev[5], ctx->stream);
   (void)hipEventSynchronize(ctx->ev[5]);
   float a = 0, b = 0, c2 = 0, d = 0, v = 0;
   (void)hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[6]);
   (void)hipEventElapsedTime(&v, ctx->ev[6], ctx->ev[2]);
+  if (accumulate) {
+    ctx->last.prefilter_ms += a;
+    ctx->last.verify_ms += v;
+    ctx->last.regex_ms += b;
+    ctx->last.order_ms += c2;
+    ctx->last.total_ms += d;
+    ctx->last.hits += ctx->last_hits;
+    return;
+  }
   ctx->last.verify_ms = v;
   ctx->last.hits = ctx->last_hits;
   (void)hipEventElapsedTime(&b, ctx->ev[3], ctx->ev[4]);
@@ -993,9 +999,9 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
     }
     s = pack_out(ctx, r, (uint32_t)a, out, cap, out_mem, total);
     if (s != WS_OK) return s;
+    finish_timings(ctx, true);
     total += r.count;
   }
-  finish_timings(ctx);
   *count = total;
   if (total > cap)
     return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(total) +

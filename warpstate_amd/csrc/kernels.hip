@@ -383,6 +383,7 @@ __global__ __launch_bounds__(kTPB) void k_hash_verify(ScanCommon c, HashParams h
   const uint32_t b = blockIdx.x / kVerifySplit, part = blockIdx.x % kVerifySplit;
   const uint2* slice = hs.hits + (size_t)b * hs.slice_cap;
   const uint32_t n = min(hs.slice_count[b], hs.slice_cap);
+  if (part == 0 && threadIdx.x == 0 && n) atomicAdd(&c.counters[8], n);  // filter hits, for timings
   for (uint32_t i = part * kTPB + threadIdx.x; i < n; i += kVerifySplit * kTPB)
     verify_hit<K, CI>(c, h, slice[i], s_n, lit, cand);
   const uint32_t no = min(*hs.overflow_count, hs.overflow_cap);

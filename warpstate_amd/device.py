@@ -81,6 +81,11 @@ class DeviceBuffer:
             _check(_rt().hipMemcpy(out.ctypes.data, self.ptr + offset, n * dt.itemsize, _D2H), "hipMemcpy D2H")
         return out
 
+    def copy_from(self, other: "DeviceBuffer", nbytes: Optional[int] = None) -> None:
+        """Device-to-device copy (the runtime's own copy kernel)."""
+        n = min(self.nbytes, other.nbytes) if nbytes is None else int(nbytes)
+        _check(_rt().hipMemcpy(self.ptr, other.ptr, n, _D2D), "hipMemcpy D2D")
+
     def zero(self) -> None:
         _check(_rt().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
 
