@@ -93,7 +93,7 @@ class Workload:
         self.args = args
         self.name = args.workload
         variant = {"auto": 0, "hash": 1, "dfa_global": 2, "dfa_lds": 3}[args.variant]
-        self.tuning = Tuning(variant=variant, blocks_per_cu=args.blocks_per_cu)
+        self.tuning = Tuning(variant=variant, bytes_per_lane=args.bytes_per_lane, blocks_per_cu=args.blocks_per_cu)
         t0 = time.time()
         self.copies = 1
         if self.name == "kmer":
@@ -206,6 +206,7 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=0, help="input bytes (default: the workload's own)")
     ap.add_argument("--variant", default="auto", choices=["auto", "hash", "dfa_global", "dfa_lds"])
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--bytes-per-lane", type=int, default=0, help="hash filter positions per lane (16/32/64)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="bytes for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--router", dest="router", action="store_true", default=None,

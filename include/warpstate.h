@@ -124,7 +124,7 @@ enum {
 
 typedef struct ws_tuning {
   uint32_t variant;       /* WS_VARIANT_* */
-  uint32_t bytes_per_lane;/* 16 or 32 (hash); 32..256 (DFA); 0 = default */
+  uint32_t bytes_per_lane;/* 16, 32 or 64 (hash); 0 = default */
   uint32_t blocks_per_cu; /* persistent grid = CUs * blocks_per_cu; 0 = default */
   uint32_t reserved;
 } ws_tuning;

@@ -222,6 +222,31 @@ def test_stream_scan_device_input_equals_whole():
     assert np.array_equal(out[:n], whole)
 
 
+def test_ordering_many_matches_spread_and_clustered():
+    # Many matches spread over the input (the bucketed ordering) and the same
+    # count packed into a few KiB of a large input (buckets overflow; the
+    # ordering falls back to the radix sort): both must equal the oracle.
+    lits = W.random_literals(300, 4, 9, W.LIT_ALPHABET, seed=31)
+    data = W.printable_input(4 << 20, 32)
+    W.plant(data, lits, 48, 33)
+    m = GpuMatcher(build([(0, l) for l in lits]))
+    got = m.scan_array(data.tobytes())
+    exp = Oracle([(0, l) for l in lits]).scan(data, 0)
+    assert len(exp) > 50_000
+    assert np.array_equal(got, exp)
+
+    big = np.zeros(64 << 20, dtype=np.uint8)
+    big[:10_000] = np.frombuffer(b"ab" * 5000, dtype=np.uint8)
+    big[(40 << 20):(40 << 20) + 6] = np.frombuffer(b"xbabax", dtype=np.uint8)
+    m2 = GpuMatcher(build([(0, b"ab"), (0, b"ba")]))
+    got2 = as_list(m2.scan_array(big.tobytes()))
+    exp2 = sorted([(0, i, i + 2) for i in range(0, 10_000, 2)] + [(1, i, i + 2) for i in range(1, 9_999, 2)] +
+                  [(1, (40 << 20) + 1, (40 << 20) + 3), (0, (40 << 20) + 2, (40 << 20) + 4),
+                   (1, (40 << 20) + 3, (40 << 20) + 5)],
+                  key=lambda t: (t[1], t[0], t[2]))
+    assert_same(got2, exp2, "clustered")
+
+
 def test_overflow_is_an_error_not_a_fallback():
     from warpstate_amd import AutoMatcherConfig, Context
     ctx = Context(0, AutoMatcherConfig(max_matches=100))

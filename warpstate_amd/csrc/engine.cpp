@@ -55,6 +55,9 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
                       const uint32_t* vin, uint32_t* vout, uint32_t n, uint32_t bits,
                       hipStream_t st);
+hipError_t bucket_sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
+                             const uint32_t* vin, uint32_t* vout, uint32_t n, uint32_t shift, uint32_t nb,
+                             uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st);
 
 namespace {
 
@@ -103,7 +106,7 @@ struct ws_ctx {
   ws_timings last{};
   hipEvent_t ev[8] = {};
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids, hits, slice_counts, lslices, cslices, scounts;
+      scratch, carried, ids, hits, hwin, slice_counts, lslices, cslices, scounts, bcnt, bcur;
   uint32_t* hcount = nullptr;  // pinned host mirror of the counters
   uint32_t last_hits = 0;      // filter hits of the last hash-variant scan
 };
@@ -115,9 +118,9 @@ struct ws_patterns {
   // Device pointers into `blob`.
   const uint8_t* fbytes = nullptr;
   const uint4* fmeta = nullptr;
-  const uint2* bent2 = nullptr;
+  const uint4 *vtab = nullptr, *vmore = nullptr;
   const uint32_t *foff = nullptr, *flen = nullptr, *fown = nullptr, *lit_pid = nullptr;
-  const uint32_t *bloom = nullptr, *bstart = nullptr, *bent = nullptr, *fhash = nullptr;
+  const uint32_t* bloom = nullptr;
   const uint32_t *ac_table32 = nullptr, *ac_out_start = nullptr, *ac_out_list = nullptr,
                  *ac_len = nullptr;
   const uint16_t* ac_table16 = nullptr;
@@ -173,13 +176,44 @@ ws_status upload_patterns(ws_patterns* p) {
     fmeta.push_back(fown[f]);
     fmeta.push_back(fc.kind ? fc.owner : h.lit_pid[fc.owner]);
   }
-  std::vector<uint32_t> bent2;
-  for (uint32_t e = 0; e < h.bent.size(); ++e) {
-    bent2.push_back(h.fhash[h.bent[e]]);
-    bent2.push_back(h.bent[e]);
+  // Verification table (see HashParams): two 32-byte slots per bucket
+  // carrying the hash, factor id, length and first 16 (folded) bytes, so a
+  // filter hit is checked with one 64-byte read; crowded buckets continue in
+  // `vmore`.
+  std::vector<uint32_t> vtab, vmore;
+  if (h.hash_ok) {
+    const uint32_t nb = 1u << h.nbits;
+    vtab.assign((size_t)nb * 16, 0);
+    auto fill = [&](uint32_t* e, uint32_t f) {
+      e[0] = h.fhash[f];
+      e[1] = f;
+      e[2] = flen[f];
+      uint8_t pre[16] = {0};
+      std::memcpy(pre, fb.data() + foff[f], std::min<uint32_t>(16, flen[f]));
+      std::memcpy(&e[3], pre, 16);
+      e[7] = 0;
+    };
+    for (uint32_t b = 0; b < nb; ++b) {
+      uint32_t* s0 = vtab.data() + (size_t)b * 16;
+      uint32_t* s1 = s0 + 8;
+      s0[0] = s1[0] = kVtabEmpty;
+      const uint32_t e0 = h.bstart[b], n = h.bstart[b + 1] - e0;
+      if (n >= 1) fill(s0, h.bent[e0]);
+      if (n == 2) fill(s1, h.bent[e0 + 1]);
+      if (n > 2) {
+        s1[0] = kVtabMore;
+        s1[1] = (uint32_t)(vmore.size() / 8);
+        s1[2] = n - 1;
+        for (uint32_t k = 1; k < n; ++k) {
+          vmore.resize(vmore.size() + 8);
+          fill(vmore.data() + vmore.size() - 8, h.bent[e0 + k]);
+        }
+      }
+    }
   }
   size_t o_fmeta = put(fmeta.data(), fmeta.size() * 4);
-  size_t o_bent2 = put(bent2.data(), bent2.size() * 4);
+  size_t o_vtab = put(vtab.data(), vtab.size() * 4);
+  size_t o_vmore = put(vmore.data(), vmore.size() * 4);
   size_t o_fbytes = put(fb.data(), fb.size());
   size_t o_foff = put(foff.data(), foff.size() * 4);
   size_t o_flen = put(flen.data(), flen.size() * 4);
@@ -187,9 +221,6 @@ ws_status upload_patterns(ws_patterns* p) {
   size_t o_aclen = put(aclen.data(), aclen.size() * 4);
   size_t o_litpid = put(h.lit_pid.data(), h.lit_pid.size() * 4);
   size_t o_bloom = put(h.bloom.data(), h.bloom.size() * 4);
-  size_t o_bstart = put(h.bstart.data(), h.bstart.size() * 4);
-  size_t o_bent = put(h.bent.data(), h.bent.size() * 4);
-  size_t o_fhash = put(h.fhash.data(), h.fhash.size() * 4);
   size_t o_ac32 = put(h.ac_table.data(), h.ac_table.size() * 4);
   std::vector<uint16_t> t16;
   p->ac_lds_ok = h.ac_ok && h.ac_states <= 65535 &&
@@ -226,16 +257,14 @@ ws_status upload_patterns(ws_patterns* p) {
   uint8_t* b = p->blob.as<uint8_t>();
   p->fbytes = b + o_fbytes;
   p->fmeta = reinterpret_cast<const uint4*>(b + o_fmeta);
-  p->bent2 = reinterpret_cast<const uint2*>(b + o_bent2);
+  p->vtab = reinterpret_cast<const uint4*>(b + o_vtab);
+  p->vmore = reinterpret_cast<const uint4*>(b + o_vmore);
   p->foff = reinterpret_cast<const uint32_t*>(b + o_foff);
   p->flen = reinterpret_cast<const uint32_t*>(b + o_flen);
   p->fown = reinterpret_cast<const uint32_t*>(b + o_fown);
   p->ac_len = reinterpret_cast<const uint32_t*>(b + o_aclen);
   p->lit_pid = reinterpret_cast<const uint32_t*>(b + o_litpid);
   p->bloom = reinterpret_cast<const uint32_t*>(b + o_bloom);
-  p->bstart = reinterpret_cast<const uint32_t*>(b + o_bstart);
-  p->bent = reinterpret_cast<const uint32_t*>(b + o_bent);
-  p->fhash = reinterpret_cast<const uint32_t*>(b + o_fhash);
   p->ac_table32 = reinterpret_cast<const uint32_t*>(b + o_ac32);
   p->ac_table16 = reinterpret_cast<const uint16_t*>(b + o_ac16);
   p->ac_cmap = b + o_accmap;
@@ -282,6 +311,8 @@ struct Plan {
 struct Result {
   uint32_t count = 0;  // records in ctx->k1/v1 (sorted keys/values)
   uint32_t sbits = 0, pbits = 0;
+  uint32_t keybits = 0;
+  bool bucketed = false;  // ordered by the bucket sort: confirm after the next sync
 };
 
 ws_status sort_into(ws_ctx* ctx, uint32_t n, uint32_t bits, const uint64_t* kin, uint64_t* kout,
@@ -291,6 +322,51 @@ ws_status sort_into(ws_ctx* ctx, uint32_t n, uint32_t bits, const uint64_t* kin,
   WS_HIP(ctx->tmp.ensure(need));
   size_t have = ctx->tmp.cap;
   WS_HIP(sort_pairs(ctx->tmp.p, &have, kin, kout, vin, vout, n, bits, ctx->stream));
+  return WS_OK;
+}
+
+// Orders n (key, value) pairs from k0/v0 into k1/v1. Position-major match
+// keys of a single input take the bucket sort (about two buckets per pair);
+// its overflow flag (counters[9]) is copied to the host mirror with the next
+// synchronization and checked by confirm_order.
+ws_status order_pairs(ws_ctx* ctx, uint32_t n, uint32_t bits, bool dense, Result* res) {
+  res->bucketed = false;
+  res->keybits = bits;
+  if (!dense || n < 4096 || bits > 63)
+    return sort_into(ctx, n, bits, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(), ctx->v0.as<uint32_t>(),
+                     ctx->v1.as<uint32_t>());
+  const uint32_t bb = std::min<uint32_t>(bits, std::min(22u, std::max(10u, bits_for(n) + 1)));
+  const uint32_t shift = bits - bb, nb = 1u << bb;
+  WS_HIP(ctx->bcnt.ensure((size_t)nb * 4));
+  WS_HIP(ctx->bcur.ensure((size_t)nb * 4));
+  size_t need = 0;
+  WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, n, shift, nb,
+                           ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), nullptr, ctx->stream));
+  WS_HIP(ctx->tmp.ensure(need));
+  size_t have = ctx->tmp.cap;
+  WS_HIP(launch_fill_u32(ctx->bcnt.as<uint32_t>(), nb, 0, ctx->stream));
+  WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>() + 9, 1, 0, ctx->stream));
+  WS_HIP(bucket_sort_pairs(ctx->tmp.p, &have, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
+                           ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), n, shift, nb,
+                           ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(),
+                           ctx->counters.as<uint32_t>() + 9, ctx->stream));
+  WS_HIP(hipMemcpyAsync(ctx->hcount + 9, ctx->counters.as<uint32_t>() + 9, 4, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  res->bucketed = true;
+  return WS_OK;
+}
+
+// After a synchronization that follows order_pairs: if a bucket overflowed,
+// redo the ordering with the radix sort. Returns whether it did.
+ws_status confirm_order(ws_ctx* ctx, Result& r, bool* redone) {
+  *redone = false;
+  if (!r.bucketed) return WS_OK;
+  r.bucketed = false;
+  if (!ctx->hcount[9]) return WS_OK;
+  ws_status s = sort_into(ctx, r.count, r.keybits, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
+                          ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>());
+  if (s != WS_OK) return s;
+  *redone = true;
   return WS_OK;
 }
 
@@ -362,13 +438,17 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   const uint32_t variant = pick_variant(p, tuning);
   if (!(supported_variants(p) & (1u << variant)))
     return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
-  // Hash filter: 16 or 32 positions per lane (32 halves per-lane overhead;
-  // 16 suits batches of small items).
+  // Hash filter: 16, 32 or 64 positions per lane. More positions amortise
+  // the per-window setup (K-1 hash steps, loads, bounds) but make work units
+  // coarser: 64 for large inputs, 16 for batches of small items.
   uint32_t lane_bytes = tuning && tuning->bytes_per_lane ? tuning->bytes_per_lane : 0;
-  if (lane_bytes != 16 && lane_bytes != 32) {
+  if (lane_bytes != 16 && lane_bytes != 32 && lane_bytes != 64) {
     uint64_t span = 0;
     for (auto& it : plan.items) span += it.y - it.x;
-    lane_bytes = (plan.batch && span / plan.items.size() < 64 * 1024) ? 16 : 32;
+    if (plan.batch && span / plan.items.size() < 64 * 1024)
+      lane_bytes = 16;
+    else
+      lane_bytes = span >= (256ull << 20) ? 64 : 32;
   }
   const uint32_t unit = variant == WS_VARIANT_HASH ? kTPB * lane_bytes : ac_unit_bytes();
   std::vector<uint2> units;
@@ -422,6 +502,7 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       WS_HIP(ctx->hits.ensure(want * sizeof(uint2)));
       hit_total = ctx->hits.cap / sizeof(uint2);
     }
+    WS_HIP(ctx->hwin.ensure(hit_total * 2 * sizeof(uint4)));
     WS_HIP(ctx->slice_counts.ensure((size_t)fgrid * 4));
   }
   // Verify output slices (literal matches, regex candidates), one per verify block.
@@ -463,12 +544,14 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     if (nunits > 0) {
       if (variant == WS_VARIANT_HASH) {
         HashParams hp{p->bloom, (uint32_t)h.bloom.size(), h.wshift, h.wbits, h.B, h.NBK, h.nbits, h.s2,
-                      lane_bytes, p->bstart, p->bent2, p->fmeta};
+                      lane_bytes, p->vtab, p->vmore, p->fmeta};
         HitSlices hs{};
         hs.overflow_cap = (uint32_t)std::min<uint64_t>(hit_total / 4, 0xFFFFFFF0ull);
         hs.slice_cap = (uint32_t)((hit_total - hs.overflow_cap) / fgrid);
         hs.hits = ctx->hits.as<uint2>();
         hs.overflow = hs.hits + (size_t)hs.slice_cap * fgrid;
+        hs.win = ctx->hwin.as<uint4>();
+        hs.overflow_win = hs.win + 2 * (size_t)hs.slice_cap * fgrid;
         hs.slice_count = ctx->slice_counts.as<uint32_t>();
         hs.overflow_count = ctx->counters.as<uint32_t>() + 4;
         auto make_slices = [&](DevBuf& buf, uint64_t total, uint32_t* counts, uint32_t* ovf_count) {
@@ -510,6 +593,7 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       // Slices and the overflow area were not enough: grow and rescan.
       WS_HIP(ctx->hits.ensure(hit_total * 2 * sizeof(uint2) + (size_t)nover * 8 * sizeof(uint2)));
       hit_total = ctx->hits.cap / sizeof(uint2);
+      WS_HIP(ctx->hwin.ensure(hit_total * 2 * sizeof(uint4)));
       if (attempt >= 2) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
       continue;
     }
@@ -658,8 +742,8 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     WS_HIP(ctx->v1.ensure((size_t)total * 4));
     WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), total, res->sbits, res->pbits,
                              ctx->k0.as<uint64_t>(), ctx->v0.as<uint32_t>(), st));
-    ws_status s = sort_into(ctx, total, res->sbits + res->pbits + ibits, ctx->k0.as<uint64_t>(),
-                            ctx->k1.as<uint64_t>(), ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>());
+    res->count = total;
+    ws_status s = order_pairs(ctx, total, res->sbits + res->pbits + ibits, !plan.batch, res);
     if (s != WS_OK) return s;
   }
   res->count = total;
@@ -730,17 +814,24 @@ ws_status pack_out(ws_ctx* ctx, const Result& r, uint32_t start_off, ws_match* o
   if (written >= cap) return WS_OK;
   if (written + n > cap) n = (uint32_t)(cap - written);
   if (n == 0) return WS_OK;
-  if (out_mem == WS_MEM_DEVICE) {
-    WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.pbits, start_off,
-                               out + written, ctx->stream));
-    WS_HIP(hipStreamSynchronize(ctx->stream));
-  } else {
-    WS_HIP(ctx->outbuf.ensure((size_t)n * sizeof(ws_match)));
-    WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.pbits, start_off,
-                               ctx->outbuf.p, ctx->stream));
-    WS_HIP(hipMemcpyAsync(out + written, ctx->outbuf.p, (size_t)n * sizeof(ws_match),
-                          hipMemcpyDeviceToHost, ctx->stream));
-    WS_HIP(hipStreamSynchronize(ctx->stream));
+  Result rr = r;
+  for (;;) {
+    if (out_mem == WS_MEM_DEVICE) {
+      WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, rr.pbits, start_off,
+                                 out + written, ctx->stream));
+      WS_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+      WS_HIP(ctx->outbuf.ensure((size_t)n * sizeof(ws_match)));
+      WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, rr.pbits, start_off,
+                                 ctx->outbuf.p, ctx->stream));
+      WS_HIP(hipMemcpyAsync(out + written, ctx->outbuf.p, (size_t)n * sizeof(ws_match),
+                            hipMemcpyDeviceToHost, ctx->stream));
+      WS_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    bool redone = false;
+    ws_status s = confirm_order(ctx, rr, &redone);
+    if (s != WS_OK) return s;
+    if (!redone) break;
   }
   return WS_OK;
 }
@@ -800,8 +891,9 @@ void ws_ctx_destroy(ws_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->recs, &ctx->cands, &ctx->raw, &ctx->k0, &ctx->v0, &ctx->k1, &ctx->v1,
                     &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
-                    &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids, &ctx->hits, &ctx->slice_counts, &ctx->lslices, &ctx->cslices,
-                    &ctx->scounts})
+                    &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids, &ctx->hits, &ctx->hwin,
+                    &ctx->slice_counts, &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
+                    &ctx->bcur})
     b->release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -1011,7 +1103,7 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
 
 static const ws_tuning kConfigTable[WS_ROUTER_CONFIGS] = {
     {WS_VARIANT_HASH, 16, 0, 0},       {WS_VARIANT_HASH, 32, 0, 0},
-    {WS_VARIANT_DFA_GLOBAL, 0, 2, 0},  {WS_VARIANT_DFA_GLOBAL, 0, 4, 0},
+    {WS_VARIANT_HASH, 64, 0, 0},       {WS_VARIANT_DFA_GLOBAL, 0, 4, 0},
     {WS_VARIANT_DFA_GLOBAL, 0, 8, 0},  {WS_VARIANT_DFA_LDS, 0, 2, 0},
     {WS_VARIANT_DFA_LDS, 0, 4, 0},     {WS_VARIANT_DFA_LDS, 0, 8, 0},
 };
@@ -1066,6 +1158,10 @@ ws_status ws_autotune(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, ui
                                    ctx->outbuf.p, ctx->stream));
       }
       WS_HIP(hipStreamSynchronize(ctx->stream));
+      bool redone = false;
+      s = confirm_order(ctx, res, &redone);  // an overflowing bucket costs the radix sort too
+      if (s != WS_OK) return s;
+      if (redone) WS_HIP(hipStreamSynchronize(ctx->stream));
       const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
       if (r == 0 || ms < tbest) tbest = ms;
     }

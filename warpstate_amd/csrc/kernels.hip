@@ -11,6 +11,7 @@
 //  k_*_keys/pack ordering keys and output packing.
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "kernels.hpp"
 #include "regex.hpp"
@@ -142,10 +143,15 @@ __device__ __forceinline__ uint32_t byte_of(const uint32_t* d, int j) {
 // position to the bucket chain (L2) and exact byte verification.
 // ---------------------------------------------------------------------------
 // Filter bits of the PL positions [p0, p0 + PL) whose window (PL + 16 bytes)
-// is in d. Per position: one byte extract, two 24-bit multiply-adds (rolling
-// hash), one LDS read of the bloom word and a two-bit test.
+// is in d, as PL/32 mask words (one word, low PL bits, when PL < 32). Per
+// position: one byte extract, two 24-bit multiply-adds (rolling hash), one
+// LDS read of the bloom word and a two-bit test.
+template <int PL>
+constexpr int mask_words() { return PL > 32 ? PL / 32 : 1; }
+
 template <int K, bool CI, int PL>
-__device__ __forceinline__ uint32_t filter_lane(const HashParams& h, const uint32_t* bloom, uint32_t* d) {
+__device__ __forceinline__ void filter_lane(const HashParams& h, const uint32_t* bloom, uint32_t* d,
+                                            uint32_t* out) {
   if (CI) {
 #pragma unroll
     for (int j = 0; j < PL / 4 + 4; ++j) d[j] = fold_word(d[j]);
@@ -160,20 +166,23 @@ __device__ __forceinline__ uint32_t filter_lane(const HashParams& h, const uint3
     // reads its offset from the low five bits of the offset operand.
     const uint32_t w = bloom[__builtin_amdgcn_ubfe(H, h.wshift, h.wbits)];
     const uint32_t hit = __builtin_amdgcn_ubfe(w, H, 1) & __builtin_amdgcn_ubfe(w, H >> h.s2, 1);
-    mask = (mask << 1) | hit;  // position i lands in bit PL - 1 - i
+    // v_alignbit shifts the mask right and brings bit 0 of `hit` in at the
+    // top: after 32 positions, position i sits in bit i.
+    mask = __builtin_amdgcn_alignbit(hit, mask, 1);
+    if ((i & 31) == 31) out[i >> 5] = mask;
   }
-  return __builtin_bitreverse32(mask) >> (32 - PL);
+  if (PL < 32) out[0] = mask >> (32 - PL);
 }
 
-// Valid positions q of [p0, p0 + PL): lo <= q, q + K <= hi, q - lo < emit_end.
-template <int PL>
-__device__ __forceinline__ uint32_t valid_lane(uint32_t p0, uint32_t lo, uint32_t hi, uint32_t K,
+// Valid positions q of [p0, p0 + n) (n <= 32): lo <= q, q + K <= hi,
+// q - lo < emit_end.
+__device__ __forceinline__ uint32_t valid_word(uint32_t p0, uint32_t n, uint32_t lo, uint32_t hi, uint32_t K,
                                                uint32_t emit_end) {
   if (hi < K) return 0;
   uint32_t qmin = p0 > lo ? p0 : lo;
   uint32_t qmax = hi - K + 1;
   if (emit_end < qmax - lo) qmax = lo + emit_end;
-  if (qmax > p0 + PL) qmax = p0 + PL;
+  if (qmax > p0 + n) qmax = p0 + n;
   if (qmax <= qmin) return 0;
   const uint32_t a0 = qmin - p0, a1 = qmax - p0;
   const uint32_t top = a1 >= 32 ? 0xFFFFFFFFu : ((1u << a1) - 1u);
@@ -201,6 +210,7 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
   const uint32_t sub = threadIdx.x / kTPB, ltid = threadIdx.x % kTPB;
   __shared__ uint32_t s_n;
   uint2* slice = hs.hits + (size_t)blockIdx.x * hs.slice_cap;
+  uint4* win = hs.win + 2 * (size_t)blockIdx.x * hs.slice_cap;
 
   for (uint32_t k = threadIdx.x; k < h.nwords; k += kFilterTPB) bloom[k] = h.bloom[k];
   if (threadIdx.x == 0) s_n = 0;
@@ -214,7 +224,8 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
   for (uint32_t g = next_group(c, 0, true, &s_slot, ahead); g < groups;
        g = next_group(c, g, false, &s_slot, ahead)) {
     uint32_t d[kUnitsPerIter][NW];
-    uint32_t p0[kUnitsPerIter], lo[kUnitsPerIter], hi[kUnitsPerIter], item[kUnitsPerIter];
+    uint32_t p0[kUnitsPerIter], lo[kUnitsPerIter], hi[kUnitsPerIter], item[kUnitsPerIter],
+        lend[kUnitsPerIter];
     bool inner[kUnitsPerIter];
 #pragma unroll
     for (int j = 0; j < kUnitsPerIter; ++j) {
@@ -238,38 +249,61 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
       // position may start a match (the common case of a large scan).
       inner[j] = tile_base >= lo[j] && tile_base + kUnit + 16 <= hi[j] &&
                  tile_base + kUnit - lo[j] + 0u <= c.emit_end;
-      uint32_t lend = (hi[j] + 15) & ~15u;
-      if (lend > load_end) lend = load_end;
+      lend[j] = (hi[j] + 15) & ~15u;
+      if (lend[j] > load_end) lend[j] = load_end;
 #pragma unroll
       for (int v = 0; v < NW / 4; ++v) {
         uint4 x = make_uint4(0, 0, 0, 0);
         const uint32_t a = p0[j] + 16 * v;
-        if (inner[j] || (a + 16 <= lend && a + 16 > (lo[j] & ~15u))) x = *reinterpret_cast<const uint4*>(c.base + a);
+        if (inner[j] || (a + 16 <= lend[j] && a + 16 > (lo[j] & ~15u))) x = *reinterpret_cast<const uint4*>(c.base + a);
         d[j][4 * v] = x.x; d[j][4 * v + 1] = x.y; d[j][4 * v + 2] = x.z; d[j][4 * v + 3] = x.w;
       }
     }
-    uint32_t mask[kUnitsPerIter];
+    constexpr int NM = mask_words<PL>();
+    constexpr uint32_t WPOS = PL < 32 ? PL : 32;  // positions per mask word
+    uint32_t mask[kUnitsPerIter][NM];
     uint32_t nh = 0;
 #pragma unroll
     for (int j = 0; j < kUnitsPerIter; ++j) {
-      mask[j] = 0;
+#pragma unroll
+      for (int k = 0; k < NM; ++k) mask[j][k] = 0;
       if (inner[j] || (p0[j] < hi[j] && p0[j] + PL > lo[j])) {
-        mask[j] = filter_lane<K, CI, PL>(h, bloom, d[j]);
-        if (!inner[j]) mask[j] &= valid_lane<PL>(p0[j], lo[j], hi[j], K, c.emit_end);
+        filter_lane<K, CI, PL>(h, bloom, d[j], mask[j]);
+        if (!inner[j]) {
+#pragma unroll
+          for (int k = 0; k < NM; ++k)
+            mask[j][k] &= valid_word(p0[j] + WPOS * k, WPOS, lo[j], hi[j], K, c.emit_end);
+        }
       }
-      nh += __builtin_popcount(mask[j]);
+#pragma unroll
+      for (int k = 0; k < NM; ++k) nh += __builtin_popcount(mask[j][k]);
     }
     if (__any(nh != 0u)) {
       uint32_t slot = wave_reserve(nh, &s_n);
 #pragma unroll
       for (int j = 0; j < kUnitsPerIter; ++j) {
-        for (uint32_t m = mask[j]; m; m &= m - 1, ++slot) {
-          const uint2 rec = make_uint2(p0[j] + __builtin_ctz(m), item[j]);
+#pragma unroll
+        for (int k = 0; k < NM; ++k)
+        for (uint32_t m = mask[j][k]; m; m &= m - 1, ++slot) {
+          const uint32_t q = p0[j] + WPOS * k + __builtin_ctz(m);
+          const uint2 rec = make_uint2(q, item[j]);
+          // The aligned 32-byte span holding q was just streamed: an L2 hit.
+          const uint32_t a = q & ~15u;
+          uint4 x0 = make_uint4(0, 0, 0, 0), x1 = make_uint4(0, 0, 0, 0);
+          if (a + 16 <= lend[j]) x0 = *reinterpret_cast<const uint4*>(c.base + a);
+          if (a + 32 <= lend[j]) x1 = *reinterpret_cast<const uint4*>(c.base + a + 16);
           if (slot < hs.slice_cap) {
             slice[slot] = rec;
+            uint4* wv = win + 2 * (size_t)slot;
+            wv[0] = x0;
+            wv[1] = x1;
           } else {
             const uint32_t gi = atomicAdd(hs.overflow_count, 1u);
-            if (gi < hs.overflow_cap) hs.overflow[gi] = rec;
+            if (gi < hs.overflow_cap) {
+              hs.overflow[gi] = rec;
+              hs.overflow_win[2 * (size_t)gi] = x0;
+              hs.overflow_win[2 * (size_t)gi + 1] = x1;
+            }
           }
         }
       }
@@ -277,33 +311,6 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
   }
   __syncthreads();
   if (threadIdx.x == 0) hs.slice_count[blockIdx.x] = s_n;
-}
-
-// Bytes [q, q + 32) of the segment (zero past lend), from two or three
-// aligned 16-byte loads.
-template <bool CI>
-__device__ __forceinline__ void load32_at(const uint8_t* base, uint64_t q, uint64_t lend, uint32_t* w) {
-  const uint64_t a = q & ~15ull;
-  uint32_t r[12];
-#pragma unroll
-  for (int v = 0; v < 3; ++v) {
-    uint4 x = make_uint4(0, 0, 0, 0);
-    if (a + 16 * v + 16 <= lend) x = *reinterpret_cast<const uint4*>(base + a + 16 * v);
-    r[4 * v] = x.x; r[4 * v + 1] = x.y; r[4 * v + 2] = x.z; r[4 * v + 3] = x.w;
-  }
-  const uint32_t sh = (uint32_t)(q & 15);
-  const uint32_t dw = sh >> 2, bs = (sh & 3) * 8;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    // dynamic dword select from a small register array (compiles to v_cndmask chains)
-    uint32_t lo = r[0], hi = r[1];
-#pragma unroll
-    for (int t = 1; t < 4; ++t)
-      if (dw == (uint32_t)t) { lo = r[k + t]; hi = r[k + t + 1]; }
-    if (dw == 0) { lo = r[k]; hi = r[k + 1]; }
-    const uint32_t v = bs ? ((lo >> bs) | (hi << (32 - bs))) : lo;
-    w[k] = CI ? fold_word(v) : v;
-  }
 }
 
 __device__ __forceinline__ void slice_put(const SliceOut& so, uint32_t slice, uint32_t slot, Rec r) {
@@ -315,36 +322,35 @@ __device__ __forceinline__ void slice_put(const SliceOut& so, uint32_t slice, ui
   }
 }
 
-template <int K, bool CI>
-__device__ __forceinline__ void verify_hit(const ScanCommon& c, const HashParams& h, uint2 hr, uint32_t* s_n,
-                                           const SliceOut& lit, const SliceOut& cand) {
-  const uint64_t q = hr.x;
-  const uint32_t item = hr.y;
-  const uint2 seg = c.items[item];
-  uint64_t lend = ((uint64_t)seg.y + 15) & ~15ull;
-  if (lend > c.load_end) lend = c.load_end;
-  uint32_t w[8];
-  load32_at<CI>(c.base, q, lend, w);
-  uint32_t H = 0;
+// Exact comparison of one table entry ({hash, factor, length, bytes 0-3},
+// {bytes 4-15, 0}) at position q, then the record. w holds the input bytes
+// from q (avail of them valid, at least 17).
+template <bool CI>
+__device__ __forceinline__ void verify_entry(const ScanCommon& c, const HashParams& h, uint4 e0, uint4 e1,
+                                             const uint32_t* w, uint32_t avail, uint64_t q, uint2 seg,
+                                             uint32_t item, uint32_t* s_n, const SliceOut& lit,
+                                             const SliceOut& cand) {
+  const uint32_t len = e0.z;
+  if (q + len > seg.y) return;
+  const uint32_t pre[4] = {e0.w, e1.x, e1.y, e1.z};
 #pragma unroll
-  for (int t = 0; t < K; ++t) H = (H * h.B + ((w[t >> 2] >> ((t & 3) * 8)) & 0xFFu)) & 0xFFFFFFu;
-  const uint32_t bk = (H * 0x9E3779B1u) >> (32 - h.nbits);
-  const uint32_t e1 = h.bstart[bk + 1];
-  for (uint32_t e = h.bstart[bk]; e < e1; ++e) {
-    const uint2 ent = h.bent2[e];  // {hash, factor}
-    if (ent.x != H) continue;
-    const uint4 m = h.fmeta[ent.y];  // {offset, length, owner word, output id}
-    if (q + m.y > seg.y) continue;
-    // Compare in dwords: the first 32 bytes from registers, the rest from memory.
+  for (int k = 0; k < 4; ++k) {
+    if (4u * k >= len) break;
+    const uint32_t rem = len - 4 * k;
+    const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
+    if ((w[k] ^ pre[k]) & msk) return;
+  }
+  const uint4 m = h.fmeta[e0.y];  // {offset, length, kind << 31 | owner, output id}
+  if (len > 16) {
+    // The tail: window registers while they last, then the input itself.
     const uint32_t* pat = reinterpret_cast<const uint32_t*>(c.fbytes + m.x);
-    bool ok = true;
-    const uint32_t nw = (m.y + 3) >> 2;
-    for (uint32_t k = 0; k < nw && ok; ++k) {
+    const uint32_t nw = (len + 3) >> 2;
+    for (uint32_t k = 4; k < nw; ++k) {
       uint32_t dv;
-      if (k < 8) {
-        dv = w[0];
+      if (4 * k + 4 <= avail) {
+        dv = w[4];
 #pragma unroll
-        for (int t = 1; t < 8; ++t) if (k == (uint32_t)t) dv = w[t];
+        for (int t = 5; t < 8; ++t) if (k == (uint32_t)t) dv = w[t];
       } else {
         dv = 0;
         for (int t = 0; t < 4; ++t) {
@@ -354,16 +360,57 @@ __device__ __forceinline__ void verify_hit(const ScanCommon& c, const HashParams
           dv |= (uint32_t)bb << (8 * t);
         }
       }
-      const uint32_t rem = m.y - 4 * k;
+      const uint32_t rem = len - 4 * k;
       const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
-      ok = ((dv ^ pat[k]) & msk) == 0;
+      if ((dv ^ pat[k]) & msk) return;
     }
-    if (!ok) continue;
-    const uint32_t start = (uint32_t)(q - seg.x);
-    if (m.z >> 31)
-      slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), make_uint4(m.w, start, 0u, item));
-    else
-      slice_put(lit, blockIdx.x, atomicAdd(&s_n[0], 1u), make_uint4(m.w, start, start + m.y, item));
+  }
+  const uint32_t start = (uint32_t)(q - seg.x);
+  if (m.z >> 31)
+    slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), make_uint4(m.w, start, 0u, item));
+  else
+    slice_put(lit, blockIdx.x, atomicAdd(&s_n[0], 1u), make_uint4(m.w, start, start + len, item));
+}
+
+template <int K, bool CI>
+__device__ __forceinline__ void verify_hit(const ScanCommon& c, const HashParams& h, uint2 hr, uint4 x0, uint4 x1,
+                                           uint32_t* s_n, const SliceOut& lit, const SliceOut& cand) {
+  const uint64_t q = hr.x;
+  const uint32_t item = hr.y;
+  const uint2 seg = c.items[item];
+  // Bytes [q, q + avail) come from the hit record's 32-byte aligned span
+  // (avail >= 17, so the K <= 16 window is always there).
+  const uint32_t r[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+  const uint32_t sh = (uint32_t)(q & 15), avail = 32 - sh;
+  const uint32_t dw = sh >> 2, bs = (sh & 3) * 8;
+  uint32_t w[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    // dynamic dword select from a small register array (v_cndmask chains)
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      if (dw == (uint32_t)t) {
+        lo = k + t < 8 ? r[k + t] : 0u;
+        hi = k + t + 1 < 8 ? r[k + t + 1] : 0u;
+      }
+    const uint32_t v = bs ? ((lo >> bs) | (hi << (32 - bs))) : lo;
+    w[k] = CI ? fold_word(v) : v;
+  }
+  uint32_t H = 0;
+#pragma unroll
+  for (int t = 0; t < K; ++t) H = (H * h.B + ((w[t >> 2] >> ((t & 3) * 8)) & 0xFFu)) & 0xFFFFFFu;
+  const uint32_t bk = (H * 0x9E3779B1u) >> (32 - h.nbits);
+  const uint4* slot = h.vtab + 4 * (size_t)bk;
+  const uint4 a0 = slot[0], a1 = slot[1], b0 = slot[2], b1 = slot[3];
+  if (a0.x == H) verify_entry<CI>(c, h, a0, a1, w, avail, q, seg, item, s_n, lit, cand);
+  if (b0.x == H) {
+    verify_entry<CI>(c, h, b0, b1, w, avail, q, seg, item, s_n, lit, cand);
+  } else if (b0.x == kVtabMore) {
+    for (uint32_t e = b0.y; e < b0.y + b0.z; ++e) {
+      const uint4 e0 = h.vmore[2 * (size_t)e];
+      if (e0.x == H) verify_entry<CI>(c, h, e0, h.vmore[2 * (size_t)e + 1], w, avail, q, seg, item, s_n, lit, cand);
+    }
   }
 }
 
@@ -383,12 +430,14 @@ __global__ __launch_bounds__(kTPB) void k_hash_verify(ScanCommon c, HashParams h
   const uint32_t b = blockIdx.x / kVerifySplit, part = blockIdx.x % kVerifySplit;
   const uint2* slice = hs.hits + (size_t)b * hs.slice_cap;
   const uint32_t n = min(hs.slice_count[b], hs.slice_cap);
+  const uint4* win = hs.win + 2 * (size_t)b * hs.slice_cap;
   if (part == 0 && threadIdx.x == 0 && n) atomicAdd(&c.counters[8], n);  // filter hits, for timings
   for (uint32_t i = part * kTPB + threadIdx.x; i < n; i += kVerifySplit * kTPB)
-    verify_hit<K, CI>(c, h, slice[i], s_n, lit, cand);
+    verify_hit<K, CI>(c, h, slice[i], win[2 * (size_t)i], win[2 * (size_t)i + 1], s_n, lit, cand);
   const uint32_t no = min(*hs.overflow_count, hs.overflow_cap);
   for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < no; i += gridDim.x * kTPB)
-    verify_hit<K, CI>(c, h, hs.overflow[i], s_n, lit, cand);
+    verify_hit<K, CI>(c, h, hs.overflow[i], hs.overflow_win[2 * (size_t)i], hs.overflow_win[2 * (size_t)i + 1],
+                      s_n, lit, cand);
   __syncthreads();
   if (threadIdx.x == 0) {
     lit.counts[blockIdx.x] = s_n[0];
@@ -726,7 +775,11 @@ static hipError_t launch_hash_kc(uint32_t grid, hipStream_t st, const ScanCommon
                                  bool verify_only) {
   const size_t shmem = hash_shmem_bytes(h.nwords);
   if (!verify_only) {
-    if (h.lane_bytes == 32) {
+    if (h.lane_bytes == 64) {
+      hipError_t e = allow_lds(k_hash_filter<K, CI, 64>, shmem);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_hash_filter<K, CI, 64>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, hs);
+    } else if (h.lane_bytes == 32) {
       hipError_t e = allow_lds(k_hash_filter<K, CI, 32>, shmem);
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL((k_hash_filter<K, CI, 32>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, hs);
@@ -875,6 +928,74 @@ hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_
                       const uint32_t* vin, uint32_t* vout, uint32_t n, uint32_t bits,
                       hipStream_t st) {
   return rocprim::radix_sort_pairs(tmp, *tmp_bytes, kin, kout, vin, vout, (size_t)n, 0u, bits, st);
+}
+
+// ---------------------------------------------------------------------------
+// Bucket sort of (key, value) pairs whose keys spread over their range (match
+// keys: position-major). Bucket = key >> shift, about two buckets per pair:
+// count, exclusive scan (rocPRIM), scatter, then one lane sorts each bucket
+// in place. A bucket over kBucketMax pairs is left unsorted and raises *flag;
+// the caller then falls back to the radix sort.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBucketMax = 64;
+
+__global__ void k_bucket_count(const uint64_t* keys, uint32_t n, uint32_t shift, uint32_t* cnt) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(&cnt[keys[i] >> shift], 1u);
+}
+
+__global__ void k_bucket_scatter(const uint64_t* kin, const uint32_t* vin, uint32_t n, uint32_t shift,
+                                 uint32_t* cursor, uint64_t* kout, uint32_t* vout) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t k = kin[i];
+    const uint32_t pos = atomicAdd(&cursor[k >> shift], 1u);
+    kout[pos] = k;
+    vout[pos] = vin[i];
+  }
+}
+
+__global__ void k_bucket_sort(uint64_t* keys, uint32_t* vals, const uint32_t* end, const uint32_t* cnt,
+                              uint32_t nb, uint32_t* flag) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+    const uint32_t c = cnt[b];
+    if (c < 2) continue;
+    if (c > kBucketMax) {
+      atomicOr(flag, 1u);
+      continue;
+    }
+    const uint32_t lo = end[b] - c;
+    for (uint32_t i = lo + 1; i < lo + c; ++i) {
+      const uint64_t k = keys[i];
+      const uint32_t v = vals[i];
+      uint32_t j = i;
+      while (j > lo && keys[j - 1] > k) {
+        keys[j] = keys[j - 1];
+        vals[j] = vals[j - 1];
+        --j;
+      }
+      keys[j] = k;
+      vals[j] = v;
+    }
+  }
+}
+
+// Sizes the scan's temporary storage (tmp = nullptr) or runs
+// count -> scan -> scatter -> per-bucket sort. cnt and cursor hold nb words;
+// cnt must be zero on entry.
+hipError_t bucket_sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
+                             const uint32_t* vin, uint32_t* vout, uint32_t n, uint32_t shift, uint32_t nb,
+                             uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st) {
+  if (!tmp)
+    return rocprim::exclusive_scan(nullptr, *tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(),
+                                   st);
+  const uint32_t g = std::min(4096u, (n + 255) / 256);
+  hipLaunchKernelGGL(k_bucket_count, dim3(g), dim3(256), 0, st, kin, n, shift, cnt);
+  hipError_t e = rocprim::exclusive_scan(tmp, *tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(), st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bucket_scatter, dim3(g), dim3(256), 0, st, kin, vin, n, shift, cursor, kout, vout);
+  const uint32_t gb = std::min(4096u, (nb + 255) / 256);
+  hipLaunchKernelGGL(k_bucket_sort, dim3(gb), dim3(256), 0, st, kout, vout, cursor, cnt, nb, flag);
+  return hipGetLastError();
 }
 
 // Explicit instantiations used by the host runtime.

@@ -43,19 +43,30 @@ struct HashParams {
   const uint32_t* bloom;
   uint32_t nwords, wshift, wbits, B, NBK, nbits;
   uint32_t s2;  // shift of the second bloom bit (0 = single-bit exact filter)
-  uint32_t lane_bytes;  // positions per lane per work unit: 16 or 32
-  const uint32_t* bstart;
-  const uint2* bent2;   // bucket entries {hash, factor}
+  uint32_t lane_bytes;  // positions per lane per work unit: 16, 32 or 64
+  // Verification table: 2^nbits buckets of two 32-byte slots. A slot is
+  // {hash, factor, length, bytes 0-3}, {bytes 4-15, 0} (hash 0xFFFFFFFF =
+  // empty); a bucket with more than two factors keeps one in slot 0 and turns
+  // slot 1 into {kVtabMore, first, count, 0} over vmore (same slot format).
+  const uint4* vtab;
+  const uint4* vmore;
   const uint4* fmeta;   // per factor {byte offset (4-aligned), length, kind << 31 | owner, output id}
 };
+constexpr uint32_t kVtabEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kVtabMore = 0x80000000u;
 
 // Filter hits {position, item}: one private slice per filter block plus a
 // shared overflow area.
+// Filter hits: {position, item} plus the 32 input bytes of the aligned
+// 32-byte span holding the position (read while the span is still in L2),
+// so verification does not re-read the input from HBM.
 struct HitSlices {
   uint2* hits;
+  uint4* win;             // two uint4 per hit slot, parallel to hits
   uint32_t slice_cap;
   uint32_t* slice_count;  // per block
   uint2* overflow;
+  uint4* overflow_win;
   uint32_t overflow_cap;
   uint32_t* overflow_count;
 };

@@ -248,13 +248,14 @@ def make_requests(n: int, specs, pools: DevicePools, seed: int) -> List[Request]
     return [Request(i, int(sets[i]), int(dens[i]), int(offs[i]), int(sizes[i])) for i in range(n)]
 
 
-def default_config_index(info: dict) -> int:
+def default_config_index(info: dict, nbytes: int) -> int:
     """The configuration the library picks without a router (WS_VARIANT_AUTO on
-    a single scan): the hash filter at 32 positions per lane, or the LDS factor
-    DFA when the hash window is at most 2 bytes."""
+    a single scan): the LDS factor DFA when the hash window is at most 2
+    bytes, else the hash filter at 64 positions per lane from 256 MiB up and
+    32 below."""
     if info["hash_window"] <= 2 and info["variants_supported"] & (1 << N.VARIANT_DFA_LDS):
         return 6
-    return 1
+    return 2 if nbytes >= (256 << 20) else 1
 
 
 def harvest(reqs, specs, pools, matchers, repeats: int = 2):
@@ -272,7 +273,7 @@ def harvest(reqs, specs, pools, matchers, repeats: int = 2):
         Y.append(np.where(mask > 0, np.log(np.maximum(times, 1e-3)), 0.0))
         M.append(mask)
         T.append(times)
-        D.append(default_config_index(info))
+        D.append(default_config_index(info, q.nbytes))
     return (np.array(X, np.float32), np.array(Y, np.float32), np.array(M, np.float32), np.array(T, np.float32),
             np.array(D, np.int64))
 
@@ -310,7 +311,7 @@ def _route_fn(router: Optional[Router], specs, pools, infos, masks):
         if router is None:
             out = []
             for j in jobs:
-                c = default_config_index(infos[j.patterns])
+                c = default_config_index(infos[j.patterns], j.nbytes)
                 out.append((c, tunings[c]))
             return out
         X = np.stack([features(infos[j.patterns], j.nbytes, j.density) for j in jobs])
@@ -389,7 +390,7 @@ def run_study(n_requests: int = 50_000, n_train: int = 1500, n_test: int = 600, 
     for s in loop.slots:
         for i in range(len(sets)):
             s.matcher(i).scan_device_array(pools.ptr(specs[i].kind, 0, 0), pool_size,
-                                           tuning=config_tuning(default_config_index(infos[i])))
+                                           tuning=config_tuning(default_config_index(infos[i], pool_size)))
     say(f"slots warmed in {time.perf_counter() - t:.1f}s")
     n_burst = min(n_requests, 5000)
     burst_jobs = jobs_for(np.zeros(n_requests))[:n_burst]
