@@ -156,7 +156,7 @@ ws_status upload_patterns(ws_patterns* p) {
   std::string fb;
   std::vector<uint32_t> foff, flen, fown, aclen;
   for (const auto& f : h.fac) {
-    while (fb.size() % 4) fb.push_back(0);  // dword-aligned factor bytes
+    while (fb.size() % 16) fb.push_back(0);  // 16-byte aligned factor bytes
     foff.push_back((uint32_t)fb.size());
     flen.push_back((uint32_t)f.bytes.size());
     fown.push_back((f.kind << 31) | f.owner);
@@ -167,7 +167,7 @@ ws_status upload_patterns(ws_patterns* p) {
       fb.push_back((char)b);
     }
   }
-  fb.append(8, '\0');
+  fb.append(32, '\0');  // 16-byte loads past the last factor stay in bounds
   std::vector<uint32_t> fmeta;
   for (size_t f = 0; f < h.fac.size(); ++f) {
     const auto& fc = h.fac[f];
@@ -185,13 +185,14 @@ ws_status upload_patterns(ws_patterns* p) {
     const uint32_t nb = 1u << h.nbits;
     vtab.assign((size_t)nb * 16, 0);
     auto fill = [&](uint32_t* e, uint32_t f) {
+      const Factor& fc = h.fac[f];
       e[0] = h.fhash[f];
-      e[1] = f;
-      e[2] = flen[f];
+      e[1] = foff[f];
+      e[2] = flen[f] | (fc.kind << 31);
       uint8_t pre[16] = {0};
       std::memcpy(pre, fb.data() + foff[f], std::min<uint32_t>(16, flen[f]));
       std::memcpy(&e[3], pre, 16);
-      e[7] = 0;
+      e[7] = fc.kind ? fc.owner : h.lit_pid[fc.owner];
     };
     for (uint32_t b = 0; b < nb; ++b) {
       uint32_t* s0 = vtab.data() + (size_t)b * 16;

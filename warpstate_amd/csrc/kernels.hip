@@ -322,96 +322,171 @@ __device__ __forceinline__ void slice_put(const SliceOut& so, uint32_t slice, ui
   }
 }
 
-// Exact comparison of one table entry ({hash, factor, length, bytes 0-3},
-// {bytes 4-15, 0}) at position q, then the record. w holds the input bytes
-// from q (avail of them valid, at least 17).
+// Dword k of the byte stream that starts at byte 4*dw + bs/8 of r (nr dwords;
+// dw is dynamic, so the select compiles to v_cndmask chains).
+template <int NR>
+__device__ __forceinline__ uint32_t stream_dword(const uint32_t* r, uint32_t dw, uint32_t bs, int k) {
+  uint32_t lo = 0, hi = 0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (dw == (uint32_t)t) {
+      lo = k + t < NR ? r[k + t] : 0u;
+      hi = k + t + 1 < NR ? r[k + t + 1] : 0u;
+    }
+  return bs ? ((lo >> bs) | (hi << (32 - bs))) : lo;
+}
+
+// Exact comparison of one table entry ({hash, factor byte offset (16-byte
+// aligned), length | kind << 31, bytes 0-3}, {bytes 4-15, output id}) at
+// position q. On a match *rec is the output record and the result is
+// 1 (literal) or 2 (regex candidate); 0 otherwise. w holds the input bytes
+// from q (avail of them, at least 17); r is the hit's raw aligned span.
 template <bool CI>
-__device__ __forceinline__ void verify_entry(const ScanCommon& c, const HashParams& h, uint4 e0, uint4 e1,
-                                             const uint32_t* w, uint32_t avail, uint64_t q, uint2 seg,
-                                             uint32_t item, uint32_t* s_n, const SliceOut& lit,
-                                             const SliceOut& cand) {
-  const uint32_t len = e0.z;
-  if (q + len > seg.y) return;
+__device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, uint4 e1, const uint32_t* w,
+                                                 const uint32_t* r, uint32_t avail, uint64_t q, uint2 seg,
+                                                 uint32_t item, uint4* rec) {
+  const uint32_t len = e0.z & 0x7FFFFFFFu;
+  if (q + len > seg.y) return 0;
   const uint32_t pre[4] = {e0.w, e1.x, e1.y, e1.z};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (4u * k >= len) break;
     const uint32_t rem = len - 4 * k;
     const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
-    if ((w[k] ^ pre[k]) & msk) return;
+    if ((w[k] ^ pre[k]) & msk) return 0;
   }
-  const uint4 m = h.fmeta[e0.y];  // {offset, length, kind << 31 | owner, output id}
+  const uint32_t start = (uint32_t)(q - seg.x);
+  const uint32_t kind = (e0.z >> 31) ? 2u : 1u;
+  *rec = make_uint4(e1.w, start, kind == 2u ? 0u : start + len, item);
   if (len > 16) {
-    // The tail: window registers while they last, then the input itself.
-    const uint32_t* pat = reinterpret_cast<const uint32_t*>(c.fbytes + m.x);
-    const uint32_t nw = (len + 3) >> 2;
-    for (uint32_t k = 4; k < nw; ++k) {
-      uint32_t dv;
-      if (4 * k + 4 <= avail) {
-        dv = w[4];
+    // Bytes 16..31 of the factor in one load, against the record span
+    // extended by one aligned 16-byte input load when the factor runs past
+    // it (q + len <= seg.y <= load_end, which is 16-byte aligned).
+    const uint4 p1 = *reinterpret_cast<const uint4*>(c.fbytes + e0.y + 16);
+    uint32_t rr[12];
 #pragma unroll
-        for (int t = 5; t < 8; ++t) if (k == (uint32_t)t) dv = w[t];
-      } else {
-        dv = 0;
-        for (int t = 0; t < 4; ++t) {
-          const uint64_t p = q + 4 * k + t;
-          uint8_t bb = p < seg.y ? c.base[p] : 0;
-          if (CI) bb = fold_byte(bb);
-          dv |= (uint32_t)bb << (8 * t);
-        }
+    for (int t = 0; t < 8; ++t) rr[t] = r[t];
+    rr[8] = rr[9] = rr[10] = rr[11] = 0;
+    if (len > avail) {
+      const uint64_t a = (q & ~15ull) + 32;
+      if (a + 16 <= c.load_end) {
+        const uint4 x2 = *reinterpret_cast<const uint4*>(c.base + a);
+        rr[8] = x2.x; rr[9] = x2.y; rr[10] = x2.z; rr[11] = x2.w;
+      }
+    }
+    const uint32_t sh = (uint32_t)(q & 15), dw = sh >> 2, bs = (sh & 3) * 8;
+    const uint32_t pw[4] = {p1.x, p1.y, p1.z, p1.w};
+    uint32_t diff = 0;
+#pragma unroll
+    for (int k = 4; k < 8; ++k) {
+      uint32_t dv = stream_dword<12>(rr, dw, bs, k);
+      if (CI) dv = fold_word(dv);
+      const uint32_t rem = len > 4u * k ? len - 4 * k : 0u;
+      const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
+      diff |= (dv ^ pw[k - 4]) & msk;
+    }
+    if (diff) return 0;
+    // Factors longer than 32 bytes: the rest byte by byte.
+    const uint32_t nw = (len + 3) >> 2;
+    const uint32_t* pat = reinterpret_cast<const uint32_t*>(c.fbytes + e0.y);
+    for (uint32_t k = 8; k < nw; ++k) {
+      uint32_t dv = 0;
+      for (int t = 0; t < 4; ++t) {
+        const uint64_t p = q + 4 * k + t;
+        uint8_t bb = p < seg.y ? c.base[p] : 0;
+        if (CI) bb = fold_byte(bb);
+        dv |= (uint32_t)bb << (8 * t);
       }
       const uint32_t rem = len - 4 * k;
       const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
-      if ((dv ^ pat[k]) & msk) return;
+      if ((dv ^ pat[k]) & msk) return 0;
     }
   }
-  const uint32_t start = (uint32_t)(q - seg.x);
-  if (m.z >> 31)
-    slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), make_uint4(m.w, start, 0u, item));
-  else
-    slice_put(lit, blockIdx.x, atomicAdd(&s_n[0], 1u), make_uint4(m.w, start, start + len, item));
+  return kind;
 }
 
+// Appends rec for every lane with `take` to the block's slice of `so`: one
+// LDS atomic per wave (ballot + prefix count). Call with every active lane.
+__device__ __forceinline__ void emit_wave(const SliceOut& so, uint32_t* s_cnt, bool take, uint4 rec) {
+  const uint64_t m = __ballot(take);
+  if (m == 0) return;
+  const uint32_t lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)lane == leader) base = atomicAdd(s_cnt, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (take) slice_put(so, blockIdx.x, base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)), rec);
+}
+
+// A filter hit being verified: its position, item, segment, the raw span and
+// the first 16 bytes from q, and the window hash.
+struct HitWork {
+  uint64_t q;
+  uint32_t item;
+  uint2 seg;
+  uint32_t r[8];
+  uint32_t w[4];
+  uint32_t avail, H;
+  uint4 s0, s1, s2, s3;  // its bucket's two slots
+  bool live;             // false for a lane's unused second hit of a round
+};
+
 template <int K, bool CI>
-__device__ __forceinline__ void verify_hit(const ScanCommon& c, const HashParams& h, uint2 hr, uint4 x0, uint4 x1,
-                                           uint32_t* s_n, const SliceOut& lit, const SliceOut& cand) {
-  const uint64_t q = hr.x;
-  const uint32_t item = hr.y;
-  const uint2 seg = c.items[item];
+__device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParams& h, uint2 hr, uint4 x0, uint4 x1,
+                                            HitWork& hw) {
+  hw.live = true;
+  hw.q = hr.x;
+  hw.item = hr.y;
+  hw.seg = c.items[hw.item];
   // Bytes [q, q + avail) come from the hit record's 32-byte aligned span
   // (avail >= 17, so the K <= 16 window is always there).
-  const uint32_t r[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-  const uint32_t sh = (uint32_t)(q & 15), avail = 32 - sh;
+  hw.r[0] = x0.x; hw.r[1] = x0.y; hw.r[2] = x0.z; hw.r[3] = x0.w;
+  hw.r[4] = x1.x; hw.r[5] = x1.y; hw.r[6] = x1.z; hw.r[7] = x1.w;
+  const uint32_t sh = (uint32_t)(hw.q & 15);
+  hw.avail = 32 - sh;
   const uint32_t dw = sh >> 2, bs = (sh & 3) * 8;
-  uint32_t w[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    // dynamic dword select from a small register array (v_cndmask chains)
-    uint32_t lo = 0, hi = 0;
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-      if (dw == (uint32_t)t) {
-        lo = k + t < 8 ? r[k + t] : 0u;
-        hi = k + t + 1 < 8 ? r[k + t + 1] : 0u;
-      }
-    const uint32_t v = bs ? ((lo >> bs) | (hi << (32 - bs))) : lo;
-    w[k] = CI ? fold_word(v) : v;
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t v = stream_dword<8>(hw.r, dw, bs, k);
+    hw.w[k] = CI ? fold_word(v) : v;
   }
   uint32_t H = 0;
 #pragma unroll
-  for (int t = 0; t < K; ++t) H = (H * h.B + ((w[t >> 2] >> ((t & 3) * 8)) & 0xFFu)) & 0xFFFFFFu;
-  const uint32_t bk = (H * 0x9E3779B1u) >> (32 - h.nbits);
-  const uint4* slot = h.vtab + 4 * (size_t)bk;
-  const uint4 a0 = slot[0], a1 = slot[1], b0 = slot[2], b1 = slot[3];
-  if (a0.x == H) verify_entry<CI>(c, h, a0, a1, w, avail, q, seg, item, s_n, lit, cand);
-  if (b0.x == H) {
-    verify_entry<CI>(c, h, b0, b1, w, avail, q, seg, item, s_n, lit, cand);
-  } else if (b0.x == kVtabMore) {
-    for (uint32_t e = b0.y; e < b0.y + b0.z; ++e) {
+  for (int t = 0; t < K; ++t) H = (H * h.B + ((hw.w[t >> 2] >> ((t & 3) * 8)) & 0xFFu)) & 0xFFFFFFu;
+  hw.H = H;
+  const uint4* slot = h.vtab + 4 * (size_t)((H * 0x9E3779B1u) >> (32 - h.nbits));
+  hw.s0 = slot[0];
+  hw.s1 = slot[1];
+  hw.s2 = slot[2];
+  hw.s3 = slot[3];
+}
+
+template <bool CI>
+__device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams& h, const HitWork& hw,
+                                           uint32_t* s_n, const SliceOut& lit, const SliceOut& cand) {
+  uint4 r0 = make_uint4(0, 0, 0, 0), r1 = r0;
+  uint32_t k0 = 0, k1 = 0;
+  if (hw.live && hw.s0.x == hw.H)
+    k0 = verify_entry<CI>(c, hw.s0, hw.s1, hw.w, hw.r, hw.avail, hw.q, hw.seg, hw.item, &r0);
+  if (!hw.live) {
+  } else if (hw.s2.x == hw.H) {
+    k1 = verify_entry<CI>(c, hw.s2, hw.s3, hw.w, hw.r, hw.avail, hw.q, hw.seg, hw.item, &r1);
+  } else if (hw.s2.x == kVtabMore) {
+    // Crowded bucket (rare): per-lane appends.
+    for (uint32_t e = hw.s2.y; e < hw.s2.y + hw.s2.z; ++e) {
       const uint4 e0 = h.vmore[2 * (size_t)e];
-      if (e0.x == H) verify_entry<CI>(c, h, e0, h.vmore[2 * (size_t)e + 1], w, avail, q, seg, item, s_n, lit, cand);
+      if (e0.x != hw.H) continue;
+      uint4 rec;
+      const uint32_t k = verify_entry<CI>(c, e0, h.vmore[2 * (size_t)e + 1], hw.w, hw.r, hw.avail, hw.q, hw.seg,
+                                          hw.item, &rec);
+      if (k == 1) slice_put(lit, blockIdx.x, atomicAdd(&s_n[0], 1u), rec);
+      if (k == 2) slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), rec);
     }
   }
+  emit_wave(lit, &s_n[0], k0 == 1, r0);
+  emit_wave(cand, &s_n[1], k0 == 2, r0);
+  emit_wave(lit, &s_n[0], k1 == 1, r1);
+  emit_wave(cand, &s_n[1], k1 == 2, r1);
 }
 
 // Exact check of every filter hit: rolling hash of the K-byte window, its
@@ -432,12 +507,24 @@ __global__ __launch_bounds__(kTPB) void k_hash_verify(ScanCommon c, HashParams h
   const uint32_t n = min(hs.slice_count[b], hs.slice_cap);
   const uint4* win = hs.win + 2 * (size_t)b * hs.slice_cap;
   if (part == 0 && threadIdx.x == 0 && n) atomicAdd(&c.counters[8], n);  // filter hits, for timings
-  for (uint32_t i = part * kTPB + threadIdx.x; i < n; i += kVerifySplit * kTPB)
-    verify_hit<K, CI>(c, h, slice[i], win[2 * (size_t)i], win[2 * (size_t)i + 1], s_n, lit, cand);
+  // Two hits per lane per round, their loads interleaved.
+  constexpr uint32_t kStride = kVerifySplit * kTPB;
+  for (uint32_t i = part * kTPB + threadIdx.x; i < n; i += 2 * kStride) {
+    const bool two = i + kStride < n;
+    const uint32_t j = two ? i + kStride : i;
+    HitWork a, bw;
+    hit_prepare<K, CI>(c, h, slice[i], win[2 * (size_t)i], win[2 * (size_t)i + 1], a);
+    hit_prepare<K, CI>(c, h, slice[j], win[2 * (size_t)j], win[2 * (size_t)j + 1], bw);
+    bw.live = two;
+    hit_finish<CI>(c, h, a, s_n, lit, cand);
+    hit_finish<CI>(c, h, bw, s_n, lit, cand);
+  }
   const uint32_t no = min(*hs.overflow_count, hs.overflow_cap);
-  for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < no; i += gridDim.x * kTPB)
-    verify_hit<K, CI>(c, h, hs.overflow[i], hs.overflow_win[2 * (size_t)i], hs.overflow_win[2 * (size_t)i + 1],
-                      s_n, lit, cand);
+  for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < no; i += gridDim.x * kTPB) {
+    HitWork a;
+    hit_prepare<K, CI>(c, h, hs.overflow[i], hs.overflow_win[2 * (size_t)i], hs.overflow_win[2 * (size_t)i + 1], a);
+    hit_finish<CI>(c, h, a, s_n, lit, cand);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     lit.counts[blockIdx.x] = s_n[0];

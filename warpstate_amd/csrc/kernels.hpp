@@ -45,9 +45,10 @@ struct HashParams {
   uint32_t s2;  // shift of the second bloom bit (0 = single-bit exact filter)
   uint32_t lane_bytes;  // positions per lane per work unit: 16, 32 or 64
   // Verification table: 2^nbits buckets of two 32-byte slots. A slot is
-  // {hash, factor, length, bytes 0-3}, {bytes 4-15, 0} (hash 0xFFFFFFFF =
-  // empty); a bucket with more than two factors keeps one in slot 0 and turns
-  // slot 1 into {kVtabMore, first, count, 0} over vmore (same slot format).
+  // {hash, factor byte offset, length | kind << 31, bytes 0-3},
+  // {bytes 4-15, output id} (hash 0xFFFFFFFF = empty); a bucket with more than
+  // two factors keeps one in slot 0 and turns slot 1 into
+  // {kVtabMore, first, count, 0} over vmore (same slot format).
   const uint4* vtab;
   const uint4* vmore;
   const uint4* fmeta;   // per factor {byte offset (4-aligned), length, kind << 31 | owner, output id}
