@@ -455,13 +455,25 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   std::vector<uint2> units;
   uint32_t nunits;
   if (plan.batch) {
-    // Work-queue order: largest items first (longest-processing-time first),
-    // each item's tiles contiguous.
-    std::vector<uint32_t> by_size(plan.items.size());
-    std::iota(by_size.begin(), by_size.end(), 0u);
-    std::stable_sort(by_size.begin(), by_size.end(), [&](uint32_t x, uint32_t y) {
-      return plan.items[x].y - plan.items[x].x > plan.items[y].y - plan.items[y].x;
-    });
+    // Work-queue order: largest items first (longest-processing-time first,
+    // by power-of-two size class: a counting sort, O(items)), each item's
+    // tiles contiguous.
+    const size_t ni = plan.items.size();
+    auto cls = [&](size_t i) {
+      const uint32_t len = plan.items[i].y - plan.items[i].x;
+      return 32u - (uint32_t)__builtin_clz(len | 1u);  // 1..32
+    };
+    std::vector<uint32_t> start(34, 0);
+    uint64_t total_units = 0;
+    for (size_t i = 0; i < ni; ++i) {
+      start[33 - cls(i)]++;
+      const uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
+      total_units += hi > lo ? (hi - lo + unit - 1) / unit : 0;
+    }
+    for (uint32_t k = 0, acc = 0; k < 34; ++k) { const uint32_t c = start[k]; start[k] = acc; acc += c; }
+    std::vector<uint32_t> by_size(ni);
+    for (size_t i = 0; i < ni; ++i) by_size[start[33 - cls(i)]++] = (uint32_t)i;
+    units.reserve(total_units);
     for (uint32_t i : by_size) {
       uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
       for (uint64_t t = lo; t < hi; t += unit) units.push_back(make_uint2((uint32_t)t, i));
@@ -1191,8 +1203,15 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
   // (batch output order is (source_id, start, pattern_id, end)).
   std::vector<uint32_t> order(n_items);
   std::iota(order.begin(), order.end(), 0u);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint32_t x, uint32_t y) { return items[x].id < items[y].id; });
+  bool sorted = true;
+  for (size_t k = 1; k < n_items && sorted; ++k) sorted = items[k - 1].id <= items[k].id;
+  if (!sorted) {
+    // (id, index) pairs sort contiguously; equal ids keep their input order.
+    std::vector<std::pair<uint64_t, uint32_t>> key(n_items);
+    for (size_t k = 0; k < n_items; ++k) key[k] = {items[k].id, (uint32_t)k};
+    std::sort(key.begin(), key.end());
+    for (size_t k = 0; k < n_items; ++k) order[k] = key[k].second;
+  }
   Plan plan;
   plan.batch = true;
   std::vector<uint64_t> ids;
