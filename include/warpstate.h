@@ -122,11 +122,18 @@ enum {
   WS_VARIANT_DFA_LDS = 3     /* dense factor DFA, table resident in LDS */
 };
 
+/* ws_tuning.flags: WS_TUNING_CHECKED runs the host-checked pipeline (the
+ * host reads counts back between passes and grows buffers on demand) instead
+ * of the device-sized one (every pass sized on the device, one
+ * synchronisation per call; overflowing buffers fall back to the checked
+ * pipeline automatically). Results are identical. */
+enum { WS_TUNING_CHECKED = 1u };
+
 typedef struct ws_tuning {
   uint32_t variant;       /* WS_VARIANT_* */
   uint32_t bytes_per_lane;/* 16, 32 or 64 (hash); 0 = default */
   uint32_t blocks_per_cu; /* persistent grid = CUs * blocks_per_cu; 0 = default */
-  uint32_t reserved;
+  uint32_t flags;         /* WS_TUNING_* */
 } ws_tuning;
 
 /* Compiled-set facts (router features; core/src/pattern/ir.rs:6-33). */

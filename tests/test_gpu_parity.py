@@ -247,6 +247,37 @@ def test_ordering_many_matches_spread_and_clustered():
     assert_same(got2, exp2, "clustered")
 
 
+def test_device_sized_and_checked_pipelines_agree():
+    # Overlapping scans run device-sized by default (one synchronisation per
+    # call); the host-checked pipeline must give the same lists, whole and
+    # chunked, and both must equal the oracle.
+    pats, _, specs = W.mixed_set(150, 60, seed=41)
+    rng = np.random.default_rng(42)
+    pieces = [p for k, p in pats if k == 0] + [s.sample(rng) for s in specs]
+    data = W.printable_input(3 << 20, 43)
+    W.plant(data, pieces, 700, 44)
+    m = GpuMatcher(build(pats))
+    dev = m.scan_array(data.tobytes())
+    chk = m.scan_array(data.tobytes(), tuning=Tuning(checked=True))
+    exp = Oracle(pats).scan(data, 0)
+    assert len(exp) > 3000
+    assert np.array_equal(dev, exp) and np.array_equal(chk, exp)
+    maxlen = max(s.max_len for s in specs) + 24
+    sp = StreamPipeline(m, maxlen)
+    sd = sp.scan_array(data.tobytes(), chunk_size=300_000)
+    sc = sp.scan_array(data.tobytes(), chunk_size=300_000, tuning=Tuning(checked=True))
+    assert np.array_equal(sd, exp) and np.array_equal(sc, exp)
+
+
+def test_device_sized_pipeline_redoes_on_overflow():
+    # 500,000 regex candidates overflow the device-sized capacity for a
+    # 1 MB input; the scan is redone on the checked pipeline, same answer.
+    data = b"ab" * 500_000 + b"ab7"
+    m = GpuMatcher(build([(1, b"ab[0-9]")]))
+    got = as_list(m.scan_array(data))
+    assert got == [(0, 1_000_000, 1_000_003)]
+
+
 def test_overflow_is_an_error_not_a_fallback():
     from warpstate_amd import AutoMatcherConfig, Context
     ctx = Context(0, AutoMatcherConfig(max_matches=100))

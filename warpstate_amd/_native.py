@@ -62,12 +62,15 @@ class WsConfig(Structure):
     ]
 
 
+TUNING_CHECKED = 1  # ws_tuning.flags: host-checked pipeline
+
+
 class WsTuning(Structure):
     _fields_ = [
         ("variant", c_uint32),
         ("bytes_per_lane", c_uint32),
         ("blocks_per_cu", c_uint32),
-        ("reserved", c_uint32),
+        ("flags", c_uint32),
     ]
 
 

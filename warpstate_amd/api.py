@@ -79,9 +79,11 @@ class Tuning:
     variant: int = N.VARIANT_AUTO
     bytes_per_lane: int = 0
     blocks_per_cu: int = 0
+    checked: bool = False  # host-checked pipeline instead of the device-sized one
 
     def to_c(self) -> N.WsTuning:
-        return N.WsTuning(self.variant, self.bytes_per_lane, self.blocks_per_cu, 0)
+        return N.WsTuning(self.variant, self.bytes_per_lane, self.blocks_per_cu,
+                          N.TUNING_CHECKED if self.checked else 0)
 
 
 class Context:

@@ -31,15 +31,15 @@ size_t ac_shmem_bytes(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCommon& c,
                      const AcParams& a);
 hipError_t launch_walk(uint32_t ncand_host, hipStream_t st, const WalkParams& w);
-hipError_t launch_order_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint32_t pbits,
+hipError_t launch_order_keys(const Rec* recs, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t pbits,
                              uint64_t* keys, uint32_t* vals, hipStream_t st);
-hipError_t launch_chain_keys(const Rec* raw, uint32_t n, uint32_t sbits, uint32_t rbits,
+hipError_t launch_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t rbits,
                              uint64_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t launch_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
-                              uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all, uint8_t* head,
-                              hipStream_t st);
+                              const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all,
+                              uint8_t* head, hipStream_t st);
 hipError_t launch_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
-                             uint32_t gshift, const uint8_t* head, const uint32_t* carried,
+                             const uint32_t* nd, uint32_t gshift, const uint8_t* head, const uint32_t* carried,
                              uint32_t seg_off, const uint32_t* rx_pid, uint32_t ties, Rec* out,
                              uint32_t out_cap, uint32_t* out_count, uint32_t* carried_out,
                              hipStream_t st);
@@ -47,17 +47,21 @@ hipError_t launch_lit_keys(const Rec* recs, uint32_t n, uint32_t lbits, uint32_t
                            uint32_t len_desc, uint64_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t launch_pid_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint64_t* keys, uint32_t* vals,
                            hipStream_t st);
-hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t pbits,
-                               uint32_t start_off, void* out, hipStream_t st);
+hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint32_t n, const uint32_t* nd,
+                               uint32_t pbits, uint32_t start_off, void* out, uint64_t out_cap,
+                               const unsigned long long* base_dev, hipStream_t st);
 hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t sbits,
                               uint32_t pbits, const uint64_t* ids, void* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
+hipError_t launch_run_close(const uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
+                            unsigned long long* total, hipStream_t st);
+hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t st);
 hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
                       const uint32_t* vin, uint32_t* vout, uint32_t n, uint32_t bits,
                       hipStream_t st);
 hipError_t bucket_sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
-                             const uint32_t* vin, uint32_t* vout, uint32_t n, uint32_t shift, uint32_t nb,
-                             uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st);
+                             const uint32_t* vin, uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift,
+                             uint32_t nb, uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st);
 
 namespace {
 
@@ -106,7 +110,8 @@ struct ws_ctx {
   ws_timings last{};
   hipEvent_t ev[8] = {};
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids, hits, hwin, slice_counts, lslices, cslices, scounts, bcnt, bcur;
+      scratch, carried, ids, hits, hwin, slice_counts, lslices, cslices, scounts, bcnt, bcur, dstate;
+  std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
   uint32_t* hcount = nullptr;  // pinned host mirror of the counters
   uint32_t last_hits = 0;      // filter hits of the last hash-variant scan
 };
@@ -341,14 +346,13 @@ ws_status order_pairs(ws_ctx* ctx, uint32_t n, uint32_t bits, bool dense, Result
   WS_HIP(ctx->bcnt.ensure((size_t)nb * 4));
   WS_HIP(ctx->bcur.ensure((size_t)nb * 4));
   size_t need = 0;
-  WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, n, shift, nb,
+  WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, n, nullptr, shift, nb,
                            ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), nullptr, ctx->stream));
   WS_HIP(ctx->tmp.ensure(need));
   size_t have = ctx->tmp.cap;
-  WS_HIP(launch_fill_u32(ctx->bcnt.as<uint32_t>(), nb, 0, ctx->stream));
   WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>() + 9, 1, 0, ctx->stream));
   WS_HIP(bucket_sort_pairs(ctx->tmp.p, &have, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
-                           ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), n, shift, nb,
+                           ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), n, nullptr, shift, nb,
                            ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(),
                            ctx->counters.as<uint32_t>() + 9, ctx->stream));
   WS_HIP(hipMemcpyAsync(ctx->hcount + 9, ctx->counters.as<uint32_t>() + 9, 4, hipMemcpyDeviceToHost,
@@ -383,10 +387,10 @@ ws_status read_counters(ws_ctx* ctx) {
 ws_status greedy_pass(ws_ctx* ctx, uint32_t n, uint32_t gshift, uint32_t maxlen, uint32_t ties,
                       uint32_t* kept) {
   const hipStream_t st = ctx->stream;
-  WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->scratch.as<Rec>(), n,
+  WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->scratch.as<Rec>(), n, nullptr,
                             gshift, nullptr, maxlen, ctx->head.as<uint8_t>(), st));
   WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 1, 0, st));
-  WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->scratch.as<Rec>(), n,
+  WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->scratch.as<Rec>(), n, nullptr,
                            gshift, ctx->head.as<uint8_t>(), nullptr, 0, nullptr, ties,
                            ctx->recs.as<Rec>(), n, ctx->counters.as<uint32_t>(), nullptr, st));
   ws_status s = read_counters(ctx);
@@ -432,6 +436,106 @@ ws_status reduce_literals(ws_ctx* ctx, uint32_t strategy, uint32_t nlit, uint32_
   return WS_OK;
 }
 
+// Hash filter: 16, 32 or 64 positions per lane. More positions amortise the
+// per-window setup (K-1 hash steps, loads, bounds) but make work units
+// coarser: 64 for large inputs, 16 for batches of small items.
+uint32_t choose_lane_bytes(const Plan& plan, const ws_tuning* tuning) {
+  uint32_t lane_bytes = tuning && tuning->bytes_per_lane ? tuning->bytes_per_lane : 0;
+  if (lane_bytes == 16 || lane_bytes == 32 || lane_bytes == 64) return lane_bytes;
+  uint64_t span = 0;
+  for (auto& it : plan.items) span += it.y - it.x;
+  if (plan.batch && span / plan.items.size() < 64 * 1024) return 16;
+  return span >= (256ull << 20) ? 64 : 32;
+}
+
+// Work units of `unit` bytes. A single input strides statically; a batch
+// lists its units in work-queue order: largest items first
+// (longest-processing-time first, by power-of-two size class: a counting
+// sort, O(items)), each item's tiles contiguous.
+uint32_t plan_units(const Plan& plan, uint32_t unit, std::vector<uint2>* units) {
+  if (!plan.batch) return (uint32_t)((plan.items[0].y + unit - 1) / unit);
+  const size_t ni = plan.items.size();
+  auto cls = [&](size_t i) {
+    const uint32_t len = plan.items[i].y - plan.items[i].x;
+    return 32u - (uint32_t)__builtin_clz(len | 1u);  // 1..32
+  };
+  std::vector<uint32_t> start(34, 0);
+  uint64_t total_units = 0;
+  for (size_t i = 0; i < ni; ++i) {
+    start[33 - cls(i)]++;
+    const uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
+    total_units += hi > lo ? (hi - lo + unit - 1) / unit : 0;
+  }
+  for (uint32_t k = 0, acc = 0; k < 34; ++k) { const uint32_t c = start[k]; start[k] = acc; acc += c; }
+  std::vector<uint32_t> by_size(ni);
+  for (size_t i = 0; i < ni; ++i) by_size[start[33 - cls(i)]++] = (uint32_t)i;
+  units->reserve(total_units);
+  for (uint32_t i : by_size) {
+    uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
+    for (uint64_t t = lo; t < hi; t += unit) units->push_back(make_uint2((uint32_t)t, i));
+  }
+  return (uint32_t)units->size();
+}
+
+// Per-block output slices over `buf` (`total` records): a quarter goes to the
+// shared overflow area.
+SliceOut make_slices(DevBuf& buf, uint64_t total, uint32_t nvb, uint32_t* counts, uint32_t* ovf_count) {
+  SliceOut so{};
+  so.ovf_cap = (uint32_t)std::min<uint64_t>(total / 4, 0xFFFFFFF0ull);
+  so.cap = (uint32_t)((total - so.ovf_cap) / nvb);
+  so.buf = buf.as<Rec>();
+  so.ovf = so.buf + (size_t)so.cap * nvb;
+  so.counts = counts;
+  so.ovf_count = ovf_count;
+  return so;
+}
+
+// Filter hit slices over ctx->hits / ctx->hwin (`hit_total` slots).
+HitSlices make_hits(ws_ctx* ctx, uint64_t hit_total, uint32_t fgrid) {
+  HitSlices hs{};
+  hs.overflow_cap = (uint32_t)std::min<uint64_t>(hit_total / 4, 0xFFFFFFF0ull);
+  hs.slice_cap = (uint32_t)((hit_total - hs.overflow_cap) / fgrid);
+  hs.hits = ctx->hits.as<uint2>();
+  hs.overflow = hs.hits + (size_t)hs.slice_cap * fgrid;
+  hs.win = ctx->hwin.as<uint4>();
+  hs.overflow_win = hs.win + 2 * (size_t)hs.slice_cap * fgrid;
+  hs.slice_count = ctx->slice_counts.as<uint32_t>();
+  hs.overflow_count = ctx->counters.as<uint32_t>() + 4;
+  return hs;
+}
+
+ScanCommon make_common(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t nunits, uint32_t lit_cap,
+                       uint32_t cand_cap) {
+  ScanCommon c{};
+  c.base = plan.base;
+  c.load_end = plan.load_end;
+  c.units = plan.batch ? ctx->units.as<uint2>() : nullptr;
+  c.nunits = nunits;
+  c.items = ctx->items.as<uint2>();
+  c.emit_end = plan.emit_end;
+  c.ci = p->h.ci ? 1u : 0u;
+  c.fbytes = p->fbytes;
+  c.foff = p->foff;
+  c.flen = p->flen;
+  c.fown = p->fown;
+  c.lit_pid = p->lit_pid;
+  c.lit_out = ctx->recs.as<Rec>();
+  c.lit_cap = lit_cap;
+  c.cand_out = ctx->cands.as<Rec>();
+  c.cand_cap = cand_cap;
+  c.counters = ctx->counters.as<uint32_t>();
+  // Batches pull uneven work from a device queue (zeroed with the counters);
+  // a single input's equal tiles use a static stride.
+  c.queue = plan.batch ? ctx->counters.as<uint32_t>() + 7 : nullptr;
+  return c;
+}
+
+HashParams hash_params(const ws_patterns* p, uint32_t lane_bytes) {
+  const PatternsHost& h = p->h;
+  return HashParams{p->bloom, (uint32_t)h.bloom.size(), h.wshift, h.wbits, h.B, h.NBK, h.nbits, h.s2,
+                    lane_bytes, p->vtab, p->vmore, p->fmeta};
+}
+
 ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t mode,
                    const ws_tuning* tuning, Result* res) {
   const PatternsHost& h = p->h;
@@ -439,49 +543,10 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   const uint32_t variant = pick_variant(p, tuning);
   if (!(supported_variants(p) & (1u << variant)))
     return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
-  // Hash filter: 16, 32 or 64 positions per lane. More positions amortise
-  // the per-window setup (K-1 hash steps, loads, bounds) but make work units
-  // coarser: 64 for large inputs, 16 for batches of small items.
-  uint32_t lane_bytes = tuning && tuning->bytes_per_lane ? tuning->bytes_per_lane : 0;
-  if (lane_bytes != 16 && lane_bytes != 32 && lane_bytes != 64) {
-    uint64_t span = 0;
-    for (auto& it : plan.items) span += it.y - it.x;
-    if (plan.batch && span / plan.items.size() < 64 * 1024)
-      lane_bytes = 16;
-    else
-      lane_bytes = span >= (256ull << 20) ? 64 : 32;
-  }
+  const uint32_t lane_bytes = choose_lane_bytes(plan, tuning);
   const uint32_t unit = variant == WS_VARIANT_HASH ? kTPB * lane_bytes : ac_unit_bytes();
   std::vector<uint2> units;
-  uint32_t nunits;
-  if (plan.batch) {
-    // Work-queue order: largest items first (longest-processing-time first,
-    // by power-of-two size class: a counting sort, O(items)), each item's
-    // tiles contiguous.
-    const size_t ni = plan.items.size();
-    auto cls = [&](size_t i) {
-      const uint32_t len = plan.items[i].y - plan.items[i].x;
-      return 32u - (uint32_t)__builtin_clz(len | 1u);  // 1..32
-    };
-    std::vector<uint32_t> start(34, 0);
-    uint64_t total_units = 0;
-    for (size_t i = 0; i < ni; ++i) {
-      start[33 - cls(i)]++;
-      const uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
-      total_units += hi > lo ? (hi - lo + unit - 1) / unit : 0;
-    }
-    for (uint32_t k = 0, acc = 0; k < 34; ++k) { const uint32_t c = start[k]; start[k] = acc; acc += c; }
-    std::vector<uint32_t> by_size(ni);
-    for (size_t i = 0; i < ni; ++i) by_size[start[33 - cls(i)]++] = (uint32_t)i;
-    units.reserve(total_units);
-    for (uint32_t i : by_size) {
-      uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
-      for (uint64_t t = lo; t < hi; t += unit) units.push_back(make_uint2((uint32_t)t, i));
-    }
-    nunits = (uint32_t)units.size();
-  } else {
-    nunits = (uint32_t)((plan.items[0].y + unit - 1) / unit);
-  }
+  const uint32_t nunits = plan_units(plan, unit, &units);
   WS_HIP(ctx->items.ensure(plan.items.size() * sizeof(uint2)));
   WS_HIP(hipMemcpyAsync(ctx->items.p, plan.items.data(), plan.items.size() * sizeof(uint2),
                         hipMemcpyHostToDevice, st));
@@ -531,55 +596,16 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[0], st));
   for (int attempt = 0;; ++attempt) {
     WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 9, 0, st));
-    ScanCommon c{};
-    c.base = plan.base;
-    c.load_end = plan.load_end;
-    c.units = plan.batch ? ctx->units.as<uint2>() : nullptr;
-    c.nunits = nunits;
-    c.items = ctx->items.as<uint2>();
-    c.emit_end = plan.emit_end;
-    c.ci = h.ci ? 1u : 0u;
-    c.fbytes = p->fbytes;
-    c.foff = p->foff;
-    c.flen = p->flen;
-    c.fown = p->fown;
-    c.lit_pid = p->lit_pid;
-    c.lit_out = ctx->recs.as<Rec>();
-    c.lit_cap = lit_cap;
-    c.cand_out = ctx->cands.as<Rec>();
-    c.cand_cap = cand_cap;
-    c.counters = ctx->counters.as<uint32_t>();
-    // Batches pull uneven work from a device queue (zeroed with the
-    // counters); a single input's equal tiles use a static stride.
-    c.queue = plan.batch ? ctx->counters.as<uint32_t>() + 7 : nullptr;
+    ScanCommon c = make_common(ctx, p, plan, nunits, lit_cap, cand_cap);
     if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[1], st));
     if (nunits == 0 && ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
     if (nunits > 0) {
       if (variant == WS_VARIANT_HASH) {
-        HashParams hp{p->bloom, (uint32_t)h.bloom.size(), h.wshift, h.wbits, h.B, h.NBK, h.nbits, h.s2,
-                      lane_bytes, p->vtab, p->vmore, p->fmeta};
-        HitSlices hs{};
-        hs.overflow_cap = (uint32_t)std::min<uint64_t>(hit_total / 4, 0xFFFFFFF0ull);
-        hs.slice_cap = (uint32_t)((hit_total - hs.overflow_cap) / fgrid);
-        hs.hits = ctx->hits.as<uint2>();
-        hs.overflow = hs.hits + (size_t)hs.slice_cap * fgrid;
-        hs.win = ctx->hwin.as<uint4>();
-        hs.overflow_win = hs.win + 2 * (size_t)hs.slice_cap * fgrid;
-        hs.slice_count = ctx->slice_counts.as<uint32_t>();
-        hs.overflow_count = ctx->counters.as<uint32_t>() + 4;
-        auto make_slices = [&](DevBuf& buf, uint64_t total, uint32_t* counts, uint32_t* ovf_count) {
-          SliceOut so{};
-          so.ovf_cap = (uint32_t)std::min<uint64_t>(total / 4, 0xFFFFFFF0ull);
-          so.cap = (uint32_t)((total - so.ovf_cap) / nvb);
-          so.buf = buf.as<Rec>();
-          so.ovf = so.buf + (size_t)so.cap * nvb;
-          so.counts = counts;
-          so.ovf_count = ovf_count;
-          return so;
-        };
+        const HashParams hp = hash_params(p, lane_bytes);
+        const HitSlices hs = make_hits(ctx, hit_total, fgrid);
         uint32_t* sc = ctx->scounts.as<uint32_t>();
-        SliceOut lso = make_slices(ctx->lslices, lslice_total, sc, ctx->counters.as<uint32_t>() + 5);
-        SliceOut cso = make_slices(ctx->cslices, cslice_total, sc + nvb, ctx->counters.as<uint32_t>() + 6);
+        SliceOut lso = make_slices(ctx->lslices, lslice_total, nvb, sc, ctx->counters.as<uint32_t>() + 5);
+        SliceOut cso = make_slices(ctx->cslices, cslice_total, nvb, sc + nvb, ctx->counters.as<uint32_t>() + 6);
         WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, false));
         if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
         WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, true));
@@ -709,13 +735,13 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       WS_HIP(ctx->v0.ensure((size_t)nraw * 4));
       WS_HIP(ctx->v1.ensure((size_t)nraw * 4));
       WS_HIP(ctx->head.ensure(nraw));
-      WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), nraw, res->sbits, rbits, ctx->k0.as<uint64_t>(),
+      WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), nraw, nullptr, res->sbits, rbits, ctx->k0.as<uint64_t>(),
                                ctx->v0.as<uint32_t>(), st));
       ws_status s2 = sort_into(ctx, nraw, cbits, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
                                ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>());
       if (s2 != WS_OK) return s2;
       WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(),
-                                nraw, res->sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st));
+                                nraw, nullptr, res->sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st));
       // The chain appends kept matches after the literal matches.
       if ((size_t)nlit + nraw > ctx->recs.cap / sizeof(Rec)) {
         // Grow while keeping the literal matches.
@@ -729,7 +755,7 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       }
       WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 1, nlit, st));
       WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(),
-                               nraw, res->sbits, ctx->head.as<uint8_t>(),
+                               nraw, nullptr, res->sbits, ctx->head.as<uint8_t>(),
                                plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr,
                                plan.seg_off, p->rx_pid, 0u, ctx->recs.as<Rec>(),
                                (uint32_t)(ctx->recs.cap / sizeof(Rec)), ctx->counters.as<uint32_t>(),
@@ -753,7 +779,7 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     WS_HIP(ctx->k1.ensure((size_t)total * 8));
     WS_HIP(ctx->v0.ensure((size_t)total * 4));
     WS_HIP(ctx->v1.ensure((size_t)total * 4));
-    WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), total, res->sbits, res->pbits,
+    WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), total, nullptr, res->sbits, res->pbits,
                              ctx->k0.as<uint64_t>(), ctx->v0.as<uint32_t>(), st));
     res->count = total;
     ws_status s = order_pairs(ctx, total, res->sbits + res->pbits + ibits, !plan.batch, res);
@@ -821,6 +847,272 @@ ws_status plan_single(ws_ctx* ctx, const uint8_t* data, uint64_t len, uint32_t d
   return WS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Device-sized runs (overlapping mode). Every pass after the filter reads its
+// element count from device memory, so a run -- or every chunk of a stream --
+// is enqueued without a host synchronisation. Buffers are sized to
+// capacities up front; k_run_close notes any overflow (the scan is then
+// redone on the checked path, whose buffers grow on demand) and appends the
+// run's match count to the output offset.
+// ---------------------------------------------------------------------------
+struct DeviceRunState {
+  void* out = nullptr;       // device ws_match array
+  uint64_t out_cap = 0;
+  unsigned long long* total = nullptr;
+  uint32_t* sticky = nullptr;
+};
+
+uint32_t bucket_bits(uint32_t keybits, uint32_t cap) {
+  return std::min<uint32_t>(keybits, std::min(22u, std::max(10u, bits_for(cap) + 1)));
+}
+
+ws_status enqueue_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const ws_tuning* tuning,
+                      uint32_t start_off, const DeviceRunState& ds, uint32_t chunk) {
+  const PatternsHost& h = p->h;
+  const hipStream_t st = ctx->stream;
+  const uint32_t variant = pick_variant(p, tuning);
+  if (!(supported_variants(p) & (1u << variant)))
+    return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
+  const uint32_t lane_bytes = choose_lane_bytes(plan, tuning);
+  const uint32_t unit = variant == WS_VARIANT_HASH ? kTPB * lane_bytes : ac_unit_bytes();
+  std::vector<uint2> units;
+  const uint32_t nunits = plan_units(plan, unit, &units);
+  WS_HIP(ctx->items.ensure(plan.items.size() * sizeof(uint2)));
+  WS_HIP(hipMemcpyAsync(ctx->items.p, plan.items.data(), plan.items.size() * sizeof(uint2),
+                        hipMemcpyHostToDevice, st));
+  if (plan.batch && !units.empty()) {
+    WS_HIP(ctx->units.ensure(units.size() * sizeof(uint2)));
+    WS_HIP(hipMemcpyAsync(ctx->units.p, units.data(), units.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+  }
+  uint64_t span = 0;
+  for (auto& it : plan.items) span += it.y - it.x;
+  // Capacities.
+  const uint32_t max_matches = (uint32_t)std::min<uint64_t>(ctx->cfg.max_matches, 0xFFFFFFF0ull);
+  const uint32_t lit_cap = (uint32_t)std::min<uint64_t>(max_matches, std::max<uint64_t>(1u << 16, span / 16));
+  const uint32_t cand_cap = (uint32_t)std::min<uint64_t>(0x0FFFFFFFull, std::max<uint64_t>(1u << 16, span / 64));
+  const uint32_t nrx = (uint32_t)h.rx.size();
+  const uint32_t raw_cap = nrx ? cand_cap : 0;
+  const uint32_t rec_cap = lit_cap + raw_cap;
+  WS_HIP(ctx->recs.ensure((size_t)rec_cap * sizeof(Rec)));
+  WS_HIP(ctx->cands.ensure((size_t)cand_cap * sizeof(Rec)));
+  if (nrx) WS_HIP(ctx->raw.ensure((size_t)raw_cap * sizeof(Rec)));
+  WS_HIP(ctx->k0.ensure((size_t)rec_cap * 8));
+  WS_HIP(ctx->k1.ensure((size_t)rec_cap * 8));
+  WS_HIP(ctx->v0.ensure((size_t)rec_cap * 4));
+  WS_HIP(ctx->v1.ensure((size_t)rec_cap * 4));
+  if (nrx) WS_HIP(ctx->head.ensure(raw_cap));
+  const uint32_t per_cu = tuning && tuning->blocks_per_cu ? tuning->blocks_per_cu : 4;
+  const uint32_t grid = std::max(1u, std::min(nunits, ctx->cus * per_cu));
+  const uint32_t fgrid = std::max(1u, std::min((nunits + 7) / 8, ctx->cus * (per_cu > 4 ? 2u : 1u)));
+  const uint32_t nvb = verify_blocks(fgrid);
+  const uint64_t hit_total = std::min<uint64_t>(std::max<uint64_t>(1u << 20, span / 64), 0x3FFFFFFFull);
+  const uint64_t slice_total = (uint64_t)nvb * 256 + (1u << 16);
+  if (variant == WS_VARIANT_HASH) {
+    WS_HIP(ctx->hits.ensure(hit_total * sizeof(uint2)));
+    WS_HIP(ctx->hwin.ensure(hit_total * 2 * sizeof(uint4)));
+    WS_HIP(ctx->slice_counts.ensure((size_t)fgrid * 4));
+    WS_HIP(ctx->lslices.ensure(slice_total * sizeof(Rec)));
+    WS_HIP(ctx->cslices.ensure(slice_total * sizeof(Rec)));
+    WS_HIP(ctx->scounts.ensure(((size_t)nvb * 2 + 4) * 4 * 2));
+  }
+  const uint32_t sbits = bits_for(plan.max_seg_len);
+  const uint32_t pbits = bits_for(h.npat ? h.npat - 1 : 0);
+  const uint32_t rbits = nrx ? bits_for(nrx - 1) : 0;
+  const uint32_t ibits = plan.batch ? bits_for(plan.items.size() - 1) : 0;
+  const uint32_t obits = sbits + pbits + ibits, cbits = sbits + rbits + ibits;
+  if (obits > 63 || cbits > 63) return fail(WS_ERR_INPUT_TOO_LARGE, "match key does not fit 63 bits");
+  // Bucket counts follow the capacities up to 2^18 buckets (a few pairs per
+  // bucket at a million matches); denser runs fall back through the
+  // overflow flag.
+  const uint32_t ob = bucket_bits(obits, std::min(rec_cap, 1u << 17));
+  const uint32_t cb = nrx ? bucket_bits(cbits, std::min(raw_cap, 1u << 17)) : 0;
+  const uint32_t nb = 1u << std::max(ob, cb);
+  WS_HIP(ctx->bcnt.ensure((size_t)nb * 4));
+  WS_HIP(ctx->bcur.ensure((size_t)nb * 4));
+  size_t need = 0, need2 = 0;
+  WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, rec_cap, nullptr, 0, 1u << ob,
+                           nullptr, nullptr, nullptr, st));
+  if (nrx)
+    WS_HIP(bucket_sort_pairs(nullptr, &need2, nullptr, nullptr, nullptr, nullptr, raw_cap, nullptr, 0, 1u << cb,
+                             nullptr, nullptr, nullptr, st));
+  WS_HIP(ctx->tmp.ensure(std::max(need, need2)));
+  size_t have = ctx->tmp.cap;
+  uint32_t* cnt = ctx->counters.as<uint32_t>();
+  hipEvent_t* ev = ctx->profiling ? &ctx->pev[5 * chunk] : nullptr;
+
+  // Filter and verification.
+  if (ev) WS_HIP(hipEventRecord(ev[0], st));
+  WS_HIP(launch_fill_u32(cnt, 11, 0, st));
+  ScanCommon c = make_common(ctx, p, plan, nunits, lit_cap, cand_cap);
+  RunCaps caps{};
+  caps.lit_cap = lit_cap;
+  caps.cand_cap = cand_cap;
+  caps.raw_cap = raw_cap;
+  caps.rec_cap = rec_cap;
+  caps.max_matches = max_matches;
+  caps.hit_ovf_cap = caps.lit_ovf_cap = caps.cand_ovf_cap = 0xFFFFFFFFu;
+  if (nunits > 0) {
+    if (variant == WS_VARIANT_HASH) {
+      const HashParams hp = hash_params(p, lane_bytes);
+      const HitSlices hs = make_hits(ctx, hit_total, fgrid);
+      uint32_t* sc = ctx->scounts.as<uint32_t>();
+      SliceOut lso = make_slices(ctx->lslices, slice_total, nvb, sc, cnt + 5);
+      SliceOut cso = make_slices(ctx->cslices, slice_total, nvb, sc + nvb, cnt + 6);
+      WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, false));
+      if (ev) WS_HIP(hipEventRecord(ev[1], st));
+      WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, true));
+      uint32_t* offs = sc + 2 * nvb;
+      WS_HIP(launch_slice_compact(lso, nvb, offs, cnt + 0, c.lit_out, c.lit_cap, st));
+      WS_HIP(launch_slice_compact(cso, nvb, offs + nvb + 2, cnt + 1, c.cand_out, c.cand_cap, st));
+      caps.hit_ovf_cap = hs.overflow_cap;
+      caps.lit_ovf_cap = lso.ovf_cap;
+      caps.cand_ovf_cap = cso.ovf_cap;
+    } else {
+      AcParams ap{p->ac_table32, p->ac_table16, p->ac_cmap, p->ac_out_start, p->ac_out_list,
+                  p->ac_len, h.ac_states, h.ac_classes, h.ac_first_out, h.ac_maxlen};
+      WS_HIP(launch_ac(variant == WS_VARIANT_DFA_LDS, grid, st, c, ap));
+      if (ev) WS_HIP(hipEventRecord(ev[1], st));
+    }
+  } else if (ev) {
+    WS_HIP(hipEventRecord(ev[1], st));
+  }
+  WS_HIP(launch_copy_u32(cnt + kCounterLitSnapshot, cnt + 0, st));
+  if (ev) WS_HIP(hipEventRecord(ev[2], st));
+
+  // Regex candidates: anchored walks, chain ordering, greedy chains appended
+  // after the literal matches.
+  if (nrx) {
+    WalkParams w{};
+    w.base = plan.base;
+    w.items = ctx->items.as<uint2>();
+    w.cands = ctx->cands.as<Rec>();
+    w.ncand = cnt + 1;
+    w.cand_cap = cand_cap;
+    w.tables = p->rx_tables;
+    w.toff = p->rx_toff;
+    w.ncls = p->rx_ncls;
+    w.cmaps = p->rx_cmaps;
+    w.rx_pid = p->rx_pid;
+    w.raw = ctx->raw.as<Rec>();
+    w.raw_cap = raw_cap;
+    w.counters = cnt;
+    w.window_truncated = plan.window_truncated ? 1u : 0u;
+    WS_HIP(launch_walk(0, st, w));
+    WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), raw_cap, cnt + 2, sbits, rbits, ctx->k0.as<uint64_t>(),
+                             ctx->v0.as<uint32_t>(), st));
+    WS_HIP(bucket_sort_pairs(ctx->tmp.p, &have, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
+                             ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), raw_cap, cnt + 2, cbits - cb, 1u << cb,
+                             ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, st));
+    WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), raw_cap,
+                              cnt + 2, sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st));
+    WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), raw_cap,
+                             cnt + 2, sbits, ctx->head.as<uint8_t>(),
+                             plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, plan.seg_off, p->rx_pid, 0u,
+                             ctx->recs.as<Rec>(), rec_cap, cnt + 0,
+                             plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, st));
+  }
+  if (ev) WS_HIP(hipEventRecord(ev[3], st));
+
+  // Order by (item, start, pattern id), pack at the running output offset.
+  WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), rec_cap, cnt + 0, sbits, pbits, ctx->k0.as<uint64_t>(),
+                           ctx->v0.as<uint32_t>(), st));
+  have = ctx->tmp.cap;
+  WS_HIP(bucket_sort_pairs(ctx->tmp.p, &have, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
+                           ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), rec_cap, cnt + 0, obits - ob, 1u << ob,
+                           ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, st));
+  WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), rec_cap, cnt + 0, pbits, start_off,
+                             ds.out, ds.out_cap, ds.total, st));
+  WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, st));
+  if (ev) WS_HIP(hipEventRecord(ev[4], st));
+  return WS_OK;
+}
+
+enum DeviceScanResult { kDeviceScanDone, kDeviceScanRedo };
+
+// Overlapping-mode scans take the device-sized path unless the caller asks
+// for the host-checked pipeline (WS_TUNING_CHECKED).
+bool device_path_ok(uint32_t mode, const ws_tuning* tuning) {
+  return mode == WS_MODE_OVERLAPPING && !(tuning && (tuning->flags & WS_TUNING_CHECKED));
+}
+
+// Runs the plans (one per chunk) as device-sized runs into `out` and reports
+// the total. *outcome = kDeviceScanRedo when a buffer overflowed; nothing was
+// then written to a host `out`.
+ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>& plans,
+                      const std::vector<uint32_t>& offs, const ws_tuning* tuning, ws_match* out, uint64_t cap,
+                      uint32_t out_mem, uint64_t* count, DeviceScanResult* outcome) {
+  const hipStream_t st = ctx->stream;
+  *outcome = kDeviceScanDone;
+  WS_HIP(ctx->dstate.ensure(64));
+  uint32_t* sticky = ctx->dstate.as<uint32_t>();
+  auto* total = reinterpret_cast<unsigned long long*>(sticky + 8);
+  WS_HIP(launch_fill_u32(sticky, 10, 0, st));
+  DeviceRunState ds;
+  ds.sticky = sticky;
+  ds.total = total;
+  const uint64_t max_matches = ctx->cfg.max_matches;
+  if (out_mem == WS_MEM_DEVICE) {
+    ds.out = out;
+    ds.out_cap = cap;
+  } else {
+    ds.out_cap = std::min<uint64_t>(cap, max_matches);
+    WS_HIP(ctx->outbuf.ensure(std::max<uint64_t>(ds.out_cap, 1) * sizeof(ws_match)));
+    ds.out = ctx->outbuf.p;
+  }
+  if (ctx->profiling && ctx->pev.size() < 5 * plans.size()) {
+    const size_t had = ctx->pev.size();
+    ctx->pev.resize(5 * plans.size(), nullptr);
+    for (size_t k = had; k < ctx->pev.size(); ++k) WS_HIP(hipEventCreate(&ctx->pev[k]));
+  }
+  for (size_t k = 0; k < plans.size(); ++k) {
+    ws_status s = enqueue_run(ctx, p, plans[k], tuning, offs[k], ds, (uint32_t)k);
+    if (s != WS_OK) return s;
+  }
+  WS_HIP(hipMemcpyAsync(ctx->hcount + 11, sticky, 10 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  WS_HIP(hipStreamSynchronize(st));
+  const uint32_t* hs = ctx->hcount + 11;
+  if (hs[kStickyRedo]) {
+    *outcome = kDeviceScanRedo;
+    return WS_OK;
+  }
+  uint64_t tot = 0;
+  std::memcpy(&tot, hs + 8, 8);
+  if (ctx->profiling) {
+    ws_timings t{};
+    for (size_t k = 0; k < plans.size(); ++k) {
+      hipEvent_t* ev = &ctx->pev[5 * k];
+      float a = 0, b = 0, c2 = 0, d = 0;
+      (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+      (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+      (void)hipEventElapsedTime(&c2, ev[2], ev[3]);
+      (void)hipEventElapsedTime(&d, ev[3], ev[4]);
+      t.prefilter_ms += a;
+      t.verify_ms += b;
+      t.regex_ms += c2;
+      t.order_ms += d;
+    }
+    float all = 0;
+    (void)hipEventElapsedTime(&all, ctx->pev[0], ctx->pev[5 * (plans.size() - 1) + 4]);
+    t.total_ms = all;
+    t.hits = hs[kStickyHits];
+    t.candidates = hs[kStickyCands];
+    t.raw_matches = tot;
+    ctx->last = t;
+  }
+  *count = tot;
+  if (hs[kStickyError] & 1u)
+    return fail(WS_ERR_INPUT_TOO_LARGE,
+                "a regex match may extend past the chunk overlap; raise the overlap to the longest possible match");
+  if (hs[kStickyError] & 2u || tot > cap)
+    return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(tot) + " exceeds " +
+                                                  std::to_string(std::min<uint64_t>(cap, max_matches)) + ")");
+  if (out_mem == WS_MEM_HOST && tot) {
+    WS_HIP(hipMemcpyAsync(out, ds.out, tot * sizeof(ws_match), hipMemcpyDeviceToHost, st));
+    WS_HIP(hipStreamSynchronize(st));
+  }
+  return WS_OK;
+}
+
 ws_status pack_out(ws_ctx* ctx, const Result& r, uint32_t start_off, ws_match* out, uint64_t cap,
                    uint32_t out_mem, uint64_t written) {
   uint32_t n = r.count;
@@ -830,13 +1122,13 @@ ws_status pack_out(ws_ctx* ctx, const Result& r, uint32_t start_off, ws_match* o
   Result rr = r;
   for (;;) {
     if (out_mem == WS_MEM_DEVICE) {
-      WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, rr.pbits, start_off,
-                                 out + written, ctx->stream));
+      WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, nullptr, rr.pbits, start_off,
+                                 out + written, n, nullptr, ctx->stream));
       WS_HIP(hipStreamSynchronize(ctx->stream));
     } else {
       WS_HIP(ctx->outbuf.ensure((size_t)n * sizeof(ws_match)));
-      WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, rr.pbits, start_off,
-                                 ctx->outbuf.p, ctx->stream));
+      WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, nullptr, rr.pbits, start_off,
+                                 ctx->outbuf.p, n, nullptr, ctx->stream));
       WS_HIP(hipMemcpyAsync(out + written, ctx->outbuf.p, (size_t)n * sizeof(ws_match),
                             hipMemcpyDeviceToHost, ctx->stream));
       WS_HIP(hipStreamSynchronize(ctx->stream));
@@ -890,7 +1182,7 @@ ws_status ws_ctx_create(int device_ordinal, const ws_config* cfg, ws_ctx** out) 
     e = hipEventCreate(&ev);
     if (e != hipSuccess) return bail("hipEventCreate", e);
   }
-  e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hcount), 16 * sizeof(uint32_t), hipHostMallocDefault);
+  e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hcount), 32 * sizeof(uint32_t), hipHostMallocDefault);
   if (e != hipSuccess) return bail("hipHostMalloc", e);
   e = ctx->counters.ensure(16 * sizeof(uint32_t));
   if (e != hipSuccess) return bail("hipMalloc", e);
@@ -906,9 +1198,11 @@ void ws_ctx_destroy(ws_ctx* ctx) {
                     &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
                     &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids, &ctx->hits, &ctx->hwin,
                     &ctx->slice_counts, &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
-                    &ctx->bcur})
+                    &ctx->bcur, &ctx->dstate})
     b->release();
   for (auto& ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto& ev : ctx->pev)
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->hcount) (void)hipHostFree(ctx->hcount);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1044,6 +1338,12 @@ ws_status ws_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64
   Plan plan;
   ws_status s = plan_single(ctx, data, len, data_mem, &plan);
   if (s != WS_OK) return s;
+  if (device_path_ok(mode, tuning)) {
+    DeviceScanResult oc;
+    s = device_scan(ctx, p, {plan}, {0u}, tuning, out, cap, out_mem, count, &oc);
+    if (s != WS_OK || oc == kDeviceScanDone) return s;
+    ctx->last = ws_timings{};  // a buffer overflowed: redo on the checked path
+  }
   Result r;
   s = run_plan(ctx, p, plan, mode, tuning, &r);
   if (s != WS_OK) {
@@ -1081,10 +1381,34 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
   WS_HIP(hipSetDevice(ctx->dev));
   ctx->last = ws_timings{};
   const uint32_t nrx = (uint32_t)p->h.rx.size();
-  if (nrx) {
-    WS_HIP(ctx->carried.ensure(nrx * sizeof(uint32_t)));
-    WS_HIP(launch_fill_u32(ctx->carried.as<uint32_t>(), nrx, 0, ctx->stream));
+  if (nrx) WS_HIP(ctx->carried.ensure(nrx * sizeof(uint32_t)));
+  if (device_path_ok(mode, tuning)) {
+    // Every chunk enqueued back to back, one synchronisation at the end.
+    if (nrx) WS_HIP(launch_fill_u32(ctx->carried.as<uint32_t>(), nrx, 0, ctx->stream));
+    std::vector<Plan> plans;
+    std::vector<uint32_t> offs;
+    for (uint64_t a = 0; a < len; a += chunk_size) {
+      const uint64_t b = std::min(len, a + chunk_size);
+      const uint64_t e = std::min(len, b + overlap);
+      Plan plan;
+      ws_status s = plan_single(ctx, data + a, e - a, data_mem, &plan);
+      if (s != WS_OK) return s;
+      plan.emit_end = (uint32_t)(b - a);
+      plan.window_truncated = e < len;
+      plan.seg_off = (uint32_t)a;
+      plan.use_carried = nrx > 0;
+      plans.push_back(std::move(plan));
+      offs.push_back((uint32_t)a);
+      if (data_mem == WS_MEM_HOST) break;  // host chunks share one staging buffer: checked path
+    }
+    if (data_mem != WS_MEM_HOST) {
+      DeviceScanResult oc;
+      ws_status s = device_scan(ctx, p, plans, offs, tuning, out, cap, out_mem, count, &oc);
+      if (s != WS_OK || oc == kDeviceScanDone) return s;
+      ctx->last = ws_timings{};
+    }
   }
+  if (nrx) WS_HIP(launch_fill_u32(ctx->carried.as<uint32_t>(), nrx, 0, ctx->stream));
   uint64_t total = 0;
   for (uint64_t a = 0; a < len; a += chunk_size) {
     const uint64_t b = std::min(len, a + chunk_size);
@@ -1167,8 +1491,8 @@ ws_status ws_autotune(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, ui
       if (s != WS_OK) return s;
       if (res.count) {
         WS_HIP(ctx->outbuf.ensure((size_t)res.count * sizeof(ws_match)));
-        WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), res.count, res.pbits, 0,
-                                   ctx->outbuf.p, ctx->stream));
+        WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), res.count, nullptr, res.pbits,
+                                   0, ctx->outbuf.p, res.count, nullptr, ctx->stream));
       }
       WS_HIP(hipStreamSynchronize(ctx->stream));
       bool redone = false;

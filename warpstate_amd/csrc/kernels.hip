@@ -703,9 +703,14 @@ __global__ __launch_bounds__(kTPB) void k_rx_walk(WalkParams w) {
   }
 }
 
+// Element count of a pass: the host's n, or (device-sized runs) the count
+// a previous kernel left in device memory, capped at n.
+__device__ __forceinline__ uint32_t count_of(uint32_t n, const uint32_t* nd) { return nd ? min(*nd, n) : n; }
+
 // Chain keys: group (item, regex) in the high bits, start in the low bits.
-__global__ void k_chain_keys(const Rec* raw, uint32_t n, uint32_t sbits, uint32_t rbits,
+__global__ void k_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t rbits,
                              uint64_t* keys, uint32_t* vals) {
+  n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const Rec r = raw[i];
     keys[i] = (((uint64_t)r.w << rbits | r.x) << sbits) | r.y;
@@ -722,8 +727,9 @@ __device__ __forceinline__ uint64_t group_of(uint64_t k, uint32_t gshift) {
 }
 
 __global__ void k_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw,
-                              uint32_t n, uint32_t gshift, const uint32_t* maxlen,
+                              uint32_t n, const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen,
                               uint32_t maxlen_all, uint8_t* head) {
+  n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint64_t g = group_of(keys[i], gshift);
     const Rec ri = raw[vals[i]];
@@ -750,10 +756,11 @@ __global__ void k_chain_heads(const uint64_t* keys, const uint32_t* vals, const 
 // (HashScanner reports every id of the shortest length at a position).
 // Output pattern id is rx_pid[rec.x] when rx_pid is set, else rec.x.
 __global__ void k_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw,
-                             uint32_t n, uint32_t gshift, const uint8_t* head,
+                             uint32_t n, const uint32_t* nd, uint32_t gshift, const uint8_t* head,
                              const uint32_t* carried, uint32_t seg_off, const uint32_t* rx_pid,
                              uint32_t ties, Rec* out, uint32_t out_cap, uint32_t* out_count,
                              uint32_t* carried_out) {
+  n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (!head[i]) continue;
     const uint64_t g = group_of(keys[i], gshift);
@@ -799,8 +806,9 @@ __global__ void k_pid_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint64_t
 }
 
 // Final ordering keys: (item, start, pattern_id); the end rides as value.
-__global__ void k_order_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint32_t pbits,
+__global__ void k_order_keys(const Rec* recs, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t pbits,
                              uint64_t* keys, uint32_t* vals) {
+  n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const Rec r = recs[i];
     keys[i] = ((((uint64_t)r.w << sbits) | r.y) << pbits) | r.x;
@@ -811,16 +819,22 @@ __global__ void k_order_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint32
 struct MatchOut { uint32_t pid, start, end; };
 struct TaggedOut { uint64_t id; uint32_t pid, start, end, pad; };
 
-__global__ void k_pack_matches(const uint64_t* keys, const uint32_t* vals, uint32_t n,
-                               uint32_t pbits, uint32_t start_off, MatchOut* out) {
+// Writes out[base + i] for i < n (base = *base_dev for device-sized runs,
+// else 0), dropping records at or past out_cap.
+__global__ void k_pack_matches(const uint64_t* keys, const uint32_t* vals, uint32_t n, const uint32_t* nd,
+                               uint32_t pbits, uint32_t start_off, MatchOut* out, uint64_t out_cap,
+                               const unsigned long long* base_dev) {
+  n = count_of(n, nd);
+  const uint64_t base = base_dev ? *base_dev : 0ull;
   const uint64_t pmask = (1ull << pbits) - 1;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (base + i >= out_cap) break;
     const uint64_t k = keys[i];
     MatchOut m;
     m.pid = (uint32_t)(k & pmask);
     m.start = (uint32_t)(k >> pbits) + start_off;
     m.end = vals[i] + start_off;
-    out[i] = m;
+    out[base + i] = m;
   }
 }
 
@@ -924,42 +938,51 @@ hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCo
   return hipGetLastError();
 }
 
+// Grid of a pass whose element count lives on the device (grid-stride).
+constexpr uint32_t kDeviceGrid = 2048;
+
 static uint32_t grid_for(uint32_t n) {
   uint32_t g = (n + kTPB - 1) / kTPB;
   return g < 1 ? 1 : (g > 65535 ? 65535 : g);
 }
 
+// ncand_host sizes the grid; 0 = the count is only known on the device
+// (k_rx_walk reads it), so launch a fixed grid-stride grid.
 hipError_t launch_walk(uint32_t ncand_host, hipStream_t st, const WalkParams& w) {
-  hipLaunchKernelGGL(k_rx_walk, dim3(grid_for(ncand_host)), dim3(kTPB), 0, st, w);
+  hipLaunchKernelGGL(k_rx_walk, dim3(ncand_host ? grid_for(ncand_host) : kDeviceGrid), dim3(kTPB), 0, st, w);
   return hipGetLastError();
 }
 
-hipError_t launch_order_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint32_t pbits,
+// The launchers below take a host count n; with nd set, n is a capacity and
+// the kernel reads the actual count from *nd.
+static uint32_t grid_of(uint32_t n, const uint32_t* nd) { return nd ? std::min(kDeviceGrid, grid_for(n)) : grid_for(n); }
+
+hipError_t launch_order_keys(const Rec* recs, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t pbits,
                              uint64_t* keys, uint32_t* vals, hipStream_t st) {
-  hipLaunchKernelGGL(k_order_keys, dim3(grid_for(n)), dim3(kTPB), 0, st, recs, n, sbits, pbits, keys, vals);
+  hipLaunchKernelGGL(k_order_keys, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, recs, n, nd, sbits, pbits, keys, vals);
   return hipGetLastError();
 }
 
-hipError_t launch_chain_keys(const Rec* raw, uint32_t n, uint32_t sbits, uint32_t rbits,
+hipError_t launch_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t rbits,
                              uint64_t* keys, uint32_t* vals, hipStream_t st) {
-  hipLaunchKernelGGL(k_chain_keys, dim3(grid_for(n)), dim3(kTPB), 0, st, raw, n, sbits, rbits, keys, vals);
+  hipLaunchKernelGGL(k_chain_keys, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, raw, n, nd, sbits, rbits, keys, vals);
   return hipGetLastError();
 }
 
 hipError_t launch_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
-                              uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all, uint8_t* head,
-                              hipStream_t st) {
-  hipLaunchKernelGGL(k_chain_heads, dim3(grid_for(n)), dim3(kTPB), 0, st, keys, vals, raw, n, gshift, maxlen,
-                     maxlen_all, head);
+                              const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all,
+                              uint8_t* head, hipStream_t st) {
+  hipLaunchKernelGGL(k_chain_heads, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, keys, vals, raw, n, nd, gshift,
+                     maxlen, maxlen_all, head);
   return hipGetLastError();
 }
 
 hipError_t launch_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
-                             uint32_t gshift, const uint8_t* head, const uint32_t* carried,
+                             const uint32_t* nd, uint32_t gshift, const uint8_t* head, const uint32_t* carried,
                              uint32_t seg_off, const uint32_t* rx_pid, uint32_t ties, Rec* out,
                              uint32_t out_cap, uint32_t* out_count, uint32_t* carried_out,
                              hipStream_t st) {
-  hipLaunchKernelGGL(k_chain_walk, dim3(grid_for(n)), dim3(kTPB), 0, st, keys, vals, raw, n, gshift, head,
+  hipLaunchKernelGGL(k_chain_walk, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, keys, vals, raw, n, nd, gshift, head,
                      carried, seg_off, rx_pid, ties, out, out_cap, out_count, carried_out);
   return hipGetLastError();
 }
@@ -977,10 +1000,11 @@ hipError_t launch_pid_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint64_t
   return hipGetLastError();
 }
 
-hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t pbits,
-                               uint32_t start_off, void* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_pack_matches, dim3(grid_for(n)), dim3(kTPB), 0, st, keys, vals, n, pbits, start_off,
-                     reinterpret_cast<MatchOut*>(out));
+hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint32_t n, const uint32_t* nd,
+                               uint32_t pbits, uint32_t start_off, void* out, uint64_t out_cap,
+                               const unsigned long long* base_dev, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_matches, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, keys, vals, n, nd, pbits, start_off,
+                     reinterpret_cast<MatchOut*>(out), out_cap, base_dev);
   return hipGetLastError();
 }
 
@@ -1017,6 +1041,43 @@ hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_
   return rocprim::radix_sort_pairs(tmp, *tmp_bytes, kin, kout, vin, vout, (size_t)n, 0u, bits, st);
 }
 
+// End of a device-sized run: record overflows and errors, accumulate
+// statistics, and advance the output offset by this run's match count.
+__global__ void k_run_close(const uint32_t* counters, RunCaps caps, uint32_t* sticky,
+                            unsigned long long* total) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint32_t redo = 0;
+  if (counters[4] > caps.hit_ovf_cap) redo |= 1u;
+  if (counters[5] > caps.lit_ovf_cap || counters[6] > caps.cand_ovf_cap) redo |= 2u;
+  if (counters[kCounterLitSnapshot] > caps.lit_cap) redo |= 4u;
+  if (counters[1] > caps.cand_cap) redo |= 8u;
+  if (counters[2] > caps.raw_cap) redo |= 16u;
+  if (counters[0] > caps.rec_cap) redo |= 32u;
+  if (counters[9]) redo |= 64u;  // a sort bucket overflowed
+  sticky[kStickyRedo] |= redo;
+  if (counters[3] & kFlagTruncatedWalk) sticky[kStickyError] |= 1u;
+  if (counters[0] > caps.max_matches) sticky[kStickyError] |= 2u;
+  sticky[kStickyHits] += counters[8];
+  sticky[kStickyCands] += counters[1];
+  sticky[kStickyRaw] += counters[2];
+  *total += counters[0];
+}
+
+__global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = *src;
+}
+
+hipError_t launch_run_close(const uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
+                            unsigned long long* total, hipStream_t st) {
+  hipLaunchKernelGGL(k_run_close, dim3(1), dim3(64), 0, st, counters, caps, sticky, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(64), 0, st, dst, src);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Bucket sort of (key, value) pairs whose keys spread over their range (match
 // keys: position-major). Bucket = key >> shift, about two buckets per pair:
@@ -1026,13 +1087,16 @@ hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBucketMax = 64;
 
-__global__ void k_bucket_count(const uint64_t* keys, uint32_t n, uint32_t shift, uint32_t* cnt) {
+__global__ void k_bucket_count(const uint64_t* keys, uint32_t n, const uint32_t* nd, uint32_t shift,
+                               uint32_t* cnt) {
+  n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     atomicAdd(&cnt[keys[i] >> shift], 1u);
 }
 
-__global__ void k_bucket_scatter(const uint64_t* kin, const uint32_t* vin, uint32_t n, uint32_t shift,
-                                 uint32_t* cursor, uint64_t* kout, uint32_t* vout) {
+__global__ void k_bucket_scatter(const uint64_t* kin, const uint32_t* vin, uint32_t n, const uint32_t* nd,
+                                 uint32_t shift, uint32_t* cursor, uint64_t* kout, uint32_t* vout) {
+  n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint64_t k = kin[i];
     const uint32_t pos = atomicAdd(&cursor[k >> shift], 1u);
@@ -1067,19 +1131,20 @@ __global__ void k_bucket_sort(uint64_t* keys, uint32_t* vals, const uint32_t* en
 }
 
 // Sizes the scan's temporary storage (tmp = nullptr) or runs
-// count -> scan -> scatter -> per-bucket sort. cnt and cursor hold nb words;
-// cnt must be zero on entry.
+// clear -> count -> scan -> scatter -> per-bucket sort. cnt and cursor hold
+// nb words. With nd set, n is a capacity and the count is *nd.
 hipError_t bucket_sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
-                             const uint32_t* vin, uint32_t* vout, uint32_t n, uint32_t shift, uint32_t nb,
-                             uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st) {
+                             const uint32_t* vin, uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift,
+                             uint32_t nb, uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st) {
   if (!tmp)
     return rocprim::exclusive_scan(nullptr, *tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(),
                                    st);
-  const uint32_t g = std::min(4096u, (n + 255) / 256);
-  hipLaunchKernelGGL(k_bucket_count, dim3(g), dim3(256), 0, st, kin, n, shift, cnt);
+  const uint32_t g = std::max(1u, std::min(nd ? kDeviceGrid : 4096u, (n + 255) / 256));
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::min(4096u, (nb + 255) / 256)), dim3(256), 0, st, cnt, nb, 0u);
+  hipLaunchKernelGGL(k_bucket_count, dim3(g), dim3(256), 0, st, kin, n, nd, shift, cnt);
   hipError_t e = rocprim::exclusive_scan(tmp, *tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(), st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bucket_scatter, dim3(g), dim3(256), 0, st, kin, vin, n, shift, cursor, kout, vout);
+  hipLaunchKernelGGL(k_bucket_scatter, dim3(g), dim3(256), 0, st, kin, vin, n, nd, shift, cursor, kout, vout);
   const uint32_t gb = std::min(4096u, (nb + 255) / 256);
   hipLaunchKernelGGL(k_bucket_sort, dim3(gb), dim3(256), 0, st, kout, vout, cursor, cnt, nb, flag);
   return hipGetLastError();

@@ -112,4 +112,17 @@ struct WalkParams {
 
 enum : uint32_t { kFlagTruncatedWalk = 1u };
 
+// Capacities of a device-sized run (no host readback between passes), and
+// the counters slot that keeps the literal count before regex matches are
+// appended.
+struct RunCaps {
+  uint32_t hit_ovf_cap, lit_ovf_cap, cand_ovf_cap;
+  uint32_t lit_cap, cand_cap, raw_cap, rec_cap, max_matches;
+};
+constexpr uint32_t kCounterLitSnapshot = 10;
+// sticky[0]: a buffer overflowed (redo on the checked path); sticky[1]: error
+// bits (1 = regex window truncated, 2 = more than max_matches);
+// sticky[2..4]: running sums of filter hits, candidates, raw regex matches.
+enum : uint32_t { kStickyRedo = 0, kStickyError = 1, kStickyHits = 2, kStickyCands = 3, kStickyRaw = 4 };
+
 }  // namespace ws
