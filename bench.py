@@ -278,6 +278,7 @@ def main() -> int:
                      "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
         "pct_hbm_peak_whole_step": round(100 * value / HBM_PEAK_GBS, 2),
+        "graphs_replayed": w.ctx.graph_count,
         "step_breakdown_ms": {"prefilter": round(k_ms, 4), "verify": round(float(np.mean(ver_ms)), 4),
                               "regex": round(float(np.mean(rx_ms)), 4),
                               "order": round(float(np.mean(order_ms)), 4)},

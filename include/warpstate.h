@@ -174,6 +174,10 @@ void ws_ctx_destroy(ws_ctx* ctx);
 void* ws_ctx_stream(ws_ctx* ctx);
 /* Number of compute units of the context's device. */
 uint32_t ws_ctx_cu_count(ws_ctx* ctx);
+/* Captured launch sequences the context holds. An overlapping-mode scan of
+ * device memory that repeats (same patterns, input, output and tuning) is
+ * captured into a hipGraph on its second call and replayed from then on. */
+uint32_t ws_ctx_graph_count(ws_ctx* ctx);
 /* Enable per-call HIP-event timing (ws_ctx_last_timings). */
 void ws_ctx_set_profiling(ws_ctx* ctx, int on);
 ws_status ws_ctx_last_timings(ws_ctx* ctx, ws_timings* out);

@@ -278,6 +278,43 @@ def test_device_sized_pipeline_redoes_on_overflow():
     assert got == [(0, 1_000_000, 1_000_003)]
 
 
+def test_graph_replays_match_eager():
+    # A repeated device scan is captured into a hipGraph on its second call
+    # and replayed afterwards; every replay must equal the eager result, with
+    # other eager work and synchronisations between the replays.
+    from warpstate_amd import AutoMatcherConfig, Context
+    from warpstate_amd.device import DeviceBuffer
+
+    pats, _, specs = W.mixed_set(120, 40, seed=51)
+    rng = np.random.default_rng(52)
+    pieces = [p for k, p in pats if k == 0] + [s.sample(rng) for s in specs]
+    data = W.printable_input(2 << 20, 53)
+    W.plant(data, pieces, 500, 54)
+    other = W.printable_input(1 << 20, 55)
+    W.plant(other, pieces, 300, 56)
+    ctx = Context(0, AutoMatcherConfig())
+    m = GpuMatcher(build(pats), context=ctx)
+    exp = Oracle(pats).scan(data, 0)
+    exp_other = Oracle(pats).scan(other, 0)
+    dev, dev_other = DeviceBuffer.from_array(data), DeviceBuffer.from_array(other)
+    for k in range(5):
+        got = m.scan_device_array(dev.ptr, len(data))
+        assert np.array_equal(got, exp), k
+        assert np.array_equal(m.scan_array(other.tobytes()), exp_other), k
+    assert ctx.graph_count >= 1
+    # Stream steady state: chunked scans of device memory replay too.
+    maxlen = max(s.max_len for s in specs) + 24
+    sp = StreamPipeline(m, maxlen)
+    out = DeviceBuffer(12 * (len(exp) + 64))
+    for k in range(4):
+        n = sp.scan_device(dev.ptr, len(data), out.ptr, len(exp) + 64, chunk_size=400_000)
+        assert n == len(exp)
+        got = out.download(np.uint8, 12 * n).view(exp.dtype)
+        assert np.array_equal(got, exp), k
+        assert np.array_equal(m.scan_device_array(dev_other.ptr, len(other)), exp_other), k
+    assert ctx.graph_count >= 3
+
+
 def test_overflow_is_an_error_not_a_fallback():
     from warpstate_amd import AutoMatcherConfig, Context
     ctx = Context(0, AutoMatcherConfig(max_matches=100))

@@ -117,6 +117,11 @@ class Context:
     def cu_count(self) -> int:
         return int(N.load().ws_ctx_cu_count(self.handle))
 
+    @property
+    def graph_count(self) -> int:
+        """Captured launch sequences (hipGraphs) this context replays."""
+        return int(N.load().ws_ctx_graph_count(self.handle))
+
     def set_profiling(self, on: bool) -> None:
         N.load().ws_ctx_set_profiling(self.handle, 1 if on else 0)
 

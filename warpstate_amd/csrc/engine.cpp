@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <map>
+#include <tuple>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -53,6 +56,8 @@ hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint3
 hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t sbits,
                               uint32_t pbits, const uint64_t* ids, void* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
+hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords);
+hipError_t prepare_ac(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_run_close(const uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
                             unsigned long long* total, hipStream_t st);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t st);
@@ -74,6 +79,11 @@ ws_status fail(ws_status s, const std::string& msg) {
 
 uint32_t bits_for(uint64_t v) { return v == 0 ? 1u : 64u - (uint32_t)__builtin_clzll(v); }
 
+// Bumped whenever a workspace is reallocated: captured graphs hold raw
+// device addresses, so a graph recorded under an older epoch is stale.
+std::atomic<uint64_t> g_buf_epoch{1};
+std::atomic<uint64_t> g_pattern_serial{1};
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
@@ -82,6 +92,7 @@ struct DevBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
+    g_buf_epoch.fetch_add(1);
     size_t want = std::max(bytes, (size_t)256);
     hipError_t e = hipMalloc(&p, want);
     if (e == hipSuccess) cap = want;
@@ -93,6 +104,26 @@ struct DevBuf {
     cap = 0;
   }
   template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Identity of a device scan's launch sequence: every kernel argument follows
+// from these, so an equal key can replay a captured graph.
+struct GraphKey {
+  uint64_t pat_serial = 0, len = 0, chunk = 0, overlap = 0, cap = 0, epoch = 0;
+  const void* data = nullptr;
+  const void* out = nullptr;
+  uint32_t out_mem = 0, variant = 0, lane = 0, bpc = 0, profiling = 0;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(pat_serial, len, chunk, overlap, cap, epoch, data, out, out_mem, variant, lane, bpc, profiling) <
+           std::tie(o.pat_serial, o.len, o.chunk, o.overlap, o.cap, o.epoch, o.data, o.out, o.out_mem, o.variant,
+                    o.lane, o.bpc, o.profiling);
+  }
+};
+
+struct GraphEntry {
+  hipGraphExec_t exec = nullptr;
+  uint32_t seen = 0;
+  bool broken = false;  // capture failed once: stay eager
 };
 
 }  // namespace
@@ -112,12 +143,16 @@ struct ws_ctx {
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
       scratch, carried, ids, hits, hwin, slice_counts, lslices, cslices, scounts, bcnt, bcur, dstate;
   std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
+  uint8_t* hstage = nullptr;    // pinned staging read by captured copies
+  size_t hstage_cap = 0;
+  std::map<GraphKey, GraphEntry> graphs;
   uint32_t* hcount = nullptr;  // pinned host mirror of the counters
   uint32_t last_hits = 0;      // filter hits of the last hash-variant scan
 };
 
 struct ws_patterns {
   int dev = 0;
+  uint64_t serial = 0;  // unique per compiled set (keys captured graphs)
   PatternsHost h;
   DevBuf blob;
   // Device pointers into `blob`.
@@ -855,6 +890,19 @@ ws_status plan_single(ws_ctx* ctx, const uint8_t* data, uint64_t len, uint32_t d
 // redone on the checked path, whose buffers grow on demand) and appends the
 // run's match count to the output offset.
 // ---------------------------------------------------------------------------
+// Grows the pinned staging area (reallocation makes captured graphs stale).
+hipError_t ensure_stage(ws_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->hstage_cap) return hipSuccess;
+  if (ctx->hstage) (void)hipHostFree(ctx->hstage);
+  ctx->hstage = nullptr;
+  ctx->hstage_cap = 0;
+  g_buf_epoch.fetch_add(1);
+  const size_t want = std::max<size_t>(bytes, 4096);
+  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hstage), want, hipHostMallocDefault);
+  if (e == hipSuccess) ctx->hstage_cap = want;
+  return e;
+}
+
 struct DeviceRunState {
   void* out = nullptr;       // device ws_match array
   uint64_t out_cap = 0;
@@ -866,113 +914,127 @@ uint32_t bucket_bits(uint32_t keybits, uint32_t cap) {
   return std::min<uint32_t>(keybits, std::min(22u, std::max(10u, bits_for(cap) + 1)));
 }
 
-ws_status enqueue_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const ws_tuning* tuning,
-                      uint32_t start_off, const DeviceRunState& ds, uint32_t chunk) {
+// Everything a device-sized run needs at launch time, decided (and every
+// buffer sized) before any launch, so the launches can be captured.
+struct RunSetup {
+  uint32_t variant = 0, lane_bytes = 0, nunits = 0, grid = 0, fgrid = 0, nvb = 0;
+  uint32_t lit_cap = 0, cand_cap = 0, raw_cap = 0, rec_cap = 0, max_matches = 0;
+  uint64_t hit_total = 0, slice_total = 0;
+  uint32_t sbits = 0, pbits = 0, rbits = 0, obits = 0, cbits = 0, ob = 0, cb = 0;
+  size_t items_off = 0;  // this run's items in the pinned staging area (uint2 units)
+};
+
+ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const ws_tuning* tuning,
+                      RunSetup* rs) {
   const PatternsHost& h = p->h;
-  const hipStream_t st = ctx->stream;
-  const uint32_t variant = pick_variant(p, tuning);
-  if (!(supported_variants(p) & (1u << variant)))
+  if (plan.batch) return fail(WS_ERR_INVALID_ARGUMENT, "device-sized runs take single segments");
+  rs->variant = pick_variant(p, tuning);
+  if (!(supported_variants(p) & (1u << rs->variant)))
     return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
-  const uint32_t lane_bytes = choose_lane_bytes(plan, tuning);
-  const uint32_t unit = variant == WS_VARIANT_HASH ? kTPB * lane_bytes : ac_unit_bytes();
-  std::vector<uint2> units;
-  const uint32_t nunits = plan_units(plan, unit, &units);
-  WS_HIP(ctx->items.ensure(plan.items.size() * sizeof(uint2)));
-  WS_HIP(hipMemcpyAsync(ctx->items.p, plan.items.data(), plan.items.size() * sizeof(uint2),
-                        hipMemcpyHostToDevice, st));
-  if (plan.batch && !units.empty()) {
-    WS_HIP(ctx->units.ensure(units.size() * sizeof(uint2)));
-    WS_HIP(hipMemcpyAsync(ctx->units.p, units.data(), units.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
-  }
-  uint64_t span = 0;
-  for (auto& it : plan.items) span += it.y - it.x;
-  // Capacities.
-  const uint32_t max_matches = (uint32_t)std::min<uint64_t>(ctx->cfg.max_matches, 0xFFFFFFF0ull);
-  const uint32_t lit_cap = (uint32_t)std::min<uint64_t>(max_matches, std::max<uint64_t>(1u << 16, span / 16));
-  const uint32_t cand_cap = (uint32_t)std::min<uint64_t>(0x0FFFFFFFull, std::max<uint64_t>(1u << 16, span / 64));
+  rs->lane_bytes = choose_lane_bytes(plan, tuning);
+  const uint32_t unit = rs->variant == WS_VARIANT_HASH ? kTPB * rs->lane_bytes : ac_unit_bytes();
+  std::vector<uint2> none;
+  rs->nunits = plan_units(plan, unit, &none);
+  const uint64_t span = plan.items[0].y - plan.items[0].x;
+  rs->max_matches = (uint32_t)std::min<uint64_t>(ctx->cfg.max_matches, 0xFFFFFFF0ull);
+  rs->lit_cap = (uint32_t)std::min<uint64_t>(rs->max_matches, std::max<uint64_t>(1u << 16, span / 16));
+  rs->cand_cap = (uint32_t)std::min<uint64_t>(0x0FFFFFFFull, std::max<uint64_t>(1u << 16, span / 64));
   const uint32_t nrx = (uint32_t)h.rx.size();
-  const uint32_t raw_cap = nrx ? cand_cap : 0;
-  const uint32_t rec_cap = lit_cap + raw_cap;
-  WS_HIP(ctx->recs.ensure((size_t)rec_cap * sizeof(Rec)));
-  WS_HIP(ctx->cands.ensure((size_t)cand_cap * sizeof(Rec)));
-  if (nrx) WS_HIP(ctx->raw.ensure((size_t)raw_cap * sizeof(Rec)));
-  WS_HIP(ctx->k0.ensure((size_t)rec_cap * 8));
-  WS_HIP(ctx->k1.ensure((size_t)rec_cap * 8));
-  WS_HIP(ctx->v0.ensure((size_t)rec_cap * 4));
-  WS_HIP(ctx->v1.ensure((size_t)rec_cap * 4));
-  if (nrx) WS_HIP(ctx->head.ensure(raw_cap));
+  rs->raw_cap = nrx ? rs->cand_cap : 0;
+  rs->rec_cap = rs->lit_cap + rs->raw_cap;
+  WS_HIP(ctx->items.ensure(8 * sizeof(uint2)));
+  WS_HIP(ctx->recs.ensure((size_t)rs->rec_cap * sizeof(Rec)));
+  WS_HIP(ctx->cands.ensure((size_t)rs->cand_cap * sizeof(Rec)));
+  if (nrx) WS_HIP(ctx->raw.ensure((size_t)rs->raw_cap * sizeof(Rec)));
+  WS_HIP(ctx->k0.ensure((size_t)rs->rec_cap * 8));
+  WS_HIP(ctx->k1.ensure((size_t)rs->rec_cap * 8));
+  WS_HIP(ctx->v0.ensure((size_t)rs->rec_cap * 4));
+  WS_HIP(ctx->v1.ensure((size_t)rs->rec_cap * 4));
+  if (nrx) WS_HIP(ctx->head.ensure(rs->raw_cap));
   const uint32_t per_cu = tuning && tuning->blocks_per_cu ? tuning->blocks_per_cu : 4;
-  const uint32_t grid = std::max(1u, std::min(nunits, ctx->cus * per_cu));
-  const uint32_t fgrid = std::max(1u, std::min((nunits + 7) / 8, ctx->cus * (per_cu > 4 ? 2u : 1u)));
-  const uint32_t nvb = verify_blocks(fgrid);
-  const uint64_t hit_total = std::min<uint64_t>(std::max<uint64_t>(1u << 20, span / 64), 0x3FFFFFFFull);
-  const uint64_t slice_total = (uint64_t)nvb * 256 + (1u << 16);
-  if (variant == WS_VARIANT_HASH) {
-    WS_HIP(ctx->hits.ensure(hit_total * sizeof(uint2)));
-    WS_HIP(ctx->hwin.ensure(hit_total * 2 * sizeof(uint4)));
-    WS_HIP(ctx->slice_counts.ensure((size_t)fgrid * 4));
-    WS_HIP(ctx->lslices.ensure(slice_total * sizeof(Rec)));
-    WS_HIP(ctx->cslices.ensure(slice_total * sizeof(Rec)));
-    WS_HIP(ctx->scounts.ensure(((size_t)nvb * 2 + 4) * 4 * 2));
+  rs->grid = std::max(1u, std::min(rs->nunits, ctx->cus * per_cu));
+  rs->fgrid = std::max(1u, std::min((rs->nunits + 7) / 8, ctx->cus * (per_cu > 4 ? 2u : 1u)));
+  rs->nvb = verify_blocks(rs->fgrid);
+  rs->hit_total = std::min<uint64_t>(std::max<uint64_t>(1u << 20, span / 64), 0x3FFFFFFFull);
+  rs->slice_total = (uint64_t)rs->nvb * 256 + (1u << 16);
+  if (rs->variant == WS_VARIANT_HASH) {
+    WS_HIP(ctx->hits.ensure(rs->hit_total * sizeof(uint2)));
+    WS_HIP(ctx->hwin.ensure(rs->hit_total * 2 * sizeof(uint4)));
+    WS_HIP(ctx->slice_counts.ensure((size_t)rs->fgrid * 4));
+    WS_HIP(ctx->lslices.ensure(rs->slice_total * sizeof(Rec)));
+    WS_HIP(ctx->cslices.ensure(rs->slice_total * sizeof(Rec)));
+    WS_HIP(ctx->scounts.ensure(((size_t)rs->nvb * 2 + 4) * 4 * 2));
   }
-  const uint32_t sbits = bits_for(plan.max_seg_len);
-  const uint32_t pbits = bits_for(h.npat ? h.npat - 1 : 0);
-  const uint32_t rbits = nrx ? bits_for(nrx - 1) : 0;
-  const uint32_t ibits = plan.batch ? bits_for(plan.items.size() - 1) : 0;
-  const uint32_t obits = sbits + pbits + ibits, cbits = sbits + rbits + ibits;
-  if (obits > 63 || cbits > 63) return fail(WS_ERR_INPUT_TOO_LARGE, "match key does not fit 63 bits");
+  rs->sbits = bits_for(plan.max_seg_len);
+  rs->pbits = bits_for(h.npat ? h.npat - 1 : 0);
+  rs->rbits = nrx ? bits_for(nrx - 1) : 0;
+  rs->obits = rs->sbits + rs->pbits;
+  rs->cbits = rs->sbits + rs->rbits;
+  if (rs->obits > 63 || rs->cbits > 63) return fail(WS_ERR_INPUT_TOO_LARGE, "match key does not fit 63 bits");
   // Bucket counts follow the capacities up to 2^18 buckets (a few pairs per
   // bucket at a million matches); denser runs fall back through the
   // overflow flag.
-  const uint32_t ob = bucket_bits(obits, std::min(rec_cap, 1u << 17));
-  const uint32_t cb = nrx ? bucket_bits(cbits, std::min(raw_cap, 1u << 17)) : 0;
-  const uint32_t nb = 1u << std::max(ob, cb);
+  rs->ob = bucket_bits(rs->obits, std::min(rs->rec_cap, 1u << 17));
+  rs->cb = nrx ? bucket_bits(rs->cbits, std::min(rs->raw_cap, 1u << 17)) : 0;
+  const uint32_t nb = 1u << std::max(rs->ob, rs->cb);
   WS_HIP(ctx->bcnt.ensure((size_t)nb * 4));
   WS_HIP(ctx->bcur.ensure((size_t)nb * 4));
   size_t need = 0, need2 = 0;
-  WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, rec_cap, nullptr, 0, 1u << ob,
-                           nullptr, nullptr, nullptr, st));
+  WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, rs->rec_cap, nullptr, 0,
+                           1u << rs->ob, nullptr, nullptr, nullptr, ctx->stream));
   if (nrx)
-    WS_HIP(bucket_sort_pairs(nullptr, &need2, nullptr, nullptr, nullptr, nullptr, raw_cap, nullptr, 0, 1u << cb,
-                             nullptr, nullptr, nullptr, st));
+    WS_HIP(bucket_sort_pairs(nullptr, &need2, nullptr, nullptr, nullptr, nullptr, rs->raw_cap, nullptr, 0,
+                             1u << rs->cb, nullptr, nullptr, nullptr, ctx->stream));
   WS_HIP(ctx->tmp.ensure(std::max(need, need2)));
-  size_t have = ctx->tmp.cap;
-  uint32_t* cnt = ctx->counters.as<uint32_t>();
-  hipEvent_t* ev = ctx->profiling ? &ctx->pev[5 * chunk] : nullptr;
+  if (rs->variant == WS_VARIANT_HASH) {
+    // Dynamic LDS attribute of the filter kernel: set outside any capture.
+    WS_HIP(prepare_hash(h.K, h.ci, rs->lane_bytes, (uint32_t)h.bloom.size()));
+  } else {
+    WS_HIP(prepare_ac(rs->variant == WS_VARIANT_DFA_LDS, h.ac_states, h.ac_classes));
+  }
+  return WS_OK;
+}
 
+// Launches one prepared run: stream operations only (capturable).
+ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const RunSetup& rs, uint32_t start_off,
+                     const DeviceRunState& ds, hipEvent_t* ev) {
+  const PatternsHost& h = p->h;
+  const hipStream_t st = ctx->stream;
+  const uint32_t nrx = (uint32_t)h.rx.size();
+  uint32_t* cnt = ctx->counters.as<uint32_t>();
+  WS_HIP(hipMemcpyAsync(ctx->items.p, ctx->hstage + rs.items_off * sizeof(uint2), sizeof(uint2),
+                        hipMemcpyHostToDevice, st));
   // Filter and verification.
   if (ev) WS_HIP(hipEventRecord(ev[0], st));
   WS_HIP(launch_fill_u32(cnt, 11, 0, st));
-  ScanCommon c = make_common(ctx, p, plan, nunits, lit_cap, cand_cap);
+  ScanCommon c = make_common(ctx, p, plan, rs.nunits, rs.lit_cap, rs.cand_cap);
   RunCaps caps{};
-  caps.lit_cap = lit_cap;
-  caps.cand_cap = cand_cap;
-  caps.raw_cap = raw_cap;
-  caps.rec_cap = rec_cap;
-  caps.max_matches = max_matches;
+  caps.lit_cap = rs.lit_cap;
+  caps.cand_cap = rs.cand_cap;
+  caps.raw_cap = rs.raw_cap;
+  caps.rec_cap = rs.rec_cap;
+  caps.max_matches = rs.max_matches;
   caps.hit_ovf_cap = caps.lit_ovf_cap = caps.cand_ovf_cap = 0xFFFFFFFFu;
-  if (nunits > 0) {
-    if (variant == WS_VARIANT_HASH) {
-      const HashParams hp = hash_params(p, lane_bytes);
-      const HitSlices hs = make_hits(ctx, hit_total, fgrid);
-      uint32_t* sc = ctx->scounts.as<uint32_t>();
-      SliceOut lso = make_slices(ctx->lslices, slice_total, nvb, sc, cnt + 5);
-      SliceOut cso = make_slices(ctx->cslices, slice_total, nvb, sc + nvb, cnt + 6);
-      WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, false));
-      if (ev) WS_HIP(hipEventRecord(ev[1], st));
-      WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, true));
-      uint32_t* offs = sc + 2 * nvb;
-      WS_HIP(launch_slice_compact(lso, nvb, offs, cnt + 0, c.lit_out, c.lit_cap, st));
-      WS_HIP(launch_slice_compact(cso, nvb, offs + nvb + 2, cnt + 1, c.cand_out, c.cand_cap, st));
-      caps.hit_ovf_cap = hs.overflow_cap;
-      caps.lit_ovf_cap = lso.ovf_cap;
-      caps.cand_ovf_cap = cso.ovf_cap;
-    } else {
-      AcParams ap{p->ac_table32, p->ac_table16, p->ac_cmap, p->ac_out_start, p->ac_out_list,
-                  p->ac_len, h.ac_states, h.ac_classes, h.ac_first_out, h.ac_maxlen};
-      WS_HIP(launch_ac(variant == WS_VARIANT_DFA_LDS, grid, st, c, ap));
-      if (ev) WS_HIP(hipEventRecord(ev[1], st));
-    }
+  if (rs.nunits > 0 && rs.variant == WS_VARIANT_HASH) {
+    const HashParams hp = hash_params(p, rs.lane_bytes);
+    const HitSlices hs = make_hits(ctx, rs.hit_total, rs.fgrid);
+    uint32_t* sc = ctx->scounts.as<uint32_t>();
+    SliceOut lso = make_slices(ctx->lslices, rs.slice_total, rs.nvb, sc, cnt + 5);
+    SliceOut cso = make_slices(ctx->cslices, rs.slice_total, rs.nvb, sc + rs.nvb, cnt + 6);
+    WS_HIP(launch_hash(h.K, h.ci, rs.fgrid, st, c, hp, hs, lso, cso, false));
+    if (ev) WS_HIP(hipEventRecord(ev[1], st));
+    WS_HIP(launch_hash(h.K, h.ci, rs.fgrid, st, c, hp, hs, lso, cso, true));
+    uint32_t* offs = sc + 2 * rs.nvb;
+    WS_HIP(launch_slice_compact(lso, rs.nvb, offs, cnt + 0, c.lit_out, c.lit_cap, st));
+    WS_HIP(launch_slice_compact(cso, rs.nvb, offs + rs.nvb + 2, cnt + 1, c.cand_out, c.cand_cap, st));
+    caps.hit_ovf_cap = hs.overflow_cap;
+    caps.lit_ovf_cap = lso.ovf_cap;
+    caps.cand_ovf_cap = cso.ovf_cap;
+  } else if (rs.nunits > 0) {
+    AcParams ap{p->ac_table32, p->ac_table16, p->ac_cmap, p->ac_out_start, p->ac_out_list,
+                p->ac_len, h.ac_states, h.ac_classes, h.ac_first_out, h.ac_maxlen};
+    WS_HIP(launch_ac(rs.variant == WS_VARIANT_DFA_LDS, rs.grid, st, c, ap));
+    if (ev) WS_HIP(hipEventRecord(ev[1], st));
   } else if (ev) {
     WS_HIP(hipEventRecord(ev[1], st));
   }
@@ -987,41 +1049,42 @@ ws_status enqueue_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
     w.items = ctx->items.as<uint2>();
     w.cands = ctx->cands.as<Rec>();
     w.ncand = cnt + 1;
-    w.cand_cap = cand_cap;
+    w.cand_cap = rs.cand_cap;
     w.tables = p->rx_tables;
     w.toff = p->rx_toff;
     w.ncls = p->rx_ncls;
     w.cmaps = p->rx_cmaps;
     w.rx_pid = p->rx_pid;
     w.raw = ctx->raw.as<Rec>();
-    w.raw_cap = raw_cap;
+    w.raw_cap = rs.raw_cap;
     w.counters = cnt;
     w.window_truncated = plan.window_truncated ? 1u : 0u;
     WS_HIP(launch_walk(0, st, w));
-    WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), raw_cap, cnt + 2, sbits, rbits, ctx->k0.as<uint64_t>(),
+    WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), rs.raw_cap, cnt + 2, rs.sbits, rs.rbits, ctx->k0.as<uint64_t>(),
                              ctx->v0.as<uint32_t>(), st));
+    size_t have = ctx->tmp.cap;
     WS_HIP(bucket_sort_pairs(ctx->tmp.p, &have, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
-                             ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), raw_cap, cnt + 2, cbits - cb, 1u << cb,
-                             ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, st));
-    WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), raw_cap,
-                              cnt + 2, sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st));
-    WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), raw_cap,
-                             cnt + 2, sbits, ctx->head.as<uint8_t>(),
+                             ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), rs.raw_cap, cnt + 2, rs.cbits - rs.cb,
+                             1u << rs.cb, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, st));
+    WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), rs.raw_cap,
+                              cnt + 2, rs.sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st));
+    WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), rs.raw_cap,
+                             cnt + 2, rs.sbits, ctx->head.as<uint8_t>(),
                              plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, plan.seg_off, p->rx_pid, 0u,
-                             ctx->recs.as<Rec>(), rec_cap, cnt + 0,
+                             ctx->recs.as<Rec>(), rs.rec_cap, cnt + 0,
                              plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, st));
   }
   if (ev) WS_HIP(hipEventRecord(ev[3], st));
 
-  // Order by (item, start, pattern id), pack at the running output offset.
-  WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), rec_cap, cnt + 0, sbits, pbits, ctx->k0.as<uint64_t>(),
+  // Order by (start, pattern id), pack at the running output offset.
+  WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), rs.rec_cap, cnt + 0, rs.sbits, rs.pbits, ctx->k0.as<uint64_t>(),
                            ctx->v0.as<uint32_t>(), st));
-  have = ctx->tmp.cap;
+  size_t have = ctx->tmp.cap;
   WS_HIP(bucket_sort_pairs(ctx->tmp.p, &have, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
-                           ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), rec_cap, cnt + 0, obits - ob, 1u << ob,
-                           ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, st));
-  WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), rec_cap, cnt + 0, pbits, start_off,
-                             ds.out, ds.out_cap, ds.total, st));
+                           ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), rs.rec_cap, cnt + 0, rs.obits - rs.ob,
+                           1u << rs.ob, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, st));
+  WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), rs.rec_cap, cnt + 0, rs.pbits,
+                             start_off, ds.out, ds.out_cap, ds.total, st));
   WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, st));
   if (ev) WS_HIP(hipEventRecord(ev[4], st));
   return WS_OK;
@@ -1036,21 +1099,22 @@ bool device_path_ok(uint32_t mode, const ws_tuning* tuning) {
 }
 
 // Runs the plans (one per chunk) as device-sized runs into `out` and reports
-// the total. *outcome = kDeviceScanRedo when a buffer overflowed; nothing was
+// the total. A launch sequence seen before is captured once into a hipGraph
+// and replayed from then on (the steady state of repeated scans and
+// streams). *outcome = kDeviceScanRedo when a buffer overflowed; nothing was
 // then written to a host `out`.
 ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>& plans,
                       const std::vector<uint32_t>& offs, const ws_tuning* tuning, ws_match* out, uint64_t cap,
-                      uint32_t out_mem, uint64_t* count, DeviceScanResult* outcome) {
+                      uint32_t out_mem, uint64_t* count, DeviceScanResult* outcome, const GraphKey* key_in) {
   const hipStream_t st = ctx->stream;
   *outcome = kDeviceScanDone;
+  // Everything sized and staged before the first launch.
+  std::vector<RunSetup> setups(plans.size());
   WS_HIP(ctx->dstate.ensure(64));
-  uint32_t* sticky = ctx->dstate.as<uint32_t>();
-  auto* total = reinterpret_cast<unsigned long long*>(sticky + 8);
-  WS_HIP(launch_fill_u32(sticky, 10, 0, st));
-  DeviceRunState ds;
-  ds.sticky = sticky;
-  ds.total = total;
   const uint64_t max_matches = ctx->cfg.max_matches;
+  DeviceRunState ds;
+  ds.sticky = ctx->dstate.as<uint32_t>();
+  ds.total = reinterpret_cast<unsigned long long*>(ds.sticky + 8);
   if (out_mem == WS_MEM_DEVICE) {
     ds.out = out;
     ds.out_cap = cap;
@@ -1059,16 +1123,67 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     WS_HIP(ctx->outbuf.ensure(std::max<uint64_t>(ds.out_cap, 1) * sizeof(ws_match)));
     ds.out = ctx->outbuf.p;
   }
+  WS_HIP(ensure_stage(ctx, plans.size() * sizeof(uint2)));
+  for (size_t k = 0; k < plans.size(); ++k) {
+    ws_status s = prepare_run(ctx, p, plans[k], tuning, &setups[k]);
+    if (s != WS_OK) return s;
+    setups[k].items_off = k;
+    std::memcpy(ctx->hstage + k * sizeof(uint2), plans[k].items.data(), sizeof(uint2));
+  }
   if (ctx->profiling && ctx->pev.size() < 5 * plans.size()) {
     const size_t had = ctx->pev.size();
     ctx->pev.resize(5 * plans.size(), nullptr);
     for (size_t k = had; k < ctx->pev.size(); ++k) WS_HIP(hipEventCreate(&ctx->pev[k]));
   }
-  for (size_t k = 0; k < plans.size(); ++k) {
-    ws_status s = enqueue_run(ctx, p, plans[k], tuning, offs[k], ds, (uint32_t)k);
+  auto launch_all = [&]() -> ws_status {
+    WS_HIP(launch_fill_u32(ds.sticky, 10, 0, st));
+    for (size_t k = 0; k < plans.size(); ++k) {
+      ws_status s = launch_run(ctx, p, plans[k], setups[k], offs[k], ds,
+                               ctx->profiling ? &ctx->pev[5 * k] : nullptr);
+      if (s != WS_OK) return s;
+    }
+    return WS_OK;
+  };
+  GraphEntry* ge = nullptr;
+  GraphKey key;
+  if (key_in) {
+    key = *key_in;
+    key.epoch = g_buf_epoch.load();
+    ge = &ctx->graphs[key];
+  }
+  if (ge && ge->exec) {
+    WS_HIP(hipGraphLaunch(ge->exec, st));
+  } else if (ge && ge->seen >= 1 && !ge->broken) {
+    // Second sighting: capture and replay from now on.
+    if (ctx->graphs.size() > 64) {
+      for (auto& kv : ctx->graphs)
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+      ctx->graphs.clear();
+      ge = &ctx->graphs[key];
+      ge->seen = 1;
+    }
+    hipGraph_t graph = nullptr;
+    WS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    ws_status s = launch_all();
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    hipGraphExec_t exec = nullptr;
+    if (s == WS_OK && e == hipSuccess && graph) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (s != WS_OK || e != hipSuccess || !exec) {
+      (void)hipGetLastError();
+      ge->broken = true;
+      ws_status s2 = launch_all();
+      if (s2 != WS_OK) return s2;
+    } else {
+      ge->exec = exec;
+      WS_HIP(hipGraphLaunch(exec, st));
+    }
+  } else {
+    if (ge) ge->seen++;
+    ws_status s = launch_all();
     if (s != WS_OK) return s;
   }
-  WS_HIP(hipMemcpyAsync(ctx->hcount + 11, sticky, 10 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  WS_HIP(hipMemcpyAsync(ctx->hcount + 11, ds.sticky, 10 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   WS_HIP(hipStreamSynchronize(st));
   const uint32_t* hs = ctx->hcount + 11;
   if (hs[kStickyRedo]) {
@@ -1147,6 +1262,14 @@ extern "C" {
 
 uint32_t ws_abi_version(void) { return WS_ABI_VERSION; }
 
+uint32_t ws_ctx_graph_count(ws_ctx* ctx) {
+  if (!ctx) return 0;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  uint32_t n = 0;
+  for (auto& kv : ctx->graphs) n += kv.second.exec ? 1u : 0u;
+  return n;
+}
+
 const char* ws_last_error(void) { return g_err.c_str(); }
 
 ws_status ws_ctx_create(int device_ordinal, const ws_config* cfg, ws_ctx** out) {
@@ -1204,6 +1327,9 @@ void ws_ctx_destroy(ws_ctx* ctx) {
     if (ev) (void)hipEventDestroy(ev);
   for (auto& ev : ctx->pev)
     if (ev) (void)hipEventDestroy(ev);
+  for (auto& kv : ctx->graphs)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+  if (ctx->hstage) (void)hipHostFree(ctx->hstage);
   if (ctx->hcount) (void)hipHostFree(ctx->hcount);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1236,6 +1362,7 @@ ws_status ws_patterns_compile(ws_ctx* ctx, const ws_pattern* pats, size_t count,
     src[i].bytes.assign(reinterpret_cast<const char*>(pats[i].bytes), pats[i].len);
   }
   ws_patterns* p = new ws_patterns();
+  p->serial = g_pattern_serial.fetch_add(1);
   try {
     p->h = compile_patterns(src, (flags & WS_FLAG_CASE_INSENSITIVE) != 0);
   } catch (const CompileError& e) {
@@ -1340,7 +1467,21 @@ ws_status ws_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64
   if (s != WS_OK) return s;
   if (device_path_ok(mode, tuning)) {
     DeviceScanResult oc;
-    s = device_scan(ctx, p, {plan}, {0u}, tuning, out, cap, out_mem, count, &oc);
+    GraphKey key;
+    key.pat_serial = p->serial;
+    key.data = data;
+    key.len = len;
+    key.cap = cap;
+    key.out = out_mem == WS_MEM_DEVICE ? out : nullptr;  // host output is copied after the launches
+    key.out_mem = out_mem;
+    key.profiling = ctx->profiling ? 1u : 0u;
+    if (tuning) {
+      key.variant = tuning->variant;
+      key.lane = tuning->bytes_per_lane;
+      key.bpc = tuning->blocks_per_cu;
+    }
+    s = device_scan(ctx, p, {plan}, {0u}, tuning, out, cap, out_mem, count, &oc,
+                    data_mem == WS_MEM_DEVICE ? &key : nullptr);
     if (s != WS_OK || oc == kDeviceScanDone) return s;
     ctx->last = ws_timings{};  // a buffer overflowed: redo on the checked path
   }
@@ -1403,7 +1544,22 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
     }
     if (data_mem != WS_MEM_HOST) {
       DeviceScanResult oc;
-      ws_status s = device_scan(ctx, p, plans, offs, tuning, out, cap, out_mem, count, &oc);
+      GraphKey key;
+      key.pat_serial = p->serial;
+      key.data = data;
+      key.len = len;
+      key.chunk = chunk_size;
+      key.overlap = overlap;
+      key.cap = cap;
+      key.out = out_mem == WS_MEM_DEVICE ? out : nullptr;
+      key.out_mem = out_mem;
+      key.profiling = ctx->profiling ? 1u : 0u;
+      if (tuning) {
+        key.variant = tuning->variant;
+        key.lane = tuning->bytes_per_lane;
+        key.bpc = tuning->blocks_per_cu;
+      }
+      ws_status s = device_scan(ctx, p, plans, offs, tuning, out, cap, out_mem, count, &oc, &key);
       if (s != WS_OK || oc == kDeviceScanDone) return s;
       ctx->last = ws_timings{};
     }

@@ -10,6 +10,9 @@
 //  k_chain_*     per-group greedy non-overlap chains (find_iter semantics).
 //  k_*_keys/pack ordering keys and output packing.
 #include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -861,14 +864,43 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
 // ---------------------------------------------------------------------------
 // Host launch wrappers (the runtime in engine.cpp only calls these).
 // ---------------------------------------------------------------------------
+// Raises a kernel's dynamic LDS limit once (remembered per kernel), so
+// launches inside a graph capture make no attribute calls.
+static std::mutex g_lds_mu;
+static std::map<const void*, size_t> g_lds_set;
+
 template <typename F>
 static hipError_t allow_lds(F* fn, size_t bytes) {
   if (bytes <= 65536) return hipSuccess;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  const void* key = reinterpret_cast<const void*>(fn);
+  std::lock_guard<std::mutex> lk(g_lds_mu);
+  auto it = g_lds_set.find(key);
+  if (it != g_lds_set.end() && it->second >= bytes) return hipSuccess;
+  hipError_t e = hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) g_lds_set[key] = bytes;
+  return e;
 }
 
 size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u; }
+
+template <int K, bool CI>
+static hipError_t prepare_hash_kc(uint32_t lane_bytes, size_t shmem) {
+  if (lane_bytes == 64) return allow_lds(k_hash_filter<K, CI, 64>, shmem);
+  if (lane_bytes == 32) return allow_lds(k_hash_filter<K, CI, 32>, shmem);
+  return allow_lds(k_hash_filter<K, CI, 16>, shmem);
+}
+
+// Sets the filter kernel's dynamic LDS limit ahead of launches that may be
+// captured into a graph.
+hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords) {
+  const size_t shmem = hash_shmem_bytes(nwords);
+  switch (K) {
+#define WS_K(k) case k: return ci ? prepare_hash_kc<k, true>(lane_bytes, shmem) : prepare_hash_kc<k, false>(lane_bytes, shmem);
+    WS_K(1) WS_K(2) WS_K(3) WS_K(4) WS_K(6) WS_K(8) WS_K(12) WS_K(16)
+#undef WS_K
+    default: return hipErrorInvalidValue;
+  }
+}
 
 template <int K, bool CI>
 static hipError_t launch_hash_kc(uint32_t grid, hipStream_t st, const ScanCommon& c, const HashParams& h,
@@ -921,6 +953,11 @@ size_t ac_shmem_bytes(bool lds_table, uint32_t states, uint32_t classes) {
   size_t b = kTPB * kAcChunk + 16 + 256 + 2 * kRecCap * sizeof(Rec);
   if (lds_table) b += (size_t)states * classes * 2;
   return b;
+}
+
+hipError_t prepare_ac(bool lds_table, uint32_t states, uint32_t classes) {
+  const size_t shmem = ac_shmem_bytes(lds_table, states, classes);
+  return lds_table ? allow_lds(k_ac_scan<true, kAcChunk>, shmem) : allow_lds(k_ac_scan<false, kAcChunk>, shmem);
 }
 
 hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCommon& c,
