@@ -567,8 +567,23 @@ ScanCommon make_common(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint
 
 HashParams hash_params(const ws_patterns* p, uint32_t lane_bytes) {
   const PatternsHost& h = p->h;
-  return HashParams{p->bloom, (uint32_t)h.bloom.size(), h.wshift, h.wbits, h.B, h.NBK, h.nbits, h.s2,
-                    lane_bytes, p->vtab, p->vmore, p->fmeta};
+  HashParams hp{};
+  hp.bloom = p->bloom;
+  hp.nwords = (uint32_t)h.bloom.size();
+  hp.wshift = h.wshift;
+  hp.wbits = h.wbits;
+  hp.B = h.B;
+  hp.NBK = h.NBK;
+  hp.nbits = h.nbits;
+  hp.s2 = h.s2;
+  hp.lane_bytes = lane_bytes;
+  hp.BB = h.BB;
+  hp.NBKK = h.NBKK;
+  hp.amask = (hp.nwords - 1u) << 2;
+  hp.vtab = p->vtab;
+  hp.vmore = p->vmore;
+  hp.fmeta = p->fmeta;
+  return hp;
 }
 
 ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t mode,

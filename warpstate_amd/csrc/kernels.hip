@@ -177,6 +177,43 @@ __device__ __forceinline__ void filter_lane(const HashParams& h, const uint32_t*
   if (PL < 32) out[0] = mask >> (32 - PL);
 }
 
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk2(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ u16x2 upk2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+
+// K >= 3: the same filter over the 32-bit dual hash D (two 16-bit
+// polynomial hashes rolled together by v_pk_mad_u16). The bloom word's byte
+// address is D & amask, its two bits D[20:16] and D[28:24], which the shifts
+// take straight from the upper half and top byte of D (SDWA operand
+// selects): per position one byte extract, two packed multiply-adds, one
+// address op, the LDS read, two shifts, an and and the mask update.
+template <int K, bool CI, int PL>
+__device__ __forceinline__ void filter_lane_dual(const HashParams& h, const uint32_t* bloom, uint32_t* d,
+                                                 uint32_t* out) {
+  if (CI) {
+#pragma unroll
+    for (int j = 0; j < PL / 4 + 4; ++j) d[j] = fold_word(d[j]);
+  }
+  const u16x2 bb = upk2(h.BB), nbk = upk2(h.NBKK);
+  auto splat = [](uint32_t b) { return u16x2{(unsigned short)b, (unsigned short)b}; };
+  uint32_t D = pk2(splat(byte_of(d, 0))), mask = 0;
+#pragma unroll
+  for (int t = 1; t < K; ++t) D = pk2(upk2(D) * bb + splat(byte_of(d, t)));
+  // The bloom words sit at LDS address 0 (k_hash_filter has no static LDS),
+  // so the masked hash is the ds_read address as it stands.
+  (void)bloom;
+  const __attribute__((address_space(3))) char* lds = nullptr;
+#pragma unroll
+  for (int i = 0; i < PL; ++i) {
+    if (i > 0) D = pk2(upk2(D) * bb + (splat(byte_of(d, i - 1)) * nbk + splat(byte_of(d, i + K - 1))));
+    const uint32_t w = *reinterpret_cast<const __attribute__((address_space(3))) uint32_t*>(lds + (D & h.amask));
+    const uint32_t hit = (w >> ((D >> 16) & 31u)) & (w >> ((D >> 24) & 31u));
+    mask = __builtin_amdgcn_alignbit(hit, mask, 1);
+    if ((i & 31) == 31) out[i >> 5] = mask;
+  }
+  if (PL < 32) out[0] = mask >> (32 - PL);
+}
+
 // Valid positions q of [p0, p0 + n) (n <= 32): lo <= q, q + K <= hi,
 // q - lo < emit_end.
 __device__ __forceinline__ uint32_t valid_word(uint32_t p0, uint32_t n, uint32_t lo, uint32_t hi, uint32_t K,
@@ -206,12 +243,16 @@ constexpr int kSubBlocks = kFilterTPB / kTPB;      // 256-lane sub-blocks, one u
 // into a shared overflow area.
 template <int K, bool CI, int PL>
 __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashParams h, HitSlices hs) {
+  // LDS: the bloom words from address 0 (the kernel has no static LDS, so
+  // a word's address is the masked hash itself), then the block's hit
+  // counter and work-queue slot.
   extern __shared__ uint32_t smem[];
   uint32_t* bloom = smem;
+  uint32_t& s_n = smem[h.nwords];
+  uint32_t* s_slot = &smem[h.nwords + 1];
   constexpr int NW = PL / 4 + 4;  // window dwords
   constexpr uint32_t kUnit = kTPB * PL;
   const uint32_t sub = threadIdx.x / kTPB, ltid = threadIdx.x % kTPB;
-  __shared__ uint32_t s_n;
   uint2* slice = hs.hits + (size_t)blockIdx.x * hs.slice_cap;
   uint4* win = hs.win + 2 * (size_t)blockIdx.x * hs.slice_cap;
 
@@ -222,10 +263,9 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
   const uint32_t load_end = (uint32_t)c.load_end;
   constexpr uint32_t kPerGroup = kSubBlocks * kUnitsPerIter;
   const uint32_t groups = (c.nunits + kPerGroup - 1) / kPerGroup;
-  __shared__ uint32_t s_slot;
   uint32_t ahead = 0;
-  for (uint32_t g = next_group(c, 0, true, &s_slot, ahead); g < groups;
-       g = next_group(c, g, false, &s_slot, ahead)) {
+  for (uint32_t g = next_group(c, 0, true, s_slot, ahead); g < groups;
+       g = next_group(c, g, false, s_slot, ahead)) {
     uint32_t d[kUnitsPerIter][NW];
     uint32_t p0[kUnitsPerIter], lo[kUnitsPerIter], hi[kUnitsPerIter], item[kUnitsPerIter],
         lend[kUnitsPerIter];
@@ -271,7 +311,10 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
 #pragma unroll
       for (int k = 0; k < NM; ++k) mask[j][k] = 0;
       if (inner[j] || (p0[j] < hi[j] && p0[j] + PL > lo[j])) {
-        filter_lane<K, CI, PL>(h, bloom, d[j], mask[j]);
+        if constexpr (K >= 3)
+          filter_lane_dual<K, CI, PL>(h, bloom, d[j], mask[j]);
+        else
+          filter_lane<K, CI, PL>(h, bloom, d[j], mask[j]);
         if (!inner[j]) {
 #pragma unroll
           for (int k = 0; k < NM; ++k)
@@ -881,7 +924,7 @@ static hipError_t allow_lds(F* fn, size_t bytes) {
   return e;
 }
 
-size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u; }
+size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u + 16u; }
 
 template <int K, bool CI>
 static hipError_t prepare_hash_kc(uint32_t lane_bytes, size_t shmem) {

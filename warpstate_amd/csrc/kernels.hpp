@@ -44,6 +44,9 @@ struct HashParams {
   uint32_t nwords, wshift, wbits, B, NBK, nbits;
   uint32_t s2;  // shift of the second bloom bit (0 = single-bit exact filter)
   uint32_t lane_bytes;  // positions per lane per work unit: 16, 32 or 64
+  // Bloom hash for K >= 3 (two 16-bit lanes): multipliers, -multiplier^K,
+  // and the bloom byte-address mask ((nwords - 1) << 2).
+  uint32_t BB, NBKK, amask;
   // Verification table: 2^nbits buckets of two 32-byte slots. A slot is
   // {hash, factor byte offset, length | kind << 31, bytes 0-3},
   // {bytes 4-15, output id} (hash 0xFFFFFFFF = empty); a bucket with more than

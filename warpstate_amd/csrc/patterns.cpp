@@ -37,17 +37,29 @@ void build_hash(PatternsHost& ps) {
   uint64_t bk = 1;
   for (uint32_t t = 0; t < K; ++t) bk = (bk * ps.B) & 0xFFFFFFu;
   ps.NBK = (uint32_t)((0x1000000u - bk) & 0xFFFFFFu);
+  ps.dual = K >= 3;
+  if (ps.dual) {
+    const uint32_t Bl = 0xA3B1u, Bh = 0xC6A5u;
+    uint32_t pl = 1, ph = 1;
+    for (uint32_t t = 0; t < K; ++t) {
+      pl = (pl * Bl) & 0xFFFFu;
+      ph = (ph * Bh) & 0xFFFFu;
+    }
+    ps.BB = Bl | (Bh << 16);
+    ps.NBKK = ((0x10000u - pl) & 0xFFFFu) | (((0x10000u - ph) & 0xFFFFu) << 16);
+  }
   if (K <= 2) {
     // Exact filter over the raw 1- or 2-byte key: one bit per key value.
     ps.wbits = 8 * K - 5;
     ps.wshift = 5;
     ps.s2 = 0;
   } else {
-    // Blocked bloom filter: the word comes from H's top bits, two bits inside
-    // it from H[4:0] and H[9:5]; about 256 bits per factor, 1 KiB .. 128 KiB.
+    // Blocked bloom filter over the 32-bit dual hash D: the word is
+    // D[wbits+1:2], its two bits D[20:16] and D[28:24]; about 256 bits per
+    // factor, 1 KiB .. 128 KiB.
     uint32_t lb = std::min(20u, std::max(13u, ceil_log2(F) + 8));
     ps.wbits = lb - 5;
-    ps.wshift = 24 - ps.wbits;
+    ps.wshift = 2;
     ps.s2 = 5;
   }
   ps.bloom.assign(1u << ps.wbits, 0);
@@ -58,9 +70,16 @@ void build_hash(PatternsHost& ps) {
   for (uint32_t f = 0; f < F; ++f) {
     uint32_t h = rolling_hash((const uint8_t*)ps.fac[f].bytes.data(), K, ps.B, ps.ci);
     ps.fhash[f] = h;
-    uint32_t w = (h >> ps.wshift) & ((1u << ps.wbits) - 1);
-    ps.bloom[w] |= 1u << (h & 31);
-    ps.bloom[w] |= 1u << ((h >> ps.s2) & 31);
+    if (ps.dual) {
+      const uint32_t d = dual_hash((const uint8_t*)ps.fac[f].bytes.data(), K, ps.BB, ps.ci);
+      const uint32_t w = (d >> 2) & ((1u << ps.wbits) - 1);
+      ps.bloom[w] |= 1u << ((d >> 16) & 31);
+      ps.bloom[w] |= 1u << ((d >> 24) & 31);
+    } else {
+      const uint32_t w = (h >> ps.wshift) & ((1u << ps.wbits) - 1);
+      ps.bloom[w] |= 1u << (h & 31);
+      ps.bloom[w] |= 1u << ((h >> ps.s2) & 31);
+    }
     bucket[f] = (uint32_t)((h * 0x9E3779B1u) >> (32 - ps.nbits));
     cnt[bucket[f] + 1]++;
   }
@@ -173,6 +192,17 @@ void build_ac(PatternsHost& ps) {
 }
 
 }  // namespace
+
+uint32_t dual_hash(const uint8_t* p, uint32_t K, uint32_t BB, bool ci) {
+  const uint32_t Bl = BB & 0xFFFFu, Bh = BB >> 16;
+  uint32_t hl = 0, hh = 0;
+  for (uint32_t t = 0; t < K; ++t) {
+    const uint32_t b = (ci && p[t] >= 'A' && p[t] <= 'Z') ? p[t] + 32u : p[t];
+    hl = (hl * Bl + b) & 0xFFFFu;
+    hh = (hh * Bh + b) & 0xFFFFu;
+  }
+  return hl | (hh << 16);
+}
 
 uint32_t rolling_hash(const uint8_t* p, uint32_t K, uint32_t B, bool ci) {
   uint32_t h = 0;

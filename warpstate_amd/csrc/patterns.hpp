@@ -46,6 +46,10 @@ struct PatternsHost {
   // Rolling-hash prefilter.
   bool hash_ok = false;
   uint32_t K = 0, B = 0, NBK = 0, wshift = 0, wbits = 0, nbits = 0, s2 = 0;
+  // Bloom hash for K >= 3: two 16-bit polynomial hashes side by side
+  // (multipliers packed low | high << 16, and -multiplier^K likewise).
+  bool dual = false;
+  uint32_t BB = 0, NBKK = 0;
   std::vector<uint32_t> bloom, bstart, bent, fhash;
 
   // Factor DFA (dense Aho-Corasick automaton over byte classes).
@@ -70,5 +74,8 @@ PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_inse
 
 // Rolling hash of the first K bytes (mod 2^24), shared by host and kernel.
 uint32_t rolling_hash(const uint8_t* p, uint32_t K, uint32_t B, bool ci);
+// The 32-bit bloom hash of a K-byte window: low half sum b[t] Bl^(K-1-t),
+// high half with Bh, each mod 2^16 (BB = Bl | Bh << 16).
+uint32_t dual_hash(const uint8_t* p, uint32_t K, uint32_t BB, bool ci);
 
 }  // namespace ws
