@@ -27,8 +27,6 @@ hipError_t launch_hash(uint32_t K, bool ci, uint32_t grid, hipStream_t st, const
                        const HashParams& h, const HitSlices& hs, const SliceOut& lit, const SliceOut& cand,
                        bool verify_only);
 uint32_t verify_blocks(uint32_t filter_grid);
-hipError_t launch_slice_compact(const SliceOut& so, uint32_t n, uint32_t* offsets, uint32_t* total, Rec* out,
-                                uint32_t out_cap, hipStream_t st);
 uint32_t ac_unit_bytes();
 size_t ac_shmem_bytes(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCommon& c,
@@ -58,9 +56,12 @@ hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords);
 hipError_t prepare_ac(bool lds_table, uint32_t states, uint32_t classes);
-hipError_t launch_run_close(const uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
+hipError_t launch_run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
                             unsigned long long* total, hipStream_t st);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t st);
+hipError_t launch_slice_compact2(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
+                                 uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total, uint32_t* lit_snap,
+                                 Rec* lout, uint32_t lcap, Rec* cout, uint32_t ccap, hipStream_t st);
 hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
                       const uint32_t* vin, uint32_t* vout, uint32_t n, uint32_t bits,
                       hipStream_t st);
@@ -370,6 +371,16 @@ ws_status sort_into(ws_ctx* ctx, uint32_t n, uint32_t bits, const uint64_t* kin,
 // keys of a single input take the bucket sort (about two buckets per pair);
 // its overflow flag (counters[9]) is copied to the host mirror with the next
 // synchronization and checked by confirm_order.
+// The bucket sort's counts: zero on allocation, and every sort leaves them
+// zero again.
+hipError_t ensure_bucket_counts(ws_ctx* ctx, uint32_t nb) {
+  const size_t bytes = (size_t)nb * 4;
+  if (bytes <= ctx->bcnt.cap) return hipSuccess;
+  hipError_t e = ctx->bcnt.ensure(bytes);
+  if (e != hipSuccess) return e;
+  return hipMemsetAsync(ctx->bcnt.p, 0, ctx->bcnt.cap, ctx->stream);
+}
+
 ws_status order_pairs(ws_ctx* ctx, uint32_t n, uint32_t bits, bool dense, Result* res) {
   res->bucketed = false;
   res->keybits = bits;
@@ -378,7 +389,7 @@ ws_status order_pairs(ws_ctx* ctx, uint32_t n, uint32_t bits, bool dense, Result
                      ctx->v1.as<uint32_t>());
   const uint32_t bb = std::min<uint32_t>(bits, std::min(22u, std::max(10u, bits_for(n) + 1)));
   const uint32_t shift = bits - bb, nb = 1u << bb;
-  WS_HIP(ctx->bcnt.ensure((size_t)nb * 4));
+  WS_HIP(ensure_bucket_counts(ctx, nb));
   WS_HIP(ctx->bcur.ensure((size_t)nb * 4));
   size_t need = 0;
   WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, n, nullptr, shift, nb,
@@ -660,9 +671,9 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
         if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
         WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, true));
         uint32_t* offs = sc + 2 * nvb;
-        WS_HIP(launch_slice_compact(lso, nvb, offs, ctx->counters.as<uint32_t>() + 0, c.lit_out, c.lit_cap, st));
-        WS_HIP(launch_slice_compact(cso, nvb, offs + nvb + 2, ctx->counters.as<uint32_t>() + 1, c.cand_out,
-                                    c.cand_cap, st));
+        WS_HIP(launch_slice_compact2(lso, cso, nvb, offs, offs + nvb + 2, ctx->counters.as<uint32_t>() + 0,
+                                     ctx->counters.as<uint32_t>() + 1, nullptr, c.lit_out, c.lit_cap, c.cand_out,
+                                     c.cand_cap, st));
         hit_overflow_cap = hs.overflow_cap;
         lit_ovf_cap = lso.ovf_cap;
         cand_ovf_cap = cso.ovf_cap;
@@ -678,33 +689,31 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     if (s != WS_OK) return s;
     const uint32_t nlit = ctx->hcount[0], ncand = ctx->hcount[1], nover = ctx->hcount[4];
     ctx->last_hits = nover;
-    if (variant == WS_VARIANT_HASH && nover > hit_overflow_cap) {
-      // Slices and the overflow area were not enough: grow and rescan.
-      WS_HIP(ctx->hits.ensure(hit_total * 2 * sizeof(uint2) + (size_t)nover * 8 * sizeof(uint2)));
-      hit_total = ctx->hits.cap / sizeof(uint2);
-      WS_HIP(ctx->hwin.ensure(hit_total * 2 * sizeof(uint4)));
-      if (attempt >= 2) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
-      continue;
-    }
-    if (variant == WS_VARIANT_HASH && (ctx->hcount[5] > lit_ovf_cap || ctx->hcount[6] > cand_ovf_cap)) {
-      if (ctx->hcount[5] > lit_ovf_cap) {
-        lslice_total = lslice_total * 2 + (uint64_t)ctx->hcount[5] * 8;
-        WS_HIP(ctx->lslices.ensure(lslice_total * sizeof(Rec)));
-      }
-      if (ctx->hcount[6] > cand_ovf_cap) {
-        cslice_total = cslice_total * 2 + (uint64_t)ctx->hcount[6] * 8;
-        WS_HIP(ctx->cslices.ensure(cslice_total * sizeof(Rec)));
-      }
-      if (attempt >= 3) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
-      continue;
-    }
     if (nlit > max_matches) {
       res->count = nlit;
       return fail(WS_ERR_MATCH_BUFFER_OVERFLOW,
                   "too many matches (" + std::to_string(nlit) + " exceeds " +
                       std::to_string(max_matches) + ")");
     }
+    // Grow every buffer that overflowed, then rescan; each category grows
+    // past its need at once, so this converges in one or two rounds.
     bool retry = false;
+    if (variant == WS_VARIANT_HASH && nover > hit_overflow_cap) {
+      WS_HIP(ctx->hits.ensure(hit_total * 2 * sizeof(uint2) + (size_t)nover * 8 * sizeof(uint2)));
+      hit_total = ctx->hits.cap / sizeof(uint2);
+      WS_HIP(ctx->hwin.ensure(hit_total * 2 * sizeof(uint4)));
+      retry = true;
+    }
+    if (variant == WS_VARIANT_HASH && ctx->hcount[5] > lit_ovf_cap) {
+      lslice_total = lslice_total * 2 + (uint64_t)ctx->hcount[5] * 8;
+      WS_HIP(ctx->lslices.ensure(lslice_total * sizeof(Rec)));
+      retry = true;
+    }
+    if (variant == WS_VARIANT_HASH && ctx->hcount[6] > cand_ovf_cap) {
+      cslice_total = cslice_total * 2 + (uint64_t)ctx->hcount[6] * 8;
+      WS_HIP(ctx->cslices.ensure(cslice_total * sizeof(Rec)));
+      retry = true;
+    }
     if (nlit > lit_cap) {
       WS_HIP(ctx->recs.ensure((size_t)nlit * 2 * sizeof(Rec) > (size_t)max_matches * sizeof(Rec) * 2
                                   ? (size_t)max_matches * 2 * sizeof(Rec)
@@ -718,7 +727,7 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       retry = true;
     }
     if (!retry) break;
-    if (attempt >= 2) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
+    if (attempt >= 4) return fail(WS_ERR_GPU_DEVICE, "scan buffers did not converge");
   }
   if (variant == WS_VARIANT_HASH)
     ctx->last_hits = (uint32_t)std::min<uint64_t>((uint64_t)ctx->last_hits + ctx->hcount[8], 0xFFFFFFFFull);
@@ -992,7 +1001,7 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   rs->ob = bucket_bits(rs->obits, std::min(rs->rec_cap, 1u << 17));
   rs->cb = nrx ? bucket_bits(rs->cbits, std::min(rs->raw_cap, 1u << 17)) : 0;
   const uint32_t nb = 1u << std::max(rs->ob, rs->cb);
-  WS_HIP(ctx->bcnt.ensure((size_t)nb * 4));
+  WS_HIP(ensure_bucket_counts(ctx, nb));
   WS_HIP(ctx->bcur.ensure((size_t)nb * 4));
   size_t need = 0, need2 = 0;
   WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, rs->rec_cap, nullptr, 0,
@@ -1021,7 +1030,8 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
                         hipMemcpyHostToDevice, st));
   // Filter and verification.
   if (ev) WS_HIP(hipEventRecord(ev[0], st));
-  WS_HIP(launch_fill_u32(cnt, 11, 0, st));
+  // cnt[0..10] are zero here: cleared before the first run, and by
+  // k_run_close after each one.
   ScanCommon c = make_common(ctx, p, plan, rs.nunits, rs.lit_cap, rs.cand_cap);
   RunCaps caps{};
   caps.lit_cap = rs.lit_cap;
@@ -1040,8 +1050,8 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     if (ev) WS_HIP(hipEventRecord(ev[1], st));
     WS_HIP(launch_hash(h.K, h.ci, rs.fgrid, st, c, hp, hs, lso, cso, true));
     uint32_t* offs = sc + 2 * rs.nvb;
-    WS_HIP(launch_slice_compact(lso, rs.nvb, offs, cnt + 0, c.lit_out, c.lit_cap, st));
-    WS_HIP(launch_slice_compact(cso, rs.nvb, offs + rs.nvb + 2, cnt + 1, c.cand_out, c.cand_cap, st));
+    WS_HIP(launch_slice_compact2(lso, cso, rs.nvb, offs, offs + rs.nvb + 2, cnt + 0, cnt + 1,
+                                 cnt + kCounterLitSnapshot, c.lit_out, c.lit_cap, c.cand_out, c.cand_cap, st));
     caps.hit_ovf_cap = hs.overflow_cap;
     caps.lit_ovf_cap = lso.ovf_cap;
     caps.cand_ovf_cap = cso.ovf_cap;
@@ -1050,10 +1060,10 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
                 p->ac_len, h.ac_states, h.ac_classes, h.ac_first_out, h.ac_maxlen};
     WS_HIP(launch_ac(rs.variant == WS_VARIANT_DFA_LDS, rs.grid, st, c, ap));
     if (ev) WS_HIP(hipEventRecord(ev[1], st));
+    WS_HIP(launch_copy_u32(cnt + kCounterLitSnapshot, cnt + 0, st));
   } else if (ev) {
     WS_HIP(hipEventRecord(ev[1], st));
   }
-  WS_HIP(launch_copy_u32(cnt + kCounterLitSnapshot, cnt + 0, st));
   if (ev) WS_HIP(hipEventRecord(ev[2], st));
 
   // Regex candidates: anchored walks, chain ordering, greedy chains appended
@@ -1152,6 +1162,7 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
   }
   auto launch_all = [&]() -> ws_status {
     WS_HIP(launch_fill_u32(ds.sticky, 10, 0, st));
+    WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), kCounterLitSnapshot + 1, 0, st));
     for (size_t k = 0; k < plans.size(); ++k) {
       ws_status s = launch_run(ctx, p, plans[k], setups[k], offs[k], ds,
                                ctx->profiling ? &ctx->pev[5 * k] : nullptr);
@@ -1236,7 +1247,7 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
   if (hs[kStickyError] & 2u || tot > cap)
     return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(tot) + " exceeds " +
                                                   std::to_string(std::min<uint64_t>(cap, max_matches)) + ")");
-  if (out_mem == WS_MEM_HOST && tot) {
+  if (out_mem == WS_MEM_HOST && tot && out) {  // (no host array: timing runs)
     WS_HIP(hipMemcpyAsync(out, ds.out, tot * sizeof(ws_match), hipMemcpyDeviceToHost, st));
     WS_HIP(hipStreamSynchronize(st));
   }
@@ -1657,6 +1668,19 @@ ws_status ws_autotune(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, ui
       if (s != WS_OK) return s;
       WS_HIP(hipStreamSynchronize(ctx->stream));
       const auto t0 = std::chrono::steady_clock::now();
+      if (device_path_ok(mode, &kConfigTable[i])) {
+        // The path a scan of this request takes: device-sized, eager.
+        DeviceScanResult oc;
+        uint64_t n = 0;
+        s = device_scan(ctx, p, {plan}, {0u}, &kConfigTable[i], nullptr, ctx->cfg.max_matches, WS_MEM_HOST, &n,
+                        &oc, nullptr);
+        if (s != WS_OK && s != WS_ERR_MATCH_BUFFER_OVERFLOW) return s;
+        if (oc == kDeviceScanDone) {
+          const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+          if (r == 0 || ms < tbest) tbest = ms;
+          continue;
+        }
+      }
       Result res;
       s = run_plan(ctx, p, plan, mode, &kConfigTable[i], &res);
       if (s != WS_OK) return s;

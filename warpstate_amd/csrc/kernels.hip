@@ -578,20 +578,26 @@ __global__ __launch_bounds__(kTPB) void k_hash_verify(ScanCommon c, HashParams h
   }
 }
 
-// Exclusive scan of min(count, cap) over the slices (one 1024-lane block);
-// offsets[n] and *total receive the sum including the overflow area.
-__global__ __launch_bounds__(1024) void k_slice_offsets(const uint32_t* counts, uint32_t n, uint32_t cap,
-                                                         const uint32_t* ovf_count, uint32_t ovf_cap,
-                                                         uint32_t* offsets, uint32_t* total) {
+// Slice compaction of both verify outputs. One 1024-lane block: lanes
+// [0, 512) take an exclusive scan of min(count, cap) over the literal
+// slices, lanes [512, 1024) over the candidate slices; offsets[n] is the
+// slices' sum, and the total (overflow area included) lands in *lit_total /
+// *cand_total, the literal total also in *lit_snap when set.
+__global__ __launch_bounds__(1024) void k_slice_offsets2(SliceOut lit, SliceOut cand, uint32_t n, uint32_t* loffs,
+                                                          uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total,
+                                                          uint32_t* lit_snap) {
   __shared__ uint32_t part[1024];
-  const uint32_t per = (n + 1023) / 1024;
-  const uint32_t b0 = threadIdx.x * per;
+  const uint32_t half = threadIdx.x >> 9, t = threadIdx.x & 511;
+  const SliceOut& so = half ? cand : lit;
+  uint32_t* offsets = half ? coffs : loffs;
+  const uint32_t per = (n + 511) / 512;
+  const uint32_t b0 = t * per;
   uint32_t sum = 0;
-  for (uint32_t k = 0; k < per && b0 + k < n; ++k) sum += min(counts[b0 + k], cap);
+  for (uint32_t k = 0; k < per && b0 + k < n; ++k) sum += min(so.counts[b0 + k], so.cap);
   part[threadIdx.x] = sum;
   __syncthreads();
-  for (uint32_t d = 1; d < 1024; d <<= 1) {
-    uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+  for (uint32_t d = 1; d < 512; d <<= 1) {
+    uint32_t v = t >= d ? part[threadIdx.x - d] : 0u;
     __syncthreads();
     part[threadIdx.x] += v;
     __syncthreads();
@@ -599,25 +605,40 @@ __global__ __launch_bounds__(1024) void k_slice_offsets(const uint32_t* counts, 
   uint32_t run = part[threadIdx.x] - sum;
   for (uint32_t k = 0; k < per && b0 + k < n; ++k) {
     offsets[b0 + k] = run;
-    run += min(counts[b0 + k], cap);
+    run += min(so.counts[b0 + k], so.cap);
   }
-  if (threadIdx.x == 1023) {
-    offsets[n] = part[1023];
-    *total = part[1023] + min(*ovf_count, ovf_cap);
+  if (t == 511) {
+    offsets[n] = part[threadIdx.x];
+    const uint32_t tot = part[threadIdx.x] + min(*so.ovf_count, so.ovf_cap);
+    if (half) {
+      *cand_total = tot;
+    } else {
+      *lit_total = tot;
+      if (lit_snap) *lit_snap = tot;
+    }
   }
 }
 
-// Copies slice b (block b) and the overflow area into one contiguous list.
-__global__ __launch_bounds__(kTPB) void k_slice_gather(SliceOut so, uint32_t n, const uint32_t* offsets,
-                                                        Rec* out, uint32_t out_cap) {
-  if (blockIdx.x < n) {
-    const uint32_t b = blockIdx.x;
-    const uint32_t m = min(so.counts[b], so.cap), o = offsets[b];
+// Gathers both outputs into contiguous lists: blocks [0, n + 64) serve the
+// literal slices, the rest the candidate slices; in each half block b < n
+// copies slice b and the 64 blocks after them the overflow area.
+__global__ __launch_bounds__(kTPB) void k_slice_gather2(SliceOut lit, SliceOut cand, uint32_t n,
+                                                         const uint32_t* loffs, const uint32_t* coffs, Rec* lout,
+                                                         uint32_t lcap, Rec* cout, uint32_t ccap) {
+  const uint32_t span = n + 64;
+  const bool second = blockIdx.x >= span;
+  const SliceOut& so = second ? cand : lit;
+  const uint32_t* offsets = second ? coffs : loffs;
+  Rec* out = second ? cout : lout;
+  const uint32_t out_cap = second ? ccap : lcap;
+  const uint32_t bx = second ? blockIdx.x - span : blockIdx.x;
+  if (bx < n) {
+    const uint32_t m = min(so.counts[bx], so.cap), o = offsets[bx];
     for (uint32_t k = threadIdx.x; k < m; k += kTPB)
-      if (o + k < out_cap) out[o + k] = so.buf[(size_t)b * so.cap + k];
+      if (o + k < out_cap) out[o + k] = so.buf[(size_t)bx * so.cap + k];
   } else {
     const uint32_t no = min(*so.ovf_count, so.ovf_cap), o = offsets[n];
-    for (uint32_t k = (blockIdx.x - n) * kTPB + threadIdx.x; k < no; k += (gridDim.x - n) * kTPB)
+    for (uint32_t k = (bx - n) * kTPB + threadIdx.x; k < no; k += 64 * kTPB)
       if (o + k < out_cap) out[o + k] = so.ovf[k];
   }
 }
@@ -1097,15 +1118,16 @@ hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32
 
 uint32_t verify_blocks(uint32_t filter_grid) { return filter_grid * kVerifySplit; }
 
-// Compacts per-block slices (and the overflow area) into `out`; the total
-// lands in *total.
-hipError_t launch_slice_compact(const SliceOut& so, uint32_t n, uint32_t* offsets, uint32_t* total, Rec* out,
-                                uint32_t out_cap, hipStream_t st) {
-  hipLaunchKernelGGL(k_slice_offsets, dim3(1), dim3(1024), 0, st, so.counts, n, so.cap, so.ovf_count, so.ovf_cap,
-                     offsets, total);
+// Compacts the literal and candidate slices (one scan kernel, one gather).
+hipError_t launch_slice_compact2(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
+                                 uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total, uint32_t* lit_snap,
+                                 Rec* lout, uint32_t lcap, Rec* cout, uint32_t ccap, hipStream_t st) {
+  hipLaunchKernelGGL(k_slice_offsets2, dim3(1), dim3(1024), 0, st, lit, cand, n, loffs, coffs, lit_total,
+                     cand_total, lit_snap);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_slice_gather, dim3(n + 64), dim3(kTPB), 0, st, so, n, offsets, out, out_cap);
+  hipLaunchKernelGGL(k_slice_gather2, dim3(2 * (n + 64)), dim3(kTPB), 0, st, lit, cand, n, loffs, coffs, lout,
+                     lcap, cout, ccap);
   return hipGetLastError();
 }
 
@@ -1122,9 +1144,9 @@ hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_
 }
 
 // End of a device-sized run: record overflows and errors, accumulate
-// statistics, and advance the output offset by this run's match count.
-__global__ void k_run_close(const uint32_t* counters, RunCaps caps, uint32_t* sticky,
-                            unsigned long long* total) {
+// statistics, advance the output offset by this run's match count, and zero
+// the run's counters for the next run.
+__global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, unsigned long long* total) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   uint32_t redo = 0;
   if (counters[4] > caps.hit_ovf_cap) redo |= 1u;
@@ -1141,13 +1163,14 @@ __global__ void k_run_close(const uint32_t* counters, RunCaps caps, uint32_t* st
   sticky[kStickyCands] += counters[1];
   sticky[kStickyRaw] += counters[2];
   *total += counters[0];
+  for (uint32_t k = 0; k <= kCounterLitSnapshot; ++k) counters[k] = 0;
 }
 
 __global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *dst = *src;
 }
 
-hipError_t launch_run_close(const uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
+hipError_t launch_run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
                             unsigned long long* total, hipStream_t st) {
   hipLaunchKernelGGL(k_run_close, dim3(1), dim3(64), 0, st, counters, caps, sticky, total);
   return hipGetLastError();
@@ -1185,10 +1208,11 @@ __global__ void k_bucket_scatter(const uint64_t* kin, const uint32_t* vin, uint3
   }
 }
 
-__global__ void k_bucket_sort(uint64_t* keys, uint32_t* vals, const uint32_t* end, const uint32_t* cnt,
+__global__ void k_bucket_sort(uint64_t* keys, uint32_t* vals, const uint32_t* end, uint32_t* cnt,
                               uint32_t nb, uint32_t* flag) {
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
     const uint32_t c = cnt[b];
+    if (c) cnt[b] = 0;  // the counts start at zero again for the next sort
     if (c < 2) continue;
     if (c > kBucketMax) {
       atomicOr(flag, 1u);
@@ -1211,8 +1235,9 @@ __global__ void k_bucket_sort(uint64_t* keys, uint32_t* vals, const uint32_t* en
 }
 
 // Sizes the scan's temporary storage (tmp = nullptr) or runs
-// clear -> count -> scan -> scatter -> per-bucket sort. cnt and cursor hold
-// nb words. With nd set, n is a capacity and the count is *nd.
+// count -> scan -> scatter -> per-bucket sort. cnt and cursor hold nb words;
+// cnt must be zero on entry and is zero again on exit. With nd set, n is a
+// capacity and the count is *nd.
 hipError_t bucket_sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
                              const uint32_t* vin, uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift,
                              uint32_t nb, uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st) {
@@ -1220,7 +1245,6 @@ hipError_t bucket_sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, 
     return rocprim::exclusive_scan(nullptr, *tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(),
                                    st);
   const uint32_t g = std::max(1u, std::min(nd ? kDeviceGrid : 4096u, (n + 255) / 256));
-  hipLaunchKernelGGL(k_fill_u32, dim3(std::min(4096u, (nb + 255) / 256)), dim3(256), 0, st, cnt, nb, 0u);
   hipLaunchKernelGGL(k_bucket_count, dim3(g), dim3(256), 0, st, kin, n, nd, shift, cnt);
   hipError_t e = rocprim::exclusive_scan(tmp, *tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(), st);
   if (e != hipSuccess) return e;
