@@ -24,9 +24,7 @@ namespace ws {
 // Launch wrappers defined in kernels.hip.
 size_t hash_shmem_bytes(uint32_t nwords);
 hipError_t launch_hash(uint32_t K, bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c,
-                       const HashParams& h, const HitSlices& hs, const SliceOut& lit, const SliceOut& cand,
-                       bool verify_only);
-uint32_t verify_blocks(uint32_t filter_grid);
+                       const HashParams& h, const SliceOut& lit, const SliceOut& cand);
 uint32_t ac_unit_bytes();
 size_t ac_shmem_bytes(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCommon& c,
@@ -142,7 +140,7 @@ struct ws_ctx {
   ws_timings last{};
   hipEvent_t ev[8] = {};
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids, hits, hwin, slice_counts, lslices, cslices, scounts, bcnt, bcur, dstate;
+      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate;
   std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
   uint8_t* hstage = nullptr;    // pinned staging read by captured copies
   size_t hstage_cap = 0;
@@ -536,19 +534,9 @@ SliceOut make_slices(DevBuf& buf, uint64_t total, uint32_t nvb, uint32_t* counts
   return so;
 }
 
-// Filter hit slices over ctx->hits / ctx->hwin (`hit_total` slots).
-HitSlices make_hits(ws_ctx* ctx, uint64_t hit_total, uint32_t fgrid) {
-  HitSlices hs{};
-  hs.overflow_cap = (uint32_t)std::min<uint64_t>(hit_total / 4, 0xFFFFFFF0ull);
-  hs.slice_cap = (uint32_t)((hit_total - hs.overflow_cap) / fgrid);
-  hs.hits = ctx->hits.as<uint2>();
-  hs.overflow = hs.hits + (size_t)hs.slice_cap * fgrid;
-  hs.win = ctx->hwin.as<uint4>();
-  hs.overflow_win = hs.win + 2 * (size_t)hs.slice_cap * fgrid;
-  hs.slice_count = ctx->slice_counts.as<uint32_t>();
-  hs.overflow_count = ctx->counters.as<uint32_t>() + 4;
-  return hs;
-}
+// Records per output list (literal matches or regex candidates) of the
+// hash variant: 1024 per filter block plus a shared overflow area of 256 K.
+uint64_t slice_records(uint32_t nblocks) { return (uint64_t)nblocks * 1024 + (1u << 18); }
 
 ScanCommon make_common(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t nunits, uint32_t lit_cap,
                        uint32_t cand_cap) {
@@ -630,30 +618,17 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   const uint32_t per_cu = tuning && tuning->blocks_per_cu ? tuning->blocks_per_cu : 4;
   const uint32_t grid = std::max(1u, std::min(nunits, ctx->cus * per_cu));
   // Hash variant: one 1024-lane filter block per CU (its LDS holds the bloom
-  // filter); each block owns a slice of the hit buffer, plus an overflow area.
-  const uint32_t fgrid = std::max(1u, std::min((nunits + 7) / 8, ctx->cus * (per_cu > 4 ? 2u : 1u)));
-  uint64_t hit_total = (uint64_t)(ctx->hits.cap / sizeof(uint2));
-  if (variant == WS_VARIANT_HASH) {
-    uint64_t span = 0;
-    for (auto& it : plan.items) span += it.y - it.x;
-    const uint64_t want = std::min<uint64_t>(std::max<uint64_t>(1u << 20, span / 64), 0x3FFFFFFFull);
-    if (hit_total < want) {
-      WS_HIP(ctx->hits.ensure(want * sizeof(uint2)));
-      hit_total = ctx->hits.cap / sizeof(uint2);
-    }
-    WS_HIP(ctx->hwin.ensure(hit_total * 2 * sizeof(uint4)));
-    WS_HIP(ctx->slice_counts.ensure((size_t)fgrid * 4));
-  }
-  // Verify output slices (literal matches, regex candidates), one per verify block.
-  const uint32_t nvb = verify_blocks(fgrid);
+  // filter); each block owns a slice of the literal and candidate outputs,
+  // plus a shared overflow area.
+  const uint32_t nvb = std::max(1u, std::min((nunits + 7) / 8, ctx->cus * (per_cu > 4 ? 2u : 1u)));
   uint64_t lslice_total = ctx->lslices.cap / sizeof(Rec), cslice_total = ctx->cslices.cap / sizeof(Rec);
   if (variant == WS_VARIANT_HASH) {
-    const uint64_t want = (uint64_t)nvb * 256 + (1u << 16);
+    const uint64_t want = slice_records(nvb);
     if (lslice_total < want) { WS_HIP(ctx->lslices.ensure(want * sizeof(Rec))); lslice_total = want; }
     if (cslice_total < want) { WS_HIP(ctx->cslices.ensure(want * sizeof(Rec))); cslice_total = want; }
     WS_HIP(ctx->scounts.ensure(((size_t)nvb * 2 + 4) * 4 * 2));
   }
-  uint32_t hit_overflow_cap = 0, lit_ovf_cap = 0, cand_ovf_cap = 0;
+  uint32_t lit_ovf_cap = 0, cand_ovf_cap = 0;
   if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[0], st));
   for (int attempt = 0;; ++attempt) {
     WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 9, 0, st));
@@ -663,18 +638,15 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     if (nunits > 0) {
       if (variant == WS_VARIANT_HASH) {
         const HashParams hp = hash_params(p, lane_bytes);
-        const HitSlices hs = make_hits(ctx, hit_total, fgrid);
         uint32_t* sc = ctx->scounts.as<uint32_t>();
         SliceOut lso = make_slices(ctx->lslices, lslice_total, nvb, sc, ctx->counters.as<uint32_t>() + 5);
         SliceOut cso = make_slices(ctx->cslices, cslice_total, nvb, sc + nvb, ctx->counters.as<uint32_t>() + 6);
-        WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, false));
+        WS_HIP(launch_hash(h.K, h.ci, nvb, st, c, hp, lso, cso));
         if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
-        WS_HIP(launch_hash(h.K, h.ci, fgrid, st, c, hp, hs, lso, cso, true));
         uint32_t* offs = sc + 2 * nvb;
         WS_HIP(launch_slice_compact2(lso, cso, nvb, offs, offs + nvb + 2, ctx->counters.as<uint32_t>() + 0,
                                      ctx->counters.as<uint32_t>() + 1, nullptr, c.lit_out, c.lit_cap, c.cand_out,
                                      c.cand_cap, st));
-        hit_overflow_cap = hs.overflow_cap;
         lit_ovf_cap = lso.ovf_cap;
         cand_ovf_cap = cso.ovf_cap;
       } else {
@@ -687,8 +659,8 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[2], st));
     ws_status s = read_counters(ctx);
     if (s != WS_OK) return s;
-    const uint32_t nlit = ctx->hcount[0], ncand = ctx->hcount[1], nover = ctx->hcount[4];
-    ctx->last_hits = nover;
+    const uint32_t nlit = ctx->hcount[0], ncand = ctx->hcount[1];
+    ctx->last_hits = 0;
     if (nlit > max_matches) {
       res->count = nlit;
       return fail(WS_ERR_MATCH_BUFFER_OVERFLOW,
@@ -698,12 +670,6 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     // Grow every buffer that overflowed, then rescan; each category grows
     // past its need at once, so this converges in one or two rounds.
     bool retry = false;
-    if (variant == WS_VARIANT_HASH && nover > hit_overflow_cap) {
-      WS_HIP(ctx->hits.ensure(hit_total * 2 * sizeof(uint2) + (size_t)nover * 8 * sizeof(uint2)));
-      hit_total = ctx->hits.cap / sizeof(uint2);
-      WS_HIP(ctx->hwin.ensure(hit_total * 2 * sizeof(uint4)));
-      retry = true;
-    }
     if (variant == WS_VARIANT_HASH && ctx->hcount[5] > lit_ovf_cap) {
       lslice_total = lslice_total * 2 + (uint64_t)ctx->hcount[5] * 8;
       WS_HIP(ctx->lslices.ensure(lslice_total * sizeof(Rec)));
@@ -941,9 +907,9 @@ uint32_t bucket_bits(uint32_t keybits, uint32_t cap) {
 // Everything a device-sized run needs at launch time, decided (and every
 // buffer sized) before any launch, so the launches can be captured.
 struct RunSetup {
-  uint32_t variant = 0, lane_bytes = 0, nunits = 0, grid = 0, fgrid = 0, nvb = 0;
+  uint32_t variant = 0, lane_bytes = 0, nunits = 0, grid = 0, nvb = 0;
   uint32_t lit_cap = 0, cand_cap = 0, raw_cap = 0, rec_cap = 0, max_matches = 0;
-  uint64_t hit_total = 0, slice_total = 0;
+  uint64_t lslice_total = 0, cslice_total = 0;
   uint32_t sbits = 0, pbits = 0, rbits = 0, obits = 0, cbits = 0, ob = 0, cb = 0;
   size_t items_off = 0;  // this run's items in the pinned staging area (uint2 units)
 };
@@ -977,17 +943,15 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   if (nrx) WS_HIP(ctx->head.ensure(rs->raw_cap));
   const uint32_t per_cu = tuning && tuning->blocks_per_cu ? tuning->blocks_per_cu : 4;
   rs->grid = std::max(1u, std::min(rs->nunits, ctx->cus * per_cu));
-  rs->fgrid = std::max(1u, std::min((rs->nunits + 7) / 8, ctx->cus * (per_cu > 4 ? 2u : 1u)));
-  rs->nvb = verify_blocks(rs->fgrid);
-  rs->hit_total = std::min<uint64_t>(std::max<uint64_t>(1u << 20, span / 64), 0x3FFFFFFFull);
-  rs->slice_total = (uint64_t)rs->nvb * 256 + (1u << 16);
+  // Filter blocks = output slices (hash variant); the slices keep whatever
+  // capacity an earlier (checked) scan grew them to.
+  rs->nvb = std::max(1u, std::min((rs->nunits + 7) / 8, ctx->cus * (per_cu > 4 ? 2u : 1u)));
   if (rs->variant == WS_VARIANT_HASH) {
-    WS_HIP(ctx->hits.ensure(rs->hit_total * sizeof(uint2)));
-    WS_HIP(ctx->hwin.ensure(rs->hit_total * 2 * sizeof(uint4)));
-    WS_HIP(ctx->slice_counts.ensure((size_t)rs->fgrid * 4));
-    WS_HIP(ctx->lslices.ensure(rs->slice_total * sizeof(Rec)));
-    WS_HIP(ctx->cslices.ensure(rs->slice_total * sizeof(Rec)));
+    WS_HIP(ctx->lslices.ensure(slice_records(rs->nvb) * sizeof(Rec)));
+    WS_HIP(ctx->cslices.ensure(slice_records(rs->nvb) * sizeof(Rec)));
     WS_HIP(ctx->scounts.ensure(((size_t)rs->nvb * 2 + 4) * 4 * 2));
+    rs->lslice_total = ctx->lslices.cap / sizeof(Rec);
+    rs->cslice_total = ctx->cslices.cap / sizeof(Rec);
   }
   rs->sbits = bits_for(plan.max_seg_len);
   rs->pbits = bits_for(h.npat ? h.npat - 1 : 0);
@@ -1039,20 +1003,17 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   caps.raw_cap = rs.raw_cap;
   caps.rec_cap = rs.rec_cap;
   caps.max_matches = rs.max_matches;
-  caps.hit_ovf_cap = caps.lit_ovf_cap = caps.cand_ovf_cap = 0xFFFFFFFFu;
+  caps.lit_ovf_cap = caps.cand_ovf_cap = 0xFFFFFFFFu;
   if (rs.nunits > 0 && rs.variant == WS_VARIANT_HASH) {
     const HashParams hp = hash_params(p, rs.lane_bytes);
-    const HitSlices hs = make_hits(ctx, rs.hit_total, rs.fgrid);
     uint32_t* sc = ctx->scounts.as<uint32_t>();
-    SliceOut lso = make_slices(ctx->lslices, rs.slice_total, rs.nvb, sc, cnt + 5);
-    SliceOut cso = make_slices(ctx->cslices, rs.slice_total, rs.nvb, sc + rs.nvb, cnt + 6);
-    WS_HIP(launch_hash(h.K, h.ci, rs.fgrid, st, c, hp, hs, lso, cso, false));
+    SliceOut lso = make_slices(ctx->lslices, rs.lslice_total, rs.nvb, sc, cnt + 5);
+    SliceOut cso = make_slices(ctx->cslices, rs.cslice_total, rs.nvb, sc + rs.nvb, cnt + 6);
+    WS_HIP(launch_hash(h.K, h.ci, rs.nvb, st, c, hp, lso, cso));
     if (ev) WS_HIP(hipEventRecord(ev[1], st));
-    WS_HIP(launch_hash(h.K, h.ci, rs.fgrid, st, c, hp, hs, lso, cso, true));
     uint32_t* offs = sc + 2 * rs.nvb;
     WS_HIP(launch_slice_compact2(lso, cso, rs.nvb, offs, offs + rs.nvb + 2, cnt + 0, cnt + 1,
                                  cnt + kCounterLitSnapshot, c.lit_out, c.lit_cap, c.cand_out, c.cand_cap, st));
-    caps.hit_ovf_cap = hs.overflow_cap;
     caps.lit_ovf_cap = lso.ovf_cap;
     caps.cand_ovf_cap = cso.ovf_cap;
   } else if (rs.nunits > 0) {
@@ -1345,8 +1306,8 @@ void ws_ctx_destroy(ws_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->recs, &ctx->cands, &ctx->raw, &ctx->k0, &ctx->v0, &ctx->k1, &ctx->v1,
                     &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
-                    &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids, &ctx->hits, &ctx->hwin,
-                    &ctx->slice_counts, &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
+                    &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
+                    &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
                     &ctx->bcur, &ctx->dstate})
     b->release();
   for (auto& ev : ctx->ev)

@@ -229,135 +229,7 @@ __device__ __forceinline__ uint32_t valid_word(uint32_t p0, uint32_t n, uint32_t
   return top & ~((1u << a0) - 1u);
 }
 
-constexpr int kUnitsPerIter = 2;                  // work units a lane loads before computing
-constexpr int kSubBlocks = kFilterTPB / kTPB;      // 256-lane sub-blocks, one unit each
-
-// Streaming rolling-hash filter, one 1024-lane block per CU so the bloom
-// filter can use up to 128 KiB of LDS. Each 256-lane sub-block owns one work
-// unit (256 * PL bytes) at a time; per iteration a lane loads the (PL + 16)-
-// byte windows of kUnitsPerIter units with coalesced 16-byte loads, then
-// hashes PL positions per unit. Tiles wholly inside their segment skip all
-// bounds work. Hit positions are compacted per wave (prefix sum + one LDS
-// reservation) into the block's private slice of the hit buffer, so the loop
-// has no global atomics and no block barriers; a slice that fills up spills
-// into a shared overflow area.
-template <int K, bool CI, int PL>
-__global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashParams h, HitSlices hs) {
-  // LDS: the bloom words from address 0 (the kernel has no static LDS, so
-  // a word's address is the masked hash itself), then the block's hit
-  // counter and work-queue slot.
-  extern __shared__ uint32_t smem[];
-  uint32_t* bloom = smem;
-  uint32_t& s_n = smem[h.nwords];
-  uint32_t* s_slot = &smem[h.nwords + 1];
-  constexpr int NW = PL / 4 + 4;  // window dwords
-  constexpr uint32_t kUnit = kTPB * PL;
-  const uint32_t sub = threadIdx.x / kTPB, ltid = threadIdx.x % kTPB;
-  uint2* slice = hs.hits + (size_t)blockIdx.x * hs.slice_cap;
-  uint4* win = hs.win + 2 * (size_t)blockIdx.x * hs.slice_cap;
-
-  for (uint32_t k = threadIdx.x; k < h.nwords; k += kFilterTPB) bloom[k] = h.bloom[k];
-  if (threadIdx.x == 0) s_n = 0;
-  __syncthreads();
-
-  const uint32_t load_end = (uint32_t)c.load_end;
-  constexpr uint32_t kPerGroup = kSubBlocks * kUnitsPerIter;
-  const uint32_t groups = (c.nunits + kPerGroup - 1) / kPerGroup;
-  uint32_t ahead = 0;
-  for (uint32_t g = next_group(c, 0, true, s_slot, ahead); g < groups;
-       g = next_group(c, g, false, s_slot, ahead)) {
-    uint32_t d[kUnitsPerIter][NW];
-    uint32_t p0[kUnitsPerIter], lo[kUnitsPerIter], hi[kUnitsPerIter], item[kUnitsPerIter],
-        lend[kUnitsPerIter];
-    bool inner[kUnitsPerIter];
-#pragma unroll
-    for (int j = 0; j < kUnitsPerIter; ++j) {
-      const uint32_t u = g * kPerGroup + sub * kUnitsPerIter + j;
-      uint32_t tile_base = 0;
-      item[j] = 0;
-      if (u < c.nunits) {
-        if (c.units) {
-          const uint2 uu = c.units[u];
-          tile_base = uu.x;
-          item[j] = uu.y;
-        } else {
-          tile_base = u * kUnit;
-        }
-      }
-      const uint2 seg = c.items[item[j]];
-      lo[j] = seg.x;
-      hi[j] = u < c.nunits ? seg.y : 0;
-      p0[j] = tile_base + PL * ltid;
-      // Inner tile: every window byte lies inside the segment and every
-      // position may start a match (the common case of a large scan).
-      inner[j] = tile_base >= lo[j] && tile_base + kUnit + 16 <= hi[j] &&
-                 tile_base + kUnit - lo[j] + 0u <= c.emit_end;
-      lend[j] = (hi[j] + 15) & ~15u;
-      if (lend[j] > load_end) lend[j] = load_end;
-#pragma unroll
-      for (int v = 0; v < NW / 4; ++v) {
-        uint4 x = make_uint4(0, 0, 0, 0);
-        const uint32_t a = p0[j] + 16 * v;
-        if (inner[j] || (a + 16 <= lend[j] && a + 16 > (lo[j] & ~15u))) x = *reinterpret_cast<const uint4*>(c.base + a);
-        d[j][4 * v] = x.x; d[j][4 * v + 1] = x.y; d[j][4 * v + 2] = x.z; d[j][4 * v + 3] = x.w;
-      }
-    }
-    constexpr int NM = mask_words<PL>();
-    constexpr uint32_t WPOS = PL < 32 ? PL : 32;  // positions per mask word
-    uint32_t mask[kUnitsPerIter][NM];
-    uint32_t nh = 0;
-#pragma unroll
-    for (int j = 0; j < kUnitsPerIter; ++j) {
-#pragma unroll
-      for (int k = 0; k < NM; ++k) mask[j][k] = 0;
-      if (inner[j] || (p0[j] < hi[j] && p0[j] + PL > lo[j])) {
-        if constexpr (K >= 3)
-          filter_lane_dual<K, CI, PL>(h, bloom, d[j], mask[j]);
-        else
-          filter_lane<K, CI, PL>(h, bloom, d[j], mask[j]);
-        if (!inner[j]) {
-#pragma unroll
-          for (int k = 0; k < NM; ++k)
-            mask[j][k] &= valid_word(p0[j] + WPOS * k, WPOS, lo[j], hi[j], K, c.emit_end);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < NM; ++k) nh += __builtin_popcount(mask[j][k]);
-    }
-    if (__any(nh != 0u)) {
-      uint32_t slot = wave_reserve(nh, &s_n);
-#pragma unroll
-      for (int j = 0; j < kUnitsPerIter; ++j) {
-#pragma unroll
-        for (int k = 0; k < NM; ++k)
-        for (uint32_t m = mask[j][k]; m; m &= m - 1, ++slot) {
-          const uint32_t q = p0[j] + WPOS * k + __builtin_ctz(m);
-          const uint2 rec = make_uint2(q, item[j]);
-          // The aligned 32-byte span holding q was just streamed: an L2 hit.
-          const uint32_t a = q & ~15u;
-          uint4 x0 = make_uint4(0, 0, 0, 0), x1 = make_uint4(0, 0, 0, 0);
-          if (a + 16 <= lend[j]) x0 = *reinterpret_cast<const uint4*>(c.base + a);
-          if (a + 32 <= lend[j]) x1 = *reinterpret_cast<const uint4*>(c.base + a + 16);
-          if (slot < hs.slice_cap) {
-            slice[slot] = rec;
-            uint4* wv = win + 2 * (size_t)slot;
-            wv[0] = x0;
-            wv[1] = x1;
-          } else {
-            const uint32_t gi = atomicAdd(hs.overflow_count, 1u);
-            if (gi < hs.overflow_cap) {
-              hs.overflow[gi] = rec;
-              hs.overflow_win[2 * (size_t)gi] = x0;
-              hs.overflow_win[2 * (size_t)gi + 1] = x1;
-            }
-          }
-        }
-      }
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) hs.slice_count[blockIdx.x] = s_n;
-}
+constexpr int kUnitsPerIter = 2;  // quarter units a lane hashes per iteration (a wave: one half unit)
 
 __device__ __forceinline__ void slice_put(const SliceOut& so, uint32_t slice, uint32_t slot, Rec r) {
   if (slot < so.cap) {
@@ -535,46 +407,186 @@ __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams
   emit_wave(cand, &s_n[1], k1 == 2, r1);
 }
 
-// Exact check of every filter hit: rolling hash of the K-byte window, its
-// bucket chain (L2), byte verification of each candidate factor. Filter
-// slice b is shared by kVerifySplit blocks; then all blocks share the
-// overflow area. Literal matches and regex candidate starts go to block
-// staging buffers flushed with one global atomic each.
-constexpr uint32_t kVerifySplit = 8;
+// Per-wave LDS ring of filter hits waiting for verification (uint2
+// {position, item}); a wave verifies 64 at a time, one per lane.
+constexpr uint32_t kHitRing = 128;
 
+// Verifies the n (<= 64) queued hits from ring index `head`, one per lane:
+// the 32-byte aligned input span holding the position (just streamed by the
+// filter, so an L2 hit), the window hash, its bucket, the exact compare.
+// Call with the whole wave.
 template <int K, bool CI>
-__global__ __launch_bounds__(kTPB) void k_hash_verify(ScanCommon c, HashParams h, HitSlices hs, SliceOut lit,
-                                                       SliceOut cand) {
-  __shared__ uint32_t s_n[2];
-  if (threadIdx.x == 0) { s_n[0] = 0; s_n[1] = 0; }
+__device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashParams& h, const uint2* ring,
+                                              uint32_t head, uint32_t n, uint32_t* s_cnt, const SliceOut& lit,
+                                              const SliceOut& cand) {
+  const uint32_t lane = lane_id();
+  HitWork hw{};
+  hw.live = false;
+  if (lane < n) {
+    const uint2 hr = ring[(head + lane) & (kHitRing - 1)];
+    const uint2 seg = c.items[hr.y];
+    uint32_t lend = (seg.y + 15) & ~15u;
+    if (lend > (uint32_t)c.load_end) lend = (uint32_t)c.load_end;
+    const uint32_t a = hr.x & ~15u;
+    uint4 x0 = make_uint4(0, 0, 0, 0), x1 = x0;
+    if (a + 16 <= lend) x0 = *reinterpret_cast<const uint4*>(c.base + a);
+    if (a + 32 <= lend) x1 = *reinterpret_cast<const uint4*>(c.base + a + 16);
+    hit_prepare<K, CI>(c, h, hr, x0, x1, hw);
+  }
+  hit_finish<CI>(c, h, hw, s_cnt, lit, cand);
+}
+
+// Orders a wave's LDS ring writes before its lanes read each other's entries.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Streaming rolling-hash filter with in-kernel verification, one 1024-lane
+// block per CU so the bloom filter can use up to 128 KiB of LDS. Work goes
+// out in half units (a unit is 256 * PL bytes; the host lists a batch's
+// units): every wave takes a static stride of half units, and per iteration
+// a lane loads the (PL + 16)-byte windows of its two quarters with coalesced
+// 16-byte loads, then hashes PL positions in each. Tiles wholly inside their
+// segment skip all bounds work. A wave appends its hits (wave prefix sum) to
+// its own LDS ring and verifies them 64 at a time while the block's other
+// waves keep filtering, so the verification's dependent L2 reads hide
+// behind their arithmetic: no hit list goes through HBM and the loop has no
+// block barriers. Literal matches and regex candidate starts go to the
+// block's output slices.
+template <int K, bool CI, int PL>
+__global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashParams h, SliceOut lit,
+                                                             SliceOut cand) {
+  // LDS: the bloom words from address 0 (the kernel has no static LDS, so
+  // a word's address is the masked hash itself), the block's two output
+  // counters, then one hit ring per wave.
+  extern __shared__ uint32_t smem[];
+  uint32_t* bloom = smem;
+  uint32_t* s_cnt = &smem[h.nwords];
+  uint2* ring = reinterpret_cast<uint2*>(&smem[h.nwords + 4]) + (threadIdx.x / 64) * kHitRing;
+  constexpr int NW = PL / 4 + 4;  // window dwords
+  constexpr uint32_t kUnit = kTPB * PL;
+  constexpr uint32_t kWaves = kFilterTPB / 64;
+  const uint32_t lane = lane_id();
+
+  for (uint32_t k = threadIdx.x; k < h.nwords; k += kFilterTPB) bloom[k] = h.bloom[k];
+  if (threadIdx.x == 0) {
+    s_cnt[0] = 0;
+    s_cnt[1] = 0;
+  }
   __syncthreads();
-  const uint32_t b = blockIdx.x / kVerifySplit, part = blockIdx.x % kVerifySplit;
-  const uint2* slice = hs.hits + (size_t)b * hs.slice_cap;
-  const uint32_t n = min(hs.slice_count[b], hs.slice_cap);
-  const uint4* win = hs.win + 2 * (size_t)b * hs.slice_cap;
-  if (part == 0 && threadIdx.x == 0 && n) atomicAdd(&c.counters[8], n);  // filter hits, for timings
-  // Two hits per lane per round, their loads interleaved.
-  constexpr uint32_t kStride = kVerifySplit * kTPB;
-  for (uint32_t i = part * kTPB + threadIdx.x; i < n; i += 2 * kStride) {
-    const bool two = i + kStride < n;
-    const uint32_t j = two ? i + kStride : i;
-    HitWork a, bw;
-    hit_prepare<K, CI>(c, h, slice[i], win[2 * (size_t)i], win[2 * (size_t)i + 1], a);
-    hit_prepare<K, CI>(c, h, slice[j], win[2 * (size_t)j], win[2 * (size_t)j + 1], bw);
-    bw.live = two;
-    hit_finish<CI>(c, h, a, s_n, lit, cand);
-    hit_finish<CI>(c, h, bw, s_n, lit, cand);
+
+  const uint32_t load_end = (uint32_t)c.load_end;
+  const uint32_t nslots = 2 * c.nunits;
+  uint32_t tail = 0, pend = 0, nhits = 0;  // the wave's ring (same in every lane)
+  for (uint32_t sl = blockIdx.x * kWaves + threadIdx.x / 64; sl < nslots; sl += gridDim.x * kWaves) {
+    const uint32_t u = sl >> 1;
+    uint32_t tile_base, it = 0;
+    if (c.units) {
+      const uint2 uu = c.units[u];
+      tile_base = uu.x;
+      it = uu.y;
+    } else {
+      tile_base = u * kUnit;
+    }
+    const uint2 seg = c.items[it];
+    // Inner tile: every window byte lies inside the segment and every
+    // position may start a match (the common case of a large scan).
+    const bool in_unit = tile_base >= seg.x && tile_base + kUnit + 16 <= seg.y &&
+                         tile_base + kUnit - seg.x + 0u <= c.emit_end;
+    uint32_t le = (seg.y + 15) & ~15u;
+    if (le > load_end) le = load_end;
+    uint32_t d[kUnitsPerIter][NW];
+    uint32_t p0[kUnitsPerIter], lo[kUnitsPerIter], hi[kUnitsPerIter], item[kUnitsPerIter],
+        lend[kUnitsPerIter];
+    bool inner[kUnitsPerIter];
+#pragma unroll
+    for (int j = 0; j < kUnitsPerIter; ++j) {
+      // Quarter 2 * (sl & 1) + j of unit u: 64 lanes x PL bytes.
+      p0[j] = tile_base + PL * (64 * (2 * (sl & 1) + j) + lane);
+      item[j] = it;
+      lo[j] = seg.x;
+      hi[j] = seg.y;
+      inner[j] = in_unit;
+      lend[j] = le;
+#pragma unroll
+      for (int v = 0; v < NW / 4; ++v) {
+        uint4 x = make_uint4(0, 0, 0, 0);
+        const uint32_t a = p0[j] + 16 * v;
+        if (inner[j] || (a + 16 <= lend[j] && a + 16 > (lo[j] & ~15u))) x = *reinterpret_cast<const uint4*>(c.base + a);
+        d[j][4 * v] = x.x; d[j][4 * v + 1] = x.y; d[j][4 * v + 2] = x.z; d[j][4 * v + 3] = x.w;
+      }
+    }
+    constexpr int NM = mask_words<PL>();
+    constexpr uint32_t WPOS = PL < 32 ? PL : 32;  // positions per mask word
+    uint32_t mask[kUnitsPerIter][NM];
+    uint32_t nh = 0;
+#pragma unroll
+    for (int j = 0; j < kUnitsPerIter; ++j) {
+#pragma unroll
+      for (int k = 0; k < NM; ++k) mask[j][k] = 0;
+      if (inner[j] || (p0[j] < hi[j] && p0[j] + PL > lo[j])) {
+        if constexpr (K >= 3)
+          filter_lane_dual<K, CI, PL>(h, bloom, d[j], mask[j]);
+        else
+          filter_lane<K, CI, PL>(h, bloom, d[j], mask[j]);
+        if (!inner[j]) {
+#pragma unroll
+          for (int k = 0; k < NM; ++k)
+            mask[j][k] &= valid_word(p0[j] + WPOS * k, WPOS, lo[j], hi[j], K, c.emit_end);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NM; ++k) nh += __builtin_popcount(mask[j][k]);
+    }
+    if (__any(nh != 0u)) {
+      // Rank of the lane's first hit among the wave's T hits.
+      uint32_t x = nh;
+#pragma unroll
+      for (int dd = 1; dd < 64; dd <<= 1) {
+        const uint32_t y = __shfl_up(x, dd, 64);
+        if (lane >= (uint32_t)dd) x += y;
+      }
+      const uint32_t T = __shfl(x, 63, 64), off = x - nh;
+      nhits += T;
+      // Ranks [lo_r, lo_r + take) go to the ring per pass (one pass unless
+      // the ring is short of room), each pass followed by full rounds.
+      for (uint32_t lo_r = 0; lo_r < T;) {
+        const uint32_t take = min(kHitRing - pend, T - lo_r);
+        if (off < lo_r + take && off + nh > lo_r) {
+          uint32_t r = off;
+#pragma unroll
+          for (int j = 0; j < kUnitsPerIter; ++j) {
+#pragma unroll
+            for (int k = 0; k < NM; ++k)
+              for (uint32_t m = mask[j][k]; m; m &= m - 1, ++r)
+                if (r - lo_r < take)
+                  ring[(tail + r - lo_r) & (kHitRing - 1)] = make_uint2(p0[j] + WPOS * k + __builtin_ctz(m), item[j]);
+          }
+        }
+        tail += take;
+        pend += take;
+        lo_r += take;
+        wave_lds_sync();
+        while (pend >= 64) {
+          verify_queued<K, CI>(c, h, ring, tail - pend, 64, s_cnt, lit, cand);
+          pend -= 64;
+        }
+        wave_lds_sync();
+      }
+    }
   }
-  const uint32_t no = min(*hs.overflow_count, hs.overflow_cap);
-  for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < no; i += gridDim.x * kTPB) {
-    HitWork a;
-    hit_prepare<K, CI>(c, h, hs.overflow[i], hs.overflow_win[2 * (size_t)i], hs.overflow_win[2 * (size_t)i + 1], a);
-    hit_finish<CI>(c, h, a, s_n, lit, cand);
+  while (pend > 0) {
+    const uint32_t n = min(pend, 64u);
+    verify_queued<K, CI>(c, h, ring, tail - pend, n, s_cnt, lit, cand);
+    pend -= n;
   }
+  if (lane == 0 && nhits) atomicAdd(&c.counters[8], nhits);  // filter hits, for timings
   __syncthreads();
   if (threadIdx.x == 0) {
-    lit.counts[blockIdx.x] = s_n[0];
-    cand.counts[blockIdx.x] = s_n[1];
+    lit.counts[blockIdx.x] = s_cnt[0];
+    cand.counts[blockIdx.x] = s_cnt[1];
   }
 }
 
@@ -945,7 +957,7 @@ static hipError_t allow_lds(F* fn, size_t bytes) {
   return e;
 }
 
-size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u + 16u; }
+size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u + 16u + (kFilterTPB / 64) * kHitRing * sizeof(uint2); }
 
 template <int K, bool CI>
 static hipError_t prepare_hash_kc(uint32_t lane_bytes, size_t shmem) {
@@ -968,43 +980,31 @@ hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nword
 
 template <int K, bool CI>
 static hipError_t launch_hash_kc(uint32_t grid, hipStream_t st, const ScanCommon& c, const HashParams& h,
-                                 const HitSlices& hs, const SliceOut& lit, const SliceOut& cand,
-                                 bool verify_only) {
+                                 const SliceOut& lit, const SliceOut& cand) {
   const size_t shmem = hash_shmem_bytes(h.nwords);
-  if (!verify_only) {
-    if (h.lane_bytes == 64) {
-      hipError_t e = allow_lds(k_hash_filter<K, CI, 64>, shmem);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL((k_hash_filter<K, CI, 64>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, hs);
-    } else if (h.lane_bytes == 32) {
-      hipError_t e = allow_lds(k_hash_filter<K, CI, 32>, shmem);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL((k_hash_filter<K, CI, 32>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, hs);
-    } else {
-      hipError_t e = allow_lds(k_hash_filter<K, CI, 16>, shmem);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL((k_hash_filter<K, CI, 16>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, hs);
-    }
-    return hipGetLastError();
+  if (h.lane_bytes == 64) {
+    hipError_t e = allow_lds(k_hash_filter<K, CI, 64>, shmem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_hash_filter<K, CI, 64>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, lit, cand);
+  } else if (h.lane_bytes == 32) {
+    hipError_t e = allow_lds(k_hash_filter<K, CI, 32>, shmem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_hash_filter<K, CI, 32>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, lit, cand);
+  } else {
+    hipError_t e = allow_lds(k_hash_filter<K, CI, 16>, shmem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_hash_filter<K, CI, 16>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, lit, cand);
   }
-  hipLaunchKernelGGL((k_hash_verify<K, CI>), dim3(grid * kVerifySplit), dim3(kTPB), 0, st, c, h, hs, lit, cand);
   return hipGetLastError();
 }
 
-template <int K>
-static hipError_t launch_hash_k(bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c, const HashParams& h,
-                                const HitSlices& hs, const SliceOut& lit, const SliceOut& cand, bool verify_only) {
-  return ci ? launch_hash_kc<K, true>(grid, st, c, h, hs, lit, cand, verify_only)
-            : launch_hash_kc<K, false>(grid, st, c, h, hs, lit, cand, verify_only);
-}
-
-// verify_only = false launches the streaming filter, true the verify pass;
-// both use `grid` blocks (verify block b checks filter slice b).
+// The filter-and-verify kernel on `grid` blocks; block b writes output
+// slice b of lit and cand.
 hipError_t launch_hash(uint32_t K, bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c,
-                       const HashParams& h, const HitSlices& hs, const SliceOut& lit, const SliceOut& cand,
-                       bool verify_only) {
+                       const HashParams& h, const SliceOut& lit, const SliceOut& cand) {
   switch (K) {
-#define WS_K(k) case k: return launch_hash_k<k>(ci, grid, st, c, h, hs, lit, cand, verify_only);
+#define WS_K(k) \
+  case k: return ci ? launch_hash_kc<k, true>(grid, st, c, h, lit, cand) : launch_hash_kc<k, false>(grid, st, c, h, lit, cand);
     WS_K(1) WS_K(2) WS_K(3) WS_K(4) WS_K(6) WS_K(8) WS_K(12) WS_K(16)
 #undef WS_K
     default: return hipErrorInvalidValue;
@@ -1116,7 +1116,6 @@ hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32
   return hipGetLastError();
 }
 
-uint32_t verify_blocks(uint32_t filter_grid) { return filter_grid * kVerifySplit; }
 
 // Compacts the literal and candidate slices (one scan kernel, one gather).
 hipError_t launch_slice_compact2(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
@@ -1149,7 +1148,6 @@ hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_
 __global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, unsigned long long* total) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   uint32_t redo = 0;
-  if (counters[4] > caps.hit_ovf_cap) redo |= 1u;
   if (counters[5] > caps.lit_ovf_cap || counters[6] > caps.cand_ovf_cap) redo |= 2u;
   if (counters[kCounterLitSnapshot] > caps.lit_cap) redo |= 4u;
   if (counters[1] > caps.cand_cap) redo |= 8u;

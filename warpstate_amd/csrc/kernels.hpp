@@ -59,24 +59,8 @@ struct HashParams {
 constexpr uint32_t kVtabEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kVtabMore = 0x80000000u;
 
-// Filter hits {position, item}: one private slice per filter block plus a
-// shared overflow area.
-// Filter hits: {position, item} plus the 32 input bytes of the aligned
-// 32-byte span holding the position (read while the span is still in L2),
-// so verification does not re-read the input from HBM.
-struct HitSlices {
-  uint2* hits;
-  uint4* win;             // two uint4 per hit slot, parallel to hits
-  uint32_t slice_cap;
-  uint32_t* slice_count;  // per block
-  uint2* overflow;
-  uint4* overflow_win;
-  uint32_t overflow_cap;
-  uint32_t* overflow_count;
-};
-
 // Per-block output slices (no shared counter in the hot path) plus a shared
-// overflow area; k_slice_offsets + k_slice_gather compact them afterwards.
+// overflow area; k_slice_offsets2 + k_slice_gather2 compact them afterwards.
 struct SliceOut {
   Rec* buf;
   uint32_t cap;        // records per slice
@@ -119,7 +103,7 @@ enum : uint32_t { kFlagTruncatedWalk = 1u };
 // the counters slot that keeps the literal count before regex matches are
 // appended.
 struct RunCaps {
-  uint32_t hit_ovf_cap, lit_ovf_cap, cand_ovf_cap;
+  uint32_t lit_ovf_cap, cand_ovf_cap;
   uint32_t lit_cap, cand_cap, raw_cap, rec_cap, max_matches;
 };
 constexpr uint32_t kCounterLitSnapshot = 10;
