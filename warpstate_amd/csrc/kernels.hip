@@ -446,9 +446,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Streaming rolling-hash filter with in-kernel verification, one 1024-lane
 // block per CU so the bloom filter can use up to 128 KiB of LDS. Work goes
 // out in half units (a unit is 256 * PL bytes; the host lists a batch's
-// units): every wave takes a static stride of half units, and per iteration
-// a lane loads the (PL + 16)-byte windows of its two quarters with coalesced
-// 16-byte loads, then hashes PL positions in each. Tiles wholly inside their
+// units), in groups of 16 per block striding over the blocks; inside a block
+// the waves claim half units one at a time from an LDS counter, so waves the
+// scheduler serves less often take fewer and all finish together. Per half
+// unit a lane loads the (PL + 16)-byte windows of its two quarters with
+// coalesced 16-byte loads, then hashes PL positions in each. Tiles wholly inside their
 // segment skip all bounds work. A wave appends its hits (wave prefix sum) to
 // its own LDS ring and verifies them 64 at a time while the block's other
 // waves keep filtering, so the verification's dependent L2 reads hide
@@ -460,10 +462,11 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
                                                              SliceOut cand) {
   // LDS: the bloom words from address 0 (the kernel has no static LDS, so
   // a word's address is the masked hash itself), the block's two output
-  // counters, then one hit ring per wave.
+  // counters and its work counter, then one hit ring per wave.
   extern __shared__ uint32_t smem[];
   uint32_t* bloom = smem;
   uint32_t* s_cnt = &smem[h.nwords];
+  uint32_t* s_work = &smem[h.nwords + 2];
   uint2* ring = reinterpret_cast<uint2*>(&smem[h.nwords + 4]) + (threadIdx.x / 64) * kHitRing;
   constexpr int NW = PL / 4 + 4;  // window dwords
   constexpr uint32_t kUnit = kTPB * PL;
@@ -474,13 +477,25 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
   if (threadIdx.x == 0) {
     s_cnt[0] = 0;
     s_cnt[1] = 0;
+    *s_work = 0;
   }
   __syncthreads();
 
   const uint32_t load_end = (uint32_t)c.load_end;
   const uint32_t nslots = 2 * c.nunits;
+  // The block's jw-th half unit: slot jw % 16 of group blockIdx.x +
+  // (jw / 16) * gridDim.x (increasing in jw). The next claim is issued a
+  // half unit ahead.
+  auto claim = [&]() {
+    uint32_t v = 0;
+    if (lane == 0) v = atomicAdd(s_work, 1u);
+    return __builtin_amdgcn_readfirstlane(v);
+  };
   uint32_t tail = 0, pend = 0, nhits = 0;  // the wave's ring (same in every lane)
-  for (uint32_t sl = blockIdx.x * kWaves + threadIdx.x / 64; sl < nslots; sl += gridDim.x * kWaves) {
+  for (uint32_t jw = claim();;) {
+    const uint32_t sl = (blockIdx.x + (jw / kWaves) * gridDim.x) * kWaves + jw % kWaves;
+    if (sl >= nslots) break;
+    jw = claim();
     const uint32_t u = sl >> 1;
     uint32_t tile_base, it = 0;
     if (c.units) {
