@@ -185,29 +185,45 @@ __device__ __forceinline__ u16x2 upk2(uint32_t v) { return __builtin_bit_cast(u1
 // polynomial hashes rolled together by v_pk_mad_u16). The bloom word's byte
 // address is D & amask, its two bits D[20:16] and D[28:24], which the shifts
 // take straight from the upper half and top byte of D (SDWA operand
-// selects): per position one byte extract, two packed multiply-adds, one
-// address op, the LDS read, two shifts, an and and the mask update.
+// selects). The window bytes are first spread into 16-bit pairs (one v_perm
+// per two bytes), so a multiply-add takes its byte as either half of a pair
+// (VOP3P operand selects): per position about half a perm, two packed
+// multiply-adds, one address op, the LDS read, two shifts, an and and the
+// mask update.
 template <int K, bool CI, int PL>
 __device__ __forceinline__ void filter_lane_dual(const HashParams& h, const uint32_t* bloom, uint32_t* d,
                                                  uint32_t* out) {
+  constexpr int NW = PL / 4 + 4;
   if (CI) {
 #pragma unroll
-    for (int j = 0; j < PL / 4 + 4; ++j) d[j] = fold_word(d[j]);
+    for (int j = 0; j < NW; ++j) d[j] = fold_word(d[j]);
   }
-  const u16x2 bb = upk2(h.BB), nbk = upk2(h.NBKK);
-  auto splat = [](uint32_t b) { return u16x2{(unsigned short)b, (unsigned short)b}; };
-  uint32_t D = pk2(splat(byte_of(d, 0))), mask = 0;
+  // pr[2m] = {byte 4m, byte 4m+1}, pr[2m+1] = {byte 4m+2, byte 4m+3}.
+  u16x2 pr[2 * NW];
 #pragma unroll
-  for (int t = 1; t < K; ++t) D = pk2(upk2(D) * bb + splat(byte_of(d, t)));
+  for (int m = 0; m < NW; ++m) {
+    pr[2 * m] = upk2(__builtin_amdgcn_perm(0u, d[m], 0x0c010c00u));
+    pr[2 * m + 1] = upk2(__builtin_amdgcn_perm(0u, d[m], 0x0c030c02u));
+  }
+  auto at = [&](int t) -> u16x2 {
+    const u16x2 v = pr[t >> 1];
+    return (t & 1) ? __builtin_shufflevector(v, v, 1, 1) : __builtin_shufflevector(v, v, 0, 0);
+  };
+  const u16x2 bb = upk2(h.BB), nbk = upk2(h.NBKK);
+  u16x2 D = at(0);
+  uint32_t mask = 0;
+#pragma unroll
+  for (int t = 1; t < K; ++t) D = D * bb + at(t);
   // The bloom words sit at LDS address 0 (k_hash_filter has no static LDS),
   // so the masked hash is the ds_read address as it stands.
   (void)bloom;
   const __attribute__((address_space(3))) char* lds = nullptr;
 #pragma unroll
   for (int i = 0; i < PL; ++i) {
-    if (i > 0) D = pk2(upk2(D) * bb + (splat(byte_of(d, i - 1)) * nbk + splat(byte_of(d, i + K - 1))));
-    const uint32_t w = *reinterpret_cast<const __attribute__((address_space(3))) uint32_t*>(lds + (D & h.amask));
-    const uint32_t hit = (w >> ((D >> 16) & 31u)) & (w >> ((D >> 24) & 31u));
+    if (i > 0) D = D * bb + (at(i - 1) * nbk + at(i + K - 1));
+    const uint32_t Dw = pk2(D);
+    const uint32_t w = *reinterpret_cast<const __attribute__((address_space(3))) uint32_t*>(lds + (Dw & h.amask));
+    const uint32_t hit = (w >> ((Dw >> 16) & 31u)) & (w >> ((Dw >> 24) & 31u));
     mask = __builtin_amdgcn_alignbit(hit, mask, 1);
     if ((i & 31) == 31) out[i >> 5] = mask;
   }
@@ -240,29 +256,30 @@ __device__ __forceinline__ void slice_put(const SliceOut& so, uint32_t slice, ui
   }
 }
 
-// Dword k of the byte stream that starts at byte 4*dw + bs/8 of r (nr dwords;
-// dw is dynamic, so the select compiles to v_cndmask chains).
-template <int NR>
-__device__ __forceinline__ uint32_t stream_dword(const uint32_t* r, uint32_t dw, uint32_t bs, int k) {
-  uint32_t lo = 0, hi = 0;
+// The 32 bytes from q as eight dwords, out of the 48-byte aligned span
+// raw[0..11] that holds them (q - (q & ~15) = sh): two operand selects by
+// the dword shift, then one v_alignbyte per output dword.
+// (Bit selects through all-ones / zero masks, v_bfi_b32, so the compiler
+// keeps the span in registers instead of indexing a stack copy.)
+__device__ __forceinline__ void realign32(const uint32_t* raw, uint32_t sh, uint32_t* W) {
+  const uint32_t m2 = 0u - ((sh >> 3) & 1u), m1 = 0u - ((sh >> 2) & 1u);
+  uint32_t t[10], u[9];
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
-    if (dw == (uint32_t)t) {
-      lo = k + t < NR ? r[k + t] : 0u;
-      hi = k + t + 1 < NR ? r[k + t + 1] : 0u;
-    }
-  return bs ? ((lo >> bs) | (hi << (32 - bs))) : lo;
+  for (int i = 0; i < 10; ++i) t[i] = (raw[i + 2] & m2) | (raw[i] & ~m2);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) u[i] = (t[i + 1] & m1) | (t[i] & ~m1);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) W[i] = __builtin_amdgcn_alignbyte(u[i + 1], u[i], sh & 3);
 }
 
 // Exact comparison of one table entry ({hash, factor byte offset (16-byte
 // aligned), length | kind << 31, bytes 0-3}, {bytes 4-15, output id}) at
-// position q. On a match *rec is the output record and the result is
-// 1 (literal) or 2 (regex candidate); 0 otherwise. w holds the input bytes
-// from q (avail of them, at least 17); r is the hit's raw aligned span.
+// position q. W holds the (folded, when CI) input bytes q..q+31. On a match
+// *rec is the output record and the result is 1 (literal) or 2 (regex
+// candidate); 0 otherwise.
 template <bool CI>
-__device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, uint4 e1, const uint32_t* w,
-                                                 const uint32_t* r, uint32_t avail, uint64_t q, uint2 seg,
-                                                 uint32_t item, uint4* rec) {
+__device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, uint4 e1, const uint32_t* W,
+                                                 uint64_t q, uint2 seg, uint32_t item, uint4* rec) {
   const uint32_t len = e0.z & 0x7FFFFFFFu;
   if (q + len > seg.y) return 0;
   const uint32_t pre[4] = {e0.w, e1.x, e1.y, e1.z};
@@ -271,37 +288,21 @@ __device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, 
     if (4u * k >= len) break;
     const uint32_t rem = len - 4 * k;
     const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
-    if ((w[k] ^ pre[k]) & msk) return 0;
+    if ((W[k] ^ pre[k]) & msk) return 0;
   }
   const uint32_t start = (uint32_t)(q - seg.x);
   const uint32_t kind = (e0.z >> 31) ? 2u : 1u;
   *rec = make_uint4(e1.w, start, kind == 2u ? 0u : start + len, item);
   if (len > 16) {
-    // Bytes 16..31 of the factor in one load, against the record span
-    // extended by one aligned 16-byte input load when the factor runs past
-    // it (q + len <= seg.y <= load_end, which is 16-byte aligned).
+    // Bytes 16..31 of the factor in one load.
     const uint4 p1 = *reinterpret_cast<const uint4*>(c.fbytes + e0.y + 16);
-    uint32_t rr[12];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) rr[t] = r[t];
-    rr[8] = rr[9] = rr[10] = rr[11] = 0;
-    if (len > avail) {
-      const uint64_t a = (q & ~15ull) + 32;
-      if (a + 16 <= c.load_end) {
-        const uint4 x2 = *reinterpret_cast<const uint4*>(c.base + a);
-        rr[8] = x2.x; rr[9] = x2.y; rr[10] = x2.z; rr[11] = x2.w;
-      }
-    }
-    const uint32_t sh = (uint32_t)(q & 15), dw = sh >> 2, bs = (sh & 3) * 8;
     const uint32_t pw[4] = {p1.x, p1.y, p1.z, p1.w};
     uint32_t diff = 0;
 #pragma unroll
     for (int k = 4; k < 8; ++k) {
-      uint32_t dv = stream_dword<12>(rr, dw, bs, k);
-      if (CI) dv = fold_word(dv);
       const uint32_t rem = len > 4u * k ? len - 4 * k : 0u;
       const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
-      diff |= (dv ^ pw[k - 4]) & msk;
+      diff |= (W[k] ^ pw[k - 4]) & msk;
     }
     if (diff) return 0;
     // Factors longer than 32 bytes: the rest byte by byte.
@@ -336,41 +337,37 @@ __device__ __forceinline__ void emit_wave(const SliceOut& so, uint32_t* s_cnt, b
   if (take) slice_put(so, blockIdx.x, base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)), rec);
 }
 
-// A filter hit being verified: its position, item, segment, the raw span and
-// the first 16 bytes from q, and the window hash.
+// A filter hit being verified: its position, item, segment, the input
+// bytes from q, and the window hash.
 struct HitWork {
   uint64_t q;
   uint32_t item;
   uint2 seg;
-  uint32_t r[8];
-  uint32_t w[4];
-  uint32_t avail, H;
+  uint32_t W[8];
+  uint32_t H;
   uint4 s0, s1, s2, s3;  // its bucket's two slots
-  bool live;             // false for a lane's unused second hit of a round
+  bool live;             // false for a lane without a hit this round
 };
 
+// raw: the 48-byte aligned span from q & ~15 (zeros past the loadable end).
 template <int K, bool CI>
-__device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParams& h, uint2 hr, uint4 x0, uint4 x1,
+__device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParams& h, uint2 hr, const uint32_t* raw,
                                             HitWork& hw) {
   hw.live = true;
   hw.q = hr.x;
   hw.item = hr.y;
   hw.seg = c.items[hw.item];
-  // Bytes [q, q + avail) come from the hit record's 32-byte aligned span
-  // (avail >= 17, so the K <= 16 window is always there).
-  hw.r[0] = x0.x; hw.r[1] = x0.y; hw.r[2] = x0.z; hw.r[3] = x0.w;
-  hw.r[4] = x1.x; hw.r[5] = x1.y; hw.r[6] = x1.z; hw.r[7] = x1.w;
-  const uint32_t sh = (uint32_t)(hw.q & 15);
-  hw.avail = 32 - sh;
-  const uint32_t dw = sh >> 2, bs = (sh & 3) * 8;
+  realign32(raw, (uint32_t)(hw.q & 15), hw.W);
+  if (CI) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t v = stream_dword<8>(hw.r, dw, bs, k);
-    hw.w[k] = CI ? fold_word(v) : v;
+    for (int k = 0; k < 8; ++k) hw.W[k] = fold_word(hw.W[k]);
   }
+  // The verification table's 24-bit window hash (v_mad_u32_u24 reads the
+  // low 24 bits of the running value).
   uint32_t H = 0;
 #pragma unroll
-  for (int t = 0; t < K; ++t) H = (H * h.B + ((hw.w[t >> 2] >> ((t & 3) * 8)) & 0xFFu)) & 0xFFFFFFu;
+  for (int t = 0; t < K; ++t) H = mad24_s(H, h.B, (hw.W[t >> 2] >> ((t & 3) * 8)) & 0xFFu);
+  H &= 0xFFFFFFu;
   hw.H = H;
   const uint4* slot = h.vtab + 4 * (size_t)((H * 0x9E3779B1u) >> (32 - h.nbits));
   hw.s0 = slot[0];
@@ -385,18 +382,17 @@ __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams
   uint4 r0 = make_uint4(0, 0, 0, 0), r1 = r0;
   uint32_t k0 = 0, k1 = 0;
   if (hw.live && hw.s0.x == hw.H)
-    k0 = verify_entry<CI>(c, hw.s0, hw.s1, hw.w, hw.r, hw.avail, hw.q, hw.seg, hw.item, &r0);
+    k0 = verify_entry<CI>(c, hw.s0, hw.s1, hw.W, hw.q, hw.seg, hw.item, &r0);
   if (!hw.live) {
   } else if (hw.s2.x == hw.H) {
-    k1 = verify_entry<CI>(c, hw.s2, hw.s3, hw.w, hw.r, hw.avail, hw.q, hw.seg, hw.item, &r1);
+    k1 = verify_entry<CI>(c, hw.s2, hw.s3, hw.W, hw.q, hw.seg, hw.item, &r1);
   } else if (hw.s2.x == kVtabMore) {
     // Crowded bucket (rare): per-lane appends.
     for (uint32_t e = hw.s2.y; e < hw.s2.y + hw.s2.z; ++e) {
       const uint4 e0 = h.vmore[2 * (size_t)e];
       if (e0.x != hw.H) continue;
       uint4 rec;
-      const uint32_t k = verify_entry<CI>(c, e0, h.vmore[2 * (size_t)e + 1], hw.w, hw.r, hw.avail, hw.q, hw.seg,
-                                          hw.item, &rec);
+      const uint32_t k = verify_entry<CI>(c, e0, h.vmore[2 * (size_t)e + 1], hw.W, hw.q, hw.seg, hw.item, &rec);
       if (k == 1) slice_put(lit, blockIdx.x, atomicAdd(&s_n[0], 1u), rec);
       if (k == 2) slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), rec);
     }
@@ -412,7 +408,7 @@ __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams
 constexpr uint32_t kHitRing = 128;
 
 // Verifies the n (<= 64) queued hits from ring index `head`, one per lane:
-// the 32-byte aligned input span holding the position (just streamed by the
+// the 48-byte aligned input span holding bytes q..q+31 (just streamed by the
 // filter, so an L2 hit), the window hash, its bucket, the exact compare.
 // Call with the whole wave.
 template <int K, bool CI>
@@ -428,10 +424,14 @@ __device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashPar
     uint32_t lend = (seg.y + 15) & ~15u;
     if (lend > (uint32_t)c.load_end) lend = (uint32_t)c.load_end;
     const uint32_t a = hr.x & ~15u;
-    uint4 x0 = make_uint4(0, 0, 0, 0), x1 = x0;
-    if (a + 16 <= lend) x0 = *reinterpret_cast<const uint4*>(c.base + a);
-    if (a + 32 <= lend) x1 = *reinterpret_cast<const uint4*>(c.base + a + 16);
-    hit_prepare<K, CI>(c, h, hr, x0, x1, hw);
+    uint32_t raw[12];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      uint4 x = make_uint4(0, 0, 0, 0);
+      if (a + 16 * v + 16 <= lend) x = *reinterpret_cast<const uint4*>(c.base + a + 16 * v);
+      raw[4 * v] = x.x; raw[4 * v + 1] = x.y; raw[4 * v + 2] = x.z; raw[4 * v + 3] = x.w;
+    }
+    hit_prepare<K, CI>(c, h, hr, raw, hw);
   }
   hit_finish<CI>(c, h, hw, s_cnt, lit, cand);
 }
@@ -525,12 +525,22 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
       hi[j] = seg.y;
       inner[j] = in_unit;
       lend[j] = le;
+      if (in_unit) {
+        // One address, immediate offsets, no per-load predicate.
+        const uint4* src = reinterpret_cast<const uint4*>(c.base + p0[j]);
 #pragma unroll
-      for (int v = 0; v < NW / 4; ++v) {
-        uint4 x = make_uint4(0, 0, 0, 0);
-        const uint32_t a = p0[j] + 16 * v;
-        if (inner[j] || (a + 16 <= lend[j] && a + 16 > (lo[j] & ~15u))) x = *reinterpret_cast<const uint4*>(c.base + a);
-        d[j][4 * v] = x.x; d[j][4 * v + 1] = x.y; d[j][4 * v + 2] = x.z; d[j][4 * v + 3] = x.w;
+        for (int v = 0; v < NW / 4; ++v) {
+          const uint4 x = src[v];
+          d[j][4 * v] = x.x; d[j][4 * v + 1] = x.y; d[j][4 * v + 2] = x.z; d[j][4 * v + 3] = x.w;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NW / 4; ++v) {
+          uint4 x = make_uint4(0, 0, 0, 0);
+          const uint32_t a = p0[j] + 16 * v;
+          if (a + 16 <= lend[j] && a + 16 > (lo[j] & ~15u)) x = *reinterpret_cast<const uint4*>(c.base + a);
+          d[j][4 * v] = x.x; d[j][4 * v + 1] = x.y; d[j][4 * v + 2] = x.z; d[j][4 * v + 3] = x.w;
+        }
       }
     }
     constexpr int NM = mask_words<PL>();
