@@ -496,7 +496,7 @@ uint32_t choose_lane_bytes(const Plan& plan, const ws_tuning* tuning) {
 // lists its units in work-queue order: largest items first
 // (longest-processing-time first, by power-of-two size class: a counting
 // sort, O(items)), each item's tiles contiguous.
-uint32_t plan_units(const Plan& plan, uint32_t unit, std::vector<uint2>* units) {
+uint32_t plan_units(const Plan& plan, uint32_t unit, std::vector<uint4>* units) {
   if (!plan.batch) return (uint32_t)((plan.items[0].y + unit - 1) / unit);
   const size_t ni = plan.items.size();
   auto cls = [&](size_t i) {
@@ -516,7 +516,8 @@ uint32_t plan_units(const Plan& plan, uint32_t unit, std::vector<uint2>* units) 
   units->reserve(total_units);
   for (uint32_t i : by_size) {
     uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
-    for (uint64_t t = lo; t < hi; t += unit) units->push_back(make_uint2((uint32_t)t, i));
+    for (uint64_t t = lo; t < hi; t += unit)
+      units->push_back(make_uint4((uint32_t)t, i, plan.items[i].x, plan.items[i].y));
   }
   return (uint32_t)units->size();
 }
@@ -543,7 +544,7 @@ ScanCommon make_common(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint
   ScanCommon c{};
   c.base = plan.base;
   c.load_end = plan.load_end;
-  c.units = plan.batch ? ctx->units.as<uint2>() : nullptr;
+  c.units = plan.batch ? ctx->units.as<uint4>() : nullptr;
   c.nunits = nunits;
   c.items = ctx->items.as<uint2>();
   c.emit_end = plan.emit_end;
@@ -561,6 +562,7 @@ ScanCommon make_common(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint
   // Batches pull uneven work from a device queue (zeroed with the counters);
   // a single input's equal tiles use a static stride.
   c.queue = plan.batch ? ctx->counters.as<uint32_t>() + 7 : nullptr;
+  c.groups = ctx->counters.as<uint32_t>() + 7;
   return c;
 }
 
@@ -594,14 +596,14 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
   const uint32_t lane_bytes = choose_lane_bytes(plan, tuning);
   const uint32_t unit = variant == WS_VARIANT_HASH ? kTPB * lane_bytes : ac_unit_bytes();
-  std::vector<uint2> units;
+  std::vector<uint4> units;
   const uint32_t nunits = plan_units(plan, unit, &units);
   WS_HIP(ctx->items.ensure(plan.items.size() * sizeof(uint2)));
   WS_HIP(hipMemcpyAsync(ctx->items.p, plan.items.data(), plan.items.size() * sizeof(uint2),
                         hipMemcpyHostToDevice, st));
   if (plan.batch && !units.empty()) {
-    WS_HIP(ctx->units.ensure(units.size() * sizeof(uint2)));
-    WS_HIP(hipMemcpyAsync(ctx->units.p, units.data(), units.size() * sizeof(uint2),
+    WS_HIP(ctx->units.ensure(units.size() * sizeof(uint4)));
+    WS_HIP(hipMemcpyAsync(ctx->units.p, units.data(), units.size() * sizeof(uint4),
                           hipMemcpyHostToDevice, st));
   }
   const uint32_t max_matches = (uint32_t)std::min<uint64_t>(ctx->cfg.max_matches, 0xFFFFFFF0ull);
@@ -923,7 +925,7 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
     return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
   rs->lane_bytes = choose_lane_bytes(plan, tuning);
   const uint32_t unit = rs->variant == WS_VARIANT_HASH ? kTPB * rs->lane_bytes : ac_unit_bytes();
-  std::vector<uint2> none;
+  std::vector<uint4> none;
   rs->nunits = plan_units(plan, unit, &none);
   const uint64_t span = plan.items[0].y - plan.items[0].x;
   rs->max_matches = (uint32_t)std::min<uint64_t>(ctx->cfg.max_matches, 0xFFFFFFF0ull);

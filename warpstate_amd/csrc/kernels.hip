@@ -406,6 +406,9 @@ __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams
 // Per-wave LDS ring of filter hits waiting for verification (uint2
 // {position, item}); a wave verifies 64 at a time, one per lane.
 constexpr uint32_t kHitRing = 128;
+// Per-block table of the groups (16 half units) the block has taken, by
+// batch number modulo its size: {batch, group} in one 64-bit LDS word.
+constexpr uint32_t kGroupTab = 16;
 
 // Verifies the n (<= 64) queued hits from ring index `head`, one per lane:
 // the 48-byte aligned input span holding bytes q..q+31 (just streamed by the
@@ -446,11 +449,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Streaming rolling-hash filter with in-kernel verification, one 1024-lane
 // block per CU so the bloom filter can use up to 128 KiB of LDS. Work goes
 // out in half units (a unit is 256 * PL bytes; the host lists a batch's
-// units), in groups of 16 per block striding over the blocks; inside a block
-// the waves claim half units one at a time from an LDS counter, so waves the
-// scheduler serves less often take fewer and all finish together. Per half
-// unit a lane loads the (PL + 16)-byte windows of its two quarters with
-// coalesced 16-byte loads, then hashes PL positions in each. Tiles wholly inside their
+// units), 16 to a group. Each block takes groups one after another (its
+// first two statically, the rest from a device counter) and its waves take
+// the half units of those groups one at a time from an LDS counter, so
+// faster CUs, and the waves the scheduler serves more often, do more and
+// all finish together. Per half unit a lane loads the (PL + 16)-byte
+// windows of its two quarters with coalesced 16-byte loads, then hashes PL
+// positions in each. Tiles wholly inside their
 // segment skip all bounds work. A wave appends its hits (wave prefix sum) to
 // its own LDS ring and verifies them 64 at a time while the block's other
 // waves keep filtering, so the verification's dependent L2 reads hide
@@ -462,18 +467,25 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
                                                              SliceOut cand) {
   // LDS: the bloom words from address 0 (the kernel has no static LDS, so
   // a word's address is the masked hash itself), the block's two output
-  // counters and its work counter, then one hit ring per wave.
+  // counters and its work counter, its group table, then one hit ring per
+  // wave.
   extern __shared__ uint32_t smem[];
   uint32_t* bloom = smem;
   uint32_t* s_cnt = &smem[h.nwords];
   uint32_t* s_work = &smem[h.nwords + 2];
-  uint2* ring = reinterpret_cast<uint2*>(&smem[h.nwords + 4]) + (threadIdx.x / 64) * kHitRing;
+  uint64_t* s_tab = reinterpret_cast<uint64_t*>(&smem[h.nwords + 4]);
+  uint2* ring = reinterpret_cast<uint2*>(&smem[h.nwords + 4 + 2 * kGroupTab]) + (threadIdx.x / 64) * kHitRing;
   constexpr int NW = PL / 4 + 4;  // window dwords
   constexpr uint32_t kUnit = kTPB * PL;
   constexpr uint32_t kWaves = kFilterTPB / 64;
   const uint32_t lane = lane_id();
 
   for (uint32_t k = threadIdx.x; k < h.nwords; k += kFilterTPB) bloom[k] = h.bloom[k];
+  if (threadIdx.x < kGroupTab) {
+    // Batches 0 and 1 are the block's static groups.
+    const uint64_t b = threadIdx.x;
+    s_tab[threadIdx.x] = b < 2 ? (uint64_t)(blockIdx.x + b * gridDim.x) << 32 | b : ~0ull;
+  }
   if (threadIdx.x == 0) {
     s_cnt[0] = 0;
     s_cnt[1] = 0;
@@ -483,87 +495,142 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
 
   const uint32_t load_end = (uint32_t)c.load_end;
   const uint32_t nslots = 2 * c.nunits;
-  // The block's jw-th half unit: slot jw % 16 of group blockIdx.x +
-  // (jw / 16) * gridDim.x (increasing in jw). The next claim is issued a
-  // half unit ahead.
-  auto claim = [&]() {
+  // The block's jw-th half unit is slot jw % 16 of its batch jw / 16. The
+  // wave that takes batch k's first slot publishes the group of batch k + 2
+  // from the device counter, after batch k + 1's (so a block's groups, and
+  // with them its slots, increase along its claims: the first slot past
+  // the end means the block is done). A batch is thus published, or its
+  // counter read in flight, by the time any of its slots is taken, and the
+  // waits below cover only that flight. A table entry is rewritten 14
+  // batches (224 claims) after the one it holds, long after every claim of
+  // its batch has read it (the read follows the claim at once).
+  auto batch_group = [&](uint32_t k) -> uint32_t {
+    uint64_t e = __atomic_load_n(&s_tab[k % kGroupTab], __ATOMIC_RELAXED);
+    while ((uint32_t)e != k) {
+      __builtin_amdgcn_s_sleep(1);
+      e = __atomic_load_n(&s_tab[k % kGroupTab], __ATOMIC_RELAXED);
+    }
+    return (uint32_t)(e >> 32);
+  };
+  auto claim = [&]() -> uint32_t {
     uint32_t v = 0;
     if (lane == 0) v = atomicAdd(s_work, 1u);
-    return __builtin_amdgcn_readfirstlane(v);
-  };
-  uint32_t tail = 0, pend = 0, nhits = 0;  // the wave's ring (same in every lane)
-  for (uint32_t jw = claim();;) {
-    const uint32_t sl = (blockIdx.x + (jw / kWaves) * gridDim.x) * kWaves + jw % kWaves;
-    if (sl >= nslots) break;
-    jw = claim();
-    const uint32_t u = sl >> 1;
-    uint32_t tile_base, it = 0;
-    if (c.units) {
-      const uint2 uu = c.units[u];
-      tile_base = uu.x;
-      it = uu.y;
-    } else {
-      tile_base = u * kUnit;
+    const uint32_t jw = __builtin_amdgcn_readfirstlane(v);
+    const uint32_t k = jw / kWaves;
+    uint32_t grp = 0;
+    if (lane == 0) {
+      if (jw % kWaves == 0) {
+        (void)batch_group(k + 1);
+        const uint32_t g = 2 * gridDim.x + atomicAdd(c.groups, 1u);
+        __atomic_store_n(&s_tab[(k + 2) % kGroupTab], (uint64_t)g << 32 | (k + 2), __ATOMIC_RELAXED);
+      }
+      grp = batch_group(k);
     }
-    const uint2 seg = c.items[it];
-    // Inner tile: every window byte lies inside the segment and every
-    // position may start a match (the common case of a large scan).
-    const bool in_unit = tile_base >= seg.x && tile_base + kUnit + 16 <= seg.y &&
-                         tile_base + kUnit - seg.x + 0u <= c.emit_end;
-    uint32_t le = (seg.y + 15) & ~15u;
-    if (le > load_end) le = load_end;
-    uint32_t d[kUnitsPerIter][NW];
-    uint32_t p0[kUnitsPerIter], lo[kUnitsPerIter], hi[kUnitsPerIter], item[kUnitsPerIter],
-        lend[kUnitsPerIter];
-    bool inner[kUnitsPerIter];
+    return __builtin_amdgcn_readfirstlane(grp) * kWaves + jw % kWaves;
+  };
+  // A half unit's bounds (wave-uniform): from its unit's record in a
+  // batch, else from the one segment, read once.
+  struct Half {
+    uint32_t s, tile_base, item, lo, hi, lend;
+    uint32_t inner;  // every window byte inside the segment, every position may start a match
+  };
+  const uint2 seg0 = c.units ? make_uint2(0, 0) : c.items[0];
+  auto half_info = [&](uint32_t sv, uint4 rec) {
+    Half hf;
+    hf.s = sv;
+    if (c.units) {
+      hf.tile_base = rec.x;
+      hf.item = rec.y;
+      hf.lo = rec.z;
+      hf.hi = rec.w;
+    } else {
+      hf.tile_base = (sv >> 1) * kUnit;
+      hf.item = 0;
+      hf.lo = seg0.x;
+      hf.hi = seg0.y;
+    }
+    hf.inner = hf.tile_base >= hf.lo && hf.tile_base + kUnit + 16 <= hf.hi &&
+               hf.tile_base + kUnit - hf.lo + 0u <= c.emit_end;
+    hf.lend = (hf.hi + 15) & ~15u;
+    if (hf.lend > load_end) hf.lend = load_end;
+    return hf;
+  };
+  // A batch unit's record, loaded one half unit ahead of its use.
+  auto unit_rec = [&](uint32_t sv) { return c.units && sv < nslots ? c.units[sv >> 1] : make_uint4(0, 0, 0, 0); };
+  // Quarter 2 * (s & 1) + j of unit s / 2: 64 lanes x PL bytes.
+  auto quarter_p0 = [&](const Half& hf, int j) { return hf.tile_base + PL * (64 * (2 * (hf.s & 1) + j) + lane); };
+  // Granules outside the segment are clamped to its first or last one: one
+  // load per granule on every path (so the waits before each use can count
+  // them), and positions whose window leaves the segment are masked out
+  // afterwards, whatever bytes they hashed.
+  auto load_quarter = [&](const Half& hf, int j, uint32_t* dj) {
+    const uint32_t p0 = quarter_p0(hf, j);
+    const uint32_t amin = hf.lo & ~15u, amax = hf.lend - 16;
 #pragma unroll
-    for (int j = 0; j < kUnitsPerIter; ++j) {
-      // Quarter 2 * (sl & 1) + j of unit u: 64 lanes x PL bytes.
-      p0[j] = tile_base + PL * (64 * (2 * (sl & 1) + j) + lane);
-      item[j] = it;
-      lo[j] = seg.x;
-      hi[j] = seg.y;
-      inner[j] = in_unit;
-      lend[j] = le;
-      if (in_unit) {
-        // One address, immediate offsets, no per-load predicate.
-        const uint4* src = reinterpret_cast<const uint4*>(c.base + p0[j]);
+    for (int v = 0; v < NW / 4; ++v) {
+      uint32_t a = p0 + 16 * v;
+      a = a < amin ? amin : (a > amax ? amax : a);
+      const uint4 x = *reinterpret_cast<const uint4*>(c.base + a);
+      dj[4 * v] = x.x; dj[4 * v + 1] = x.y; dj[4 * v + 2] = x.z; dj[4 * v + 3] = x.w;
+    }
+  };
+  constexpr int NM = mask_words<PL>();
+  constexpr uint32_t WPOS = PL < 32 ? PL : 32;  // positions per mask word
+  auto filter_quarter = [&](const Half& hf, int j, uint32_t* dj, uint32_t* mj) {
+    const uint32_t p0 = quarter_p0(hf, j);
 #pragma unroll
-        for (int v = 0; v < NW / 4; ++v) {
-          const uint4 x = src[v];
-          d[j][4 * v] = x.x; d[j][4 * v + 1] = x.y; d[j][4 * v + 2] = x.z; d[j][4 * v + 3] = x.w;
-        }
-      } else {
+    for (int k = 0; k < NM; ++k) mj[k] = 0;
+    if (hf.inner || (p0 < hf.hi && p0 + PL > hf.lo)) {
+      if constexpr (K >= 3)
+        filter_lane_dual<K, CI, PL>(h, bloom, dj, mj);
+      else
+        filter_lane<K, CI, PL>(h, bloom, dj, mj);
+      if (!hf.inner) {
 #pragma unroll
-        for (int v = 0; v < NW / 4; ++v) {
-          uint4 x = make_uint4(0, 0, 0, 0);
-          const uint32_t a = p0[j] + 16 * v;
-          if (a + 16 <= lend[j] && a + 16 > (lo[j] & ~15u)) x = *reinterpret_cast<const uint4*>(c.base + a);
-          d[j][4 * v] = x.x; d[j][4 * v + 1] = x.y; d[j][4 * v + 2] = x.z; d[j][4 * v + 3] = x.w;
-        }
+        for (int k = 0; k < NM; ++k) mj[k] &= valid_word(p0 + WPOS * k, WPOS, hf.lo, hf.hi, K, c.emit_end);
       }
     }
-    constexpr int NM = mask_words<PL>();
-    constexpr uint32_t WPOS = PL < 32 ? PL : 32;  // positions per mask word
+  };
+
+  // Software pipeline over the wave's half units: a quarter's window is
+  // reloaded with the next half unit's as soon as it has been hashed, so
+  // one quarter's loads are always in flight behind the arithmetic. (After
+  // the last half unit the loads repeat it: every path issues the same
+  // loads, so the waits before each use can count them.)
+  uint32_t tail = 0, pend = 0, nhits = 0;  // the wave's ring (same in every lane)
+  uint32_t d[kUnitsPerIter][NW];
+  uint32_t cur_s = claim(), nxt_s = nslots;
+  Half cur{};
+  uint4 nxt_rec = make_uint4(0, 0, 0, 0);
+  if (cur_s < nslots) {
+    cur = half_info(cur_s, unit_rec(cur_s));
+    load_quarter(cur, 0, d[0]);
+    load_quarter(cur, 1, d[1]);
+    nxt_s = claim();
+    nxt_rec = unit_rec(nxt_s);
+  }
+  while (cur_s < nslots) {
+    const bool more = nxt_s < nslots;
+    const Half nxt = more ? half_info(nxt_s, nxt_rec) : cur;
     uint32_t mask[kUnitsPerIter][NM];
+    filter_quarter(cur, 0, d[0], mask[0]);
+    load_quarter(nxt, 0, d[0]);
+    filter_quarter(cur, 1, d[1], mask[1]);
+    load_quarter(nxt, 1, d[1]);
+    uint32_t p0[kUnitsPerIter], item[kUnitsPerIter];
     uint32_t nh = 0;
 #pragma unroll
     for (int j = 0; j < kUnitsPerIter; ++j) {
-#pragma unroll
-      for (int k = 0; k < NM; ++k) mask[j][k] = 0;
-      if (inner[j] || (p0[j] < hi[j] && p0[j] + PL > lo[j])) {
-        if constexpr (K >= 3)
-          filter_lane_dual<K, CI, PL>(h, bloom, d[j], mask[j]);
-        else
-          filter_lane<K, CI, PL>(h, bloom, d[j], mask[j]);
-        if (!inner[j]) {
-#pragma unroll
-          for (int k = 0; k < NM; ++k)
-            mask[j][k] &= valid_word(p0[j] + WPOS * k, WPOS, lo[j], hi[j], K, c.emit_end);
-        }
-      }
+      p0[j] = quarter_p0(cur, j);
+      item[j] = cur.item;
 #pragma unroll
       for (int k = 0; k < NM; ++k) nh += __builtin_popcount(mask[j][k]);
+    }
+    cur_s = nxt_s;
+    cur = nxt;
+    if (more) {
+      nxt_s = claim();
+      nxt_rec = unit_rec(nxt_s);
     }
     if (__any(nh != 0u)) {
       // Rank of the lane's first hit among the wave's T hits.
@@ -713,7 +780,7 @@ __global__ __launch_bounds__(kTPB) void k_ac_scan(ScanCommon c, AcParams a) {
     uint64_t tile_base;
     uint32_t item;
     if (c.units) {
-      uint2 uu = c.units[u];
+      const uint4 uu = c.units[u];
       tile_base = uu.x;
       item = uu.y;
     } else {
@@ -982,7 +1049,9 @@ static hipError_t allow_lds(F* fn, size_t bytes) {
   return e;
 }
 
-size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u + 16u + (kFilterTPB / 64) * kHitRing * sizeof(uint2); }
+size_t hash_shmem_bytes(uint32_t nwords) {
+  return nwords * 4u + 16u + kGroupTab * sizeof(uint64_t) + (kFilterTPB / 64) * kHitRing * sizeof(uint2);
+}
 
 template <int K, bool CI>
 static hipError_t prepare_hash_kc(uint32_t lane_bytes, size_t shmem) {

@@ -19,7 +19,7 @@ using Rec = uint4;
 struct ScanCommon {
   const uint8_t* base;  // 16-byte aligned base of the scanned buffer
   uint64_t load_end;    // bytes [0, load_end) may be loaded; multiple of 16
-  const uint2* units;   // {tile_base, item} per work unit, or null (one segment)
+  const uint4* units;   // {tile_base, item, segment start, segment end} per work unit, or null (one segment)
   uint32_t nunits;
   const uint2* items;   // {lo, hi} segment bounds in base coordinates
   uint32_t emit_end;    // only starts (relative to lo) below this are emitted
@@ -37,6 +37,7 @@ struct ScanCommon {
   uint32_t cand_cap;
   uint32_t* counters;  // [0] matches, [1] candidates, [2] raw regex matches, [3] flags
   uint32_t* queue;     // device work queue head (zeroed per launch), or null: static schedule
+  uint32_t* groups;    // hash filter's group counter (zeroed per launch)
 };
 
 struct HashParams {
