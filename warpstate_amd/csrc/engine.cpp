@@ -52,6 +52,8 @@ hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint3
 hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t sbits,
                               uint32_t pbits, const uint64_t* ids, void* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
+hipError_t launch_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counters, uint32_t ncounters,
+                            unsigned long long* stamps, uint32_t n, hipStream_t st);
 hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords);
 hipError_t prepare_ac(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
@@ -139,8 +141,10 @@ struct ws_ctx {
   bool profiling = false;
   ws_timings last{};
   hipEvent_t ev[8] = {};
+  hipEvent_t tev[2] = {};      // around a whole device-sized call (outside any graph)
+  uint32_t wall_khz = 0;       // device wall-clock rate (kernel stamps)
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate;
+      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, dstamp;
   std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
   uint8_t* hstage = nullptr;    // pinned staging read by captured copies
   size_t hstage_cap = 0;
@@ -562,7 +566,6 @@ ScanCommon make_common(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint
   // Batches pull uneven work from a device queue (zeroed with the counters);
   // a single input's equal tiles use a static stride.
   c.queue = plan.batch ? ctx->counters.as<uint32_t>() + 7 : nullptr;
-  c.groups = ctx->counters.as<uint32_t>() + 7;
   return c;
 }
 
@@ -999,6 +1002,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   // cnt[0..10] are zero here: cleared before the first run, and by
   // k_run_close after each one.
   ScanCommon c = make_common(ctx, p, plan, rs.nunits, rs.lit_cap, rs.cand_cap);
+  if (ev) c.stamp = ctx->dstamp.as<unsigned long long>() + 2 * rs.items_off;
   RunCaps caps{};
   caps.lit_cap = rs.lit_cap;
   caps.cand_cap = rs.cand_cap;
@@ -1111,21 +1115,25 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     WS_HIP(ctx->outbuf.ensure(std::max<uint64_t>(ds.out_cap, 1) * sizeof(ws_match)));
     ds.out = ctx->outbuf.p;
   }
-  WS_HIP(ensure_stage(ctx, plans.size() * sizeof(uint2)));
+  // Pinned staging: the runs' items, then (profiling) the kernel stamps read back.
+  const size_t stamp_off = (plans.size() * sizeof(uint2) + 15) & ~(size_t)15;
+  WS_HIP(ensure_stage(ctx, stamp_off + plans.size() * 2 * sizeof(unsigned long long)));
   for (size_t k = 0; k < plans.size(); ++k) {
     ws_status s = prepare_run(ctx, p, plans[k], tuning, &setups[k]);
     if (s != WS_OK) return s;
     setups[k].items_off = k;
     std::memcpy(ctx->hstage + k * sizeof(uint2), plans[k].items.data(), sizeof(uint2));
   }
+  if (ctx->profiling) WS_HIP(ctx->dstamp.ensure(plans.size() * 2 * sizeof(unsigned long long)));
   if (ctx->profiling && ctx->pev.size() < 5 * plans.size()) {
     const size_t had = ctx->pev.size();
     ctx->pev.resize(5 * plans.size(), nullptr);
     for (size_t k = had; k < ctx->pev.size(); ++k) WS_HIP(hipEventCreate(&ctx->pev[k]));
   }
   auto launch_all = [&]() -> ws_status {
-    WS_HIP(launch_fill_u32(ds.sticky, 10, 0, st));
-    WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), kCounterLitSnapshot + 1, 0, st));
+    WS_HIP(launch_call_init(ds.sticky, 10, ctx->counters.as<uint32_t>(), kCounterLitSnapshot + 1,
+                            ctx->profiling ? ctx->dstamp.as<unsigned long long>() : nullptr,
+                            (uint32_t)plans.size(), st));
     for (size_t k = 0; k < plans.size(); ++k) {
       ws_status s = launch_run(ctx, p, plans[k], setups[k], offs[k], ds,
                                ctx->profiling ? &ctx->pev[5 * k] : nullptr);
@@ -1140,8 +1148,11 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     key.epoch = g_buf_epoch.load();
     ge = &ctx->graphs[key];
   }
+  if (ctx->profiling) WS_HIP(hipEventRecord(ctx->tev[0], st));
+  bool replayed = false;
   if (ge && ge->exec) {
     WS_HIP(hipGraphLaunch(ge->exec, st));
+    replayed = true;
   } else if (ge && ge->seen >= 1 && !ge->broken) {
     // Second sighting: capture and replay from now on.
     if (ctx->graphs.size() > 64) {
@@ -1166,11 +1177,17 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     } else {
       ge->exec = exec;
       WS_HIP(hipGraphLaunch(exec, st));
+      replayed = true;
     }
   } else {
     if (ge) ge->seen++;
     ws_status s = launch_all();
     if (s != WS_OK) return s;
+  }
+  if (ctx->profiling) {
+    WS_HIP(hipEventRecord(ctx->tev[1], st));
+    WS_HIP(hipMemcpyAsync(ctx->hstage + stamp_off, ctx->dstamp.p, plans.size() * 2 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, st));
   }
   WS_HIP(hipMemcpyAsync(ctx->hcount + 11, ds.sticky, 10 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   WS_HIP(hipStreamSynchronize(st));
@@ -1182,21 +1199,30 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
   uint64_t tot = 0;
   std::memcpy(&tot, hs + 8, 8);
   if (ctx->profiling) {
+    // The prefilter from its in-kernel stamps; the other phases from the
+    // events between launches, which a replayed graph does not re-record
+    // (then -1: unknown).
     ws_timings t{};
-    for (size_t k = 0; k < plans.size(); ++k) {
-      hipEvent_t* ev = &ctx->pev[5 * k];
-      float a = 0, b = 0, c2 = 0, d = 0;
-      (void)hipEventElapsedTime(&a, ev[0], ev[1]);
-      (void)hipEventElapsedTime(&b, ev[1], ev[2]);
-      (void)hipEventElapsedTime(&c2, ev[2], ev[3]);
-      (void)hipEventElapsedTime(&d, ev[3], ev[4]);
-      t.prefilter_ms += a;
-      t.verify_ms += b;
-      t.regex_ms += c2;
-      t.order_ms += d;
+    const unsigned long long* stamps = reinterpret_cast<const unsigned long long*>(ctx->hstage + stamp_off);
+    for (size_t k = 0; k < plans.size(); ++k)
+      if (stamps[2 * k + 1] > stamps[2 * k])
+        t.prefilter_ms += (float)((double)(stamps[2 * k + 1] - stamps[2 * k]) / ctx->wall_khz);
+    if (replayed) {
+      t.verify_ms = t.regex_ms = t.order_ms = -1.0f;
+    } else {
+      for (size_t k = 0; k < plans.size(); ++k) {
+        hipEvent_t* ev = &ctx->pev[5 * k];
+        float b = 0, c2 = 0, d = 0;
+        (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&c2, ev[2], ev[3]);
+        (void)hipEventElapsedTime(&d, ev[3], ev[4]);
+        t.verify_ms += b;
+        t.regex_ms += c2;
+        t.order_ms += d;
+      }
     }
     float all = 0;
-    (void)hipEventElapsedTime(&all, ctx->pev[0], ctx->pev[5 * (plans.size() - 1) + 4]);
+    (void)hipEventElapsedTime(&all, ctx->tev[0], ctx->tev[1]);
     t.total_ms = all;
     t.hits = hs[kStickyHits];
     t.candidates = hs[kStickyCands];
@@ -1294,6 +1320,16 @@ ws_status ws_ctx_create(int device_ordinal, const ws_config* cfg, ws_ctx** out) 
     e = hipEventCreate(&ev);
     if (e != hipSuccess) return bail("hipEventCreate", e);
   }
+  for (auto& ev : ctx->tev) {
+    e = hipEventCreate(&ev);
+    if (e != hipSuccess) return bail("hipEventCreate", e);
+  }
+  {
+    int khz = 0;
+    e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_ordinal);
+    if (e != hipSuccess) return bail("hipDeviceGetAttribute(wall clock rate)", e);
+    ctx->wall_khz = khz > 0 ? (uint32_t)khz : 100000u;
+  }
   e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hcount), 32 * sizeof(uint32_t), hipHostMallocDefault);
   if (e != hipSuccess) return bail("hipHostMalloc", e);
   e = ctx->counters.ensure(16 * sizeof(uint32_t));
@@ -1310,9 +1346,11 @@ void ws_ctx_destroy(ws_ctx* ctx) {
                     &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
                     &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
                     &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
-                    &ctx->bcur, &ctx->dstate})
+                    &ctx->bcur, &ctx->dstate, &ctx->dstamp})
     b->release();
   for (auto& ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto& ev : ctx->tev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto& ev : ctx->pev)
     if (ev) (void)hipEventDestroy(ev);

@@ -406,9 +406,6 @@ __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams
 // Per-wave LDS ring of filter hits waiting for verification (uint2
 // {position, item}); a wave verifies 64 at a time, one per lane.
 constexpr uint32_t kHitRing = 128;
-// Per-block table of the groups (16 half units) the block has taken, by
-// batch number modulo its size: {batch, group} in one 64-bit LDS word.
-constexpr uint32_t kGroupTab = 16;
 
 // Verifies the n (<= 64) queued hits from ring index `head`, one per lane:
 // the 48-byte aligned input span holding bytes q..q+31 (just streamed by the
@@ -439,6 +436,16 @@ __device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashPar
   hit_finish<CI>(c, h, hw, s_cnt, lit, cand);
 }
 
+// Kernel duration for profiling without events (which a replayed graph does
+// not re-record): the first block in takes the minimum start, the last one
+// out the maximum end, on the device's constant-rate wall clock.
+__device__ __forceinline__ void stamp_start(const ScanCommon& c) {
+  if (c.stamp && threadIdx.x == 0) atomicMin(&c.stamp[0], (unsigned long long)wall_clock64());
+}
+__device__ __forceinline__ void stamp_end(const ScanCommon& c) {
+  if (c.stamp && threadIdx.x == 0) atomicMax(&c.stamp[1], (unsigned long long)wall_clock64());
+}
+
 // Orders a wave's LDS ring writes before its lanes read each other's entries.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -449,13 +456,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Streaming rolling-hash filter with in-kernel verification, one 1024-lane
 // block per CU so the bloom filter can use up to 128 KiB of LDS. Work goes
 // out in half units (a unit is 256 * PL bytes; the host lists a batch's
-// units), 16 to a group. Each block takes groups one after another (its
-// first two statically, the rest from a device counter) and its waves take
-// the half units of those groups one at a time from an LDS counter, so
-// faster CUs, and the waves the scheduler serves more often, do more and
-// all finish together. Per half unit a lane loads the (PL + 16)-byte
-// windows of its two quarters with coalesced 16-byte loads, then hashes PL
-// positions in each. Tiles wholly inside their
+// units), 16 to a group, the groups striding over the blocks; a block's
+// waves take its half units one at a time from an LDS counter, so the waves
+// the scheduler serves more often do more and all finish together. (A
+// device-wide counter was measured slower: its atomics cost more than the
+// imbalance across CUs they remove.) Per half unit a lane loads the
+// (PL + 16)-byte windows of its two quarters with coalesced 16-byte loads,
+// then hashes PL positions in each. Tiles wholly inside their
 // segment skip all bounds work. A wave appends its hits (wave prefix sum) to
 // its own LDS ring and verifies them 64 at a time while the block's other
 // waves keep filtering, so the verification's dependent L2 reads hide
@@ -467,24 +474,33 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
                                                              SliceOut cand) {
   // LDS: the bloom words from address 0 (the kernel has no static LDS, so
   // a word's address is the masked hash itself), the block's two output
-  // counters and its work counter, its group table, then one hit ring per
-  // wave.
+  // counters and its work counter, then one hit ring per wave.
   extern __shared__ uint32_t smem[];
   uint32_t* bloom = smem;
   uint32_t* s_cnt = &smem[h.nwords];
   uint32_t* s_work = &smem[h.nwords + 2];
-  uint64_t* s_tab = reinterpret_cast<uint64_t*>(&smem[h.nwords + 4]);
-  uint2* ring = reinterpret_cast<uint2*>(&smem[h.nwords + 4 + 2 * kGroupTab]) + (threadIdx.x / 64) * kHitRing;
+  uint2* ring = reinterpret_cast<uint2*>(&smem[h.nwords + 4]) + (threadIdx.x / 64) * kHitRing;
   constexpr int NW = PL / 4 + 4;  // window dwords
   constexpr uint32_t kUnit = kTPB * PL;
   constexpr uint32_t kWaves = kFilterTPB / 64;
   const uint32_t lane = lane_id();
+  stamp_start(c);
 
-  for (uint32_t k = threadIdx.x; k < h.nwords; k += kFilterTPB) bloom[k] = h.bloom[k];
-  if (threadIdx.x < kGroupTab) {
-    // Batches 0 and 1 are the block's static groups.
-    const uint64_t b = threadIdx.x;
-    s_tab[threadIdx.x] = b < 2 ? (uint64_t)(blockIdx.x + b * gridDim.x) << 32 | b : ~0ull;
+  {
+    // The bloom filter (nwords is a power of two >= 8): 16-byte loads, eight
+    // in flight per lane before any store.
+    const uint4* src = reinterpret_cast<const uint4*>(h.bloom);
+    uint4* dst = reinterpret_cast<uint4*>(bloom);
+    const uint32_t n4 = h.nwords / 4;
+    for (uint32_t k0 = threadIdx.x; k0 < n4; k0 += 8 * kFilterTPB) {
+      uint4 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u * kFilterTPB < n4) v[u] = src[k0 + u * kFilterTPB];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u * kFilterTPB < n4) dst[k0 + u * kFilterTPB] = v[u];
+    }
   }
   if (threadIdx.x == 0) {
     s_cnt[0] = 0;
@@ -495,38 +511,14 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
 
   const uint32_t load_end = (uint32_t)c.load_end;
   const uint32_t nslots = 2 * c.nunits;
-  // The block's jw-th half unit is slot jw % 16 of its batch jw / 16. The
-  // wave that takes batch k's first slot publishes the group of batch k + 2
-  // from the device counter, after batch k + 1's (so a block's groups, and
-  // with them its slots, increase along its claims: the first slot past
-  // the end means the block is done). A batch is thus published, or its
-  // counter read in flight, by the time any of its slots is taken, and the
-  // waits below cover only that flight. A table entry is rewritten 14
-  // batches (224 claims) after the one it holds, long after every claim of
-  // its batch has read it (the read follows the claim at once).
-  auto batch_group = [&](uint32_t k) -> uint32_t {
-    uint64_t e = __atomic_load_n(&s_tab[k % kGroupTab], __ATOMIC_RELAXED);
-    while ((uint32_t)e != k) {
-      __builtin_amdgcn_s_sleep(1);
-      e = __atomic_load_n(&s_tab[k % kGroupTab], __ATOMIC_RELAXED);
-    }
-    return (uint32_t)(e >> 32);
-  };
+  // The block's jw-th half unit: slot jw % 16 of group blockIdx.x +
+  // (jw / 16) * gridDim.x, increasing in jw, so the first one past the end
+  // means the block is done.
   auto claim = [&]() -> uint32_t {
     uint32_t v = 0;
     if (lane == 0) v = atomicAdd(s_work, 1u);
     const uint32_t jw = __builtin_amdgcn_readfirstlane(v);
-    const uint32_t k = jw / kWaves;
-    uint32_t grp = 0;
-    if (lane == 0) {
-      if (jw % kWaves == 0) {
-        (void)batch_group(k + 1);
-        const uint32_t g = 2 * gridDim.x + atomicAdd(c.groups, 1u);
-        __atomic_store_n(&s_tab[(k + 2) % kGroupTab], (uint64_t)g << 32 | (k + 2), __ATOMIC_RELAXED);
-      }
-      grp = batch_group(k);
-    }
-    return __builtin_amdgcn_readfirstlane(grp) * kWaves + jw % kWaves;
+    return (blockIdx.x + (jw / kWaves) * gridDim.x) * kWaves + jw % kWaves;
   };
   // A half unit's bounds (wave-uniform): from its unit's record in a
   // batch, else from the one segment, read once.
@@ -680,6 +672,7 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
     lit.counts[blockIdx.x] = s_cnt[0];
     cand.counts[blockIdx.x] = s_cnt[1];
   }
+  stamp_end(c);
 }
 
 // Slice compaction of both verify outputs. One 1024-lane block: lanes
@@ -764,6 +757,7 @@ __global__ __launch_bounds__(kTPB) void k_ac_scan(ScanCommon c, AcParams a) {
   uint16_t* table = reinterpret_cast<uint16_t*>(s_cand + kRecCap);
   __shared__ uint32_t s_n[2];
   __shared__ uint32_t s_base[4];
+  stamp_start(c);
 
   for (uint32_t k = threadIdx.x; k < 256; k += kTPB) cmap[k] = a.cmap[k];
   if (LDS_TABLE) {
@@ -843,6 +837,7 @@ __global__ __launch_bounds__(kTPB) void k_ac_scan(ScanCommon c, AcParams a) {
     if (s_n[0] > kRecCap / 2 || s_n[1] > kRecCap / 2) flush_block(s_n, s_lit, s_cand, s_base, c);
   }
   flush_block(s_n, s_lit, s_cand, s_base, c);
+  stamp_end(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1049,9 +1044,7 @@ static hipError_t allow_lds(F* fn, size_t bytes) {
   return e;
 }
 
-size_t hash_shmem_bytes(uint32_t nwords) {
-  return nwords * 4u + 16u + kGroupTab * sizeof(uint64_t) + (kFilterTPB / 64) * kHitRing * sizeof(uint2);
-}
+size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u + 16u + (kFilterTPB / 64) * kHitRing * sizeof(uint2); }
 
 template <int K, bool CI>
 static hipError_t prepare_hash_kc(uint32_t lane_bytes, size_t shmem) {
@@ -1221,6 +1214,27 @@ hipError_t launch_slice_compact2(const SliceOut& lit, const SliceOut& cand, uint
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_slice_gather2, dim3(2 * (n + 64)), dim3(kTPB), 0, st, lit, cand, n, loffs, coffs, lout,
                      lcap, cout, ccap);
+  return hipGetLastError();
+}
+
+// Start of a device-sized call: zeroes the sticky words and the run
+// counters, and resets the n {start, end} kernel stamp pairs (stamps may be
+// null).
+__global__ void k_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counters, uint32_t ncounters,
+                            unsigned long long* stamps, uint32_t n) {
+  const uint32_t t = threadIdx.x;
+  if (t < nsticky) sticky[t] = 0;
+  if (t < ncounters) counters[t] = 0;
+  if (stamps)
+    for (uint32_t k = t; k < n; k += blockDim.x) {
+      stamps[2 * k] = ~0ull;
+      stamps[2 * k + 1] = 0;
+    }
+}
+
+hipError_t launch_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counters, uint32_t ncounters,
+                            unsigned long long* stamps, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_call_init, dim3(1), dim3(256), 0, st, sticky, nsticky, counters, ncounters, stamps, n);
   return hipGetLastError();
 }
 

@@ -37,7 +37,9 @@ struct ScanCommon {
   uint32_t cand_cap;
   uint32_t* counters;  // [0] matches, [1] candidates, [2] raw regex matches, [3] flags
   uint32_t* queue;     // device work queue head (zeroed per launch), or null: static schedule
-  uint32_t* groups;    // hash filter's group counter (zeroed per launch)
+  // Profiling: {earliest block start, latest block end} of the prefilter
+  // kernel on the device wall clock, or null.
+  unsigned long long* stamp;
 };
 
 struct HashParams {
