@@ -52,6 +52,14 @@ hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint3
 hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t sbits,
                               uint32_t pbits, const uint64_t* ids, void* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
+hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
+                                     uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total,
+                                     uint32_t* lit_snap, uint32_t cap, uint32_t sbits, uint32_t pbits,
+                                     uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift, hipStream_t st);
+hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
+                             uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift, uint32_t nb,
+                             uint32_t* cnt, uint32_t* cursor, uint32_t* flag, uint32_t pbits, uint32_t start_off,
+                             void* out, uint64_t out_cap, const unsigned long long* base_dev, hipStream_t st);
 hipError_t launch_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counters, uint32_t ncounters,
                             unsigned long long* stamps, uint32_t n, hipStream_t st);
 hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords);
@@ -1010,6 +1018,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   caps.rec_cap = rs.rec_cap;
   caps.max_matches = rs.max_matches;
   caps.lit_ovf_cap = caps.cand_ovf_cap = 0xFFFFFFFFu;
+  bool keyed = false;  // ordering pairs already made (and counted) by the compaction
   if (rs.nunits > 0 && rs.variant == WS_VARIANT_HASH) {
     const HashParams hp = hash_params(p, rs.lane_bytes);
     uint32_t* sc = ctx->scounts.as<uint32_t>();
@@ -1018,8 +1027,17 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     WS_HIP(launch_hash(h.K, h.ci, rs.nvb, st, c, hp, lso, cso));
     if (ev) WS_HIP(hipEventRecord(ev[1], st));
     uint32_t* offs = sc + 2 * rs.nvb;
-    WS_HIP(launch_slice_compact2(lso, cso, rs.nvb, offs, offs + rs.nvb + 2, cnt + 0, cnt + 1,
-                                 cnt + kCounterLitSnapshot, c.lit_out, c.lit_cap, c.cand_out, c.cand_cap, st));
+    if (nrx == 0) {
+      // Literals only: the compaction emits ordering pairs and bucket counts.
+      WS_HIP(launch_slice_compact_keys(lso, cso, rs.nvb, offs, offs + rs.nvb + 2, cnt + 0, cnt + 1,
+                                       cnt + kCounterLitSnapshot, c.lit_cap, rs.sbits, rs.pbits,
+                                       ctx->k0.as<uint64_t>(), ctx->v0.as<uint32_t>(), ctx->bcnt.as<uint32_t>(),
+                                       rs.obits - rs.ob, st));
+      keyed = true;
+    } else {
+      WS_HIP(launch_slice_compact2(lso, cso, rs.nvb, offs, offs + rs.nvb + 2, cnt + 0, cnt + 1,
+                                   cnt + kCounterLitSnapshot, c.lit_out, c.lit_cap, c.cand_out, c.cand_cap, st));
+    }
     caps.lit_ovf_cap = lso.ovf_cap;
     caps.cand_ovf_cap = cso.ovf_cap;
   } else if (rs.nunits > 0) {
@@ -1069,6 +1087,15 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   if (ev) WS_HIP(hipEventRecord(ev[3], st));
 
   // Order by (start, pattern id), pack at the running output offset.
+  if (keyed) {
+    WS_HIP(bucket_order_pack(ctx->tmp.p, ctx->tmp.cap, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
+                             ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), rs.rec_cap, cnt + 0, rs.obits - rs.ob,
+                             1u << rs.ob, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, rs.pbits,
+                             start_off, ds.out, ds.out_cap, ds.total, st));
+    WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, st));
+    if (ev) WS_HIP(hipEventRecord(ev[4], st));
+    return WS_OK;
+  }
   WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), rs.rec_cap, cnt + 0, rs.sbits, rs.pbits, ctx->k0.as<uint64_t>(),
                            ctx->v0.as<uint32_t>(), st));
   size_t have = ctx->tmp.cap;

@@ -740,6 +740,32 @@ __global__ __launch_bounds__(kTPB) void k_slice_gather2(SliceOut lit, SliceOut c
   }
 }
 
+// Literal matches of a set without regexes, from the slices straight to
+// ordering pairs: blocks as the literal half of k_slice_gather2; record
+// {pid, start, end, item} becomes key (((item << sbits) | start) << pbits) |
+// pid with value end (as k_order_keys makes them), counted in bucket
+// key >> shift for the bucket sort that follows.
+__global__ __launch_bounds__(kTPB) void k_slice_gather_keys(SliceOut lit, uint32_t n, const uint32_t* loffs,
+                                                             uint32_t cap, uint32_t sbits, uint32_t pbits,
+                                                             uint64_t* keys, uint32_t* vals, uint32_t* cnt,
+                                                             uint32_t shift) {
+  auto put = [&](uint32_t i, const Rec& r) {
+    const uint64_t k = ((((uint64_t)r.w << sbits) | r.y) << pbits) | r.x;
+    keys[i] = k;
+    vals[i] = r.z;
+    atomicAdd(&cnt[k >> shift], 1u);
+  };
+  if (blockIdx.x < n) {
+    const uint32_t m = min(lit.counts[blockIdx.x], lit.cap), o = loffs[blockIdx.x];
+    for (uint32_t k = threadIdx.x; k < m; k += kTPB)
+      if (o + k < cap) put(o + k, lit.buf[(size_t)blockIdx.x * lit.cap + k]);
+  } else {
+    const uint32_t no = min(*lit.ovf_count, lit.ovf_cap), o = loffs[n];
+    for (uint32_t k = (blockIdx.x - n) * kTPB + threadIdx.x; k < no; k += 64 * kTPB)
+      if (o + k < cap) put(o + k, lit.ovf[k]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Factor-DFA prefilter. The block stages its tile (kTPB * CH bytes plus a
 // lookahead) in LDS with 16-byte loads; every lane walks its CH-byte chunk
@@ -1217,6 +1243,21 @@ hipError_t launch_slice_compact2(const SliceOut& lit, const SliceOut& cand, uint
   return hipGetLastError();
 }
 
+// Offsets of both slice sets, then the literal slices (the set has no
+// regexes, so no candidates) straight to ordering pairs and bucket counts.
+hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
+                                     uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total,
+                                     uint32_t* lit_snap, uint32_t cap, uint32_t sbits, uint32_t pbits,
+                                     uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift, hipStream_t st) {
+  hipLaunchKernelGGL(k_slice_offsets2, dim3(1), dim3(1024), 0, st, lit, cand, n, loffs, coffs, lit_total,
+                     cand_total, lit_snap);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_slice_gather_keys, dim3(n + 64), dim3(kTPB), 0, st, lit, n, loffs, cap, sbits, pbits, keys,
+                     vals, cnt, shift);
+  return hipGetLastError();
+}
+
 // Start of a device-sized call: zeroes the sticky words and the run
 // counters, and resets the n {start, end} kernel stamp pairs (stamps may be
 // null).
@@ -1338,6 +1379,63 @@ __global__ void k_bucket_sort(uint64_t* keys, uint32_t* vals, const uint32_t* en
       vals[j] = v;
     }
   }
+}
+
+// Final ordering step fused with packing: one lane per bucket sorts it in
+// place (as k_bucket_sort) and writes its matches at the running output
+// offset (as k_pack_matches). Leaves the counts zero.
+__global__ void k_bucket_sort_pack(uint64_t* keys, uint32_t* vals, const uint32_t* end, uint32_t* cnt, uint32_t nb,
+                                   uint32_t* flag, uint32_t pbits, uint32_t start_off, MatchOut* out,
+                                   uint64_t out_cap, const unsigned long long* base_dev) {
+  const uint64_t base = *base_dev;
+  const uint64_t pmask = (1ull << pbits) - 1;
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+    const uint32_t c = cnt[b];
+    if (c == 0) continue;
+    cnt[b] = 0;
+    if (c > kBucketMax) {
+      atomicOr(flag, 1u);
+      continue;
+    }
+    const uint32_t lo = end[b] - c;
+    for (uint32_t i = lo + 1; i < lo + c; ++i) {
+      const uint64_t k = keys[i];
+      const uint32_t v = vals[i];
+      uint32_t j = i;
+      while (j > lo && keys[j - 1] > k) {
+        keys[j] = keys[j - 1];
+        vals[j] = vals[j - 1];
+        --j;
+      }
+      keys[j] = k;
+      vals[j] = v;
+    }
+    for (uint32_t i = lo; i < lo + c; ++i) {
+      if (base + i >= out_cap) break;
+      const uint64_t k = keys[i];
+      MatchOut m;
+      m.pid = (uint32_t)(k & pmask);
+      m.start = (uint32_t)(k >> pbits) + start_off;
+      m.end = vals[i] + start_off;
+      out[base + i] = m;
+    }
+  }
+}
+
+// Pairs already counted into cnt (k_slice_gather_keys): scan -> scatter ->
+// per-bucket sort fused with packing into `out` at *base_dev.
+hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
+                             uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift, uint32_t nb,
+                             uint32_t* cnt, uint32_t* cursor, uint32_t* flag, uint32_t pbits, uint32_t start_off,
+                             void* out, uint64_t out_cap, const unsigned long long* base_dev, hipStream_t st) {
+  hipError_t e = rocprim::exclusive_scan(tmp, tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(), st);
+  if (e != hipSuccess) return e;
+  const uint32_t g = std::max(1u, std::min(nd ? kDeviceGrid : 4096u, (n + 255) / 256));
+  hipLaunchKernelGGL(k_bucket_scatter, dim3(g), dim3(256), 0, st, kin, vin, n, nd, shift, cursor, kout, vout);
+  const uint32_t gb = std::min(4096u, (nb + 255) / 256);
+  hipLaunchKernelGGL(k_bucket_sort_pack, dim3(gb), dim3(256), 0, st, kout, vout, cursor, cnt, nb, flag, pbits,
+                     start_off, reinterpret_cast<MatchOut*>(out), out_cap, base_dev);
+  return hipGetLastError();
 }
 
 // Sizes the scan's temporary storage (tmp = nullptr) or runs
