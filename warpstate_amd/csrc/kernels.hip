@@ -349,31 +349,55 @@ struct HitWork {
   bool live;             // false for a lane without a hit this round
 };
 
-// raw: the 48-byte aligned span from q & ~15 (zeros past the loadable end).
+// raw: the aligned span from q & ~15, its first 32 bytes loaded (zeros past
+// the loadable end). The window hash is checked against the bucket's two
+// tags first; only a match (or a crowded bucket) reads the slots and the
+// span's third 16 bytes (for factors that run past the first 32).
 template <int K, bool CI>
-__device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParams& h, uint2 hr, const uint32_t* raw,
-                                            HitWork& hw) {
+__device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParams& h, uint2 hr, uint32_t* raw,
+                                            uint32_t lend, HitWork& hw) {
   hw.live = true;
   hw.q = hr.x;
   hw.item = hr.y;
   hw.seg = c.items[hw.item];
-  realign32(raw, (uint32_t)(hw.q & 15), hw.W);
+  const uint32_t sh = (uint32_t)(hw.q & 15);
+  realign32(raw, sh, hw.W);
+  // The verification table's 24-bit window hash (v_mad_u32_u24 reads the
+  // low 24 bits of the running value; the window lies in the first 16 bytes).
+  uint32_t H = 0;
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const uint32_t b = (hw.W[t >> 2] >> ((t & 3) * 8)) & 0xFFu;
+    H = mad24_s(H, h.B, CI ? fold_byte((uint8_t)b) : b);
+  }
+  H &= 0xFFFFFFu;
+  hw.H = H;
+  const size_t bucket = (H * 0x9E3779B1u) >> (32 - h.nbits);
+  const uint2 tg = h.vtag[bucket];
+  hw.s0.x = kVtabEmpty;
+  hw.s2.x = kVtabEmpty;
+  const bool m0 = tg.x == H, m1 = tg.y == H || tg.y == kVtabMore;
+  if (m0 || m1) {
+    const uint32_t a = (uint32_t)(hw.q & ~15ull) + 32;
+    if (a + 16 <= lend) {
+      const uint4 x2 = *reinterpret_cast<const uint4*>(c.base + a);
+      raw[8] = x2.x; raw[9] = x2.y; raw[10] = x2.z; raw[11] = x2.w;
+      realign32(raw, sh, hw.W);
+    }
+    const uint4* slot = h.vtab + 4 * bucket;
+    if (m0) {
+      hw.s0 = slot[0];
+      hw.s1 = slot[1];
+    }
+    if (m1) {
+      hw.s2 = slot[2];
+      hw.s3 = slot[3];
+    }
+  }
   if (CI) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) hw.W[k] = fold_word(hw.W[k]);
   }
-  // The verification table's 24-bit window hash (v_mad_u32_u24 reads the
-  // low 24 bits of the running value).
-  uint32_t H = 0;
-#pragma unroll
-  for (int t = 0; t < K; ++t) H = mad24_s(H, h.B, (hw.W[t >> 2] >> ((t & 3) * 8)) & 0xFFu);
-  H &= 0xFFFFFFu;
-  hw.H = H;
-  const uint4* slot = h.vtab + 4 * (size_t)((H * 0x9E3779B1u) >> (32 - h.nbits));
-  hw.s0 = slot[0];
-  hw.s1 = slot[1];
-  hw.s2 = slot[2];
-  hw.s3 = slot[3];
 }
 
 template <bool CI>
@@ -408,8 +432,9 @@ __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams
 constexpr uint32_t kHitRing = 128;
 
 // Verifies the n (<= 64) queued hits from ring index `head`, one per lane:
-// the 48-byte aligned input span holding bytes q..q+31 (just streamed by the
-// filter, so an L2 hit), the window hash, its bucket, the exact compare.
+// the 32-byte aligned input span holding the window (streamed by the filter
+// shortly before), the window hash, its bucket's tags, and for a tag match
+// the slots and the exact compare.
 // Call with the whole wave.
 template <int K, bool CI>
 __device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashParams& h, const uint2* ring,
@@ -426,12 +451,13 @@ __device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashPar
     const uint32_t a = hr.x & ~15u;
     uint32_t raw[12];
 #pragma unroll
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < 2; ++v) {
       uint4 x = make_uint4(0, 0, 0, 0);
       if (a + 16 * v + 16 <= lend) x = *reinterpret_cast<const uint4*>(c.base + a + 16 * v);
       raw[4 * v] = x.x; raw[4 * v + 1] = x.y; raw[4 * v + 2] = x.z; raw[4 * v + 3] = x.w;
     }
-    hit_prepare<K, CI>(c, h, hr, raw, hw);
+    raw[8] = raw[9] = raw[10] = raw[11] = 0;
+    hit_prepare<K, CI>(c, h, hr, raw, lend, hw);
   }
   hit_finish<CI>(c, h, hw, s_cnt, lit, cand);
 }
@@ -487,19 +513,22 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
   stamp_start(c);
 
   {
-    // The bloom filter (nwords is a power of two >= 8): 16-byte loads, eight
-    // in flight per lane before any store.
+    // The bloom filter (nwords is a power of two >= 8) in 16-byte pieces;
+    // whole passes of eight loads per lane put them all in flight before
+    // any store.
     const uint4* src = reinterpret_cast<const uint4*>(h.bloom);
     uint4* dst = reinterpret_cast<uint4*>(bloom);
     const uint32_t n4 = h.nwords / 4;
-    for (uint32_t k0 = threadIdx.x; k0 < n4; k0 += 8 * kFilterTPB) {
-      uint4 v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (k0 + u * kFilterTPB < n4) v[u] = src[k0 + u * kFilterTPB];
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (k0 + u * kFilterTPB < n4) dst[k0 + u * kFilterTPB] = v[u];
+    if (n4 % (8 * kFilterTPB) == 0) {
+      constexpr uint32_t T = kFilterTPB;
+      for (uint32_t k0 = threadIdx.x; k0 < n4; k0 += 8 * T) {
+        const uint4 a0 = src[k0], a1 = src[k0 + T], a2 = src[k0 + 2 * T], a3 = src[k0 + 3 * T];
+        const uint4 a4 = src[k0 + 4 * T], a5 = src[k0 + 5 * T], a6 = src[k0 + 6 * T], a7 = src[k0 + 7 * T];
+        dst[k0] = a0; dst[k0 + T] = a1; dst[k0 + 2 * T] = a2; dst[k0 + 3 * T] = a3;
+        dst[k0 + 4 * T] = a4; dst[k0 + 5 * T] = a5; dst[k0 + 6 * T] = a6; dst[k0 + 7 * T] = a7;
+      }
+    } else {
+      for (uint32_t k = threadIdx.x; k < n4; k += kFilterTPB) dst[k] = src[k];
     }
   }
   if (threadIdx.x == 0) {
