@@ -243,9 +243,11 @@ def main() -> int:
     ms_per_step = elapsed * 1e3 / args.steps
     value = w.n / (ms_per_step * 1e-3) / 1e9
     k_ms = float(np.mean(kern_ms))
-    # k_hash_filter reads every input byte once and writes only the match
-    # records and regex candidates it verified: N algorithmic bytes a launch.
-    alg_bytes = w.n
+    # Algorithmic bytes of a step: the input read once, each match written
+    # once (12-byte Match; 24-byte TaggedMatch in a batch), 16 bytes per
+    # batch item descriptor.
+    n_items = int(w.config.get("items", 0))
+    alg_bytes = w.n + (24 if n_items else 12) * int(count) + 16 * n_items
     # The stream workload launches the filter once per chunk; kernel_ms and
     # the algorithmic bytes cover the whole step, traffic one launch.
     launches = -(-w.n // w.config["chunk"]) if "chunk" in w.config else 1

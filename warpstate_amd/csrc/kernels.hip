@@ -1,9 +1,9 @@
 // HIP kernels of the scan path, written for gfx950 (CDNA4, wave64).
 //
-//  k_hash_filter rolling-hash prefilter over 16 positions per lane with the
-//                bloom filter in LDS; hit positions are wave prefix-sum
-//                compacted into the block's private slice of the hit list.
-//  k_hash_verify one lane per hit: bucket chain (L2) and exact verification.
+//  k_hash_filter rolling-hash prefilter (16-64 positions per lane, bloom
+//                filter in LDS) with the verification of its hits in the
+//                same kernel: per-wave LDS hit rings, 64 hits a round.
+//  k_slice_*     compaction of the per-block output slices.
 //  k_ac_scan     dense factor-DFA (Aho-Corasick over byte classes) traversal,
 //                one chunk per lane, table in LDS or L2.
 //  k_rx_walk     anchored leftmost-first DFA walk from each regex candidate.
