@@ -41,7 +41,7 @@ hipError_t launch_chain_walk(const uint64_t* keys, const uint32_t* vals, const R
                              const uint32_t* nd, uint32_t gshift, const uint8_t* head, const uint32_t* carried,
                              uint32_t seg_off, const uint32_t* rx_pid, uint32_t ties, Rec* out,
                              uint32_t out_cap, uint32_t* out_count, uint32_t* carried_out,
-                             hipStream_t st);
+                             hipStream_t st, const OrderOut* oo = nullptr);
 hipError_t launch_lit_keys(const Rec* recs, uint32_t n, uint32_t lbits, uint32_t pbits, uint32_t maxlen,
                            uint32_t len_desc, uint64_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t launch_pid_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint64_t* keys, uint32_t* vals,
@@ -55,7 +55,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
                                      uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total,
                                      uint32_t* lit_snap, uint32_t cap, uint32_t sbits, uint32_t pbits,
-                                     uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift, hipStream_t st);
+                                     uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift, Rec* cout,
+                                     uint32_t ccap, hipStream_t st);
 hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
                              uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift, uint32_t nb,
                              uint32_t* cnt, uint32_t* cursor, uint32_t* flag, uint32_t pbits, uint32_t start_off,
@@ -75,7 +76,8 @@ hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_
                       hipStream_t st);
 hipError_t bucket_sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
                              const uint32_t* vin, uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift,
-                             uint32_t nb, uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st);
+                             uint32_t nb, uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st,
+                             bool counted = false);
 
 namespace {
 
@@ -152,7 +154,7 @@ struct ws_ctx {
   hipEvent_t tev[2] = {};      // around a whole device-sized call (outside any graph)
   uint32_t wall_khz = 0;       // device wall-clock rate (kernel stamps)
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, dstamp;
+      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, dstamp, k2, v2, bcnt2;
   std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
   uint8_t* hstage = nullptr;    // pinned staging read by captured copies
   size_t hstage_cap = 0;
@@ -391,13 +393,14 @@ ws_status sort_into(ws_ctx* ctx, uint32_t n, uint32_t bits, const uint64_t* kin,
 // synchronization and checked by confirm_order.
 // The bucket sort's counts: zero on allocation, and every sort leaves them
 // zero again.
-hipError_t ensure_bucket_counts(ws_ctx* ctx, uint32_t nb) {
+hipError_t ensure_zeroed_counts(ws_ctx* ctx, DevBuf& b, uint32_t nb) {
   const size_t bytes = (size_t)nb * 4;
-  if (bytes <= ctx->bcnt.cap) return hipSuccess;
-  hipError_t e = ctx->bcnt.ensure(bytes);
+  if (bytes <= b.cap) return hipSuccess;
+  hipError_t e = b.ensure(bytes);
   if (e != hipSuccess) return e;
-  return hipMemsetAsync(ctx->bcnt.p, 0, ctx->bcnt.cap, ctx->stream);
+  return hipMemsetAsync(b.p, 0, b.cap, ctx->stream);
 }
+hipError_t ensure_bucket_counts(ws_ctx* ctx, uint32_t nb) { return ensure_zeroed_counts(ctx, ctx->bcnt, nb); }
 
 ws_status order_pairs(ws_ctx* ctx, uint32_t n, uint32_t bits, bool dense, Result* res) {
   res->bucketed = false;
@@ -954,7 +957,6 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   const uint32_t nrx = (uint32_t)h.rx.size();
   rs->raw_cap = nrx ? rs->cand_cap : 0;
   rs->rec_cap = rs->lit_cap + rs->raw_cap;
-  WS_HIP(ctx->items.ensure(8 * sizeof(uint2)));
   WS_HIP(ctx->recs.ensure((size_t)rs->rec_cap * sizeof(Rec)));
   WS_HIP(ctx->cands.ensure((size_t)rs->cand_cap * sizeof(Rec)));
   if (nrx) WS_HIP(ctx->raw.ensure((size_t)rs->raw_cap * sizeof(Rec)));
@@ -988,6 +990,13 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   rs->cb = nrx ? bucket_bits(rs->cbits, std::min(rs->raw_cap, 1u << 17)) : 0;
   const uint32_t nb = 1u << std::max(rs->ob, rs->cb);
   WS_HIP(ensure_bucket_counts(ctx, nb));
+  if (nrx && rs->variant == WS_VARIANT_HASH) {
+    // Regex sets on the hash variant: final ordering pairs (k2, v2) and their
+    // counts (bcnt) are made while the chain sort uses k0..v1 and bcnt2.
+    WS_HIP(ensure_zeroed_counts(ctx, ctx->bcnt2, 1u << rs->cb));
+    WS_HIP(ctx->k2.ensure((size_t)rs->rec_cap * 8));
+    WS_HIP(ctx->v2.ensure((size_t)rs->rec_cap * 4));
+  }
   WS_HIP(ctx->bcur.ensure((size_t)nb * 4));
   size_t need = 0, need2 = 0;
   WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, rs->rec_cap, nullptr, 0,
@@ -1012,13 +1021,14 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   const hipStream_t st = ctx->stream;
   const uint32_t nrx = (uint32_t)h.rx.size();
   uint32_t* cnt = ctx->counters.as<uint32_t>();
-  WS_HIP(hipMemcpyAsync(ctx->items.p, ctx->hstage + rs.items_off * sizeof(uint2), sizeof(uint2),
-                        hipMemcpyHostToDevice, st));
+  // This run's segment (all runs' segments were copied in one go).
+  const uint2* items = ctx->items.as<uint2>() + rs.items_off;
   // Filter and verification.
   if (ev) WS_HIP(hipEventRecord(ev[0], st));
   // cnt[0..10] are zero here: cleared before the first run, and by
   // k_run_close after each one.
   ScanCommon c = make_common(ctx, p, plan, rs.nunits, rs.lit_cap, rs.cand_cap);
+  c.items = items;
   if (ev) c.stamp = ctx->dstamp.as<unsigned long long>() + 2 * rs.items_off;
   RunCaps caps{};
   caps.lit_cap = rs.lit_cap;
@@ -1036,17 +1046,15 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     WS_HIP(launch_hash(h.K, h.ci, rs.nvb, st, c, hp, lso, cso));
     if (ev) WS_HIP(hipEventRecord(ev[1], st));
     uint32_t* offs = sc + 2 * rs.nvb;
-    if (nrx == 0) {
-      // Literals only: the compaction emits ordering pairs and bucket counts.
-      WS_HIP(launch_slice_compact_keys(lso, cso, rs.nvb, offs, offs + rs.nvb + 2, cnt + 0, cnt + 1,
-                                       cnt + kCounterLitSnapshot, c.lit_cap, rs.sbits, rs.pbits,
-                                       ctx->k0.as<uint64_t>(), ctx->v0.as<uint32_t>(), ctx->bcnt.as<uint32_t>(),
-                                       rs.obits - rs.ob, st));
-      keyed = true;
-    } else {
-      WS_HIP(launch_slice_compact2(lso, cso, rs.nvb, offs, offs + rs.nvb + 2, cnt + 0, cnt + 1,
-                                   cnt + kCounterLitSnapshot, c.lit_out, c.lit_cap, c.cand_out, c.cand_cap, st));
-    }
+    // The compaction emits the literals' ordering pairs and bucket counts
+    // (into k0/v0, or k2/v2 when a chain sort will use k0..v1) and, for
+    // regex sets, the candidate records.
+    WS_HIP(launch_slice_compact_keys(lso, cso, rs.nvb, offs, offs + rs.nvb + 2, cnt + 0, cnt + 1,
+                                     cnt + kCounterLitSnapshot, c.lit_cap, rs.sbits, rs.pbits,
+                                     (nrx ? ctx->k2 : ctx->k0).as<uint64_t>(), (nrx ? ctx->v2 : ctx->v0).as<uint32_t>(),
+                                     ctx->bcnt.as<uint32_t>(), rs.obits - rs.ob, nrx ? c.cand_out : nullptr,
+                                     c.cand_cap, st));
+    keyed = true;
     caps.lit_ovf_cap = lso.ovf_cap;
     caps.cand_ovf_cap = cso.ovf_cap;
   } else if (rs.nunits > 0) {
@@ -1065,7 +1073,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   if (nrx) {
     WalkParams w{};
     w.base = plan.base;
-    w.items = ctx->items.as<uint2>();
+    w.items = items;
     w.cands = ctx->cands.as<Rec>();
     w.ncand = cnt + 1;
     w.cand_cap = rs.cand_cap;
@@ -1078,27 +1086,53 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     w.raw_cap = rs.raw_cap;
     w.counters = cnt;
     w.window_truncated = plan.window_truncated ? 1u : 0u;
+    // Hash variant: the walk writes the chain pairs and counts them (bcnt2),
+    // the chain walk writes the kept matches' final pairs (k2/v2, bcnt).
+    uint32_t* ccnt = keyed ? ctx->bcnt2.as<uint32_t>() : ctx->bcnt.as<uint32_t>();
+    if (keyed) {
+      w.ckeys = ctx->k0.as<uint64_t>();
+      w.cvals = ctx->v0.as<uint32_t>();
+      w.ccnt = ccnt;
+      w.cshift = rs.cbits - rs.cb;
+      w.sbits = rs.sbits;
+      w.rbits = rs.rbits;
+    }
     WS_HIP(launch_walk(0, st, w));
-    WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), rs.raw_cap, cnt + 2, rs.sbits, rs.rbits, ctx->k0.as<uint64_t>(),
-                             ctx->v0.as<uint32_t>(), st));
+    if (!keyed)
+      WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), rs.raw_cap, cnt + 2, rs.sbits, rs.rbits, ctx->k0.as<uint64_t>(),
+                               ctx->v0.as<uint32_t>(), st));
     size_t have = ctx->tmp.cap;
     WS_HIP(bucket_sort_pairs(ctx->tmp.p, &have, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
                              ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), rs.raw_cap, cnt + 2, rs.cbits - rs.cb,
-                             1u << rs.cb, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, st));
+                             1u << rs.cb, ccnt, ctx->bcur.as<uint32_t>(), cnt + 9, st, keyed));
     WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), rs.raw_cap,
                               cnt + 2, rs.sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st));
+    OrderOut oo{};
+    if (keyed) {
+      oo.keys = ctx->k2.as<uint64_t>();
+      oo.vals = ctx->v2.as<uint32_t>();
+      oo.cnt = ctx->bcnt.as<uint32_t>();
+      oo.shift = rs.obits - rs.ob;
+      oo.sbits = rs.sbits;
+      oo.pbits = rs.pbits;
+    }
     WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), rs.raw_cap,
                              cnt + 2, rs.sbits, ctx->head.as<uint8_t>(),
                              plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, plan.seg_off, p->rx_pid, 0u,
                              ctx->recs.as<Rec>(), rs.rec_cap, cnt + 0,
-                             plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, st));
+                             plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, st, keyed ? &oo : nullptr));
   }
   if (ev) WS_HIP(hipEventRecord(ev[3], st));
 
   // Order by (start, pattern id), pack at the running output offset.
   if (keyed) {
-    WS_HIP(bucket_order_pack(ctx->tmp.p, ctx->tmp.cap, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
-                             ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), rs.rec_cap, cnt + 0, rs.obits - rs.ob,
+    // Pairs in k0/v0 (literal-only sets) or k2/v2 (regex sets; k0/v0 are free
+    // again after the chain walk) are scattered into the other pair.
+    uint64_t* fk = (nrx ? ctx->k2 : ctx->k0).as<uint64_t>();
+    uint32_t* fv = (nrx ? ctx->v2 : ctx->v0).as<uint32_t>();
+    WS_HIP(bucket_order_pack(ctx->tmp.p, ctx->tmp.cap, fk, nrx ? ctx->k0.as<uint64_t>() : ctx->k1.as<uint64_t>(),
+                             fv, nrx ? ctx->v0.as<uint32_t>() : ctx->v1.as<uint32_t>(), rs.rec_cap, cnt + 0,
+                             rs.obits - rs.ob,
                              1u << rs.ob, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, rs.pbits,
                              start_off, ds.out, ds.out_cap, ds.total, st));
     WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, st));
@@ -1161,6 +1195,7 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     std::memcpy(ctx->hstage + k * sizeof(uint2), plans[k].items.data(), sizeof(uint2));
   }
   if (ctx->profiling) WS_HIP(ctx->dstamp.ensure(plans.size() * 2 * sizeof(unsigned long long)));
+  WS_HIP(ctx->items.ensure(plans.size() * sizeof(uint2)));
   if (ctx->profiling && ctx->pev.size() < 5 * plans.size()) {
     const size_t had = ctx->pev.size();
     ctx->pev.resize(5 * plans.size(), nullptr);
@@ -1170,6 +1205,7 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     WS_HIP(launch_call_init(ds.sticky, 10, ctx->counters.as<uint32_t>(), kCounterLitSnapshot + 1,
                             ctx->profiling ? ctx->dstamp.as<unsigned long long>() : nullptr,
                             (uint32_t)plans.size(), st));
+    WS_HIP(hipMemcpyAsync(ctx->items.p, ctx->hstage, plans.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
     for (size_t k = 0; k < plans.size(); ++k) {
       ws_status s = launch_run(ctx, p, plans[k], setups[k], offs[k], ds,
                                ctx->profiling ? &ctx->pev[5 * k] : nullptr);
@@ -1382,7 +1418,7 @@ void ws_ctx_destroy(ws_ctx* ctx) {
                     &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
                     &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
                     &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
-                    &ctx->bcur, &ctx->dstate, &ctx->dstamp})
+                    &ctx->bcur, &ctx->dstate, &ctx->dstamp, &ctx->k2, &ctx->v2, &ctx->bcnt2})
     b->release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);

@@ -769,15 +769,31 @@ __global__ __launch_bounds__(kTPB) void k_slice_gather2(SliceOut lit, SliceOut c
   }
 }
 
-// Literal matches of a set without regexes, from the slices straight to
-// ordering pairs: blocks as the literal half of k_slice_gather2; record
+// Literal matches from the slices straight to ordering pairs (and, when
+// cout is set, the candidate slices to records as k_slice_gather2's second
+// half, on blocks n + 64 on): blocks as the literal half of k_slice_gather2; record
 // {pid, start, end, item} becomes key (((item << sbits) | start) << pbits) |
 // pid with value end (as k_order_keys makes them), counted in bucket
 // key >> shift for the bucket sort that follows.
 __global__ __launch_bounds__(kTPB) void k_slice_gather_keys(SliceOut lit, uint32_t n, const uint32_t* loffs,
                                                              uint32_t cap, uint32_t sbits, uint32_t pbits,
                                                              uint64_t* keys, uint32_t* vals, uint32_t* cnt,
-                                                             uint32_t shift) {
+                                                             uint32_t shift, SliceOut cand, const uint32_t* coffs,
+                                                             Rec* cout, uint32_t ccap) {
+  if (blockIdx.x >= n + 64) {
+    // Candidate slices (regex sets), as k_slice_gather2's second half.
+    const uint32_t bx = blockIdx.x - (n + 64);
+    if (bx < n) {
+      const uint32_t m = min(cand.counts[bx], cand.cap), o = coffs[bx];
+      for (uint32_t k = threadIdx.x; k < m; k += kTPB)
+        if (o + k < ccap) cout[o + k] = cand.buf[(size_t)bx * cand.cap + k];
+    } else {
+      const uint32_t no = min(*cand.ovf_count, cand.ovf_cap), o = coffs[n];
+      for (uint32_t k = (bx - n) * kTPB + threadIdx.x; k < no; k += 64 * kTPB)
+        if (o + k < ccap) cout[o + k] = cand.ovf[k];
+    }
+    return;
+  }
   auto put = [&](uint32_t i, const Rec& r) {
     const uint64_t k = ((((uint64_t)r.w << sbits) | r.y) << pbits) | r.x;
     keys[i] = k;
@@ -919,7 +935,15 @@ __global__ __launch_bounds__(kTPB) void k_rx_walk(WalkParams w) {
     if (!dead && w.window_truncated) atomicOr(&w.counters[3], kFlagTruncatedWalk);
     if (last != 0xFFFFFFFFu) {
       const uint32_t slot = atomicAdd(&w.counters[2], 1u);
-      if (slot < w.raw_cap) w.raw[slot] = make_uint4(rx, cd.y, last, item);
+      if (slot < w.raw_cap) {
+        w.raw[slot] = make_uint4(rx, cd.y, last, item);
+        if (w.ckeys) {
+          const uint64_t k = (((uint64_t)item << w.rbits | rx) << w.sbits) | cd.y;
+          w.ckeys[slot] = k;
+          w.cvals[slot] = slot;
+          atomicAdd(&w.ccnt[k >> w.cshift], 1u);
+        }
+      }
     }
   }
 }
@@ -980,7 +1004,7 @@ __global__ void k_chain_walk(const uint64_t* keys, const uint32_t* vals, const R
                              uint32_t n, const uint32_t* nd, uint32_t gshift, const uint8_t* head,
                              const uint32_t* carried, uint32_t seg_off, const uint32_t* rx_pid,
                              uint32_t ties, Rec* out, uint32_t out_cap, uint32_t* out_count,
-                             uint32_t* carried_out) {
+                             uint32_t* carried_out, OrderOut oo) {
   n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (!head[i]) continue;
@@ -998,7 +1022,16 @@ __global__ void k_chain_walk(const uint64_t* keys, const uint32_t* vals, const R
       last_end = r.z;
       last_start = r.y;
       const uint32_t slot = atomicAdd(out_count, 1u);
-      if (slot < out_cap) out[slot] = make_uint4(rx_pid ? rx_pid[r.x] : r.x, r.y, r.z, r.w);
+      const uint32_t pid = rx_pid ? rx_pid[r.x] : r.x;
+      if (slot < out_cap) {
+        out[slot] = make_uint4(pid, r.y, r.z, r.w);
+        if (oo.keys) {
+          const uint64_t k = ((((uint64_t)r.w << oo.sbits) | r.y) << oo.pbits) | pid;
+          oo.keys[slot] = k;
+          oo.vals[slot] = r.z;
+          atomicAdd(&oo.cnt[k >> oo.shift], 1u);
+        }
+      }
       if (carried_out) atomicMax(&carried_out[r.x], seg_off + r.z);
     }
   }
@@ -1224,9 +1257,9 @@ hipError_t launch_chain_walk(const uint64_t* keys, const uint32_t* vals, const R
                              const uint32_t* nd, uint32_t gshift, const uint8_t* head, const uint32_t* carried,
                              uint32_t seg_off, const uint32_t* rx_pid, uint32_t ties, Rec* out,
                              uint32_t out_cap, uint32_t* out_count, uint32_t* carried_out,
-                             hipStream_t st) {
+                             hipStream_t st, const OrderOut* oo) {
   hipLaunchKernelGGL(k_chain_walk, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, keys, vals, raw, n, nd, gshift, head,
-                     carried, seg_off, rx_pid, ties, out, out_cap, out_count, carried_out);
+                     carried, seg_off, rx_pid, ties, out, out_cap, out_count, carried_out, oo ? *oo : OrderOut{});
   return hipGetLastError();
 }
 
@@ -1272,18 +1305,19 @@ hipError_t launch_slice_compact2(const SliceOut& lit, const SliceOut& cand, uint
   return hipGetLastError();
 }
 
-// Offsets of both slice sets, then the literal slices (the set has no
-// regexes, so no candidates) straight to ordering pairs and bucket counts.
+// Offsets of both slice sets, then the literal slices straight to ordering
+// pairs and bucket counts, and (cout set) the candidate slices to records.
 hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
                                      uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total,
                                      uint32_t* lit_snap, uint32_t cap, uint32_t sbits, uint32_t pbits,
-                                     uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift, hipStream_t st) {
+                                     uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift, Rec* cout,
+                                     uint32_t ccap, hipStream_t st) {
   hipLaunchKernelGGL(k_slice_offsets2, dim3(1), dim3(1024), 0, st, lit, cand, n, loffs, coffs, lit_total,
                      cand_total, lit_snap);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_slice_gather_keys, dim3(n + 64), dim3(kTPB), 0, st, lit, n, loffs, cap, sbits, pbits, keys,
-                     vals, cnt, shift);
+  hipLaunchKernelGGL(k_slice_gather_keys, dim3((cout ? 2 : 1) * (n + 64)), dim3(kTPB), 0, st, lit, n, loffs, cap,
+                     sbits, pbits, keys, vals, cnt, shift, cand, coffs, cout, ccap);
   return hipGetLastError();
 }
 
@@ -1468,17 +1502,19 @@ hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, u
 }
 
 // Sizes the scan's temporary storage (tmp = nullptr) or runs
-// count -> scan -> scatter -> per-bucket sort. cnt and cursor hold nb words;
-// cnt must be zero on entry and is zero again on exit. With nd set, n is a
-// capacity and the count is *nd.
+// count -> scan -> scatter -> per-bucket sort (with `counted`, the pairs'
+// producer has counted them already). cnt and cursor hold nb words; cnt must
+// be zero on entry (apart from a producer's counts) and is zero again on
+// exit. With nd set, n is a capacity and the count is *nd.
 hipError_t bucket_sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
                              const uint32_t* vin, uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift,
-                             uint32_t nb, uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st) {
+                             uint32_t nb, uint32_t* cnt, uint32_t* cursor, uint32_t* flag, hipStream_t st,
+                             bool counted) {
   if (!tmp)
     return rocprim::exclusive_scan(nullptr, *tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(),
                                    st);
   const uint32_t g = std::max(1u, std::min(nd ? kDeviceGrid : 4096u, (n + 255) / 256));
-  hipLaunchKernelGGL(k_bucket_count, dim3(g), dim3(256), 0, st, kin, n, nd, shift, cnt);
+  if (!counted) hipLaunchKernelGGL(k_bucket_count, dim3(g), dim3(256), 0, st, kin, n, nd, shift, cnt);
   hipError_t e = rocprim::exclusive_scan(tmp, *tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(), st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_bucket_scatter, dim3(g), dim3(256), 0, st, kin, vin, n, nd, shift, cursor, kout, vout);

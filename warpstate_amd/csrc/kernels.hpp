@@ -101,6 +101,23 @@ struct WalkParams {
   uint32_t raw_cap;
   uint32_t* counters;
   uint32_t window_truncated;  // segment ends before the true input end
+  // Optional (device-sized runs): chain ordering pairs written with each
+  // raw match, key ((item << rbits | regex) << sbits) | start and value its
+  // raw index, counted in bucket key >> cshift.
+  uint64_t* ckeys;
+  uint32_t* cvals;
+  uint32_t* ccnt;
+  uint32_t cshift, sbits, rbits;
+};
+
+// Optional output of k_chain_walk: final ordering pairs of the kept matches,
+// key ((item << sbits | start) << pbits) | pattern id and value end, counted
+// in bucket key >> shift.
+struct OrderOut {
+  uint64_t* keys;
+  uint32_t* vals;
+  uint32_t* cnt;
+  uint32_t shift, sbits, pbits;
 };
 
 enum : uint32_t { kFlagTruncatedWalk = 1u };
