@@ -61,6 +61,13 @@ hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, u
                              uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift, uint32_t nb,
                              uint32_t* cnt, uint32_t* cursor, uint32_t* flag, uint32_t pbits, uint32_t start_off,
                              void* out, uint64_t out_cap, const unsigned long long* base_dev, hipStream_t st);
+hipError_t launch_order_keys_dense(const Rec* recs, uint32_t n, uint32_t pbits, const uint64_t* bases,
+                                   uint64_t* keys, uint32_t* vals, hipStream_t st);
+hipError_t launch_pack_tagged_dense(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t pbits,
+                                    const uint64_t* bases, uint32_t nitems, const uint64_t* ids, void* out,
+                                    hipStream_t st);
+hipError_t launch_expand_units(const uint2* runs, uint32_t nruns, const uint2* items, uint32_t unit, uint4* units,
+                               uint32_t nunits, hipStream_t st);
 hipError_t launch_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counters, uint32_t ncounters,
                             unsigned long long* stamps, uint32_t n, hipStream_t st);
 hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords);
@@ -154,7 +161,7 @@ struct ws_ctx {
   hipEvent_t tev[2] = {};      // around a whole device-sized call (outside any graph)
   uint32_t wall_khz = 0;       // device wall-clock rate (kernel stamps)
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, dstamp, k2, v2, bcnt2;
+      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, dstamp, k2, v2, bcnt2, bases;
   std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
   uint8_t* hstage = nullptr;    // pinned staging read by captured copies
   size_t hstage_cap = 0;
@@ -375,6 +382,7 @@ struct Result {
   uint32_t sbits = 0, pbits = 0;
   uint32_t keybits = 0;
   bool bucketed = false;  // ordered by the bucket sort: confirm after the next sync
+  bool dense_batch = false;  // batch keys are (item base + start, pid): ctx->bases
 };
 
 ws_status sort_into(ws_ctx* ctx, uint32_t n, uint32_t bits, const uint64_t* kin, uint64_t* kout,
@@ -516,10 +524,11 @@ uint32_t choose_lane_bytes(const Plan& plan, const ws_tuning* tuning) {
 }
 
 // Work units of `unit` bytes. A single input strides statically; a batch
-// lists its units in work-queue order: largest items first
-// (longest-processing-time first, by power-of-two size class: a counting
-// sort, O(items)), each item's tiles contiguous.
-uint32_t plan_units(const Plan& plan, uint32_t unit, std::vector<uint4>* units) {
+// lists its units in work order: largest items first (longest-processing-
+// time first, by power-of-two size class: a counting sort, O(items)), each
+// item's tiles contiguous. The host keeps one run per item, {item, first
+// unit}; k_expand_units writes the unit records on the device.
+uint32_t plan_units(const Plan& plan, uint32_t unit, std::vector<uint2>* runs) {
   if (!plan.batch) return (uint32_t)((plan.items[0].y + unit - 1) / unit);
   const size_t ni = plan.items.size();
   auto cls = [&](size_t i) {
@@ -527,22 +536,19 @@ uint32_t plan_units(const Plan& plan, uint32_t unit, std::vector<uint4>* units) 
     return 32u - (uint32_t)__builtin_clz(len | 1u);  // 1..32
   };
   std::vector<uint32_t> start(34, 0);
-  uint64_t total_units = 0;
-  for (size_t i = 0; i < ni; ++i) {
-    start[33 - cls(i)]++;
-    const uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
-    total_units += hi > lo ? (hi - lo + unit - 1) / unit : 0;
-  }
+  for (size_t i = 0; i < ni; ++i) start[33 - cls(i)]++;
   for (uint32_t k = 0, acc = 0; k < 34; ++k) { const uint32_t c = start[k]; start[k] = acc; acc += c; }
   std::vector<uint32_t> by_size(ni);
   for (size_t i = 0; i < ni; ++i) by_size[start[33 - cls(i)]++] = (uint32_t)i;
-  units->reserve(total_units);
+  runs->reserve(ni);
+  uint32_t first = 0;
   for (uint32_t i : by_size) {
-    uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
-    for (uint64_t t = lo; t < hi; t += unit)
-      units->push_back(make_uint4((uint32_t)t, i, plan.items[i].x, plan.items[i].y));
+    const uint64_t lo = plan.items[i].x & ~15u, hi = plan.items[i].y;
+    if (hi <= lo) continue;
+    runs->push_back(make_uint2(i, first));
+    first += (uint32_t)((hi - lo + unit - 1) / unit);
   }
-  return (uint32_t)units->size();
+  return first;
 }
 
 // Per-block output slices over `buf` (`total` records): a quarter goes to the
@@ -619,15 +625,17 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
   const uint32_t lane_bytes = choose_lane_bytes(plan, tuning);
   const uint32_t unit = variant == WS_VARIANT_HASH ? kTPB * lane_bytes : ac_unit_bytes();
-  std::vector<uint4> units;
-  const uint32_t nunits = plan_units(plan, unit, &units);
+  std::vector<uint2> runs;
+  const uint32_t nunits = plan_units(plan, unit, &runs);
   WS_HIP(ctx->items.ensure(plan.items.size() * sizeof(uint2)));
   WS_HIP(hipMemcpyAsync(ctx->items.p, plan.items.data(), plan.items.size() * sizeof(uint2),
                         hipMemcpyHostToDevice, st));
-  if (plan.batch && !units.empty()) {
-    WS_HIP(ctx->units.ensure(units.size() * sizeof(uint4)));
-    WS_HIP(hipMemcpyAsync(ctx->units.p, units.data(), units.size() * sizeof(uint4),
-                          hipMemcpyHostToDevice, st));
+  if (plan.batch && nunits) {
+    WS_HIP(ctx->units.ensure((size_t)nunits * sizeof(uint4) + runs.size() * sizeof(uint2)));
+    uint2* druns = reinterpret_cast<uint2*>(ctx->units.as<uint4>() + nunits);
+    WS_HIP(hipMemcpyAsync(druns, runs.data(), runs.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+    WS_HIP(launch_expand_units(druns, (uint32_t)runs.size(), ctx->items.as<uint2>(), unit, ctx->units.as<uint4>(),
+                               nunits, st));
   }
   const uint32_t max_matches = (uint32_t)std::min<uint64_t>(ctx->cfg.max_matches, 0xFFFFFFF0ull);
   uint32_t lit_cap = (uint32_t)std::min<uint64_t>(ctx->recs.cap / sizeof(Rec), max_matches);
@@ -829,10 +837,30 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     WS_HIP(ctx->k1.ensure((size_t)total * 8));
     WS_HIP(ctx->v0.ensure((size_t)total * 4));
     WS_HIP(ctx->v1.ensure((size_t)total * 4));
-    WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), total, nullptr, res->sbits, res->pbits,
-                             ctx->k0.as<uint64_t>(), ctx->v0.as<uint32_t>(), st));
     res->count = total;
-    ws_status s = order_pairs(ctx, total, res->sbits + res->pbits + ibits, !plan.batch, res);
+    ws_status s = WS_OK;
+    if (plan.batch) {
+      // Items end to end in id order: dense (position, pid) keys, so the
+      // bucket sort applies (k_pack_tagged_dense maps positions back).
+      std::vector<uint64_t> bases(plan.items.size());
+      uint64_t acc = 0;
+      for (size_t i = 0; i < plan.items.size(); ++i) {
+        bases[i] = acc;
+        acc += plan.items[i].y - plan.items[i].x;
+      }
+      const uint32_t gbits = bits_for(acc ? acc - 1 : 0);
+      if (gbits + res->pbits > 63) return fail(WS_ERR_INPUT_TOO_LARGE, "batch key does not fit 63 bits");
+      WS_HIP(ctx->bases.ensure(bases.size() * 8));
+      WS_HIP(hipMemcpyAsync(ctx->bases.p, bases.data(), bases.size() * 8, hipMemcpyHostToDevice, st));
+      WS_HIP(launch_order_keys_dense(ctx->recs.as<Rec>(), total, res->pbits, ctx->bases.as<uint64_t>(),
+                                     ctx->k0.as<uint64_t>(), ctx->v0.as<uint32_t>(), st));
+      res->dense_batch = true;
+      s = order_pairs(ctx, total, gbits + res->pbits, true, res);
+    } else {
+      WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), total, nullptr, res->sbits, res->pbits,
+                               ctx->k0.as<uint64_t>(), ctx->v0.as<uint32_t>(), st));
+      s = order_pairs(ctx, total, res->sbits + res->pbits + ibits, true, res);
+    }
     if (s != WS_OK) return s;
   }
   res->count = total;
@@ -948,7 +976,7 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
     return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
   rs->lane_bytes = choose_lane_bytes(plan, tuning);
   const uint32_t unit = rs->variant == WS_VARIANT_HASH ? kTPB * rs->lane_bytes : ac_unit_bytes();
-  std::vector<uint4> none;
+  std::vector<uint2> none;
   rs->nunits = plan_units(plan, unit, &none);
   const uint64_t span = plan.items[0].y - plan.items[0].x;
   rs->max_matches = (uint32_t)std::min<uint64_t>(ctx->cfg.max_matches, 0xFFFFFFF0ull);
@@ -1418,7 +1446,7 @@ void ws_ctx_destroy(ws_ctx* ctx) {
                     &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
                     &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
                     &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
-                    &ctx->bcur, &ctx->dstate, &ctx->dstamp, &ctx->k2, &ctx->v2, &ctx->bcnt2})
+                    &ctx->bcur, &ctx->dstate, &ctx->dstamp, &ctx->k2, &ctx->v2, &ctx->bcnt2, &ctx->bases})
     b->release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -1866,12 +1894,24 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
   }
   *count = r.count;
   uint32_t n = (uint32_t)std::min<uint64_t>(r.count, cap);
+  if (r.bucketed) {
+    // The bucket sort's overflow flag came back with this synchronisation.
+    WS_HIP(hipStreamSynchronize(ctx->stream));
+    bool redone = false;
+    s = confirm_order(ctx, r, &redone);
+    if (s != WS_OK) return s;
+  }
   if (n) {
     WS_HIP(ctx->ids.ensure(ids.size() * 8));
     WS_HIP(hipMemcpyAsync(ctx->ids.p, ids.data(), ids.size() * 8, hipMemcpyHostToDevice, ctx->stream));
     WS_HIP(ctx->outbuf.ensure((size_t)n * sizeof(ws_tagged_match)));
-    WS_HIP(launch_pack_tagged(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.sbits, r.pbits,
-                              ctx->ids.as<uint64_t>(), ctx->outbuf.p, ctx->stream));
+    if (r.dense_batch)
+      WS_HIP(launch_pack_tagged_dense(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.pbits,
+                                      ctx->bases.as<uint64_t>(), (uint32_t)plan.items.size(), ctx->ids.as<uint64_t>(),
+                                      ctx->outbuf.p, ctx->stream));
+    else
+      WS_HIP(launch_pack_tagged(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.sbits, r.pbits,
+                                ctx->ids.as<uint64_t>(), ctx->outbuf.p, ctx->stream));
     WS_HIP(hipMemcpyAsync(out, ctx->outbuf.p, (size_t)n * sizeof(ws_tagged_match),
                           hipMemcpyDeviceToHost, ctx->stream));
     WS_HIP(hipStreamSynchronize(ctx->stream));

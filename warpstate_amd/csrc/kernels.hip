@@ -1108,6 +1108,60 @@ __global__ void k_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32
   }
 }
 
+// Batch ordering keys over the items laid end to end in id order: item i's
+// match at start s sits at bases[i] + s (bases: exclusive prefix sums of the
+// item lengths), so keys (position << pbits | pid) are dense and the bucket
+// sort applies. Same order as (item, start, pid).
+__global__ void k_order_keys_dense(const Rec* recs, uint32_t n, uint32_t pbits, const uint64_t* bases,
+                                   uint64_t* keys, uint32_t* vals) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const Rec r = recs[i];
+    keys[i] = ((bases[r.w] + r.y) << pbits) | r.x;
+    vals[i] = r.z;
+  }
+}
+
+// Tagged output from dense batch keys: the item is the last one whose base is
+// <= the position (empty items share their successor's base and hold no
+// matches), found by binary search.
+__global__ void k_pack_tagged_dense(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t pbits,
+                                    const uint64_t* bases, uint32_t nitems, const uint64_t* ids, TaggedOut* out) {
+  const uint64_t pmask = (1ull << pbits) - 1;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t k = keys[i];
+    const uint64_t g = k >> pbits;
+    uint32_t lo = 0, hi = nitems;  // bases[lo] <= g < bases[hi] (hi = nitems: past the end)
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (bases[mid] <= g) lo = mid;
+      else hi = mid;
+    }
+    TaggedOut t;
+    t.pid = (uint32_t)(k & pmask);
+    t.start = (uint32_t)(g - bases[lo]);
+    t.id = ids[lo];
+    t.end = vals[i];
+    t.pad = 0;
+    out[i] = t;
+  }
+}
+
+hipError_t launch_order_keys_dense(const Rec* recs, uint32_t n, uint32_t pbits, const uint64_t* bases,
+                                   uint64_t* keys, uint32_t* vals, hipStream_t st) {
+  const uint32_t g = std::max(1u, std::min(4096u, (n + kTPB - 1) / kTPB));
+  hipLaunchKernelGGL(k_order_keys_dense, dim3(g), dim3(kTPB), 0, st, recs, n, pbits, bases, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_tagged_dense(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t pbits,
+                                    const uint64_t* bases, uint32_t nitems, const uint64_t* ids, void* out,
+                                    hipStream_t st) {
+  const uint32_t g = std::max(1u, std::min(4096u, (n + kTPB - 1) / kTPB));
+  hipLaunchKernelGGL(k_pack_tagged_dense, dim3(g), dim3(kTPB), 0, st, keys, vals, n, pbits, bases, nitems, ids,
+                     reinterpret_cast<TaggedOut*>(out));
+  return hipGetLastError();
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
@@ -1318,6 +1372,31 @@ hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, 
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_slice_gather_keys, dim3((cout ? 2 : 1) * (n + 64)), dim3(kTPB), 0, st, lit, n, loffs, cap,
                      sbits, pbits, keys, vals, cnt, shift, cand, coffs, cout, ccap);
+  return hipGetLastError();
+}
+
+// A batch's unit records from its runs ({item, first unit}, first units
+// increasing from 0): unit u belongs to the last run whose first unit is
+// <= u and is that item's tile (start & ~15) + (u - first) * unit.
+__global__ void k_expand_units(const uint2* runs, uint32_t nruns, const uint2* items, uint32_t unit, uint4* units,
+                               uint32_t nunits) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < nunits; u += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nruns;  // runs[lo].y <= u < runs[hi].y (hi = nruns: past the end)
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (runs[mid].y <= u) lo = mid;
+      else hi = mid;
+    }
+    const uint2 r = runs[lo];
+    const uint2 seg = items[r.x];
+    units[u] = make_uint4((seg.x & ~15u) + (u - r.y) * unit, r.x, seg.x, seg.y);
+  }
+}
+
+hipError_t launch_expand_units(const uint2* runs, uint32_t nruns, const uint2* items, uint32_t unit, uint4* units,
+                               uint32_t nunits, hipStream_t st) {
+  hipLaunchKernelGGL(k_expand_units, dim3(std::max(1u, std::min(4096u, (nunits + 255) / 256))), dim3(256), 0, st,
+                     runs, nruns, items, unit, units, nunits);
   return hipGetLastError();
 }
 
