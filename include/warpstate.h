@@ -221,6 +221,29 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t
                          uint32_t mode, const ws_tuning* tuning, ws_match* out, uint64_t cap,
                          uint32_t out_mem, uint64_t* count);
 
+/* ---- incremental stream: StreamScanner (core/src/stream.rs:14-135).
+ * Each feed scans the carried overlap plus the new chunk (on the device) in `mode`,
+ * reports the matches that end past the overlap, with offsets from the start of the
+ * stream, and carries the window's last L bytes, L = the longest literal pattern
+ * (regexes add nothing to L, as in core/src/pattern/compiler.rs:149,201). Offsets and
+ * the carried window are limited to 4 GiB (WS_ERR_INPUT_TOO_LARGE). */
+typedef struct ws_stream_scanner ws_stream_scanner;
+
+/* StreamScanner::new (core/src/stream.rs:38-57); mode WS_MODE_STANDARD is the reference's. */
+ws_status ws_stream_scanner_create(ws_ctx* ctx, const ws_patterns* patterns, uint32_t mode,
+                                   ws_stream_scanner** out);
+void ws_stream_scanner_destroy(ws_stream_scanner* scanner);
+/* StreamScanner::feed (core/src/stream.rs:59-125). chunk is host or device memory
+ * (data_mem); out is host memory. On WS_ERR_MATCH_BUFFER_OVERFLOW *count holds the
+ * required capacity and the scanner has not advanced. */
+ws_status ws_stream_scanner_feed(ws_stream_scanner* scanner, const uint8_t* chunk, uint64_t len,
+                                 uint32_t data_mem, ws_match* out, uint64_t cap, uint64_t* count);
+/* StreamScanner::finish (core/src/stream.rs:127-134): a feed of nothing. */
+ws_status ws_stream_scanner_finish(ws_stream_scanner* scanner, ws_match* out, uint64_t cap,
+                                   uint64_t* count);
+/* Bytes fed so far (StreamScanner::processed_bytes). */
+uint64_t ws_stream_scanner_processed(const ws_stream_scanner* scanner);
+
 /* ---- kernel routing: AutoMatcher (core/src/router.rs:205-291, core/src/config.rs:4-35).
  * The reference routes on one input-size threshold. Here an exhaustive autotuner
  * times every kernel configuration, and a small learned cost model (FP32 MLP,

@@ -157,3 +157,33 @@ class Oracle:
         out = np.zeros(max(int(n), 1), dtype=MATCH_DTYPE)
         lib().or_regex_find_iter(self.h, regex_index, buf.ctypes.data, len(buf), out.ctypes.data, n)
         return [(int(s), int(e)) for s, e in zip(out["start"][: int(n)], out["end"][: int(n)])]
+
+
+class StreamScannerOracle:
+    """Restatement of StreamScanner (core/src/stream.rs:38-134): each feed
+    scans window = carried tail + chunk, keeps the matches that end past the
+    tail (stream.rs:104) and shifts them by processed - len(tail)
+    (stream.rs:101-108); the new tail is the window's last `keep` bytes
+    (stream.rs:110-120), keep = the longest literal pattern (regexes add
+    nothing: core/src/pattern/compiler.rs:149,201). finish() is feed(b"")."""
+
+    def __init__(self, oracle: "Oracle", keep: int, mode: int = 1):
+        self.oracle = oracle
+        self.keep = keep
+        self.mode = mode
+        self.tail = b""
+        self.processed = 0
+
+    def feed(self, chunk):
+        chunk = _b(chunk)
+        window = self.tail + chunk
+        base = self.processed - len(self.tail)
+        found = self.oracle.scan_list(window, self.mode) if window else []
+        out = [(p, s + base, e + base) for p, s, e in found if e > len(self.tail)]
+        keep = min(self.keep, len(window))
+        self.tail = window[len(window) - keep:] if keep else b""
+        self.processed += len(chunk)
+        return out
+
+    def finish(self):
+        return self.feed(b"")

@@ -210,3 +210,66 @@ def test_threaded_overlapping_scan_equals_single_thread():
         n = o.scan_into(data, out, 0, threads)
         assert n == len(ref)
         assert np.array_equal(out[:n], ref), threads
+
+
+# --- StreamScanner (core/src/stream.rs) -------------------------------------
+
+from oracle.oracle import StreamScannerOracle  # noqa: E402
+
+
+def _keep(pats):
+    return max((len(p) for k, p in pats if k == 0), default=0)
+
+
+@pytest.mark.parametrize("kat", load("stream_kats.json"), ids=lambda k: k["name"])
+def test_stream_scanner_kats(kat):
+    pats = [(k, bytes.fromhex(h)) for k, h in kat["patterns"]]
+    sc = StreamScannerOracle(Oracle(pats), _keep(pats), mode=1)
+    for feed, exp in zip(kat["feeds"], kat["expected"]):
+        assert sc.feed(bytes.fromhex(feed)) == [tuple(e) for e in exp], kat["source"]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_stream_scanner_overlapping_literals_equal_whole_scan(seed):
+    # Every occurrence of a literal ends past the tail of exactly one window
+    # (the tail is the longest literal), so the feeds add up to the whole
+    # overlapping scan, which is brute force here.
+    r = np.random.default_rng(seed)
+    pats = [(0, bytes(r.integers(97, 100, size=int(r.integers(1, 6)), dtype=np.uint8))) for _ in range(5)]
+    data = bytes(r.integers(97, 100, size=3000, dtype=np.uint8))
+    cuts = sorted(r.integers(0, len(data), size=12).tolist())
+    sc = StreamScannerOracle(Oracle(pats), _keep(pats), mode=0)
+    got = []
+    prev = 0
+    for c in cuts + [len(data)]:
+        got += sc.feed(data[prev:c])
+        prev = c
+    got += sc.finish()
+    brute = sorted((i, s, s + len(p)) for i, (_, p) in enumerate(pats)
+                   for s in range(len(data) - len(p) + 1) if data[s:s + len(p)] == p)
+    assert sorted(got) == brute
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_stream_scanner_standard_matches_scan_like_reference_property(seed):
+    # core/tests/property/proptest_scan.rs:257-310: stream totals equal scan()
+    # on the concatenation (compared by (start, pattern id)).
+    r = np.random.default_rng(100 + seed)
+    pats = [(0, bytes(r.integers(0, 256, size=int(r.integers(1, 8)), dtype=np.uint8))) for _ in range(3)]
+    chunks = [bytes(r.integers(0, 256, size=int(r.integers(0, 200)), dtype=np.uint8)) for _ in range(10)]
+    # plant the patterns across chunk boundaries
+    data = bytearray(b"".join(chunks))
+    for k in range(0, len(data) - 8, 97):
+        p = pats[k % len(pats)][1]
+        data[k:k + len(p)] = p
+    data = bytes(data)
+    cuts = np.cumsum([len(c) for c in chunks]).tolist()
+    o = Oracle(pats)
+    sc = StreamScannerOracle(o, _keep(pats), mode=1)
+    got, prev = [], 0
+    for c in cuts:
+        got += sc.feed(data[prev:c])
+        prev = c
+    got += sc.finish()
+    whole = o.scan_list(data, 1)
+    assert sorted((s, p) for p, s, e in got) == sorted((s, p) for p, s, e in whole)

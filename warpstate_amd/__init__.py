@@ -24,6 +24,7 @@ from .api import (  # noqa: F401
     PatternSetBuilder,
     ScanItem,
     StreamPipeline,
+    StreamScanner,
     TaggedMatch,
     Tuning,
     WarpStateer,

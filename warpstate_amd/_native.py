@@ -157,6 +157,12 @@ SIGNATURES = {
         [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint64, c_uint64, c_uint32, POINTER(WsTuning),
          c_void_p, c_uint64, c_uint32, POINTER(c_uint64)],
     ),
+    "ws_stream_scanner_create": (c_int, [c_void_p, c_void_p, c_uint32, POINTER(c_void_p)]),
+    "ws_stream_scanner_destroy": (None, [c_void_p]),
+    "ws_stream_scanner_feed": (c_int, [c_void_p, c_void_p, c_uint64, c_uint32, c_void_p, c_uint64,
+                                       POINTER(c_uint64)]),
+    "ws_stream_scanner_finish": (c_int, [c_void_p, c_void_p, c_uint64, POINTER(c_uint64)]),
+    "ws_stream_scanner_processed": (c_uint64, [c_void_p]),
 }
 
 _lib = None

@@ -393,6 +393,63 @@ class StreamPipeline:
         return matches_to_list(self.scan_array(data, chunk_size=chunk_size))
 
 
+class StreamScanner:
+    """StreamScanner (core/src/stream.rs:14-135): incremental feeds. Each feed
+    scans the carried tail plus the new chunk on the device and returns the
+    matches that end past the tail, at stream offsets; the tail is the last
+    L bytes, L the longest literal pattern. mode "standard" (PatternSet::scan
+    semantics) is the reference's; "overlapping" reports every occurrence."""
+
+    def __init__(self, backend, mode: str = "standard"):
+        matcher = backend if isinstance(backend, GpuMatcher) else GpuMatcher(backend)
+        self.backend = matcher
+        lib = N.load()
+        h = ctypes.c_void_p()
+        N.check(lib.ws_stream_scanner_create(matcher.ctx.handle, matcher.handle, _mode_code(mode), ctypes.byref(h)))
+        self.handle = h.value
+
+    def _run(self, fn, *args) -> List[Match]:
+        lib = N.load()
+        cap = 1024
+        while True:
+            out = np.zeros(cap, dtype=N.MATCH_DTYPE)
+            count = ctypes.c_uint64(0)
+            st = fn(self.handle, *args, out.ctypes.data, cap, ctypes.byref(count))
+            if st == 11 and count.value > cap and count.value <= self.backend.config.max_matches:
+                cap = int(count.value)
+                continue
+            N.check(st, int(count.value))
+            return matches_to_list(out[: count.value])
+
+    def feed(self, chunk) -> List[Match]:
+        lib = N.load()
+        buf = np.frombuffer(_as_bytes(chunk), dtype=np.uint8)
+        return self._run(lib.ws_stream_scanner_feed, buf.ctypes.data if len(buf) else None, len(buf), N.MEM_HOST)
+
+    def feed_device(self, data_ptr: int, length: int) -> List[Match]:
+        """A device-resident chunk."""
+        lib = N.load()
+        return self._run(lib.ws_stream_scanner_feed, ctypes.c_void_p(data_ptr), length, N.MEM_DEVICE)
+
+    def finish(self) -> List[Match]:
+        return self._run(N.load().ws_stream_scanner_finish)
+
+    @property
+    def processed_bytes(self) -> int:
+        return int(N.load().ws_stream_scanner_processed(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            N.load().ws_stream_scanner_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _batch_array(matcher: GpuMatcher, items: Sequence[ScanItem], mode: str,
                  tuning: Optional[Tuning] = None) -> np.ndarray:
     lib = N.load()

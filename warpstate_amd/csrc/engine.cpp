@@ -1923,4 +1923,114 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
   return WS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Incremental stream scanner. The window (carried tail + new chunk) lives in
+// one of two device buffers; each feed builds the next window in the other
+// buffer and scans it with ws_scan (so repeated window shapes replay a graph).
+// ---------------------------------------------------------------------------
+struct ws_stream_scanner {
+  ws_ctx* ctx = nullptr;
+  const ws_patterns* p = nullptr;
+  uint32_t mode = WS_MODE_STANDARD;
+  uint64_t keep = 0;         // longest literal: the carried tail's maximum length
+  DevBuf win[2];
+  int cur = 0;               // buffer holding the current window
+  uint64_t win_len = 0;      // its length
+  uint64_t overlap = 0;      // its carried tail (the last `overlap` bytes)
+  uint64_t processed = 0;    // bytes fed so far
+  std::vector<ws_match> tmp; // the window's matches
+};
+
+ws_status ws_stream_scanner_create(ws_ctx* ctx, const ws_patterns* p, uint32_t mode, ws_stream_scanner** out) {
+  if (!ctx || !p || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");
+  if (p->dev != ctx->dev) return fail(WS_ERR_INVALID_ARGUMENT, "pattern set was compiled for another device");
+  auto* sc = new ws_stream_scanner();
+  sc->ctx = ctx;
+  sc->p = p;
+  sc->mode = mode;
+  for (const auto& l : p->h.lits) sc->keep = std::max<uint64_t>(sc->keep, l.size());
+  *out = sc;
+  return WS_OK;
+}
+
+void ws_stream_scanner_destroy(ws_stream_scanner* sc) {
+  if (!sc) return;
+  {
+    std::lock_guard<std::mutex> lk(sc->ctx->mu);
+    (void)hipSetDevice(sc->ctx->dev);
+    (void)hipStreamSynchronize(sc->ctx->stream);
+    sc->win[0].release();
+    sc->win[1].release();
+  }
+  delete sc;
+}
+
+uint64_t ws_stream_scanner_processed(const ws_stream_scanner* sc) { return sc ? sc->processed : 0; }
+
+ws_status ws_stream_scanner_feed(ws_stream_scanner* sc, const uint8_t* chunk, uint64_t len, uint32_t data_mem,
+                                 ws_match* out, uint64_t cap, uint64_t* count) {
+  if (!sc || !count) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (len && !chunk) return fail(WS_ERR_INVALID_ARGUMENT, "chunk is null");
+  if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
+  *count = 0;
+  ws_ctx* ctx = sc->ctx;
+  const uint64_t combined = sc->overlap + len;
+  if (combined > 0xFFFFFFFFull)
+    return fail(WS_ERR_INPUT_TOO_LARGE, "stream window of " + std::to_string(combined) + " bytes exceeds 4 GiB");
+  if (sc->processed + len > 0xFFFFFFFFull)
+    return fail(WS_ERR_INPUT_TOO_LARGE, "stream offset " + std::to_string(sc->processed + len) + " exceeds 4 GiB");
+  const int nxt = 1 - sc->cur;
+  {
+    // The next window: the current one's carried tail, then the chunk.
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    WS_HIP(hipSetDevice(ctx->dev));
+    WS_HIP(sc->win[nxt].ensure(((combined + 15) & ~15ull) + 16));
+    if (sc->overlap)
+      WS_HIP(hipMemcpyAsync(sc->win[nxt].p, sc->win[sc->cur].as<uint8_t>() + (sc->win_len - sc->overlap),
+                            sc->overlap, hipMemcpyDeviceToDevice, ctx->stream));
+    if (len)
+      WS_HIP(hipMemcpyAsync(sc->win[nxt].as<uint8_t>() + sc->overlap, chunk, len,
+                            data_mem == WS_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                            ctx->stream));
+    WS_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  // The window's matches, in order (start, pattern id, end).
+  const uint64_t mcap = std::max<uint64_t>(1024, std::min<uint64_t>(ctx->cfg.max_matches, combined + 16));
+  if (sc->tmp.size() < mcap) sc->tmp.resize(mcap);
+  uint64_t n = 0;
+  ws_status s = ws_scan(ctx, sc->p, sc->win[nxt].as<uint8_t>(), combined, WS_MEM_DEVICE, sc->mode, nullptr,
+                        sc->tmp.data(), sc->tmp.size(), WS_MEM_HOST, &n);
+  if (s == WS_ERR_MATCH_BUFFER_OVERFLOW && n <= ctx->cfg.max_matches) {
+    sc->tmp.resize(n);
+    s = ws_scan(ctx, sc->p, sc->win[nxt].as<uint8_t>(), combined, WS_MEM_DEVICE, sc->mode, nullptr,
+                sc->tmp.data(), sc->tmp.size(), WS_MEM_HOST, &n);
+  }
+  if (s != WS_OK) return s;
+  // New matches end past the carried tail; offsets become stream offsets.
+  const uint64_t base = sc->processed - sc->overlap;
+  uint64_t kept = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if (sc->tmp[i].end > sc->overlap) ++kept;
+  *count = kept;
+  if (kept > cap)
+    return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(kept) + " exceeds " +
+                                                  std::to_string(cap) + ")");
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const ws_match& m = sc->tmp[i];
+    if (m.end <= sc->overlap) continue;
+    out[k++] = ws_match{m.pattern_id, (uint32_t)(m.start + base), (uint32_t)(m.end + base)};
+  }
+  sc->cur = nxt;
+  sc->win_len = combined;
+  sc->overlap = std::min(sc->keep, combined);
+  sc->processed += len;
+  return WS_OK;
+}
+
+ws_status ws_stream_scanner_finish(ws_stream_scanner* sc, ws_match* out, uint64_t cap, uint64_t* count) {
+  return ws_stream_scanner_feed(sc, nullptr, 0, WS_MEM_HOST, out, cap, count);
+}
+
 }  // extern "C"
