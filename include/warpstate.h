@@ -189,6 +189,9 @@ uint32_t ws_abi_version(void);
  * then on: any context on that device may scan with it, concurrently. */
 ws_status ws_patterns_compile(ws_ctx* ctx, const ws_pattern* patterns, size_t count,
                               uint32_t flags, ws_patterns** out);
+/* A compiled set is reference counted: compile returns one reference, retain adds one
+ * (Arc<PatternSet>::clone), free drops one and releases the set with the last. */
+void ws_patterns_retain(ws_patterns* patterns);
 void ws_patterns_free(ws_patterns* patterns);
 /* Host-only compile (no device needed): validates the set exactly like
  * ws_patterns_compile and reports its facts; the learned router's features. */
@@ -220,6 +223,22 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t
                          uint64_t len, uint32_t data_mem, uint64_t chunk_size, uint64_t overlap,
                          uint32_t mode, const ws_tuning* tuning, ws_match* out, uint64_t cap,
                          uint32_t out_mem, uint64_t* count);
+
+/* ---- HotSwapPatternSet (core/src/pattern/mod.rs:17-70): the active compiled set,
+ * replaced atomically. A scan takes a reference to the set that is active when it
+ * starts and keeps it to the end, so a swap never changes a scan in flight. */
+typedef struct ws_hotswap ws_hotswap;
+
+/* HotSwapPatternSet::new / from_arc: takes its own reference to `initial`. */
+ws_status ws_hotswap_create(ws_patterns* initial, ws_hotswap** out);
+void ws_hotswap_destroy(ws_hotswap* hs);
+/* HotSwapPatternSet::swap_arc: 1 if `next` replaced the active set, 0 if it already
+ * was the active set (Arc::ptr_eq); takes its own reference to `next`. */
+int ws_hotswap_swap(ws_hotswap* hs, ws_patterns* next);
+/* HotSwapPatternSet::scan: ws_scan with the active set. */
+ws_status ws_hotswap_scan(ws_ctx* ctx, ws_hotswap* hs, const uint8_t* data, uint64_t len,
+                          uint32_t data_mem, uint32_t mode, const ws_tuning* tuning, ws_match* out,
+                          uint64_t cap, uint32_t out_mem, uint64_t* count);
 
 /* ---- incremental stream: StreamScanner (core/src/stream.rs:14-135).
  * Each feed scans the carried overlap plus the new chunk (on the device) in `mode`,

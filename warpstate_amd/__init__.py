@@ -19,6 +19,7 @@ from .api import (  # noqa: F401
     AutoMatcherConfig,
     Context,
     GpuMatcher,
+    HotSwapPatternSet,
     Match,
     PatternSet,
     PatternSetBuilder,

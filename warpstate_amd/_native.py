@@ -123,6 +123,15 @@ SIGNATURES = {
     "ws_ctx_last_timings": (c_int, [c_void_p, POINTER(WsTimings)]),
     "ws_patterns_compile": (c_int, [c_void_p, POINTER(WsPattern), c_size_t, c_uint32, POINTER(c_void_p)]),
     "ws_patterns_free": (None, [c_void_p]),
+    "ws_patterns_retain": (None, [c_void_p]),
+    "ws_hotswap_create": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "ws_hotswap_destroy": (None, [c_void_p]),
+    "ws_hotswap_swap": (c_int, [c_void_p, c_void_p]),
+    "ws_hotswap_scan": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, POINTER(WsTuning), c_void_p, c_uint64,
+         c_uint32, POINTER(c_uint64)],
+    ),
     "ws_patterns_info_get": (c_int, [c_void_p, POINTER(WsPatternsInfo)]),
     "ws_patterns_analyze": (c_int, [POINTER(WsPattern), c_size_t, c_uint32, POINTER(WsPatternsInfo)]),
     "ws_scan": (

@@ -393,6 +393,57 @@ class StreamPipeline:
         return matches_to_list(self.scan_array(data, chunk_size=chunk_size))
 
 
+class HotSwapPatternSet:
+    """HotSwapPatternSet (core/src/pattern/mod.rs:17-70): the active compiled
+    set, replaced atomically by swap(); a scan keeps the set that was active
+    when it started. Thread-safe: scans from several threads (each with its
+    own Context, or sharing one) may run while another thread swaps."""
+
+    def __init__(self, patterns: "PatternSet", device: int = 0, config: Optional[AutoMatcherConfig] = None):
+        self.config = config or AutoMatcherConfig()
+        self.ctx = Context(device, self.config)
+        h = ctypes.c_void_p()
+        N.check(N.load().ws_hotswap_create(patterns._compiled(self.ctx), ctypes.byref(h)))
+        self.handle = h.value
+
+    def swap(self, nxt: "PatternSet") -> bool:
+        """True if `nxt` replaced the active set, False if it already was it.
+        The hot-swap slot holds its own reference to the compiled set."""
+        return bool(N.load().ws_hotswap_swap(self.handle, nxt._compiled(self.ctx)))
+
+    def scan_array(self, data, mode="standard", ctx: Optional[Context] = None) -> np.ndarray:
+        lib = N.load()
+        c = ctx or self.ctx
+        buf = np.frombuffer(_as_bytes(data), dtype=np.uint8)
+        cap = max(1024, min(self.config.max_matches, len(buf) // 8 + 1024))
+        while True:
+            out = np.zeros(cap, dtype=N.MATCH_DTYPE)
+            count = ctypes.c_uint64(0)
+            st = lib.ws_hotswap_scan(c.handle, self.handle, buf.ctypes.data if len(buf) else None, len(buf),
+                                     N.MEM_HOST, _mode_code(mode), None, out.ctypes.data, cap, N.MEM_HOST,
+                                     ctypes.byref(count))
+            if st == 11 and count.value > cap and count.value <= self.config.max_matches:
+                cap = int(count.value)
+                continue
+            N.check(st, int(count.value))
+            return out[: count.value]
+
+    def scan(self, data, mode="standard", ctx: Optional[Context] = None) -> List[Match]:
+        """PatternSet::scan semantics by default, as HotSwapPatternSet::scan."""
+        return matches_to_list(self.scan_array(data, mode, ctx))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            N.load().ws_hotswap_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class StreamScanner:
     """StreamScanner (core/src/stream.rs:14-135): incremental feeds. Each feed
     scans the carried tail plus the new chunk on the device and returns the

@@ -171,6 +171,7 @@ struct ws_ctx {
 };
 
 struct ws_patterns {
+  std::atomic<uint32_t> refs{1};
   int dev = 0;
   uint64_t serial = 0;  // unique per compiled set (keys captured graphs)
   PatternsHost h;
@@ -1515,8 +1516,13 @@ ws_status ws_patterns_compile(ws_ctx* ctx, const ws_pattern* pats, size_t count,
   return WS_OK;
 }
 
+void ws_patterns_retain(ws_patterns* p) {
+  if (p) p->refs.fetch_add(1);
+}
+
 void ws_patterns_free(ws_patterns* p) {
   if (!p) return;
+  if (p->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(p->dev);
   (void)hipDeviceSynchronize();
   p->blob.release();
@@ -1924,6 +1930,59 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
 }
 
 // ---------------------------------------------------------------------------
+// Hot-swappable pattern set: the active set behind a mutex; every holder
+// (the swap slot, each scan in flight) owns a reference.
+// ---------------------------------------------------------------------------
+struct ws_hotswap {
+  std::mutex mu;
+  ws_patterns* cur = nullptr;
+};
+
+ws_status ws_hotswap_create(ws_patterns* initial, ws_hotswap** out) {
+  if (!initial || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  ws_patterns_retain(initial);
+  auto* hs = new ws_hotswap();
+  hs->cur = initial;
+  *out = hs;
+  return WS_OK;
+}
+
+void ws_hotswap_destroy(ws_hotswap* hs) {
+  if (!hs) return;
+  ws_patterns_free(hs->cur);
+  delete hs;
+}
+
+int ws_hotswap_swap(ws_hotswap* hs, ws_patterns* next) {
+  if (!hs || !next) return 0;
+  ws_patterns* old = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(hs->mu);
+    if (hs->cur == next) return 0;
+    ws_patterns_retain(next);
+    old = hs->cur;
+    hs->cur = next;
+  }
+  ws_patterns_free(old);
+  return 1;
+}
+
+ws_status ws_hotswap_scan(ws_ctx* ctx, ws_hotswap* hs, const uint8_t* data, uint64_t len, uint32_t data_mem,
+                          uint32_t mode, const ws_tuning* tuning, ws_match* out, uint64_t cap, uint32_t out_mem,
+                          uint64_t* count) {
+  if (!hs) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  ws_patterns* p = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(hs->mu);
+    p = hs->cur;
+    ws_patterns_retain(p);
+  }
+  const ws_status s = ws_scan(ctx, p, data, len, data_mem, mode, tuning, out, cap, out_mem, count);
+  ws_patterns_free(p);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
 // Incremental stream scanner. The window (carried tail + new chunk) lives in
 // one of two device buffers; each feed builds the next window in the other
 // buffer and scans it with ws_scan (so repeated window shapes replay a graph).
@@ -1947,6 +2006,7 @@ ws_status ws_stream_scanner_create(ws_ctx* ctx, const ws_patterns* p, uint32_t m
   if (p->dev != ctx->dev) return fail(WS_ERR_INVALID_ARGUMENT, "pattern set was compiled for another device");
   auto* sc = new ws_stream_scanner();
   sc->ctx = ctx;
+  ws_patterns_retain(const_cast<ws_patterns*>(p));
   sc->p = p;
   sc->mode = mode;
   for (const auto& l : p->h.lits) sc->keep = std::max<uint64_t>(sc->keep, l.size());
@@ -1963,6 +2023,7 @@ void ws_stream_scanner_destroy(ws_stream_scanner* sc) {
     sc->win[0].release();
     sc->win[1].release();
   }
+  ws_patterns_free(const_cast<ws_patterns*>(sc->p));
   delete sc;
 }
 
