@@ -177,6 +177,34 @@ class Workload:
                           f"leftmost-first regexes, C oracle, one thread ({dt_1:.1f} s)"}
 
 
+def index_timing(w) -> dict:
+    """Loading the workload's set from a version-4 compiled index against
+    compiling it from its patterns; both include the device upload."""
+    import ctypes
+
+    from warpstate_amd import CompiledPatternIndex
+    from warpstate_amd import _native as N
+
+    lib = N.load()
+    data = CompiledPatternIndex.build(w.ps)
+    arr, _keep = w.ps._c_patterns()
+    best = {"compile": float("inf"), "load": float("inf")}
+    for _ in range(3):
+        for what in ("compile", "load"):
+            h = ctypes.c_void_p()
+            t = time.perf_counter()
+            if what == "compile":
+                st = lib.ws_patterns_compile(w.ctx.handle, arr, len(w.ps.patterns), 0, ctypes.byref(h))
+            else:
+                st = lib.ws_index_load(w.ctx.handle, data, len(data), ctypes.byref(h))
+            dt = time.perf_counter() - t
+            N.check(st)
+            lib.ws_patterns_free(h)
+            best[what] = min(best[what], dt * 1e3)
+    return {"index_bytes": len(data), "compile_ms": round(best["compile"], 2),
+            "index_load_ms": round(best["load"], 2), "load_speedup": round(best["compile"] / best["load"], 1)}
+
+
 def copy_calibration(args) -> int:
     """1 GiB device-to-device copies, for calibrating FETCH_SIZE / WRITE_SIZE."""
     from warpstate_amd.device import DeviceBuffer, synchronize
@@ -295,6 +323,8 @@ def main() -> int:
         "graphs_replayed": w.ctx.graph_count,
         "step_breakdown_ms": breakdown,
     }
+    line["compiled_index"] = index_timing(w)
+    log(f"compiled index {line['compiled_index']}")
     run_router = args.router if args.router is not None else (w.name == "kmer")
     if run_router:
         from warpstate_amd.router import run_study

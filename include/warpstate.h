@@ -263,6 +263,59 @@ ws_status ws_stream_scanner_finish(ws_stream_scanner* scanner, ws_match* out, ui
 /* Bytes fed so far (StreamScanner::processed_bytes). */
 uint64_t ws_stream_scanner_processed(const ws_stream_scanner* scanner);
 
+/* ---- compiled pattern index: CompiledPatternIndex (core/src/compiled_index/).
+ * The WPSIDX01 little-endian format. Versions 1-3 are the reference's files:
+ * they are read, checked as CompiledPatternIndex::load checks them, and rebuilt
+ * from the literals and regex sources they store, as to_pattern_set does
+ * (core/src/compiled_index/mod.rs:287-372); their regex-automata DFA sections
+ * are checked for structure only. Version 3 is written for sets without
+ * regexes (a file the reference loads). Version 4 is this library's default:
+ * the version-3 sections, then each regex's anchored DFA and the factor
+ * automaton, so that loading skips the subset construction and the automaton
+ * build. Versions 3 and 4 end in a CRC-32 (IEEE) of the rest. The builders
+ * write into `out` (host memory): with out == NULL they set *len to the size
+ * and return WS_OK; with cap < the size they set *len and return
+ * WS_ERR_INVALID_ARGUMENT. Malformed files are WS_ERR_PATTERN_COMPILATION_FAILED. */
+#define WS_INDEX_VERSION 4u
+enum { WS_INDEX_REFERENCE_V3 = 1u }; /* index_flags: write version 3 */
+
+typedef struct ws_index_info {
+  uint32_t version;
+  uint32_t case_insensitive;
+  uint32_t pattern_count;
+  uint32_t literal_count;
+  uint32_t regex_count;
+  uint32_t hash_window_len;
+  uint32_t prebuilt;          /* 1 for version 4: tables load without compilation */
+  uint32_t reserved;
+} ws_index_info;
+
+/* CompiledPatternIndex::build (core/src/compiled_index/builder.rs:8-61) from
+ * pattern sources, on the host (flags as for ws_patterns_compile). */
+ws_status ws_index_build(const ws_pattern* patterns, size_t count, uint32_t flags,
+                         uint32_t index_flags, uint8_t* out, uint64_t cap, uint64_t* len);
+/* The same for a compiled set. */
+ws_status ws_patterns_serialize(const ws_patterns* patterns, uint32_t index_flags, uint8_t* out,
+                                uint64_t cap, uint64_t* len);
+/* CompiledPatternIndex::load (core/src/compiled_index/mod.rs:58-61, 96-271) and
+ * to_pattern_set, on the host only: checks `data`, reports its header and, when
+ * `info` is not NULL, the facts of the set it holds (as ws_patterns_analyze). */
+ws_status ws_index_inspect(const uint8_t* data, uint64_t len, ws_index_info* out,
+                           ws_patterns_info* info);
+/* load + to_pattern_set + device upload: a compiled set equal to compiling the
+ * stored patterns (data is host memory). */
+ws_status ws_index_load(ws_ctx* ctx, const uint8_t* data, uint64_t len, ws_patterns** out);
+/* CompiledPatternIndex::load_from_file (core/src/compiled_index/mod.rs:76-94):
+ * files above 512 MiB are WS_ERR_INPUT_TOO_LARGE. */
+ws_status ws_index_load_file(ws_ctx* ctx, const char* path, ws_patterns** out);
+/* CompiledPatternIndex::save_to_file (core/src/compiled_index/mod.rs:65-73). */
+ws_status ws_index_save_file(const uint8_t* data, uint64_t len, const char* path);
+/* CompiledPatternIndex::names (core/src/compiled_index/mod.rs:274-276): the name of
+ * pattern `id` into buf (not NUL-terminated); *len = UINT64_MAX when it has none.
+ * With cap < the length, *len holds it and the status is WS_ERR_INVALID_ARGUMENT. */
+ws_status ws_patterns_name(const ws_patterns* patterns, uint32_t id, char* buf, uint64_t cap,
+                           uint64_t* len);
+
 /* ---- kernel routing: AutoMatcher (core/src/router.rs:205-291, core/src/config.rs:4-35).
  * The reference routes on one input-size threshold. Here an exhaustive autotuner
  * times every kernel configuration, and a small learned cost model (FP32 MLP,

@@ -17,6 +17,7 @@ from ._native import (  # noqa: F401
 )
 from .api import (  # noqa: F401
     AutoMatcherConfig,
+    CompiledPatternIndex,
     Context,
     GpuMatcher,
     HotSwapPatternSet,

@@ -110,6 +110,26 @@ class WsTimings(Structure):
     ]
 
 
+INDEX_VERSION = 4
+INDEX_REFERENCE_V3 = 1  # ws_index_build / ws_patterns_serialize flag: write version 3
+
+
+class WsIndexInfo(Structure):
+    _fields_ = [
+        (name, c_uint32)
+        for name in (
+            "version",
+            "case_insensitive",
+            "pattern_count",
+            "literal_count",
+            "regex_count",
+            "hash_window_len",
+            "prebuilt",
+            "reserved",
+        )
+    ]
+
+
 # Every symbol include/warpstate.h declares, with its ctypes signature.
 SIGNATURES = {
     "ws_abi_version": (c_uint32, []),
@@ -172,6 +192,14 @@ SIGNATURES = {
                                        POINTER(c_uint64)]),
     "ws_stream_scanner_finish": (c_int, [c_void_p, c_void_p, c_uint64, POINTER(c_uint64)]),
     "ws_stream_scanner_processed": (c_uint64, [c_void_p]),
+    "ws_index_build": (c_int, [POINTER(WsPattern), c_size_t, c_uint32, c_uint32, c_void_p, c_uint64,
+                               POINTER(c_uint64)]),
+    "ws_patterns_serialize": (c_int, [c_void_p, c_uint32, c_void_p, c_uint64, POINTER(c_uint64)]),
+    "ws_index_inspect": (c_int, [c_void_p, c_uint64, POINTER(WsIndexInfo), POINTER(WsPatternsInfo)]),
+    "ws_index_load": (c_int, [c_void_p, c_void_p, c_uint64, POINTER(c_void_p)]),
+    "ws_index_load_file": (c_int, [c_void_p, c_char_p, POINTER(c_void_p)]),
+    "ws_index_save_file": (c_int, [c_void_p, c_uint64, c_char_p]),
+    "ws_patterns_name": (c_int, [c_void_p, c_uint32, c_void_p, c_uint64, POINTER(c_uint64)]),
 }
 
 _lib = None

@@ -188,12 +188,23 @@ class PatternSet:
         self.case_insensitive = case_insensitive
         self._handles = {}
         self._lock = threading.Lock()
+        self._index: Optional[bytes] = None  # set for a set loaded from a compiled index
+        self._index_info: Optional[dict] = None
+
+    @classmethod
+    def _from_index(cls, data: bytes, header: dict, info: dict) -> "PatternSet":
+        ps = cls([], bool(header["case_insensitive"]))
+        ps._index = data
+        ps._index_info = info
+        return ps
 
     @staticmethod
     def builder() -> PatternSetBuilder:
         return PatternSetBuilder()
 
     def __len__(self) -> int:
+        if self._index is not None:
+            return int(self._index_info["pattern_count"])
         return len(self.patterns)
 
     def _compiled(self, ctx: Context):
@@ -204,10 +215,13 @@ class PatternSet:
             h = self._handles.get(key)
             if h is None:
                 lib = N.load()
-                arr, _bufs = self._c_patterns()
                 h = ctypes.c_void_p()
-                flags = 1 if self.case_insensitive else 0
-                N.check(lib.ws_patterns_compile(ctx.handle, arr, len(self.patterns), flags, ctypes.byref(h)))
+                if self._index is not None:
+                    N.check(lib.ws_index_load(ctx.handle, self._index, len(self._index), ctypes.byref(h)))
+                else:
+                    arr, _bufs = self._c_patterns()
+                    flags = 1 if self.case_insensitive else 0
+                    N.check(lib.ws_patterns_compile(ctx.handle, arr, len(self.patterns), flags, ctypes.byref(h)))
                 self._handles[key] = (h, ctx)
                 return h
             return h[0]
@@ -224,6 +238,8 @@ class PatternSet:
 
     def analyze(self) -> dict:
         """Host-only compile (ws_patterns_analyze): validation + set facts."""
+        if self._index is not None:
+            return dict(self._index_info)
         arr, _bufs = self._c_patterns()
         i = N.WsPatternsInfo()
         N.check(N.load().ws_patterns_analyze(arr, len(self.patterns), 1 if self.case_insensitive else 0,
@@ -238,6 +254,22 @@ class PatternSet:
 
     def max_pattern_len(self) -> int:
         return int(self.analyze()["max_pattern_len"])
+
+    def names(self, ctx: Optional[Context] = None) -> List[Optional[str]]:
+        """Per pattern id, its name or None (from the compiled set)."""
+        lib = N.load()
+        h = self._compiled(ctx or default_context())
+        out: List[Optional[str]] = []
+        for i in range(len(self)):
+            n = ctypes.c_uint64(0)
+            N.check(lib.ws_patterns_name(h, i, None, 0, ctypes.byref(n)))
+            if n.value == 2**64 - 1:
+                out.append(None)
+                continue
+            buf = ctypes.create_string_buffer(max(1, n.value))
+            N.check(lib.ws_patterns_name(h, i, buf, n.value, ctypes.byref(n)))
+            out.append(buf.raw[: n.value].decode())
+        return out
 
     def scan(self, data) -> List[Match]:
         return GpuMatcher(self).scan(data, mode="standard")
@@ -255,6 +287,107 @@ class PatternSet:
                 lib.ws_patterns_free(h)
         except Exception:
             pass
+
+
+class CompiledPatternIndex:
+    """CompiledPatternIndex (core/src/compiled_index/mod.rs:22-845): a
+    serialized pattern set (WPSIDX01). `build` writes version 4 by default
+    (the reference's sections plus the compiled DFA and automaton tables, so
+    loading skips compilation) or version 3 (`version=3`, sets without
+    regexes: a file the reference crate loads). `load` reads versions 1-4;
+    files of versions 1-3 are rebuilt from their stored patterns as
+    to_pattern_set does. `scan` runs to_pattern_set().scan on the GPU: the
+    reference's index scanner returns PatternSet::scan's results (its
+    round-trip tests, core/src/compiled_index/tests.rs:5-128)."""
+
+    MAX_FILE_BYTES = 512 * 1024 * 1024  # mod.rs:19
+
+    def __init__(self, data: bytes, header: dict, info: dict):
+        self._data = data
+        self._header = header
+        self._info = info
+        self._set: Optional[PatternSet] = None
+
+    @staticmethod
+    def build(patterns: "PatternSet", version: int = N.INDEX_VERSION) -> bytes:
+        """CompiledPatternIndex::build (builder.rs:8-61); host only."""
+        flags = N.INDEX_REFERENCE_V3 if version == 3 else 0
+        if version not in (3, N.INDEX_VERSION):
+            raise ValueError(f"index version {version} cannot be written (3 or {N.INDEX_VERSION})")
+        lib = N.load()
+        n = ctypes.c_uint64(0)
+        if patterns._index is not None:
+            h = patterns._compiled(default_context())
+            N.check(lib.ws_patterns_serialize(h, flags, None, 0, ctypes.byref(n)))
+            out = ctypes.create_string_buffer(n.value)
+            N.check(lib.ws_patterns_serialize(h, flags, out, n.value, ctypes.byref(n)))
+            return out.raw[: n.value]
+        arr, _bufs = patterns._c_patterns()
+        ci = 1 if patterns.case_insensitive else 0
+        # One compilation when the first guess at the size suffices.
+        cap = 1 << 16
+        for p in patterns.patterns:
+            cap += 64 + 32 * len(p[1])
+        while True:
+            out = ctypes.create_string_buffer(cap)
+            st = lib.ws_index_build(arr, len(patterns.patterns), ci, flags, out, cap, ctypes.byref(n))
+            if st == 13 and n.value > cap:
+                cap = int(n.value)
+                continue
+            N.check(st)
+            return out.raw[: n.value]
+
+    @classmethod
+    def load(cls, data) -> "CompiledPatternIndex":
+        """CompiledPatternIndex::load (mod.rs:58-61): checks the whole file on the host."""
+        data = bytes(data)
+        hdr = N.WsIndexInfo()
+        info = N.WsPatternsInfo()
+        N.check(N.load().ws_index_inspect(data if data else None, len(data), ctypes.byref(hdr), ctypes.byref(info)))
+        return cls(data, {k: getattr(hdr, k) for k, _ in N.WsIndexInfo._fields_},
+                   {k: getattr(info, k) for k, _ in N.WsPatternsInfo._fields_})
+
+    @classmethod
+    def load_from_file(cls, path) -> "CompiledPatternIndex":
+        """CompiledPatternIndex::load_from_file (mod.rs:76-94)."""
+        import os
+
+        size = os.stat(path).st_size
+        if size > cls.MAX_FILE_BYTES:
+            raise N.WarpstateError(10, f"compiled index is {size} bytes; the limit is {cls.MAX_FILE_BYTES}")
+        with open(path, "rb") as f:
+            return cls.load(f.read())
+
+    def save_to_file(self, path) -> None:
+        """CompiledPatternIndex::save_to_file (mod.rs:65-73)."""
+        N.check(N.load().ws_index_save_file(self._data, len(self._data), str(path).encode()))
+
+    def to_bytes(self) -> bytes:
+        return self._data
+
+    @property
+    def version(self) -> int:
+        return int(self._header["version"])
+
+    @property
+    def prebuilt(self) -> bool:
+        return bool(self._header["prebuilt"])
+
+    def literal_count(self) -> int:
+        return int(self._header["literal_count"])
+
+    def to_pattern_set(self) -> "PatternSet":
+        """CompiledPatternIndex::to_pattern_set (mod.rs:287-372); compiled on
+        the device from the file's tables when the file carries them."""
+        if self._set is None:
+            self._set = PatternSet._from_index(self._data, self._header, self._info)
+        return self._set
+
+    def names(self) -> List[Optional[str]]:
+        return self.to_pattern_set().names()
+
+    def scan(self, data) -> List[Match]:
+        return self.to_pattern_set().scan(data)
 
 
 def _mode_code(mode: str) -> int:

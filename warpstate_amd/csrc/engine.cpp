@@ -7,7 +7,9 @@
 #include <atomic>
 #include <map>
 #include <tuple>
+#include <cerrno>
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -16,6 +18,7 @@
 #include <vector>
 
 #include "../../include/warpstate.h"
+#include "index.hpp"
 #include "kernels.hpp"
 #include "patterns.hpp"
 
@@ -1372,6 +1375,73 @@ ws_status pack_out(ws_ctx* ctx, const Result& r, uint32_t start_off, ws_match* o
   return WS_OK;
 }
 
+// Pattern sources from the C ABI, names included.
+ws_status read_sources(const ws_pattern* pats, size_t count, std::vector<PatternSrc>* src) {
+  if (count == 0) return fail(WS_ERR_EMPTY_PATTERN_SET, "pattern set is empty. Fix: add at least one pattern");
+  if (!pats) return fail(WS_ERR_INVALID_ARGUMENT, "patterns is null");
+  src->assign(count, PatternSrc{});
+  for (size_t i = 0; i < count; ++i) {
+    PatternSrc& d = (*src)[i];
+    d.kind = pats[i].kind;
+    if (pats[i].len && !pats[i].bytes) return fail(WS_ERR_INVALID_ARGUMENT, "pattern bytes are null");
+    d.bytes.assign(reinterpret_cast<const char*>(pats[i].bytes), pats[i].len);
+    if (pats[i].name) {
+      d.name = pats[i].name;
+      d.has_name = true;
+    }
+  }
+  return WS_OK;
+}
+
+// Runs host-side compilation or index work, mapping its exceptions to statuses.
+template <class F>
+ws_status guarded_compile(F&& f) {
+  try {
+    f();
+  } catch (const CompileError& e) {
+    return fail((ws_status)e.status, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(WS_ERR_PATTERN_SET_TOO_LARGE, "out of host memory compiling the pattern set");
+  }
+  return WS_OK;
+}
+
+// Uploads a host-compiled set to the context's device.
+ws_status make_patterns(ws_ctx* ctx, PatternsHost&& h, ws_patterns** out) {
+  ws_patterns* p = new ws_patterns();
+  p->serial = g_pattern_serial.fetch_add(1);
+  p->h = std::move(h);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  p->dev = ctx->dev;
+  if (hipSetDevice(ctx->dev) != hipSuccess) {
+    delete p;
+    return fail(WS_ERR_GPU_DEVICE, "hipSetDevice failed");
+  }
+  ws_status s = upload_patterns(p);
+  if (s != WS_OK) {
+    p->blob.release();
+    delete p;
+    return s;
+  }
+  *out = p;
+  return WS_OK;
+}
+
+// Copies a serialized index to the caller's buffer (size query when out is null).
+ws_status emit_index(const std::vector<uint8_t>& b, uint8_t* out, uint64_t cap, uint64_t* len) {
+  *len = b.size();
+  if (!out) return WS_OK;
+  if (cap < b.size())
+    return fail(WS_ERR_INVALID_ARGUMENT, "index buffer holds " + std::to_string(cap) + " bytes; the index needs " +
+                                             std::to_string(b.size()));
+  std::memcpy(out, b.data(), b.size());
+  return WS_OK;
+}
+
+uint32_t index_version(uint32_t index_flags) {
+  return (index_flags & WS_INDEX_REFERENCE_V3) ? kIndexVersionRef : kIndexVersion;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1481,39 +1551,13 @@ ws_status ws_patterns_compile(ws_ctx* ctx, const ws_pattern* pats, size_t count,
                               ws_patterns** out) {
   if (!ctx || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
-  if (count == 0) return fail(WS_ERR_EMPTY_PATTERN_SET, "pattern set is empty. Fix: add at least one pattern");
-  if (!pats) return fail(WS_ERR_INVALID_ARGUMENT, "patterns is null");
-  std::vector<PatternSrc> src(count);
-  for (size_t i = 0; i < count; ++i) {
-    src[i].kind = pats[i].kind;
-    if (pats[i].len && !pats[i].bytes) return fail(WS_ERR_INVALID_ARGUMENT, "pattern bytes are null");
-    src[i].bytes.assign(reinterpret_cast<const char*>(pats[i].bytes), pats[i].len);
-  }
-  ws_patterns* p = new ws_patterns();
-  p->serial = g_pattern_serial.fetch_add(1);
-  try {
-    p->h = compile_patterns(src, (flags & WS_FLAG_CASE_INSENSITIVE) != 0);
-  } catch (const CompileError& e) {
-    delete p;
-    return fail((ws_status)e.status, e.msg);
-  } catch (const std::bad_alloc&) {
-    delete p;
-    return fail(WS_ERR_PATTERN_SET_TOO_LARGE, "out of host memory compiling the pattern set");
-  }
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  p->dev = ctx->dev;
-  if (hipSetDevice(ctx->dev) != hipSuccess) {
-    delete p;
-    return fail(WS_ERR_GPU_DEVICE, "hipSetDevice failed");
-  }
-  ws_status s = upload_patterns(p);
-  if (s != WS_OK) {
-    p->blob.release();
-    delete p;
-    return s;
-  }
-  *out = p;
-  return WS_OK;
+  std::vector<PatternSrc> src;
+  ws_status s = read_sources(pats, count, &src);
+  if (s != WS_OK) return s;
+  PatternsHost h;
+  s = guarded_compile([&] { h = compile_patterns(src, (flags & WS_FLAG_CASE_INSENSITIVE) != 0); });
+  if (s != WS_OK) return s;
+  return make_patterns(ctx, std::move(h), out);
 }
 
 void ws_patterns_retain(ws_patterns* p) {
@@ -1548,37 +1592,151 @@ static void fill_info(const PatternsHost& h, uint32_t rx_states, uint32_t varian
   *out = i;
 }
 
+static uint32_t rx_state_total(const PatternsHost& h) {
+  uint32_t rs = 0;
+  for (auto& d : h.rx) rs += d.nstates;
+  return rs;
+}
+
+static uint32_t host_variants(const PatternsHost& h) {
+  uint32_t v = 1u << WS_VARIANT_HASH;
+  if (h.ac_ok) v |= 1u << WS_VARIANT_DFA_GLOBAL;
+  if (h.ac_ok && h.ac_states <= 65535 && (size_t)h.ac_states * h.ac_classes * 2 <= kAcLdsBudget)
+    v |= 1u << WS_VARIANT_DFA_LDS;
+  return v;
+}
+
 ws_status ws_patterns_analyze(const ws_pattern* pats, size_t count, uint32_t flags,
                               ws_patterns_info* out) {
   if (!out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
-  if (count == 0) return fail(WS_ERR_EMPTY_PATTERN_SET, "pattern set is empty. Fix: add at least one pattern");
-  if (!pats) return fail(WS_ERR_INVALID_ARGUMENT, "patterns is null");
-  std::vector<PatternSrc> src(count);
-  for (size_t i = 0; i < count; ++i) {
-    if (pats[i].len && !pats[i].bytes) return fail(WS_ERR_INVALID_ARGUMENT, "pattern bytes are null");
-    src[i].kind = pats[i].kind;
-    src[i].bytes.assign(reinterpret_cast<const char*>(pats[i].bytes), pats[i].len);
-  }
-  try {
+  std::vector<PatternSrc> src;
+  ws_status s = read_sources(pats, count, &src);
+  if (s != WS_OK) return s;
+  return guarded_compile([&] {
     PatternsHost h = compile_patterns(src, (flags & WS_FLAG_CASE_INSENSITIVE) != 0);
-    uint32_t rs = 0;
-    for (auto& d : h.rx) rs += d.nstates;
-    uint32_t v = 1u << WS_VARIANT_HASH;
-    if (h.ac_ok) v |= 1u << WS_VARIANT_DFA_GLOBAL;
-    if (h.ac_ok && h.ac_states <= 65535 && (size_t)h.ac_states * h.ac_classes * 2 <= kAcLdsBudget)
-      v |= 1u << WS_VARIANT_DFA_LDS;
-    fill_info(h, rs, v, out);
-  } catch (const CompileError& e) {
-    return fail((ws_status)e.status, e.msg);
-  } catch (const std::bad_alloc&) {
-    return fail(WS_ERR_PATTERN_SET_TOO_LARGE, "out of host memory compiling the pattern set");
-  }
-  return WS_OK;
+    fill_info(h, rx_state_total(h), host_variants(h), out);
+  });
 }
 
 ws_status ws_patterns_info_get(const ws_patterns* p, ws_patterns_info* out) {
   if (!p || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
   fill_info(p->h, p->rx_states_total, supported_variants(p), out);
+  return WS_OK;
+}
+
+ws_status ws_index_build(const ws_pattern* pats, size_t count, uint32_t flags, uint32_t index_flags,
+                         uint8_t* out, uint64_t cap, uint64_t* len) {
+  if (!len) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  std::vector<PatternSrc> src;
+  ws_status s = read_sources(pats, count, &src);
+  if (s != WS_OK) return s;
+  std::vector<uint8_t> b;
+  s = guarded_compile([&] {
+    PatternsHost h = compile_patterns(src, (flags & WS_FLAG_CASE_INSENSITIVE) != 0);
+    b = index_serialize(h, index_version(index_flags));
+  });
+  if (s != WS_OK) return s;
+  return emit_index(b, out, cap, len);
+}
+
+ws_status ws_patterns_serialize(const ws_patterns* p, uint32_t index_flags, uint8_t* out, uint64_t cap,
+                                uint64_t* len) {
+  if (!p || !len) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  std::vector<uint8_t> b;
+  ws_status s = guarded_compile([&] { b = index_serialize(p->h, index_version(index_flags)); });
+  if (s != WS_OK) return s;
+  return emit_index(b, out, cap, len);
+}
+
+ws_status ws_index_inspect(const uint8_t* data, uint64_t len, ws_index_info* out, ws_patterns_info* info) {
+  if (!out || (len && !data)) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  IndexHeader hd;
+  ws_status s = guarded_compile([&] {
+    IndexContents ic = index_parse(data, (size_t)len);
+    hd = ic.hdr;
+    // The rebuild must succeed too (to_pattern_set).
+    PatternsHost h = compile_patterns(ic.src, hd.ci, hd.prebuilt ? &ic.pre : nullptr);
+    if (info) fill_info(h, rx_state_total(h), host_variants(h), info);
+  });
+  if (s != WS_OK) return s;
+  ws_index_info i{};
+  i.version = hd.version;
+  i.case_insensitive = hd.ci ? 1 : 0;
+  i.pattern_count = hd.pattern_count;
+  i.literal_count = hd.literal_count;
+  i.regex_count = hd.regex_count;
+  i.hash_window_len = hd.hash_window_len;
+  i.prebuilt = hd.prebuilt ? 1 : 0;
+  *out = i;
+  return WS_OK;
+}
+
+ws_status ws_index_load(ws_ctx* ctx, const uint8_t* data, uint64_t len, ws_patterns** out) {
+  if (!ctx || !out || (len && !data)) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  PatternsHost h;
+  ws_status s = guarded_compile([&] {
+    IndexContents ic = index_parse(data, (size_t)len);
+    h = compile_patterns(ic.src, ic.hdr.ci, ic.hdr.prebuilt ? &ic.pre : nullptr);
+  });
+  if (s != WS_OK) return s;
+  return make_patterns(ctx, std::move(h), out);
+}
+
+ws_status ws_index_load_file(ws_ctx* ctx, const char* path, ws_patterns** out) {
+  if (!ctx || !path || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f)
+    return fail(WS_ERR_PATTERN_COMPILATION_FAILED, std::string("failed to read compiled index from ") + path +
+                                                        ": " + std::strerror(errno) +
+                                                        ". Fix: verify the file exists and is readable");
+  std::fseek(f, 0, SEEK_END);
+  const long size = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (size < 0) {
+    std::fclose(f);
+    return fail(WS_ERR_PATTERN_COMPILATION_FAILED, std::string("failed to stat compiled index ") + path);
+  }
+  if ((uint64_t)size > kIndexMaxFileBytes) {
+    std::fclose(f);
+    return fail(WS_ERR_INPUT_TOO_LARGE, "compiled index is " + std::to_string(size) + " bytes; the limit is " +
+                                            std::to_string(kIndexMaxFileBytes));
+  }
+  std::vector<uint8_t> b((size_t)size);
+  const size_t got = size ? std::fread(b.data(), 1, b.size(), f) : 0;
+  std::fclose(f);
+  if (got != b.size())
+    return fail(WS_ERR_PATTERN_COMPILATION_FAILED, std::string("short read of compiled index ") + path);
+  return ws_index_load(ctx, b.data(), b.size(), out);
+}
+
+ws_status ws_index_save_file(const uint8_t* data, uint64_t len, const char* path) {
+  if (!path || (len && !data)) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  FILE* f = std::fopen(path, "wb");
+  if (!f)
+    return fail(WS_ERR_PATTERN_COMPILATION_FAILED, std::string("failed to write compiled index to ") + path +
+                                                        ": " + std::strerror(errno) +
+                                                        ". Fix: verify the parent directory exists and is writable");
+  const size_t put = len ? std::fwrite(data, 1, (size_t)len, f) : 0;
+  const int rc = std::fclose(f);
+  if (put != len || rc != 0)
+    return fail(WS_ERR_PATTERN_COMPILATION_FAILED, std::string("failed to write compiled index to ") + path);
+  return WS_OK;
+}
+
+ws_status ws_patterns_name(const ws_patterns* p, uint32_t id, char* buf, uint64_t cap, uint64_t* len) {
+  if (!p || !len) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (id >= p->h.npat) return fail(WS_ERR_INVALID_ARGUMENT, "pattern id out of range");
+  if (!p->h.has_name[id]) {
+    *len = UINT64_MAX;
+    return WS_OK;
+  }
+  const std::string& n = p->h.names[id];
+  *len = n.size();
+  if (!buf) return WS_OK;
+  if (cap < n.size()) return fail(WS_ERR_INVALID_ARGUMENT, "name buffer is too small");
+  std::memcpy(buf, n.data(), n.size());
   return WS_OK;
 }
 

@@ -191,6 +191,29 @@ void build_ac(PatternsHost& ps) {
   ps.ac_ok = newid[0] == 0;  // the root must be state 0 and carry no output
 }
 
+// Takes a stored factor automaton. The reader has checked its shape (table
+// entries are states, classes cover the byte map, output ranges are ordered);
+// here its outputs are checked against this set's factors.
+void install_ac(PatternsHost& ps, AcTables& t) {
+  ps.ac_ok = false;
+  ps.ac_maxlen = 0;
+  for (const Factor& f : ps.fac) ps.ac_maxlen = std::max(ps.ac_maxlen, f.ac_len);
+  if (!t.ok) return;
+  const uint32_t F = (uint32_t)ps.fac.size();
+  for (uint32_t f : t.out_list)
+    if (f >= F)
+      throw CompileError{WS_ERR_PATTERN_COMPILATION_FAILED,
+                         "compiled index factor automaton names a factor out of bounds"};
+  ps.ac_states = t.states;
+  ps.ac_classes = t.classes;
+  ps.ac_first_out = t.first_out;
+  ps.ac_cmap = t.cmap;
+  ps.ac_table = std::move(t.table);
+  ps.ac_out_start = std::move(t.out_start);
+  ps.ac_out_list = std::move(t.out_list);
+  ps.ac_ok = true;
+}
+
 }  // namespace
 
 uint32_t dual_hash(const uint8_t* p, uint32_t K, uint32_t BB, bool ci) {
@@ -210,13 +233,21 @@ uint32_t rolling_hash(const uint8_t* p, uint32_t K, uint32_t B, bool ci) {
   return h;
 }
 
-PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_insensitive) {
+PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_insensitive,
+                              Prebuilt* pre) {
   if (src.empty()) throw CompileError{WS_ERR_EMPTY_PATTERN_SET, "pattern set is empty"};
   if (src.size() > 0x7FFFFFFFull)
     throw CompileError{WS_ERR_PATTERN_SET_TOO_LARGE, "too many patterns"};
   PatternsHost ps;
   ps.npat = (uint32_t)src.size();
   ps.ci = case_insensitive;
+  ps.names.resize(ps.npat);
+  ps.has_name.assign(ps.npat, 0);
+  for (uint32_t i = 0; i < ps.npat; ++i)
+    if (src[i].has_name) {
+      ps.names[i] = src[i].name;
+      ps.has_name[i] = 1;
+    }
   bool unbounded = false;
   uint64_t max_len = 0;
   for (uint32_t i = 0; i < ps.npat; ++i) {
@@ -258,13 +289,22 @@ PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_inse
                              " can match the empty string; not supported on the GPU path"};
     try {
       Nfa nfa = build_nfa(ast, kNfaMaxStates);
-      AnchoredDfa dfa = build_anchored_dfa(nfa, kDfaMaxStates);
+      const uint32_t r = (uint32_t)ps.rx.size();
+      AnchoredDfa dfa;
+      if (pre) {
+        if (r >= pre->rx.size())
+          throw CompileError{WS_ERR_PATTERN_COMPILATION_FAILED,
+                             "compiled index has fewer regex DFA sections than regex patterns"};
+        dfa = std::move(pre->rx[r]);
+      } else {
+        dfa = build_anchored_dfa(nfa, kDfaMaxStates);
+      }
       std::vector<std::string> fs = extract_factors(nfa, kFactorMaxLen, kFactorFanout, kFactorMaxCount);
       for (auto& s : fs)
         if (s.empty())
           throw ParseError{"no selective prefix (first byte matches too many values)"};
-      uint32_t r = (uint32_t)ps.rx.size();
       ps.rx.push_back(std::move(dfa));
+      ps.rx_src.push_back(p.bytes);
       ps.rx_pid.push_back(i);
       ps.rx_maxlen.push_back(mx);
       if (mx < 0) unbounded = true;
@@ -302,7 +342,14 @@ PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_inse
   else ps.strategy = STRAT_LEFTMOST_LONGEST;
 
   build_hash(ps);
-  build_ac(ps);
+  if (!pre) {
+    build_ac(ps);
+    return ps;
+  }
+  if (pre->rx.size() != ps.rx.size())
+    throw CompileError{WS_ERR_PATTERN_COMPILATION_FAILED,
+                       "compiled index has more regex DFA sections than regex patterns"};
+  install_ac(ps, pre->ac);
   return ps;
 }
 

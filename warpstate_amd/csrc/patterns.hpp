@@ -33,9 +33,13 @@ struct Factor {
 struct PatternsHost {
   uint32_t npat = 0;
   bool ci = false;
+  // Per pattern id: the optional name (PatternSetBuilder::named_*).
+  std::vector<std::string> names;
+  std::vector<uint8_t> has_name;
   std::vector<std::string> lits;
   std::vector<uint32_t> lit_pid;
   std::vector<uint32_t> rx_pid;
+  std::vector<std::string> rx_src;  // regex source text, per regex
   std::vector<AnchoredDfa> rx;
   std::vector<long> rx_maxlen;  // -1 = unbounded
   std::vector<Factor> fac;
@@ -67,10 +71,31 @@ struct CompileError {
 struct PatternSrc {
   uint32_t kind;
   std::string bytes;
+  std::string name;
+  bool has_name = false;
 };
 
-// Throws CompileError.
-PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_insensitive);
+// The factor automaton as stored in a compiled index.
+struct AcTables {
+  bool ok = false;
+  uint32_t states = 0, classes = 0, first_out = 0;
+  std::array<uint8_t, 256> cmap{};
+  std::vector<uint32_t> table, out_start, out_list;
+};
+
+// Tables read from a compiled index in place of the two expensive steps of
+// compilation: the subset construction of each regex DFA (one per regex, in
+// pattern order) and the factor automaton. Their structure has been checked
+// by the reader; compile_patterns checks them against the factor set.
+struct Prebuilt {
+  std::vector<AnchoredDfa> rx;
+  AcTables ac;
+};
+
+// Throws CompileError. With `pre`, its tables are moved into the result
+// instead of being built.
+PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_insensitive,
+                              Prebuilt* pre = nullptr);
 
 // Rolling hash of the first K bytes (mod 2^24), shared by host and kernel.
 uint32_t rolling_hash(const uint8_t* p, uint32_t K, uint32_t B, bool ci);
