@@ -55,11 +55,10 @@ hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint3
 hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t sbits,
                               uint32_t pbits, const uint64_t* ids, void* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
-hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
-                                     uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total,
-                                     uint32_t* lit_snap, uint32_t cap, uint32_t sbits, uint32_t pbits,
-                                     uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift, Rec* cout,
-                                     uint32_t ccap, hipStream_t st);
+hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* lit_total,
+                                     uint32_t* cand_total, uint32_t* lit_snap, uint32_t cap, uint32_t sbits,
+                                     uint32_t pbits, uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift,
+                                     Rec* cout, uint32_t ccap, hipStream_t st);
 hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
                              uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift, uint32_t nb,
                              uint32_t* cnt, uint32_t* cursor, uint32_t* flag, uint32_t pbits, uint32_t start_off,
@@ -72,11 +71,12 @@ hipError_t launch_pack_tagged_dense(const uint64_t* keys, const uint32_t* vals, 
 hipError_t launch_expand_units(const uint2* runs, uint32_t nruns, const uint2* items, uint32_t unit, uint4* units,
                                uint32_t nunits, hipStream_t st);
 hipError_t launch_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counters, uint32_t ncounters,
-                            unsigned long long* stamps, uint32_t n, hipStream_t st);
+                            unsigned long long* stamps, uint32_t n, const CallItems& ci, uint2* items,
+                            hipStream_t st);
 hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords);
 hipError_t prepare_ac(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
-                            unsigned long long* total, hipStream_t st);
+                            unsigned long long* total, uint32_t* publish, uint32_t pwords, hipStream_t st);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t st);
 hipError_t launch_slice_compact2(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
                                  uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total, uint32_t* lit_snap,
@@ -164,10 +164,12 @@ struct ws_ctx {
   hipEvent_t tev[2] = {};      // around a whole device-sized call (outside any graph)
   uint32_t wall_khz = 0;       // device wall-clock rate (kernel stamps)
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, dstamp, k2, v2, bcnt2, bases;
+      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, k2, v2, bcnt2, bases;
   std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
   uint8_t* hstage = nullptr;    // pinned staging read by captured copies
   size_t hstage_cap = 0;
+  uint32_t* hstate = nullptr;   // pinned, coherent: the device state block as the last kernel left it
+  size_t hstate_cap = 0;
   std::map<GraphKey, GraphEntry> graphs;
   uint32_t* hcount = nullptr;  // pinned host mirror of the counters
   uint32_t last_hits = 0;      // filter hits of the last hash-variant scan
@@ -938,6 +940,18 @@ ws_status plan_single(ws_ctx* ctx, const uint8_t* data, uint64_t len, uint32_t d
 // run's match count to the output offset.
 // ---------------------------------------------------------------------------
 // Grows the pinned staging area (reallocation makes captured graphs stale).
+hipError_t ensure_hstate(ws_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->hstate_cap) return hipSuccess;
+  if (ctx->hstate) (void)hipHostFree(ctx->hstate);
+  ctx->hstate = nullptr;
+  ctx->hstate_cap = 0;
+  g_buf_epoch.fetch_add(1);
+  const size_t want = std::max<size_t>(bytes, 4096);
+  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hstate), want, hipHostMallocCoherent);
+  if (e == hipSuccess) ctx->hstate_cap = want;
+  return e;
+}
+
 hipError_t ensure_stage(ws_ctx* ctx, size_t bytes) {
   if (bytes <= ctx->hstage_cap) return hipSuccess;
   if (ctx->hstage) (void)hipHostFree(ctx->hstage);
@@ -968,7 +982,10 @@ struct RunSetup {
   uint32_t lit_cap = 0, cand_cap = 0, raw_cap = 0, rec_cap = 0, max_matches = 0;
   uint64_t lslice_total = 0, cslice_total = 0;
   uint32_t sbits = 0, pbits = 0, rbits = 0, obits = 0, cbits = 0, ob = 0, cb = 0;
-  size_t items_off = 0;  // this run's items in the pinned staging area (uint2 units)
+  size_t items_off = 0;  // this run's index among the call's runs (its segment, its stamps)
+  unsigned long long* stamps = nullptr;  // the call's kernel stamps (profiling), two per run
+  uint32_t* publish = nullptr;           // the call's last run: pinned host copy of the state block
+  uint32_t publish_words = 0;
 };
 
 ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const ws_tuning* tuning,
@@ -1061,7 +1078,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   // k_run_close after each one.
   ScanCommon c = make_common(ctx, p, plan, rs.nunits, rs.lit_cap, rs.cand_cap);
   c.items = items;
-  if (ev) c.stamp = ctx->dstamp.as<unsigned long long>() + 2 * rs.items_off;
+  if (ev && rs.stamps) c.stamp = rs.stamps + 2 * rs.items_off;
   RunCaps caps{};
   caps.lit_cap = rs.lit_cap;
   caps.cand_cap = rs.cand_cap;
@@ -1077,15 +1094,13 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     SliceOut cso = make_slices(ctx->cslices, rs.cslice_total, rs.nvb, sc + rs.nvb, cnt + 6);
     WS_HIP(launch_hash(h.K, h.ci, rs.nvb, st, c, hp, lso, cso));
     if (ev) WS_HIP(hipEventRecord(ev[1], st));
-    uint32_t* offs = sc + 2 * rs.nvb;
     // The compaction emits the literals' ordering pairs and bucket counts
     // (into k0/v0, or k2/v2 when a chain sort will use k0..v1) and, for
     // regex sets, the candidate records.
-    WS_HIP(launch_slice_compact_keys(lso, cso, rs.nvb, offs, offs + rs.nvb + 2, cnt + 0, cnt + 1,
-                                     cnt + kCounterLitSnapshot, c.lit_cap, rs.sbits, rs.pbits,
-                                     (nrx ? ctx->k2 : ctx->k0).as<uint64_t>(), (nrx ? ctx->v2 : ctx->v0).as<uint32_t>(),
-                                     ctx->bcnt.as<uint32_t>(), rs.obits - rs.ob, nrx ? c.cand_out : nullptr,
-                                     c.cand_cap, st));
+    WS_HIP(launch_slice_compact_keys(lso, cso, rs.nvb, cnt + 0, cnt + 1, cnt + kCounterLitSnapshot, c.lit_cap,
+                                     rs.sbits, rs.pbits, (nrx ? ctx->k2 : ctx->k0).as<uint64_t>(),
+                                     (nrx ? ctx->v2 : ctx->v0).as<uint32_t>(), ctx->bcnt.as<uint32_t>(),
+                                     rs.obits - rs.ob, nrx ? c.cand_out : nullptr, c.cand_cap, st));
     keyed = true;
     caps.lit_ovf_cap = lso.ovf_cap;
     caps.cand_ovf_cap = cso.ovf_cap;
@@ -1167,7 +1182,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
                              rs.obits - rs.ob,
                              1u << rs.ob, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, rs.pbits,
                              start_off, ds.out, ds.out_cap, ds.total, st));
-    WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, st));
+    WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
     if (ev) WS_HIP(hipEventRecord(ev[4], st));
     return WS_OK;
   }
@@ -1179,7 +1194,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
                            1u << rs.ob, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, st));
   WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), rs.rec_cap, cnt + 0, rs.pbits,
                              start_off, ds.out, ds.out_cap, ds.total, st));
-  WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, st));
+  WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
   if (ev) WS_HIP(hipEventRecord(ev[4], st));
   return WS_OK;
 }
@@ -1204,11 +1219,8 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
   *outcome = kDeviceScanDone;
   // Everything sized and staged before the first launch.
   std::vector<RunSetup> setups(plans.size());
-  WS_HIP(ctx->dstate.ensure(64));
   const uint64_t max_matches = ctx->cfg.max_matches;
   DeviceRunState ds;
-  ds.sticky = ctx->dstate.as<uint32_t>();
-  ds.total = reinterpret_cast<unsigned long long*>(ds.sticky + 8);
   if (out_mem == WS_MEM_DEVICE) {
     ds.out = out;
     ds.out_cap = cap;
@@ -1217,27 +1229,53 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     WS_HIP(ctx->outbuf.ensure(std::max<uint64_t>(ds.out_cap, 1) * sizeof(ws_match)));
     ds.out = ctx->outbuf.p;
   }
-  // Pinned staging: the runs' items, then (profiling) the kernel stamps read back.
-  const size_t stamp_off = (plans.size() * sizeof(uint2) + 15) & ~(size_t)15;
-  WS_HIP(ensure_stage(ctx, stamp_off + plans.size() * 2 * sizeof(unsigned long long)));
+  // The device state block: sticky flags and the running total (64 bytes),
+  // then (profiling) two kernel stamps per run; one copy reads it back.
+  const size_t nstamp = ctx->profiling ? plans.size() : 0;
+  const size_t state_bytes = 64 + nstamp * 2 * sizeof(unsigned long long);
+  WS_HIP(ctx->dstate.ensure(state_bytes));
+  ds.sticky = ctx->dstate.as<uint32_t>();
+  ds.total = reinterpret_cast<unsigned long long*>(ds.sticky + 8);
+  unsigned long long* dstamps = ctx->profiling ? reinterpret_cast<unsigned long long*>(ds.sticky + 16) : nullptr;
+  // Pinned staging for the runs' items when they do not fit the init
+  // kernel's arguments; the state block comes back through ctx->hstate.
+  const bool items_by_arg = plans.size() <= kCallItemsMax;
+  if (!items_by_arg) WS_HIP(ensure_stage(ctx, plans.size() * sizeof(uint2)));
+  WS_HIP(ensure_hstate(ctx, state_bytes));
+  CallItems ci{};
+  ci.n = items_by_arg ? (uint32_t)plans.size() : 0u;
   for (size_t k = 0; k < plans.size(); ++k) {
     ws_status s = prepare_run(ctx, p, plans[k], tuning, &setups[k]);
     if (s != WS_OK) return s;
     setups[k].items_off = k;
-    std::memcpy(ctx->hstage + k * sizeof(uint2), plans[k].items.data(), sizeof(uint2));
+    setups[k].stamps = dstamps;
+    if (k + 1 == plans.size()) {
+      setups[k].publish = ctx->hstate;
+      setups[k].publish_words = (uint32_t)(state_bytes / 4);
+    }
+    if (items_by_arg)
+      ci.v[k] = plans[k].items[0];
+    else
+      std::memcpy(ctx->hstage + k * sizeof(uint2), plans[k].items.data(), sizeof(uint2));
   }
-  if (ctx->profiling) WS_HIP(ctx->dstamp.ensure(plans.size() * 2 * sizeof(unsigned long long)));
   WS_HIP(ctx->items.ensure(plans.size() * sizeof(uint2)));
   if (ctx->profiling && ctx->pev.size() < 5 * plans.size()) {
     const size_t had = ctx->pev.size();
     ctx->pev.resize(5 * plans.size(), nullptr);
     for (size_t k = had; k < ctx->pev.size(); ++k) WS_HIP(hipEventCreate(&ctx->pev[k]));
   }
+  // The call's init (state reset, and the runs' segments when they fit its
+  // arguments) runs eagerly ahead of the runs, captured or not: a captured
+  // graph then holds only the runs, and each replay gets this call's
+  // segments without a copy node.
+  auto launch_init = [&]() -> ws_status {
+    WS_HIP(launch_call_init(ds.sticky, 10, ctx->counters.as<uint32_t>(), kCounterLitSnapshot + 1, dstamps,
+                            (uint32_t)nstamp, ci, ctx->items.as<uint2>(), st));
+    return WS_OK;
+  };
   auto launch_all = [&]() -> ws_status {
-    WS_HIP(launch_call_init(ds.sticky, 10, ctx->counters.as<uint32_t>(), kCounterLitSnapshot + 1,
-                            ctx->profiling ? ctx->dstamp.as<unsigned long long>() : nullptr,
-                            (uint32_t)plans.size(), st));
-    WS_HIP(hipMemcpyAsync(ctx->items.p, ctx->hstage, plans.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+    if (!items_by_arg)
+      WS_HIP(hipMemcpyAsync(ctx->items.p, ctx->hstage, plans.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
     for (size_t k = 0; k < plans.size(); ++k) {
       ws_status s = launch_run(ctx, p, plans[k], setups[k], offs[k], ds,
                                ctx->profiling ? &ctx->pev[5 * k] : nullptr);
@@ -1253,6 +1291,10 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     ge = &ctx->graphs[key];
   }
   if (ctx->profiling) WS_HIP(hipEventRecord(ctx->tev[0], st));
+  {
+    ws_status s = launch_init();
+    if (s != WS_OK) return s;
+  }
   bool replayed = false;
   if (ge && ge->exec) {
     WS_HIP(hipGraphLaunch(ge->exec, st));
@@ -1288,14 +1330,9 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     ws_status s = launch_all();
     if (s != WS_OK) return s;
   }
-  if (ctx->profiling) {
-    WS_HIP(hipEventRecord(ctx->tev[1], st));
-    WS_HIP(hipMemcpyAsync(ctx->hstage + stamp_off, ctx->dstamp.p, plans.size() * 2 * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, st));
-  }
-  WS_HIP(hipMemcpyAsync(ctx->hcount + 11, ds.sticky, 10 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  if (ctx->profiling) WS_HIP(hipEventRecord(ctx->tev[1], st));
   WS_HIP(hipStreamSynchronize(st));
-  const uint32_t* hs = ctx->hcount + 11;
+  const uint32_t* hs = ctx->hstate;
   if (hs[kStickyRedo]) {
     *outcome = kDeviceScanRedo;
     return WS_OK;
@@ -1307,7 +1344,7 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     // events between launches, which a replayed graph does not re-record
     // (then -1: unknown).
     ws_timings t{};
-    const unsigned long long* stamps = reinterpret_cast<const unsigned long long*>(ctx->hstage + stamp_off);
+    const unsigned long long* stamps = reinterpret_cast<const unsigned long long*>(hs + 16);
     for (size_t k = 0; k < plans.size(); ++k)
       if (stamps[2 * k + 1] > stamps[2 * k])
         t.prefilter_ms += (float)((double)(stamps[2 * k + 1] - stamps[2 * k]) / ctx->wall_khz);
@@ -1517,7 +1554,7 @@ void ws_ctx_destroy(ws_ctx* ctx) {
                     &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
                     &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
                     &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
-                    &ctx->bcur, &ctx->dstate, &ctx->dstamp, &ctx->k2, &ctx->v2, &ctx->bcnt2, &ctx->bases})
+                    &ctx->bcur, &ctx->dstate, &ctx->k2, &ctx->v2, &ctx->bcnt2, &ctx->bases})
     b->release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -1528,6 +1565,7 @@ void ws_ctx_destroy(ws_ctx* ctx) {
   for (auto& kv : ctx->graphs)
     if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
   if (ctx->hstage) (void)hipHostFree(ctx->hstage);
+  if (ctx->hstate) (void)hipHostFree(ctx->hstate);
   if (ctx->hcount) (void)hipHostFree(ctx->hcount);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -775,20 +775,44 @@ __global__ __launch_bounds__(kTPB) void k_slice_gather2(SliceOut lit, SliceOut c
 // {pid, start, end, item} becomes key (((item << sbits) | start) << pbits) |
 // pid with value end (as k_order_keys makes them), counted in bucket
 // key >> shift for the bucket sort that follows.
-__global__ __launch_bounds__(kTPB) void k_slice_gather_keys(SliceOut lit, uint32_t n, const uint32_t* loffs,
-                                                             uint32_t cap, uint32_t sbits, uint32_t pbits,
-                                                             uint64_t* keys, uint32_t* vals, uint32_t* cnt,
-                                                             uint32_t shift, SliceOut cand, const uint32_t* coffs,
-                                                             Rec* cout, uint32_t ccap) {
+// Sum over slices [0, upto) of the records they hold, in every thread of
+// the block (one call per block).
+__device__ __forceinline__ uint32_t block_slice_sum(const SliceOut& so, uint32_t upto) {
+  __shared__ uint32_t red[kTPB / 64];
+  uint32_t v = 0;
+  for (uint32_t b = threadIdx.x; b < upto; b += kTPB) v += min(so.counts[b], so.cap);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t t = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kTPB / 64; ++w) t += red[w];
+  return t;
+}
+
+// Compaction of the filter's output slices, each block finding its own
+// output offset from the slice counts before it: blocks [0, n) copy the
+// literal slices straight to ordering pairs (and bucket counts), blocks
+// [n, n + 64) the literal overflow area (block n also writes the literal
+// total), and with cout set the same again for the candidate slices, as
+// records.
+__global__ __launch_bounds__(kTPB) void k_slice_gather_keys(SliceOut lit, uint32_t n, uint32_t cap,
+                                                             uint32_t sbits, uint32_t pbits, uint64_t* keys,
+                                                             uint32_t* vals, uint32_t* cnt, uint32_t shift,
+                                                             SliceOut cand, Rec* cout, uint32_t ccap,
+                                                             uint32_t* lit_total, uint32_t* cand_total,
+                                                             uint32_t* lit_snap) {
   if (blockIdx.x >= n + 64) {
-    // Candidate slices (regex sets), as k_slice_gather2's second half.
     const uint32_t bx = blockIdx.x - (n + 64);
+    const uint32_t o = block_slice_sum(cand, bx < n ? bx : n);
     if (bx < n) {
-      const uint32_t m = min(cand.counts[bx], cand.cap), o = coffs[bx];
+      const uint32_t m = min(cand.counts[bx], cand.cap);
       for (uint32_t k = threadIdx.x; k < m; k += kTPB)
         if (o + k < ccap) cout[o + k] = cand.buf[(size_t)bx * cand.cap + k];
     } else {
-      const uint32_t no = min(*cand.ovf_count, cand.ovf_cap), o = coffs[n];
+      const uint32_t no = min(*cand.ovf_count, cand.ovf_cap);
+      if (bx == n && threadIdx.x == 0) *cand_total = o + no;
       for (uint32_t k = (bx - n) * kTPB + threadIdx.x; k < no; k += 64 * kTPB)
         if (o + k < ccap) cout[o + k] = cand.ovf[k];
     }
@@ -800,12 +824,17 @@ __global__ __launch_bounds__(kTPB) void k_slice_gather_keys(SliceOut lit, uint32
     vals[i] = r.z;
     atomicAdd(&cnt[k >> shift], 1u);
   };
+  const uint32_t o = block_slice_sum(lit, blockIdx.x < n ? blockIdx.x : n);
   if (blockIdx.x < n) {
-    const uint32_t m = min(lit.counts[blockIdx.x], lit.cap), o = loffs[blockIdx.x];
+    const uint32_t m = min(lit.counts[blockIdx.x], lit.cap);
     for (uint32_t k = threadIdx.x; k < m; k += kTPB)
       if (o + k < cap) put(o + k, lit.buf[(size_t)blockIdx.x * lit.cap + k]);
   } else {
-    const uint32_t no = min(*lit.ovf_count, lit.ovf_cap), o = loffs[n];
+    const uint32_t no = min(*lit.ovf_count, lit.ovf_cap);
+    if (blockIdx.x == n && threadIdx.x == 0) {
+      *lit_total = o + no;
+      if (lit_snap) *lit_snap = o + no;
+    }
     for (uint32_t k = (blockIdx.x - n) * kTPB + threadIdx.x; k < no; k += 64 * kTPB)
       if (o + k < cap) put(o + k, lit.ovf[k]);
   }
@@ -1359,19 +1388,14 @@ hipError_t launch_slice_compact2(const SliceOut& lit, const SliceOut& cand, uint
   return hipGetLastError();
 }
 
-// Offsets of both slice sets, then the literal slices straight to ordering
-// pairs and bucket counts, and (cout set) the candidate slices to records.
-hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
-                                     uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total,
-                                     uint32_t* lit_snap, uint32_t cap, uint32_t sbits, uint32_t pbits,
-                                     uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift, Rec* cout,
-                                     uint32_t ccap, hipStream_t st) {
-  hipLaunchKernelGGL(k_slice_offsets2, dim3(1), dim3(1024), 0, st, lit, cand, n, loffs, coffs, lit_total,
-                     cand_total, lit_snap);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_slice_gather_keys, dim3((cout ? 2 : 1) * (n + 64)), dim3(kTPB), 0, st, lit, n, loffs, cap,
-                     sbits, pbits, keys, vals, cnt, shift, cand, coffs, cout, ccap);
+// The literal slices straight to ordering pairs and bucket counts, and
+// (cout set) the candidate slices to records, in one launch.
+hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* lit_total,
+                                     uint32_t* cand_total, uint32_t* lit_snap, uint32_t cap, uint32_t sbits,
+                                     uint32_t pbits, uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift,
+                                     Rec* cout, uint32_t ccap, hipStream_t st) {
+  hipLaunchKernelGGL(k_slice_gather_keys, dim3((cout ? 2 : 1) * (n + 64)), dim3(kTPB), 0, st, lit, n, cap, sbits,
+                     pbits, keys, vals, cnt, shift, cand, cout, ccap, lit_total, cand_total, lit_snap);
   return hipGetLastError();
 }
 
@@ -1404,10 +1428,11 @@ hipError_t launch_expand_units(const uint2* runs, uint32_t nruns, const uint2* i
 // counters, and resets the n {start, end} kernel stamp pairs (stamps may be
 // null).
 __global__ void k_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counters, uint32_t ncounters,
-                            unsigned long long* stamps, uint32_t n) {
+                            unsigned long long* stamps, uint32_t n, CallItems ci, uint2* items) {
   const uint32_t t = threadIdx.x;
   if (t < nsticky) sticky[t] = 0;
   if (t < ncounters) counters[t] = 0;
+  if (t < ci.n) items[t] = ci.v[t];
   if (stamps)
     for (uint32_t k = t; k < n; k += blockDim.x) {
       stamps[2 * k] = ~0ull;
@@ -1416,8 +1441,10 @@ __global__ void k_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counte
 }
 
 hipError_t launch_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counters, uint32_t ncounters,
-                            unsigned long long* stamps, uint32_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_call_init, dim3(1), dim3(256), 0, st, sticky, nsticky, counters, ncounters, stamps, n);
+                            unsigned long long* stamps, uint32_t n, const CallItems& ci, uint2* items,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_call_init, dim3(1), dim3(256), 0, st, sticky, nsticky, counters, ncounters, stamps, n, ci,
+                     items);
   return hipGetLastError();
 }
 
@@ -1436,23 +1463,32 @@ hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_
 // End of a device-sized run: record overflows and errors, accumulate
 // statistics, advance the output offset by this run's match count, and zero
 // the run's counters for the next run.
-__global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, unsigned long long* total) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  uint32_t redo = 0;
-  if (counters[5] > caps.lit_ovf_cap || counters[6] > caps.cand_ovf_cap) redo |= 2u;
-  if (counters[kCounterLitSnapshot] > caps.lit_cap) redo |= 4u;
-  if (counters[1] > caps.cand_cap) redo |= 8u;
-  if (counters[2] > caps.raw_cap) redo |= 16u;
-  if (counters[0] > caps.rec_cap) redo |= 32u;
-  if (counters[9]) redo |= 64u;  // a sort bucket overflowed
-  sticky[kStickyRedo] |= redo;
-  if (counters[3] & kFlagTruncatedWalk) sticky[kStickyError] |= 1u;
-  if (counters[0] > caps.max_matches) sticky[kStickyError] |= 2u;
-  sticky[kStickyHits] += counters[8];
-  sticky[kStickyCands] += counters[1];
-  sticky[kStickyRaw] += counters[2];
-  *total += counters[0];
-  for (uint32_t k = 0; k <= kCounterLitSnapshot; ++k) counters[k] = 0;
+__global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, unsigned long long* total,
+                            uint32_t* publish, uint32_t pwords) {
+  if (threadIdx.x == 0) {
+    uint32_t redo = 0;
+    if (counters[5] > caps.lit_ovf_cap || counters[6] > caps.cand_ovf_cap) redo |= 2u;
+    if (counters[kCounterLitSnapshot] > caps.lit_cap) redo |= 4u;
+    if (counters[1] > caps.cand_cap) redo |= 8u;
+    if (counters[2] > caps.raw_cap) redo |= 16u;
+    if (counters[0] > caps.rec_cap) redo |= 32u;
+    if (counters[9]) redo |= 64u;  // a sort bucket overflowed
+    sticky[kStickyRedo] |= redo;
+    if (counters[3] & kFlagTruncatedWalk) sticky[kStickyError] |= 1u;
+    if (counters[0] > caps.max_matches) sticky[kStickyError] |= 2u;
+    sticky[kStickyHits] += counters[8];
+    sticky[kStickyCands] += counters[1];
+    sticky[kStickyRaw] += counters[2];
+    *total += counters[0];
+    for (uint32_t k = 0; k <= kCounterLitSnapshot; ++k) counters[k] = 0;
+  }
+  // The call's last run hands the state block (flags, total, kernel stamps)
+  // to pinned host memory, so the host reads it without a copy.
+  if (publish) {
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < pwords; k += blockDim.x) publish[k] = sticky[k];
+    __threadfence_system();
+  }
 }
 
 __global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) {
@@ -1460,8 +1496,8 @@ __global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) {
 }
 
 hipError_t launch_run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
-                            unsigned long long* total, hipStream_t st) {
-  hipLaunchKernelGGL(k_run_close, dim3(1), dim3(64), 0, st, counters, caps, sticky, total);
+                            unsigned long long* total, uint32_t* publish, uint32_t pwords, hipStream_t st) {
+  hipLaunchKernelGGL(k_run_close, dim3(1), dim3(64), 0, st, counters, caps, sticky, total, publish, pwords);
   return hipGetLastError();
 }
 

@@ -125,6 +125,14 @@ enum : uint32_t { kFlagTruncatedWalk = 1u };
 // Capacities of a device-sized run (no host readback between passes), and
 // the counters slot that keeps the literal count before regex matches are
 // appended.
+// Up to kCallItemsMax run segments passed to k_call_init by value, so a call
+// needs no host-to-device copy for them.
+constexpr uint32_t kCallItemsMax = 32;
+struct CallItems {
+  uint32_t n;
+  uint2 v[kCallItemsMax];
+};
+
 struct RunCaps {
   uint32_t lit_ovf_cap, cand_ovf_cap;
   uint32_t lit_cap, cand_cap, raw_cap, rec_cap, max_matches;
