@@ -27,7 +27,7 @@ namespace ws {
 // Launch wrappers defined in kernels.hip.
 size_t hash_shmem_bytes(uint32_t nwords);
 hipError_t launch_hash(uint32_t K, bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c,
-                       const HashParams& h, const SliceOut& lit, const SliceOut& cand);
+                       const HashParams& h, const SliceOut& lit, const SliceOut& cand, bool rx);
 uint32_t ac_unit_bytes();
 size_t ac_shmem_bytes(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCommon& c,
@@ -73,7 +73,7 @@ hipError_t launch_expand_units(const uint2* runs, uint32_t nruns, const uint2* i
 hipError_t launch_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counters, uint32_t ncounters,
                             unsigned long long* stamps, uint32_t n, const CallItems& ci, uint2* items,
                             hipStream_t st);
-hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords);
+hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords, bool rx);
 hipError_t prepare_ac(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
                             unsigned long long* total, uint32_t* publish, uint32_t pwords, hipStream_t st);
@@ -680,7 +680,7 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
         uint32_t* sc = ctx->scounts.as<uint32_t>();
         SliceOut lso = make_slices(ctx->lslices, lslice_total, nvb, sc, ctx->counters.as<uint32_t>() + 5);
         SliceOut cso = make_slices(ctx->cslices, cslice_total, nvb, sc + nvb, ctx->counters.as<uint32_t>() + 6);
-        WS_HIP(launch_hash(h.K, h.ci, nvb, st, c, hp, lso, cso));
+        WS_HIP(launch_hash(h.K, h.ci, nvb, st, c, hp, lso, cso, !h.rx.empty()));
         if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
         uint32_t* offs = sc + 2 * nvb;
         WS_HIP(launch_slice_compact2(lso, cso, nvb, offs, offs + nvb + 2, ctx->counters.as<uint32_t>() + 0,
@@ -1056,7 +1056,7 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   WS_HIP(ctx->tmp.ensure(std::max(need, need2)));
   if (rs->variant == WS_VARIANT_HASH) {
     // Dynamic LDS attribute of the filter kernel: set outside any capture.
-    WS_HIP(prepare_hash(h.K, h.ci, rs->lane_bytes, (uint32_t)h.bloom.size()));
+    WS_HIP(prepare_hash(h.K, h.ci, rs->lane_bytes, (uint32_t)h.bloom.size(), !h.rx.empty()));
   } else {
     WS_HIP(prepare_ac(rs->variant == WS_VARIANT_DFA_LDS, h.ac_states, h.ac_classes));
   }
@@ -1092,7 +1092,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     uint32_t* sc = ctx->scounts.as<uint32_t>();
     SliceOut lso = make_slices(ctx->lslices, rs.lslice_total, rs.nvb, sc, cnt + 5);
     SliceOut cso = make_slices(ctx->cslices, rs.cslice_total, rs.nvb, sc + rs.nvb, cnt + 6);
-    WS_HIP(launch_hash(h.K, h.ci, rs.nvb, st, c, hp, lso, cso));
+    WS_HIP(launch_hash(h.K, h.ci, rs.nvb, st, c, hp, lso, cso, nrx > 0));
     if (ev) WS_HIP(hipEventRecord(ev[1], st));
     // The compaction emits the literals' ordering pairs and bucket counts
     // (into k0/v0, or k2/v2 when a chain sort will use k0..v1) and, for

@@ -354,12 +354,12 @@ struct HitWork {
 // tags first; only a match (or a crowded bucket) reads the slots and the
 // span's third 16 bytes (for factors that run past the first 32).
 template <int K, bool CI>
-__device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParams& h, uint2 hr, uint32_t* raw,
-                                            uint32_t lend, HitWork& hw) {
+__device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParams& h, uint2 hr, uint2 seg,
+                                            uint32_t* raw, uint32_t lend, HitWork& hw) {
   hw.live = true;
   hw.q = hr.x;
   hw.item = hr.y;
-  hw.seg = c.items[hw.item];
+  hw.seg = seg;
   const uint32_t sh = (uint32_t)(hw.q & 15);
   realign32(raw, sh, hw.W);
   // The verification table's 24-bit window hash (v_mad_u32_u24 reads the
@@ -400,7 +400,8 @@ __device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParam
   }
 }
 
-template <bool CI>
+// RX: the set has regex factors (kind-2 entries, candidate output).
+template <bool CI, bool RX>
 __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams& h, const HitWork& hw,
                                            uint32_t* s_n, const SliceOut& lit, const SliceOut& cand) {
   uint4 r0 = make_uint4(0, 0, 0, 0), r1 = r0;
@@ -418,13 +419,13 @@ __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams
       uint4 rec;
       const uint32_t k = verify_entry<CI>(c, e0, h.vmore[2 * (size_t)e + 1], hw.W, hw.q, hw.seg, hw.item, &rec);
       if (k == 1) slice_put(lit, blockIdx.x, atomicAdd(&s_n[0], 1u), rec);
-      if (k == 2) slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), rec);
+      if (RX && k == 2) slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), rec);
     }
   }
   emit_wave(lit, &s_n[0], k0 == 1, r0);
-  emit_wave(cand, &s_n[1], k0 == 2, r0);
+  if constexpr (RX) emit_wave(cand, &s_n[1], k0 == 2, r0);
   emit_wave(lit, &s_n[0], k1 == 1, r1);
-  emit_wave(cand, &s_n[1], k1 == 2, r1);
+  if constexpr (RX) emit_wave(cand, &s_n[1], k1 == 2, r1);
 }
 
 // Per-wave LDS ring of filter hits waiting for verification (uint2
@@ -436,16 +437,17 @@ constexpr uint32_t kHitRing = 128;
 // shortly before), the window hash, its bucket's tags, and for a tag match
 // the slots and the exact compare.
 // Call with the whole wave.
-template <int K, bool CI>
+// seg0: the segment of a single-segment scan (c.units null), wave-uniform.
+template <int K, bool CI, bool RX>
 __device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashParams& h, const uint2* ring,
                                               uint32_t head, uint32_t n, uint32_t* s_cnt, const SliceOut& lit,
-                                              const SliceOut& cand) {
+                                              const SliceOut& cand, uint2 seg0) {
   const uint32_t lane = lane_id();
   HitWork hw{};
   hw.live = false;
   if (lane < n) {
     const uint2 hr = ring[(head + lane) & (kHitRing - 1)];
-    const uint2 seg = c.items[hr.y];
+    const uint2 seg = c.units ? c.items[hr.y] : seg0;
     uint32_t lend = (seg.y + 15) & ~15u;
     if (lend > (uint32_t)c.load_end) lend = (uint32_t)c.load_end;
     const uint32_t a = hr.x & ~15u;
@@ -457,9 +459,9 @@ __device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashPar
       raw[4 * v] = x.x; raw[4 * v + 1] = x.y; raw[4 * v + 2] = x.z; raw[4 * v + 3] = x.w;
     }
     raw[8] = raw[9] = raw[10] = raw[11] = 0;
-    hit_prepare<K, CI>(c, h, hr, raw, lend, hw);
+    hit_prepare<K, CI>(c, h, hr, seg, raw, lend, hw);
   }
-  hit_finish<CI>(c, h, hw, s_cnt, lit, cand);
+  hit_finish<CI, RX>(c, h, hw, s_cnt, lit, cand);
 }
 
 // Kernel duration for profiling without events (which a replayed graph does
@@ -495,7 +497,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // behind their arithmetic: no hit list goes through HBM and the loop has no
 // block barriers. Literal matches and regex candidate starts go to the
 // block's output slices.
-template <int K, bool CI, int PL>
+template <int K, bool CI, int PL, bool RX>
 __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashParams h, SliceOut lit,
                                                              SliceOut cand) {
   // LDS: the bloom words from address 0 (the kernel has no static LDS, so
@@ -683,7 +685,7 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
         lo_r += take;
         wave_lds_sync();
         while (pend >= 64) {
-          verify_queued<K, CI>(c, h, ring, tail - pend, 64, s_cnt, lit, cand);
+          verify_queued<K, CI, RX>(c, h, ring, tail - pend, 64, s_cnt, lit, cand, seg0);
           pend -= 64;
         }
         wave_lds_sync();
@@ -692,7 +694,7 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
   }
   while (pend > 0) {
     const uint32_t n = min(pend, 64u);
-    verify_queued<K, CI>(c, h, ring, tail - pend, n, s_cnt, lit, cand);
+    verify_queued<K, CI, RX>(c, h, ring, tail - pend, n, s_cnt, lit, cand, seg0);
     pend -= n;
   }
   if (lane == 0 && nhits) atomicAdd(&c.counters[8], nhits);  // filter hits, for timings
@@ -1217,52 +1219,67 @@ static hipError_t allow_lds(F* fn, size_t bytes) {
 
 size_t hash_shmem_bytes(uint32_t nwords) { return nwords * 4u + 16u + (kFilterTPB / 64) * kHitRing * sizeof(uint2); }
 
+template <int K, bool CI, bool RX>
+static hipError_t prepare_hash_kcr(uint32_t lane_bytes, size_t shmem) {
+  if (lane_bytes == 64) return allow_lds(k_hash_filter<K, CI, 64, RX>, shmem);
+  if (lane_bytes == 32) return allow_lds(k_hash_filter<K, CI, 32, RX>, shmem);
+  return allow_lds(k_hash_filter<K, CI, 16, RX>, shmem);
+}
+
 template <int K, bool CI>
-static hipError_t prepare_hash_kc(uint32_t lane_bytes, size_t shmem) {
-  if (lane_bytes == 64) return allow_lds(k_hash_filter<K, CI, 64>, shmem);
-  if (lane_bytes == 32) return allow_lds(k_hash_filter<K, CI, 32>, shmem);
-  return allow_lds(k_hash_filter<K, CI, 16>, shmem);
+static hipError_t prepare_hash_kc(uint32_t lane_bytes, size_t shmem, bool rx) {
+  return rx ? prepare_hash_kcr<K, CI, true>(lane_bytes, shmem) : prepare_hash_kcr<K, CI, false>(lane_bytes, shmem);
 }
 
 // Sets the filter kernel's dynamic LDS limit ahead of launches that may be
 // captured into a graph.
-hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords) {
+hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords, bool rx) {
   const size_t shmem = hash_shmem_bytes(nwords);
   switch (K) {
-#define WS_K(k) case k: return ci ? prepare_hash_kc<k, true>(lane_bytes, shmem) : prepare_hash_kc<k, false>(lane_bytes, shmem);
+#define WS_K(k) \
+  case k: return ci ? prepare_hash_kc<k, true>(lane_bytes, shmem, rx) : prepare_hash_kc<k, false>(lane_bytes, shmem, rx);
     WS_K(1) WS_K(2) WS_K(3) WS_K(4) WS_K(6) WS_K(8) WS_K(12) WS_K(16)
 #undef WS_K
     default: return hipErrorInvalidValue;
   }
 }
 
-template <int K, bool CI>
-static hipError_t launch_hash_kc(uint32_t grid, hipStream_t st, const ScanCommon& c, const HashParams& h,
-                                 const SliceOut& lit, const SliceOut& cand) {
+template <int K, bool CI, bool RX>
+static hipError_t launch_hash_kcr(uint32_t grid, hipStream_t st, const ScanCommon& c, const HashParams& h,
+                                  const SliceOut& lit, const SliceOut& cand) {
   const size_t shmem = hash_shmem_bytes(h.nwords);
   if (h.lane_bytes == 64) {
-    hipError_t e = allow_lds(k_hash_filter<K, CI, 64>, shmem);
+    hipError_t e = allow_lds(k_hash_filter<K, CI, 64, RX>, shmem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_hash_filter<K, CI, 64>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, lit, cand);
+    hipLaunchKernelGGL((k_hash_filter<K, CI, 64, RX>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, lit, cand);
   } else if (h.lane_bytes == 32) {
-    hipError_t e = allow_lds(k_hash_filter<K, CI, 32>, shmem);
+    hipError_t e = allow_lds(k_hash_filter<K, CI, 32, RX>, shmem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_hash_filter<K, CI, 32>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, lit, cand);
+    hipLaunchKernelGGL((k_hash_filter<K, CI, 32, RX>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, lit, cand);
   } else {
-    hipError_t e = allow_lds(k_hash_filter<K, CI, 16>, shmem);
+    hipError_t e = allow_lds(k_hash_filter<K, CI, 16, RX>, shmem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_hash_filter<K, CI, 16>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, lit, cand);
+    hipLaunchKernelGGL((k_hash_filter<K, CI, 16, RX>), dim3(grid), dim3(kFilterTPB), shmem, st, c, h, lit, cand);
   }
   return hipGetLastError();
 }
 
+template <int K, bool CI>
+static hipError_t launch_hash_kc(uint32_t grid, hipStream_t st, const ScanCommon& c, const HashParams& h,
+                                 const SliceOut& lit, const SliceOut& cand, bool rx) {
+  return rx ? launch_hash_kcr<K, CI, true>(grid, st, c, h, lit, cand)
+            : launch_hash_kcr<K, CI, false>(grid, st, c, h, lit, cand);
+}
+
 // The filter-and-verify kernel on `grid` blocks; block b writes output
-// slice b of lit and cand.
+// slice b of lit and cand (rx: the set has regex factors).
 hipError_t launch_hash(uint32_t K, bool ci, uint32_t grid, hipStream_t st, const ScanCommon& c,
-                       const HashParams& h, const SliceOut& lit, const SliceOut& cand) {
+                       const HashParams& h, const SliceOut& lit, const SliceOut& cand, bool rx) {
   switch (K) {
-#define WS_K(k) \
-  case k: return ci ? launch_hash_kc<k, true>(grid, st, c, h, lit, cand) : launch_hash_kc<k, false>(grid, st, c, h, lit, cand);
+#define WS_K(k)                                                                          \
+  case k:                                                                                \
+    return ci ? launch_hash_kc<k, true>(grid, st, c, h, lit, cand, rx)                  \
+              : launch_hash_kc<k, false>(grid, st, c, h, lit, cand, rx);
     WS_K(1) WS_K(2) WS_K(3) WS_K(4) WS_K(6) WS_K(8) WS_K(12) WS_K(16)
 #undef WS_K
     default: return hipErrorInvalidValue;
