@@ -1241,7 +1241,11 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
   // kernel's arguments; the state block comes back through ctx->hstate.
   const bool items_by_arg = plans.size() <= kCallItemsMax;
   if (!items_by_arg) WS_HIP(ensure_stage(ctx, plans.size() * sizeof(uint2)));
-  WS_HIP(ensure_hstate(ctx, state_bytes));
+  WS_HIP(ensure_hstate(ctx, state_bytes + 4));
+  // The completion flag after the state block (set by the last kernel).
+  volatile uint32_t* done_flag = ctx->hstate + state_bytes / 4;
+  *done_flag = 0u;
+  std::atomic_thread_fence(std::memory_order_seq_cst);
   CallItems ci{};
   ci.n = items_by_arg ? (uint32_t)plans.size() : 0u;
   for (size_t k = 0; k < plans.size(); ++k) {
@@ -1331,7 +1335,23 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     if (s != WS_OK) return s;
   }
   if (ctx->profiling) WS_HIP(hipEventRecord(ctx->tev[1], st));
-  WS_HIP(hipStreamSynchronize(st));
+  // Completion: spin briefly on the flag the last kernel sets (a shorter
+  // wake-up than a blocking wait), then block on the stream. Per-phase
+  // events of an eager call need the stream finished.
+  bool flagged = false;
+  if (!(ctx->profiling && !replayed)) {
+    const auto t_spin = std::chrono::steady_clock::now();
+    for (uint32_t it = 0;; ++it) {
+      if (*done_flag == 1u) {
+        flagged = true;
+        break;
+      }
+      if ((it & 1023u) == 0 && std::chrono::steady_clock::now() - t_spin > std::chrono::milliseconds(5)) break;
+      __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+  }
+  if (!flagged) WS_HIP(hipStreamSynchronize(st));
   const uint32_t* hs = ctx->hstate;
   if (hs[kStickyRedo]) {
     *outcome = kDeviceScanRedo;
@@ -1362,8 +1382,11 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
         t.order_ms += d;
       }
     }
-    float all = 0;
-    (void)hipEventElapsedTime(&all, ctx->tev[0], ctx->tev[1]);
+    float all = -1.0f;
+    if (hipEventQuery(ctx->tev[1]) == hipSuccess)
+      (void)hipEventElapsedTime(&all, ctx->tev[0], ctx->tev[1]);
+    else
+      (void)hipGetLastError();  // a not-ready query must not read as a later launch error
     t.total_ms = all;
     t.hits = hs[kStickyHits];
     t.candidates = hs[kStickyCands];

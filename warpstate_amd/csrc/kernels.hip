@@ -1501,10 +1501,16 @@ __global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, 
   }
   // The call's last run hands the state block (flags, total, kernel stamps)
   // to pinned host memory, so the host reads it without a copy.
+  // publish[pwords] is then set to 1: the host may wait on it instead of
+  // blocking on the stream.
   if (publish) {
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < pwords; k += blockDim.x) publish[k] = sticky[k];
     __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __hip_atomic_store(&publish[pwords], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 
