@@ -63,6 +63,10 @@ hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, u
                              uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift, uint32_t nb,
                              uint32_t* cnt, uint32_t* cursor, uint32_t* flag, uint32_t pbits, uint32_t start_off,
                              void* out, uint64_t out_cap, const unsigned long long* base_dev, hipStream_t st);
+hipError_t launch_lit_order(const SliceOut& lit, uint32_t n, uint32_t pbits, uint32_t shift, uint32_t nb,
+                            uint64_t* bkeys, uint32_t* bcnt, uint32_t* bofs, uint32_t* counters,
+                            const uint32_t* flen, uint32_t start_off, void* out, uint64_t out_cap,
+                            const unsigned long long* base_dev, hipStream_t st);
 hipError_t launch_order_keys_dense(const Rec* recs, uint32_t n, uint32_t pbits, const uint64_t* bases,
                                    uint64_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t launch_pack_tagged_dense(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t pbits,
@@ -164,7 +168,7 @@ struct ws_ctx {
   hipEvent_t tev[2] = {};      // around a whole device-sized call (outside any graph)
   uint32_t wall_khz = 0;       // device wall-clock rate (kernel stamps)
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, k2, v2, bcnt2, bases;
+      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, k2, v2, bcnt2, bases, lkeys, lcnt, lofs;
   std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
   uint8_t* hstage = nullptr;    // pinned staging read by captured copies
   size_t hstage_cap = 0;
@@ -982,6 +986,7 @@ struct RunSetup {
   uint32_t lit_cap = 0, cand_cap = 0, raw_cap = 0, rec_cap = 0, max_matches = 0;
   uint64_t lslice_total = 0, cslice_total = 0;
   uint32_t sbits = 0, pbits = 0, rbits = 0, obits = 0, cbits = 0, ob = 0, cb = 0;
+  uint32_t lb = 0;       // literal-only ordering: 2^lb fixed-capacity buckets (0: not used)
   size_t items_off = 0;  // this run's index among the call's runs (its segment, its stamps)
   unsigned long long* stamps = nullptr;  // the call's kernel stamps (profiling), two per run
   uint32_t* publish = nullptr;           // the call's last run: pinned host copy of the state block
@@ -1039,6 +1044,16 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   rs->cb = nrx ? bucket_bits(rs->cbits, std::min(rs->raw_cap, 1u << 17)) : 0;
   const uint32_t nb = 1u << std::max(rs->ob, rs->cb);
   WS_HIP(ensure_bucket_counts(ctx, nb));
+  if (!nrx && rs->variant == WS_VARIANT_HASH) {
+    // Literal-only sets: about 128 matches per bucket at the capacity, so
+    // a bucket holds twice its share before the checked path takes over.
+    rs->lb = std::min(rs->obits, std::min(bits_for(kLitBuckets - 1),
+                                          std::max(6u, bits_for(rs->lit_cap) > 7 ? bits_for(rs->lit_cap) - 7 : 6u)));
+    const uint32_t lnb = 1u << rs->lb;
+    WS_HIP(ctx->lkeys.ensure((size_t)lnb * kLitBucketCap * 8));
+    WS_HIP(ensure_zeroed_counts(ctx, ctx->lcnt, kLitBuckets));
+    WS_HIP(ctx->lofs.ensure((size_t)kLitBuckets * 4));
+  }
   if (nrx && rs->variant == WS_VARIANT_HASH) {
     // Regex sets on the hash variant: final ordering pairs (k2, v2) and their
     // counts (bcnt) are made while the chain sort uses k0..v1 and bcnt2.
@@ -1094,6 +1109,19 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     SliceOut cso = make_slices(ctx->cslices, rs.cslice_total, rs.nvb, sc + rs.nvb, cnt + 6);
     WS_HIP(launch_hash(h.K, h.ci, rs.nvb, st, c, hp, lso, cso, nrx > 0));
     if (ev) WS_HIP(hipEventRecord(ev[1], st));
+    if (rs.lb) {
+      // Literal-only: slices -> fixed-capacity buckets -> ranked and packed.
+      if (ev) WS_HIP(hipEventRecord(ev[2], st));
+      if (ev) WS_HIP(hipEventRecord(ev[3], st));
+      caps.lit_ovf_cap = lso.ovf_cap;
+      caps.cand_ovf_cap = cso.ovf_cap;
+      WS_HIP(launch_lit_order(lso, rs.nvb, rs.pbits, rs.obits - rs.lb, 1u << rs.lb, ctx->lkeys.as<uint64_t>(),
+                              ctx->lcnt.as<uint32_t>(), ctx->lofs.as<uint32_t>(), cnt, p->flen, start_off, ds.out,
+                              ds.out_cap, ds.total, st));
+      WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
+      if (ev) WS_HIP(hipEventRecord(ev[4], st));
+      return WS_OK;
+    }
     // The compaction emits the literals' ordering pairs and bucket counts
     // (into k0/v0, or k2/v2 when a chain sort will use k0..v1) and, for
     // regex sets, the candidate records.
@@ -1577,7 +1605,8 @@ void ws_ctx_destroy(ws_ctx* ctx) {
                     &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
                     &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
                     &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
-                    &ctx->bcur, &ctx->dstate, &ctx->k2, &ctx->v2, &ctx->bcnt2, &ctx->bases})
+                    &ctx->bcur, &ctx->dstate, &ctx->k2, &ctx->v2, &ctx->bcnt2, &ctx->bases,
+                    &ctx->lkeys, &ctx->lcnt, &ctx->lofs})
     b->release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);

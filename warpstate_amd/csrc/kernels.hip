@@ -1623,6 +1623,160 @@ __global__ void k_bucket_sort_pack(uint64_t* keys, uint32_t* vals, const uint32_
   }
 }
 
+// ---------------------------------------------------------------------------
+// Ordering of a literal-only set's matches in three launches. A match's key is
+// start << pbits | pattern id (a literal-only set's pattern ids are its
+// literal indices, so the end is start + the literal's length). The
+// position range is cut into nb buckets of kLitBucketCap slots.
+//  1. k_gather_buckets: every slice record to a slot of its bucket (a
+//     bucket that fills up sets the overflow flag: the call is redone on
+//     the checked path).
+//  2. k_scan_buckets: the bucket counts to output offsets, and the total.
+//  3. k_rank_pack: one wave per bucket ranks its keys in LDS and writes the
+//     matches in order; it leaves the counts zero.
+// ---------------------------------------------------------------------------
+constexpr int kGatherTPB = 1024;
+
+__global__ __launch_bounds__(kGatherTPB) void k_gather_buckets(SliceOut lit, uint32_t n, uint32_t pbits,
+                                                               uint32_t shift, uint32_t nb, uint64_t* bkeys,
+                                                               uint32_t* bcnt, uint32_t* bofs, uint32_t* counters) {
+  // Called by whole waves (`live` false for lanes without a record). Lanes
+  // of a wave holding the same bucket share one atomic: consecutive slice
+  // records mostly fall in one or two buckets.
+  auto put = [&](bool live, const Rec& r) {
+    const uint64_t k = ((uint64_t)r.y << pbits) | r.x;
+    const uint32_t b = (uint32_t)(k >> shift);
+    uint64_t todo = __ballot(live);
+    while (todo) {
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const uint32_t lb = __shfl(b, leader);
+      const uint64_t same = __ballot(live && b == lb) & todo;
+      uint32_t base = 0;
+      if ((int)lane_id() == leader) base = atomicAdd(&bcnt[lb], (uint32_t)__popcll(same));
+      base = __shfl(base, leader);
+      if ((same >> lane_id()) & 1ull) {
+        const uint32_t slot = base + (uint32_t)__popcll(same & ((1ull << lane_id()) - 1ull));
+        if (slot < kLitBucketCap)
+          bkeys[(size_t)b * kLitBucketCap + slot] = k;
+        else
+          atomicOr(&counters[9], 1u);  // a bucket is full: redo on the checked path
+      }
+      todo &= ~same;
+    }
+  };
+  if (blockIdx.x < n) {
+    const uint32_t m = min(lit.counts[blockIdx.x], lit.cap);
+    for (uint32_t k0 = 0; k0 < m; k0 += kGatherTPB) {
+      const uint32_t k = k0 + threadIdx.x;
+      const bool live = k < m;
+      put(live, live ? lit.buf[(size_t)blockIdx.x * lit.cap + k] : make_uint4(0, 0, 0, 0));
+    }
+  } else {
+    const uint32_t no = min(*lit.ovf_count, lit.ovf_cap);
+    for (uint32_t k0 = (blockIdx.x - n) * kGatherTPB; k0 < no; k0 += 64 * kGatherTPB) {
+      const uint32_t k = k0 + threadIdx.x;
+      const bool live = k < no;
+      put(live, live ? lit.ovf[k] : make_uint4(0, 0, 0, 0));
+    }
+  }
+}
+
+// Exclusive scan of the kLitBuckets bucket counts (each clamped to the
+// bucket capacity) into output offsets, and the match total. One block.
+__global__ __launch_bounds__(kGatherTPB) void k_scan_buckets(const uint32_t* bcnt, uint32_t* bofs,
+                                                             uint32_t* counters) {
+  __shared__ uint32_t s_part[kGatherTPB];
+  constexpr uint32_t per = kLitBuckets / kGatherTPB;
+  const uint32_t b0 = threadIdx.x * per;
+  uint32_t v[per];
+#pragma unroll
+  for (uint32_t k = 0; k < per; k += 4) {
+    const uint4 x = *reinterpret_cast<const uint4*>(bcnt + b0 + k);
+    v[k] = min(x.x, kLitBucketCap);
+    v[k + 1] = min(x.y, kLitBucketCap);
+    v[k + 2] = min(x.z, kLitBucketCap);
+    v[k + 3] = min(x.w, kLitBucketCap);
+  }
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < per; ++k) sum += v[k];
+  s_part[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t d = 1; d < kGatherTPB; d <<= 1) {
+    const uint32_t x = threadIdx.x >= d ? s_part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    s_part[threadIdx.x] += x;
+    __syncthreads();
+  }
+  uint32_t run = s_part[threadIdx.x] - sum;
+#pragma unroll
+  for (uint32_t k = 0; k < per; k += 4) {
+    uint4 o;
+    o.x = run; run += v[k];
+    o.y = run; run += v[k + 1];
+    o.z = run; run += v[k + 2];
+    o.w = run; run += v[k + 3];
+    *reinterpret_cast<uint4*>(bofs + b0 + k) = o;
+  }
+  if (threadIdx.x == kGatherTPB - 1) {
+    counters[0] = s_part[threadIdx.x];
+    counters[kCounterLitSnapshot] = s_part[threadIdx.x];
+  }
+}
+
+constexpr int kRankWaves = 4;  // buckets (one per wave) per block of k_rank_pack
+
+__global__ __launch_bounds__(64 * kRankWaves) void k_rank_pack(const uint64_t* bkeys, uint32_t* bcnt,
+                                                               const uint32_t* bofs, uint32_t nb, uint32_t pbits,
+                                                               const uint32_t* flen, uint32_t start_off,
+                                                               MatchOut* out, uint64_t out_cap,
+                                                               const unsigned long long* base_dev) {
+  __shared__ uint64_t s_keys[kRankWaves][kLitBucketCap];
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t b = blockIdx.x * kRankWaves + w;
+  if (b >= nb) return;
+  const uint32_t c = min(bcnt[b], kLitBucketCap);
+  if (c == 0) return;
+  const uint64_t* src = bkeys + (size_t)b * kLitBucketCap;
+  for (uint32_t i = lane; i < c; i += 64) s_keys[w][i] = src[i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint64_t base = *base_dev + bofs[b];
+  const uint64_t pmask = (1ull << pbits) - 1;
+  // Keys are distinct (one match per pattern and start), so a key's rank is
+  // the number of smaller keys.
+  for (uint32_t i = lane; i < c; i += 64) {
+    const uint64_t k = s_keys[w][i];
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < c; ++j) r += s_keys[w][j] < k;
+    if (base + r < out_cap) {
+      MatchOut m;
+      m.pid = (uint32_t)(k & pmask);
+      m.start = (uint32_t)(k >> pbits) + start_off;
+      m.end = m.start + flen[m.pid];
+      out[base + r] = m;
+    }
+  }
+  if (lane == 0) bcnt[b] = 0;
+}
+
+hipError_t launch_lit_order(const SliceOut& lit, uint32_t n, uint32_t pbits, uint32_t shift, uint32_t nb,
+                            uint64_t* bkeys, uint32_t* bcnt, uint32_t* bofs, uint32_t* counters,
+                            const uint32_t* flen, uint32_t start_off, void* out, uint64_t out_cap,
+                            const unsigned long long* base_dev, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_buckets, dim3(n + 64), dim3(kGatherTPB), 0, st, lit, n, pbits, shift, nb, bkeys, bcnt,
+                     bofs, counters);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_buckets, dim3(1), dim3(kGatherTPB), 0, st, bcnt, bofs, counters);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_rank_pack, dim3((nb + kRankWaves - 1) / kRankWaves), dim3(64 * kRankWaves), 0, st, bkeys,
+                     bcnt, bofs, nb, pbits, flen, start_off, reinterpret_cast<MatchOut*>(out), out_cap, base_dev);
+  return hipGetLastError();
+}
+
 // Pairs already counted into cnt (k_slice_gather_keys): scan -> scatter ->
 // per-bucket sort fused with packing into `out` at *base_dev.
 hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,

@@ -138,6 +138,10 @@ struct RunCaps {
   uint32_t lit_cap, cand_cap, raw_cap, rec_cap, max_matches;
 };
 constexpr uint32_t kCounterLitSnapshot = 10;
+// Literal-only ordering: records per bucket of the fixed-capacity buckets,
+// and the most buckets (the count array always has this many entries).
+constexpr uint32_t kLitBucketCap = 256;
+constexpr uint32_t kLitBuckets = 8192;
 // sticky[0]: a buffer overflowed (redo on the checked path); sticky[1]: error
 // bits (1 = regex window truncated, 2 = more than max_matches);
 // sticky[2..4]: running sums of filter hits, candidates, raw regex matches.
