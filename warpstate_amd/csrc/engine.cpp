@@ -172,6 +172,7 @@ struct ws_ctx {
   std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
   uint8_t* hstage = nullptr;    // pinned staging read by captured copies
   size_t hstage_cap = 0;
+  size_t stage_used = 0;        // checked-pipeline calls: bytes of hstage in use
   uint32_t* hstate = nullptr;   // pinned, coherent: the device state block as the last kernel left it
   size_t hstate_cap = 0;
   std::map<GraphKey, GraphEntry> graphs;
@@ -626,6 +627,48 @@ HashParams hash_params(const ws_patterns* p, uint32_t lane_bytes) {
   return hp;
 }
 
+hipError_t ensure_hstate(ws_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->hstate_cap) return hipSuccess;
+  if (ctx->hstate) (void)hipHostFree(ctx->hstate);
+  ctx->hstate = nullptr;
+  ctx->hstate_cap = 0;
+  g_buf_epoch.fetch_add(1);
+  const size_t want = std::max<size_t>(bytes, 4096);
+  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hstate), want, hipHostMallocCoherent);
+  if (e == hipSuccess) ctx->hstate_cap = want;
+  return e;
+}
+
+hipError_t ensure_stage(ws_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->hstage_cap) return hipSuccess;
+  // Copies from the old buffer may still be in flight on the stream.
+  if (ctx->hstage) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return e;
+    (void)hipHostFree(ctx->hstage);
+  }
+  ctx->hstage = nullptr;
+  ctx->hstage_cap = 0;
+  g_buf_epoch.fetch_add(1);
+  const size_t want = std::max<size_t>(bytes, 4096);
+  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hstage), want, hipHostMallocDefault);
+  if (e == hipSuccess) ctx->hstage_cap = want;
+  return e;
+}
+
+// Host-to-device copy through the pinned staging area (pageable copies are
+// slower and vary). ctx->stage_used is the running offset of the call's
+// staged copies (reset when a checked-pipeline call starts).
+hipError_t h2d_staged(ws_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return hipSuccess;
+  const size_t off = (ctx->stage_used + 255) & ~(size_t)255;
+  hipError_t e = ensure_stage(ctx, off + bytes);
+  if (e != hipSuccess) return e;
+  std::memcpy(ctx->hstage + off, src, bytes);
+  ctx->stage_used = off + bytes;
+  return hipMemcpyAsync(dst, ctx->hstage + off, bytes, hipMemcpyHostToDevice, ctx->stream);
+}
+
 ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t mode,
                    const ws_tuning* tuning, Result* res) {
   const PatternsHost& h = p->h;
@@ -637,13 +680,13 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
   const uint32_t unit = variant == WS_VARIANT_HASH ? kTPB * lane_bytes : ac_unit_bytes();
   std::vector<uint2> runs;
   const uint32_t nunits = plan_units(plan, unit, &runs);
+  ctx->stage_used = 0;
   WS_HIP(ctx->items.ensure(plan.items.size() * sizeof(uint2)));
-  WS_HIP(hipMemcpyAsync(ctx->items.p, plan.items.data(), plan.items.size() * sizeof(uint2),
-                        hipMemcpyHostToDevice, st));
+  WS_HIP(h2d_staged(ctx, ctx->items.p, plan.items.data(), plan.items.size() * sizeof(uint2)));
   if (plan.batch && nunits) {
     WS_HIP(ctx->units.ensure((size_t)nunits * sizeof(uint4) + runs.size() * sizeof(uint2)));
     uint2* druns = reinterpret_cast<uint2*>(ctx->units.as<uint4>() + nunits);
-    WS_HIP(hipMemcpyAsync(druns, runs.data(), runs.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+    WS_HIP(h2d_staged(ctx, druns, runs.data(), runs.size() * sizeof(uint2)));
     WS_HIP(launch_expand_units(druns, (uint32_t)runs.size(), ctx->items.as<uint2>(), unit, ctx->units.as<uint4>(),
                                nunits, st));
   }
@@ -861,7 +904,7 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       const uint32_t gbits = bits_for(acc ? acc - 1 : 0);
       if (gbits + res->pbits > 63) return fail(WS_ERR_INPUT_TOO_LARGE, "batch key does not fit 63 bits");
       WS_HIP(ctx->bases.ensure(bases.size() * 8));
-      WS_HIP(hipMemcpyAsync(ctx->bases.p, bases.data(), bases.size() * 8, hipMemcpyHostToDevice, st));
+      WS_HIP(h2d_staged(ctx, ctx->bases.p, bases.data(), bases.size() * 8));
       WS_HIP(launch_order_keys_dense(ctx->recs.as<Rec>(), total, res->pbits, ctx->bases.as<uint64_t>(),
                                      ctx->k0.as<uint64_t>(), ctx->v0.as<uint32_t>(), st));
       res->dense_batch = true;
@@ -944,29 +987,6 @@ ws_status plan_single(ws_ctx* ctx, const uint8_t* data, uint64_t len, uint32_t d
 // run's match count to the output offset.
 // ---------------------------------------------------------------------------
 // Grows the pinned staging area (reallocation makes captured graphs stale).
-hipError_t ensure_hstate(ws_ctx* ctx, size_t bytes) {
-  if (bytes <= ctx->hstate_cap) return hipSuccess;
-  if (ctx->hstate) (void)hipHostFree(ctx->hstate);
-  ctx->hstate = nullptr;
-  ctx->hstate_cap = 0;
-  g_buf_epoch.fetch_add(1);
-  const size_t want = std::max<size_t>(bytes, 4096);
-  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hstate), want, hipHostMallocCoherent);
-  if (e == hipSuccess) ctx->hstate_cap = want;
-  return e;
-}
-
-hipError_t ensure_stage(ws_ctx* ctx, size_t bytes) {
-  if (bytes <= ctx->hstage_cap) return hipSuccess;
-  if (ctx->hstage) (void)hipHostFree(ctx->hstage);
-  ctx->hstage = nullptr;
-  ctx->hstage_cap = 0;
-  g_buf_epoch.fetch_add(1);
-  const size_t want = std::max<size_t>(bytes, 4096);
-  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hstage), want, hipHostMallocDefault);
-  if (e == hipSuccess) ctx->hstage_cap = want;
-  return e;
-}
 
 struct DeviceRunState {
   void* out = nullptr;       // device ws_match array
@@ -2157,7 +2177,7 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
   }
   if (n) {
     WS_HIP(ctx->ids.ensure(ids.size() * 8));
-    WS_HIP(hipMemcpyAsync(ctx->ids.p, ids.data(), ids.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    WS_HIP(h2d_staged(ctx, ctx->ids.p, ids.data(), ids.size() * 8));
     WS_HIP(ctx->outbuf.ensure((size_t)n * sizeof(ws_tagged_match)));
     if (r.dense_batch)
       WS_HIP(launch_pack_tagged_dense(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.pbits,
