@@ -1057,11 +1057,11 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   rs->obits = rs->sbits + rs->pbits;
   rs->cbits = rs->sbits + rs->rbits;
   if (rs->obits > 63 || rs->cbits > 63) return fail(WS_ERR_INPUT_TOO_LARGE, "match key does not fit 63 bits");
-  // Bucket counts follow the capacities up to 2^18 buckets (a few pairs per
+  // Bucket counts follow the capacities up to 2^16 buckets (16 pairs per
   // bucket at a million matches); denser runs fall back through the
   // overflow flag.
-  rs->ob = bucket_bits(rs->obits, std::min(rs->rec_cap, 1u << 17));
-  rs->cb = nrx ? bucket_bits(rs->cbits, std::min(rs->raw_cap, 1u << 17)) : 0;
+  rs->ob = bucket_bits(rs->obits, std::min(rs->rec_cap, 1u << 14));
+  rs->cb = nrx ? bucket_bits(rs->cbits, std::min(rs->raw_cap, 1u << 14)) : 0;
   const uint32_t nb = 1u << std::max(rs->ob, rs->cb);
   WS_HIP(ensure_bucket_counts(ctx, nb));
   if (!nrx && rs->variant == WS_VARIANT_HASH) {
