@@ -322,3 +322,34 @@ def test_overflow_is_an_error_not_a_fallback():
     with pytest.raises(WarpstateError) as e:
         GpuMatcher(ps, context=ctx).scan(b"a" * 1000)
     assert e.value.variant == "MatchBufferOverflow"
+
+
+@pytest.mark.parametrize("chunk", [1 << 20, 96 << 10])
+def test_literal_stream_runs_and_replays(chunk):
+    # A literal-only set over device memory in chunks: 4 runs (segments
+    # passed to the init kernel by value) or 43 runs (more than it takes:
+    # segments copied from pinned staging), each ordered through the
+    # fixed-capacity buckets.
+    # Eager, captured and replayed calls all equal the oracle's whole scan.
+    from warpstate_amd import AutoMatcherConfig, Context
+    from warpstate_amd.device import DeviceBuffer
+
+    lits = W.random_literals(400, 5, 14, W.LIT_ALPHABET, seed=61)
+    data = W.printable_input(4 << 20, 62)
+    W.plant(data, lits, 300, 63)
+    exp = Oracle([(0, l) for l in lits]).scan(data, 0)
+    assert len(exp) > 10_000
+    ctx = Context(0, AutoMatcherConfig())
+    m = GpuMatcher(build([(0, l) for l in lits]), context=ctx)
+    sp = StreamPipeline(m, max(len(l) for l in lits))
+    dev = DeviceBuffer.from_array(data)
+    out = DeviceBuffer(12 * (len(exp) + 64))
+    exp_short = Oracle([(0, l) for l in lits]).scan(data[: len(data) - 7], 0)
+    for k in range(4):
+        n = sp.scan_device(dev.ptr, len(data), out.ptr, len(exp) + 64, chunk_size=chunk)
+        assert n == len(exp), k
+        got = out.download(np.uint8, 12 * n).view(exp.dtype)
+        assert np.array_equal(got, exp), k
+        # a whole scan in between (another graph, other segments)
+        assert np.array_equal(m.scan_device_array(dev.ptr, len(data) - 7), exp_short), k
+    assert ctx.graph_count >= 2

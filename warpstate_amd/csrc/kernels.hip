@@ -1638,8 +1638,8 @@ __global__ void k_bucket_sort_pack(uint64_t* keys, uint32_t* vals, const uint32_
 constexpr int kGatherTPB = 1024;
 
 __global__ __launch_bounds__(kGatherTPB) void k_gather_buckets(SliceOut lit, uint32_t n, uint32_t pbits,
-                                                               uint32_t shift, uint32_t nb, uint64_t* bkeys,
-                                                               uint32_t* bcnt, uint32_t* bofs, uint32_t* counters) {
+                                                               uint32_t shift, uint64_t* bkeys, uint32_t* bcnt,
+                                                               uint32_t* counters) {
   // Called by whole waves (`live` false for lanes without a record). Lanes
   // of a wave holding the same bucket share one atomic: consecutive slice
   // records mostly fall in one or two buckets.
@@ -1765,8 +1765,8 @@ hipError_t launch_lit_order(const SliceOut& lit, uint32_t n, uint32_t pbits, uin
                             uint64_t* bkeys, uint32_t* bcnt, uint32_t* bofs, uint32_t* counters,
                             const uint32_t* flen, uint32_t start_off, void* out, uint64_t out_cap,
                             const unsigned long long* base_dev, hipStream_t st) {
-  hipLaunchKernelGGL(k_gather_buckets, dim3(n + 64), dim3(kGatherTPB), 0, st, lit, n, pbits, shift, nb, bkeys, bcnt,
-                     bofs, counters);
+  hipLaunchKernelGGL(k_gather_buckets, dim3(n + 64), dim3(kGatherTPB), 0, st, lit, n, pbits, shift, bkeys, bcnt,
+                     counters);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_scan_buckets, dim3(1), dim3(kGatherTPB), 0, st, bcnt, bofs, counters);
