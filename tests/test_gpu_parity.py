@@ -353,3 +353,38 @@ def test_literal_stream_runs_and_replays(chunk):
         # a whole scan in between (another graph, other segments)
         assert np.array_equal(m.scan_device_array(dev.ptr, len(data) - 7), exp_short), k
     assert ctx.graph_count >= 2
+
+
+def test_state_block_with_profiling_toggled_and_many_runs():
+    # The call's state block (flags, total, one pair of kernel stamps per
+    # run) grows with the run count when profiling is on; calls alternate
+    # profiling off and on, eager and replayed, with 1 and 64 runs. Every
+    # result equals the oracle and profiled calls report the filter time.
+    from warpstate_amd import AutoMatcherConfig, Context
+    from warpstate_amd.device import DeviceBuffer
+
+    pats, _, specs = W.mixed_set(80, 20, seed=71)
+    rng = np.random.default_rng(72)
+    pieces = [p for k, p in pats if k == 0] + [s.sample(rng) for s in specs]
+    data = W.printable_input(3 << 20, 73)
+    W.plant(data, pieces, 400, 74)
+    exp = Oracle(pats).scan(data, 0)
+    ctx = Context(0, AutoMatcherConfig())
+    m = GpuMatcher(build(pats), context=ctx)
+    sp = StreamPipeline(m, max(s.max_len for s in specs) + 24)
+    dev = DeviceBuffer.from_array(data)
+    out = DeviceBuffer(12 * (len(exp) + 64))
+    for k in range(8):
+        prof = k % 2 == 1
+        ctx.set_profiling(prof)
+        got = m.scan_device_array(dev.ptr, len(data))
+        assert np.array_equal(got, exp), k
+        if prof:
+            assert ctx.last_timings().prefilter_ms > 0, k
+        n = sp.scan_device(dev.ptr, len(data), out.ptr, len(exp) + 64, chunk_size=48 << 10)
+        assert n == len(exp), k
+        assert np.array_equal(out.download(np.uint8, 12 * n).view(exp.dtype), exp), k
+        if prof:
+            assert ctx.last_timings().prefilter_ms > 0, k
+    ctx.set_profiling(False)
+    assert ctx.graph_count >= 4
