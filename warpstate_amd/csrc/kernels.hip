@@ -502,11 +502,12 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
                                                              SliceOut cand) {
   // LDS: the bloom words from address 0 (the kernel has no static LDS, so
   // a word's address is the masked hash itself), the block's two output
-  // counters and its work counter, then one hit ring per wave.
+  // counters, its work counter and its hit count, then one hit ring per wave.
   extern __shared__ uint32_t smem[];
   uint32_t* bloom = smem;
   uint32_t* s_cnt = &smem[h.nwords];
   uint32_t* s_work = &smem[h.nwords + 2];
+  uint32_t* s_hits = &smem[h.nwords + 3];
   uint2* ring = reinterpret_cast<uint2*>(&smem[h.nwords + 4]) + (threadIdx.x / 64) * kHitRing;
   constexpr int NW = PL / 4 + 4;  // window dwords
   constexpr uint32_t kUnit = kTPB * PL;
@@ -537,6 +538,7 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
     s_cnt[0] = 0;
     s_cnt[1] = 0;
     *s_work = 0;
+    *s_hits = 0;
   }
   __syncthreads();
 
@@ -697,11 +699,15 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
     verify_queued<K, CI, RX>(c, h, ring, tail - pend, n, s_cnt, lit, cand, seg0);
     pend -= n;
   }
-  if (lane == 0 && nhits) atomicAdd(&c.counters[8], nhits);  // filter hits, for timings
+  // Filter hits (for timings): one device atomic per block, not per wave;
+  // same-address atomics from thousands of waves serialise at the end of a
+  // short launch.
+  if (lane == 0 && nhits) atomicAdd(s_hits, nhits);
   __syncthreads();
   if (threadIdx.x == 0) {
     lit.counts[blockIdx.x] = s_cnt[0];
     cand.counts[blockIdx.x] = s_cnt[1];
+    if (*s_hits) atomicAdd(&c.counters[8], *s_hits);
   }
   stamp_end(c);
 }
