@@ -948,30 +948,55 @@ __global__ __launch_bounds__(kTPB) void k_ac_scan(ScanCommon c, AcParams a) {
   stamp_end(c);
 }
 
+// Block-wide reservation of `want` slots per thread from *counter: every
+// thread of the block calls it (block-uniform control flow) and gets the
+// first of its slots. One device atomic per block: same-address device
+// atomics serialise, and one per wave costs microseconds per launch.
+// s_res: two LDS words of the caller's, reused call after call (the reset
+// of the total follows the last barrier every thread has passed).
+__device__ __forceinline__ uint32_t block_reserve(uint32_t want, uint32_t* counter, uint32_t* s_res) {
+  if (threadIdx.x == 0) s_res[0] = 0;
+  __syncthreads();
+  const uint32_t local = want ? atomicAdd(&s_res[0], want) : 0u;
+  __syncthreads();
+  if (threadIdx.x == 0) s_res[1] = s_res[0] ? atomicAdd(counter, s_res[0]) : 0u;
+  __syncthreads();
+  return s_res[1] + local;
+}
+
 // ---------------------------------------------------------------------------
 // Anchored leftmost-first walk: one lane per candidate start.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kTPB) void k_rx_walk(WalkParams w) {
+  __shared__ uint32_t s_res[2];
   const uint32_t n = min(*w.ncand, w.cand_cap);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const Rec cd = w.cands[i];
-    const uint32_t rx = cd.x, item = cd.w;
-    const uint2 seg = w.items[item];
-    const uint32_t* T = w.tables + w.toff[rx];
-    const uint32_t C = w.ncls[rx];
-    const uint8_t* cm = w.cmaps + 256u * rx;
-    uint32_t s = 1, last = 0xFFFFFFFFu;
-    uint64_t p = (uint64_t)seg.x + cd.y;
-    bool dead = false;
-    for (; p < seg.y; ++p) {
-      const uint32_t e = T[s * C + cm[w.base[p]]];
-      if (e & DFA_DEAD) { dead = true; break; }
-      s = e & DFA_STATE_MASK;
-      if (e & DFA_MATCH) last = (uint32_t)(p + 1 - seg.x);
+  // Block-uniform loop: every lane reaches the slot reservation.
+  for (uint32_t b0 = blockIdx.x * kTPB; b0 < n; b0 += gridDim.x * kTPB) {
+    const uint32_t i = b0 + threadIdx.x;
+    Rec cd = make_uint4(0, 0, 0, 0);
+    uint32_t last = 0xFFFFFFFFu;
+    if (i < n) {
+      cd = w.cands[i];
+      const uint32_t rx = cd.x;
+      const uint2 seg = w.items[cd.w];
+      const uint32_t* T = w.tables + w.toff[rx];
+      const uint32_t C = w.ncls[rx];
+      const uint8_t* cm = w.cmaps + 256u * rx;
+      uint32_t s = 1;
+      uint64_t p = (uint64_t)seg.x + cd.y;
+      bool dead = false;
+      for (; p < seg.y; ++p) {
+        const uint32_t e = T[s * C + cm[w.base[p]]];
+        if (e & DFA_DEAD) { dead = true; break; }
+        s = e & DFA_STATE_MASK;
+        if (e & DFA_MATCH) last = (uint32_t)(p + 1 - seg.x);
+      }
+      if (!dead && w.window_truncated) atomicOr(&w.counters[3], kFlagTruncatedWalk);
     }
-    if (!dead && w.window_truncated) atomicOr(&w.counters[3], kFlagTruncatedWalk);
-    if (last != 0xFFFFFFFFu) {
-      const uint32_t slot = atomicAdd(&w.counters[2], 1u);
+    const bool found = last != 0xFFFFFFFFu;
+    const uint32_t slot = block_reserve(found ? 1u : 0u, &w.counters[2], s_res);
+    if (found) {
+      const uint32_t rx = cd.x, item = cd.w;
       if (slot < w.raw_cap) {
         w.raw[slot] = make_uint4(rx, cd.y, last, item);
         if (w.ckeys) {
@@ -1037,16 +1062,22 @@ __global__ void k_chain_heads(const uint64_t* keys, const uint32_t* vals, const 
 // With `ties`, records sharing the last kept (start, end) are kept too
 // (HashScanner reports every id of the shortest length at a position).
 // Output pattern id is rx_pid[rec.x] when rx_pid is set, else rec.x.
-__global__ void k_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw,
-                             uint32_t n, const uint32_t* nd, uint32_t gshift, const uint8_t* head,
-                             const uint32_t* carried, uint32_t seg_off, const uint32_t* rx_pid,
-                             uint32_t ties, Rec* out, uint32_t out_cap, uint32_t* out_count,
-                             uint32_t* carried_out, OrderOut oo) {
+// A head lane walks its chain twice: once to count the records it keeps,
+// then, after the block has reserved all its slots at once, to write them.
+// Launched with kTPB threads per block.
+__global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw,
+                                                     uint32_t n, const uint32_t* nd, uint32_t gshift,
+                                                     const uint8_t* head, const uint32_t* carried, uint32_t seg_off,
+                                                     const uint32_t* rx_pid, uint32_t ties, Rec* out,
+                                                     uint32_t out_cap, uint32_t* out_count, uint32_t* carried_out,
+                                                     OrderOut oo) {
+  __shared__ uint32_t s_res[2];
   n = count_of(n, nd);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (!head[i]) continue;
+  // The chain from head i: with `emit`, its kept records go to slots
+  // slot, slot + 1, ...; returns how many it keeps.
+  auto walk = [&](uint32_t i, bool emit, uint32_t slot) -> uint32_t {
     const uint64_t g = group_of(keys[i], gshift);
-    uint32_t last_end = 0, last_start = 0xFFFFFFFFu;
+    uint32_t last_end = 0, last_start = 0xFFFFFFFFu, kept = 0;
     if (carried) {
       const uint32_t cv = carried[raw[vals[i]].x];
       last_end = cv > seg_off ? cv - seg_off : 0u;
@@ -1058,19 +1089,31 @@ __global__ void k_chain_walk(const uint64_t* keys, const uint32_t* vals, const R
       if (r.y < last_end && !tie) continue;
       last_end = r.z;
       last_start = r.y;
-      const uint32_t slot = atomicAdd(out_count, 1u);
-      const uint32_t pid = rx_pid ? rx_pid[r.x] : r.x;
-      if (slot < out_cap) {
-        out[slot] = make_uint4(pid, r.y, r.z, r.w);
-        if (oo.keys) {
-          const uint64_t k = ((((uint64_t)r.w << oo.sbits) | r.y) << oo.pbits) | pid;
-          oo.keys[slot] = k;
-          oo.vals[slot] = r.z;
-          atomicAdd(&oo.cnt[k >> oo.shift], 1u);
+      if (emit) {
+        const uint32_t sl = slot + kept;
+        const uint32_t pid = rx_pid ? rx_pid[r.x] : r.x;
+        if (sl < out_cap) {
+          out[sl] = make_uint4(pid, r.y, r.z, r.w);
+          if (oo.keys) {
+            const uint64_t k = ((((uint64_t)r.w << oo.sbits) | r.y) << oo.pbits) | pid;
+            oo.keys[sl] = k;
+            oo.vals[sl] = r.z;
+            atomicAdd(&oo.cnt[k >> oo.shift], 1u);
+          }
         }
+        if (carried_out) atomicMax(&carried_out[r.x], seg_off + r.z);
       }
-      if (carried_out) atomicMax(&carried_out[r.x], seg_off + r.z);
+      ++kept;
     }
+    return kept;
+  };
+  // Block-uniform loop: every lane reaches the slot reservation.
+  for (uint32_t b0 = blockIdx.x * kTPB; b0 < n; b0 += gridDim.x * kTPB) {
+    const uint32_t i = b0 + threadIdx.x;
+    const bool on = i < n && head[i];
+    const uint32_t kept = on ? walk(i, false, 0) : 0u;
+    const uint32_t slot = block_reserve(kept, out_count, s_res);
+    if (kept) walk(i, true, slot);
   }
 }
 
