@@ -676,7 +676,18 @@ def scan_batch_device(matcher: GpuMatcher, ids: np.ndarray, ptrs: np.ndarray, le
     view["data"] = ptrs
     view["len"] = lens
     total = int(np.sum(lens))
-    cap = capacity if capacity is not None else max(1024, min(matcher.config.max_matches, total // 64 + 4096))
+    # Output capacity: a quarter above the matcher's previous batch count
+    # once there is one (a bound of one record per 64 input bytes otherwise).
+    # A modest buffer comes from the allocator's reused heap pages, where a
+    # bound-sized one would be fresh pages that the copy faults in.
+    hint = getattr(matcher, "_batch_count_hint", None)
+    bound = max(1024, min(matcher.config.max_matches, total // 64 + 4096))
+    if capacity is not None:
+        cap = capacity
+    elif hint is not None:
+        cap = max(1024, min(bound, hint + hint // 4 + 4096))
+    else:
+        cap = bound
     t = tuning.to_c() if tuning else None
     while True:
         out = np.empty(cap, dtype=N.TAGGED_DTYPE)
@@ -687,6 +698,8 @@ def scan_batch_device(matcher: GpuMatcher, ids: np.ndarray, ptrs: np.ndarray, le
             cap = int(count.value)
             continue
         N.check(st, int(count.value))
+        if capacity is None:
+            matcher._batch_count_hint = int(count.value)
         return out[: count.value]
 
 

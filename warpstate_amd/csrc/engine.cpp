@@ -2117,6 +2117,7 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
   }
   Plan plan;
   plan.batch = true;
+  plan.items.reserve(n_items);
   std::vector<uint64_t> ids;
   ids.reserve(n_items);
   uint64_t max_len = 0;
