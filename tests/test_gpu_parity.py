@@ -388,3 +388,35 @@ def test_state_block_with_profiling_toggled_and_many_runs():
             assert ctx.last_timings().prefilter_ms > 0, k
     ctx.set_profiling(False)
     assert ctx.graph_count >= 4
+
+
+def test_filter_hit_count_is_summed_over_every_block():
+    # The filter sums its hits per block and adds each block's sum once to
+    # the call's counter. Every match start is a hit, the count is the same
+    # on the device-sized and checked pipelines, eager and replayed, and a
+    # scan of twice the input (the same bytes again) counts exactly twice
+    # as many hits.
+    lits, data = W.literal_workload(size=4 << 20, seed=3)
+    m = GpuMatcher(build([(0, p) for p in lits]))
+    ctx = m.ctx
+    ctx.set_profiling(True)
+    try:
+        counts = []
+        for tuning in (None, None, None, Tuning(checked=True)):
+            got = m.scan_array(data.tobytes(), tuning=tuning)
+            counts.append(int(ctx.last_timings().hits))
+        assert len(set(counts)) == 1, counts
+        starts = len(set(int(s) for s in got["start"]))
+        assert starts > 20 and counts[0] >= starts
+        # No literal spans the seam (they are 4-16 bytes over [a-z_], the
+        # input is printable with planted copies), so hits double exactly
+        # when every window of the doubled input is hashed; the seam's
+        # windows are checked by the match count instead.
+        twice = np.concatenate([data, data])
+        got2 = m.scan_array(twice.tobytes())
+        exp2 = Oracle([(0, p) for p in lits]).scan(twice, 0)
+        assert np.array_equal(got2, exp2)
+        h2 = int(ctx.last_timings().hits)
+        assert 2 * counts[0] <= h2 <= 2 * counts[0] + 16
+    finally:
+        ctx.set_profiling(False)
