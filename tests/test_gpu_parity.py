@@ -420,3 +420,37 @@ def test_filter_hit_count_is_summed_over_every_block():
         assert 2 * counts[0] <= h2 <= 2 * counts[0] + 16
     finally:
         ctx.set_profiling(False)
+
+
+def test_repeat_calls_skip_init_only_when_state_is_clean():
+    # A device-sized call whose segments equal the previous call's skips
+    # the init kernel (the previous close kernel reset the state block).
+    # Checked, batch and stream calls in between reuse the counters and the
+    # segment list, so the next call must initialise again. Every result
+    # equals the oracle's, hits included.
+    lits, data = W.literal_workload(size=2 << 20, seed=8)
+    pats = [(0, p) for p in lits]
+    m = GpuMatcher(build(pats))
+    raw = data.tobytes()
+    exp = Oracle(pats).scan(data, 0)
+    half = Oracle(pats).scan(data[: len(raw) // 2], 0)
+    ctx = m.ctx
+    ctx.set_profiling(True)
+    try:
+        hits = set()
+        for step in ["dev", "dev", "dev", "checked", "dev", "batch", "dev", "stream", "dev", "half", "dev", "dev"]:
+            if step == "checked":
+                assert np.array_equal(m.scan_array(raw, tuning=Tuning(checked=True)), exp)
+            elif step == "batch":
+                got = scan_batch(m, [ScanItem(5, raw)])
+                assert len(got) > 20 and all(t.source_id == 5 for t in got)
+            elif step == "stream":
+                assert np.array_equal(StreamPipeline(m, 32).scan_array(raw, chunk_size=700_000), exp)
+            elif step == "half":
+                assert np.array_equal(m.scan_array(raw[: len(raw) // 2]), half)
+            else:
+                assert np.array_equal(m.scan_array(raw), exp)
+                hits.add(int(ctx.last_timings().hits))
+        assert len(hits) == 1
+    finally:
+        ctx.set_profiling(False)

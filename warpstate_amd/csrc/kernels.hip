@@ -1552,6 +1552,9 @@ __global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, 
   // to pinned host memory, so the host reads it without a copy.
   // publish[pwords] is then set to 1: the host may wait on it instead of
   // blocking on the stream.
+  // It then leaves the state block as k_call_init sets it (flags and total
+  // zero, stamp pairs {all ones, 0}), so a next call on the same stream
+  // with the same segments may skip the init kernel.
   if (publish) {
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < pwords; k += blockDim.x) publish[k] = sticky[k];
@@ -1560,6 +1563,9 @@ __global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, 
     if (threadIdx.x == 0) {
       __hip_atomic_store(&publish[pwords], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+    if (threadIdx.x < kStateResetWords) sticky[threadIdx.x] = 0;
+    unsigned long long* stamps = reinterpret_cast<unsigned long long*>(sticky + 16);
+    for (uint32_t k = threadIdx.x; k < (pwords - 16) / 2; k += blockDim.x) stamps[k] = (k & 1) ? 0ull : ~0ull;
   }
 }
 

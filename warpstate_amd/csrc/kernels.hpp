@@ -146,5 +146,8 @@ constexpr uint32_t kLitBuckets = 8192;
 // bits (1 = regex window truncated, 2 = more than max_matches);
 // sticky[2..4]: running sums of filter hits, candidates, raw regex matches.
 enum : uint32_t { kStickyRedo = 0, kStickyError = 1, kStickyHits = 2, kStickyCands = 3, kStickyRaw = 4 };
+// A call's state block: sticky words, the 64-bit match total at words 8-9
+// (both zeroed by k_call_init), kernel stamp pairs from word 16.
+constexpr uint32_t kStateResetWords = 10;
 
 }  // namespace ws
