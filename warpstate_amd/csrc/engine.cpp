@@ -2135,42 +2135,48 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
   WS_HIP(hipSetDevice(ctx->dev));
   ctx->last = ws_timings{};
   // Items are ordered by source id so the ordering key carries the id rank
-  // (batch output order is (source_id, start, pattern_id, end)).
-  std::vector<uint32_t> order(n_items);
-  std::iota(order.begin(), order.end(), 0u);
+  // (batch output order is (source_id, start, pattern_id, end)). One pass
+  // checks the order and, for device items, finds their span; `order` is
+  // built only when the ids are out of order.
   bool sorted = true;
-  for (size_t k = 1; k < n_items && sorted; ++k) sorted = items[k - 1].id <= items[k].id;
+  uintptr_t lo = UINTPTR_MAX, hi = 0;
+  for (size_t k = 0; k < n_items; ++k) {
+    const ws_item& it = items[k];
+    if (k && items[k - 1].id > it.id) sorted = false;
+    if (!it.len) continue;
+    if (!it.data) return fail(WS_ERR_INVALID_ARGUMENT, "item data is null");
+    lo = std::min(lo, reinterpret_cast<uintptr_t>(it.data));
+    hi = std::max(hi, reinterpret_cast<uintptr_t>(it.data) + (uintptr_t)it.len);
+  }
+  std::vector<uint32_t> order;
   if (!sorted) {
     // (id, index) pairs sort contiguously; equal ids keep their input order.
     std::vector<std::pair<uint64_t, uint32_t>> key(n_items);
     for (size_t k = 0; k < n_items; ++k) key[k] = {items[k].id, (uint32_t)k};
     std::sort(key.begin(), key.end());
+    order.resize(n_items);
     for (size_t k = 0; k < n_items; ++k) order[k] = key[k].second;
   }
+  auto item_at = [&](size_t k) -> const ws_item& { return items[sorted ? k : order[k]]; };
   Plan plan;
   plan.batch = true;
-  plan.items.reserve(n_items);
-  std::vector<uint64_t> ids;
-  ids.reserve(n_items);
+  plan.items.resize(n_items);
+  std::vector<uint64_t> ids(n_items);
   uint64_t max_len = 0;
   if (data_mem == WS_MEM_HOST) {
     // Coalesce into the device input buffer, each item 16-byte aligned.
-    std::vector<uint64_t> off(n_items);
     uint64_t tot = 0;
     for (size_t k = 0; k < n_items; ++k) {
-      const ws_item& it = items[order[k]];
-      if (it.len && !it.data) return fail(WS_ERR_INVALID_ARGUMENT, "item data is null");
-      off[k] = tot;
-      tot += (it.len + 15) & ~15ull;
+      const uint64_t len = item_at(k).len;
+      plan.items[k] = make_uint2((uint32_t)tot, (uint32_t)(tot + len));
+      tot += (len + 15) & ~15ull;
+      if (tot > 0xFFFFFFFFull - 16) return fail(WS_ERR_INPUT_TOO_LARGE, "coalesced batch exceeds 4 GiB");
     }
-    if (tot > 0xFFFFFFFFull - 16)
-      return fail(WS_ERR_INPUT_TOO_LARGE, "coalesced batch exceeds 4 GiB");
     std::vector<uint8_t> host(tot, 0);
     for (size_t k = 0; k < n_items; ++k) {
-      const ws_item& it = items[order[k]];
-      if (it.len) std::memcpy(host.data() + off[k], it.data, it.len);
-      plan.items.push_back(make_uint2((uint32_t)off[k], (uint32_t)(off[k] + it.len)));
-      ids.push_back(it.id);
+      const ws_item& it = item_at(k);
+      if (it.len) std::memcpy(host.data() + plan.items[k].x, it.data, it.len);
+      ids[k] = it.id;
       max_len = std::max<uint64_t>(max_len, it.len);
     }
     WS_HIP(ctx->input.ensure(tot + 16));
@@ -2179,23 +2185,15 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
     plan.base = ctx->input.as<uint8_t>();
     plan.load_end = tot;
   } else {
-    uintptr_t lo = UINTPTR_MAX, hi = 0;
-    for (size_t k = 0; k < n_items; ++k) {
-      const ws_item& it = items[k];
-      if (!it.len) continue;
-      if (!it.data) return fail(WS_ERR_INVALID_ARGUMENT, "item data is null");
-      lo = std::min(lo, reinterpret_cast<uintptr_t>(it.data));
-      hi = std::max(hi, reinterpret_cast<uintptr_t>(it.data) + (uintptr_t)it.len);
-    }
     if (lo == UINTPTR_MAX) lo = hi = 0;
     const uintptr_t base = lo & ~(uintptr_t)15;
     if (hi - base > 0xFFFFFFFFull - 16)
       return fail(WS_ERR_INPUT_TOO_LARGE, "device batch items span more than 4 GiB");
     for (size_t k = 0; k < n_items; ++k) {
-      const ws_item& it = items[order[k]];
-      uint64_t o = it.len ? reinterpret_cast<uintptr_t>(it.data) - base : 0;
-      plan.items.push_back(make_uint2((uint32_t)o, (uint32_t)(o + it.len)));
-      ids.push_back(it.id);
+      const ws_item& it = item_at(k);
+      const uint64_t o = it.len ? reinterpret_cast<uintptr_t>(it.data) - base : 0;
+      plan.items[k] = make_uint2((uint32_t)o, (uint32_t)(o + it.len));
+      ids[k] = it.id;
       max_len = std::max<uint64_t>(max_len, it.len);
     }
     plan.base = reinterpret_cast<const uint8_t*>(base);
