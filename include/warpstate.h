@@ -8,7 +8,8 @@
  * Conventions
  *  - All offsets are u32 byte offsets, as in the reference (`Match` is
  *    {u32 pattern_id, u32 start, u32 end}, end exclusive;
- *    core/src/dfa/mod.rs:416-419). One scan covers at most 4 GiB - 16 bytes.
+ *    core/src/dfa/mod.rs:416-419). One scan covers at most 4 GiB - 1 MiB
+ *    (WS_ERR_INPUT_TOO_LARGE beyond).
  *  - Output lists are sorted by (start, pattern_id, end)
  *    (core/src/cpu/mod.rs:67-78); batch output by
  *    (source_id, start, pattern_id, end) (core/src/batch.rs:159-165).

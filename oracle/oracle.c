@@ -1119,3 +1119,83 @@ uint64_t or_scan_overlapping_mt(const or_set* s, const uint8_t* d, uint64_t len,
   free(tid);
   return n;
 }
+
+/* Whole-set scan over `threads` POSIX threads: the literal part as
+ * byte-range shards (mode 0, the halo rule above) or one job (mode 1, whose
+ * reductions run left to right), plus one job per regex (find_iter is per
+ * pattern, core/src/dfa/scan.rs:433-457). Threads take jobs from a shared
+ * counter; the lists are concatenated and sorted canonically. Same result as
+ * or_scan. */
+typedef struct {
+  const or_set* s;
+  const uint8_t* d;
+  uint64_t len;
+  int mode;
+  uint32_t nshard;
+  uint32_t njobs;
+  uint32_t next; /* atomic */
+  mlist* out;    /* one list per job */
+} mt_pool;
+
+static void* pool_run(void* arg) {
+  mt_pool* P = (mt_pool*)arg;
+  for (;;) {
+    uint32_t j = __atomic_fetch_add(&P->next, 1u, __ATOMIC_RELAXED);
+    if (j >= P->njobs) return NULL;
+    if (j < P->nshard) {
+      if (P->mode == 0) {
+        const uint64_t step = (P->len + P->nshard - 1) / P->nshard;
+        shard_job sj;
+        memset(&sj, 0, sizeof sj);
+        sj.s = P->s;
+        sj.d = P->d;
+        sj.len = P->len;
+        sj.lo = step * j < P->len ? step * j : P->len;
+        sj.hi = step * (j + 1) < P->len ? step * (j + 1) : P->len;
+        shard_run(&sj);
+        P->out[j] = sj.out;
+      } else if (P->s->strategy == 2) {
+        lits_multi_memchr(P->s, P->d, P->len, &P->out[j]);
+      } else if (P->s->strategy == 3) {
+        lits_hash_shortest(P->s, P->d, P->len, &P->out[j]);
+      } else if (P->s->strategy == 1) {
+        lits_leftmost_longest(P->s, P->d, P->len, &P->out[j]);
+      }
+    } else {
+      regex_find_all(P->s, j - P->nshard, P->d, P->len, &P->out[j]);
+    }
+  }
+}
+
+uint64_t or_scan_mt(const or_set* s, const uint8_t* d, uint64_t len, int mode, uint32_t threads, or_match* out,
+                    uint64_t cap) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  mt_pool P;
+  memset(&P, 0, sizeof P);
+  P.s = s;
+  P.d = d;
+  P.len = len;
+  P.mode = mode;
+  P.nshard = s->nlit ? (mode == 0 ? threads : 1) : 0;
+  P.njobs = P.nshard + s->nrx;
+  P.out = (mlist*)calloc(P.njobs ? P.njobs : 1, sizeof(mlist));
+  pthread_t* tid = (pthread_t*)calloc(threads, sizeof(pthread_t));
+  for (uint32_t t = 0; t < threads; ++t) pthread_create(&tid[t], NULL, pool_run, &P);
+  for (uint32_t t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+  uint64_t total = 0;
+  for (uint32_t j = 0; j < P.njobs; ++j) total += P.out[j].n;
+  or_match* all = (or_match*)malloc(sizeof(or_match) * (total ? total : 1));
+  uint64_t k = 0;
+  for (uint32_t j = 0; j < P.njobs; ++j) {
+    if (P.out[j].n) memcpy(all + k, P.out[j].m, sizeof(or_match) * P.out[j].n);
+    k += P.out[j].n;
+    free(P.out[j].m);
+  }
+  if (total) qsort(all, total, sizeof(or_match), cmp_canon);
+  for (uint64_t i = 0; i < total && i < cap; ++i) out[i] = all[i];
+  free(all);
+  free(P.out);
+  free(tid);
+  return total;
+}

@@ -66,6 +66,11 @@ uint64_t or_scan(const or_set* s, const uint8_t* data, uint64_t len, int mode, o
 uint64_t or_scan_overlapping_mt(const or_set* s, const uint8_t* data, uint64_t len, uint32_t threads,
                                 or_match* out, uint64_t cap);
 
+/* or_scan (either mode) over `threads` threads: literal byte-range shards
+ * (mode 0) or one literal job (mode 1), and one job per regex. Same result. */
+uint64_t or_scan_mt(const or_set* s, const uint8_t* data, uint64_t len, int mode, uint32_t threads,
+                    or_match* out, uint64_t cap);
+
 /* Leftmost-first find_iter of one regex (core/src/dfa/scan.rs:433-457). */
 uint64_t or_regex_find_iter(const or_set* s, uint32_t regex_index, const uint8_t* data,
                             uint64_t len, or_match* out, uint64_t cap);

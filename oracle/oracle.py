@@ -54,6 +54,8 @@ def lib():
         L.or_scan.argtypes = [c_void_p, c_void_p, c_uint64, c_int, c_void_p, c_uint64]
         L.or_scan_overlapping_mt.restype = c_uint64
         L.or_scan_overlapping_mt.argtypes = [c_void_p, c_void_p, c_uint64, c_uint32, c_void_p, c_uint64]
+        L.or_scan_mt.restype = c_uint64
+        L.or_scan_mt.argtypes = [c_void_p, c_void_p, c_uint64, c_int, c_uint32, c_void_p, c_uint64]
         L.or_regex_find_iter.restype = c_uint64
         L.or_regex_find_iter.argtypes = [c_void_p, c_uint32, c_void_p, c_uint64, c_void_p, c_uint64]
         L.or_regex_match_at.restype = c_int64
@@ -146,6 +148,17 @@ class Oracle:
                 raise ValueError("the threaded oracle scan is literal-only")
             return int(n)
         return int(lib().or_scan(self.h, buf.ctypes.data, len(buf), mode, out.ctypes.data, len(out)))
+
+    def scan_mt(self, data, mode: int = 0, threads: int = 16) -> np.ndarray:
+        """scan() over `threads` threads (literal shards and one job per regex)."""
+        buf = np.frombuffer(_b(data) if not isinstance(data, np.ndarray) else data, dtype=np.uint8)
+        cap = max(1 << 16, len(buf) // 64)
+        while True:
+            out = np.zeros(cap, dtype=MATCH_DTYPE)
+            n = int(lib().or_scan_mt(self.h, buf.ctypes.data, len(buf), mode, threads, out.ctypes.data, cap))
+            if n <= cap:
+                return out[:n]
+            cap = n
 
     def scan_list(self, data, mode: int = 0):
         a = self.scan(data, mode)

@@ -42,14 +42,29 @@ def variants_for(m):
     return [x for x in (VARIANT_HASH, VARIANT_DFA_GLOBAL, VARIANT_DFA_LDS) if v & (1 << x)]
 
 
+def tunings_for(m):
+    """Every kernel variant; the hash filter at each lane width (16, 32 and
+    64 positions per lane: the benchmarks' 16 MiB, 64 MiB-chunk and 1 GiB
+    shapes) on the device-sized pipeline, and at its default on the checked
+    one."""
+    out = []
+    for v in variants_for(m):
+        if v == VARIANT_HASH:
+            out += [Tuning(variant=v, bytes_per_lane=b) for b in (16, 32, 64)]
+            out.append(Tuning(variant=v, checked=True))
+        else:
+            out.append(Tuning(variant=v))
+    return out
+
+
 def check_all_variants(pats, data, mode=0, ci=False):
     ps = build(pats, ci)
     m = GpuMatcher(ps)
     exp = Oracle(pats, ci).scan_list(data, mode)
-    for v in variants_for(m):
-        print(f"variant={v} n={len(data)} pats={len(pats)} mode={mode}", flush=True)
-        got = as_list(m.scan_array(data, mode="standard" if mode else "overlapping", tuning=Tuning(variant=v)))
-        assert_same(got, exp, f"variant {v}")
+    for t in tunings_for(m):
+        print(f"tuning={t} n={len(data)} pats={len(pats)} mode={mode}", flush=True)
+        got = as_list(m.scan_array(data, mode="standard" if mode else "overlapping", tuning=t))
+        assert_same(got, exp, f"tuning {t}")
     return exp
 
 
@@ -150,6 +165,26 @@ def test_stream_pipeline_equals_whole():
         got = StreamPipeline(m, overlap).scan(data, chunk_size=chunk)
         assert_same(got, whole, f"chunk {chunk}")
     assert whole == [tuple(x) for x in Oracle(pats).scan_list(data)]
+
+
+@pytest.mark.parametrize("checked", [False, True])
+def test_stream_chains_spanning_many_blocks(checked):
+    # A frequent bounded regex gives each chunk thousands of chain segments
+    # of one (regex) group, spread over many 256-lane blocks of the chain
+    # walk; the state carried from the previous chunk must read the same in
+    # every segment and in both passes of a lane (count, then emit).
+    rng = np.random.default_rng(77)
+    data = bytearray(W.printable_input(1 << 20, 78).tobytes())
+    for pos in range(0, len(data) - 8, 11):
+        data[pos:pos + 2] = b"q" + bytes([0x30 + int(rng.integers(0, 10))])
+    data = bytes(data)
+    pats = [(1, b"q[0-9][a-z0-9]{0,3}"), (1, b"[0-9]q[0-9]"), (0, b"q1")]
+    m = GpuMatcher(build(pats))
+    exp = [tuple(x) for x in Oracle(pats).scan_list(data)]
+    assert len(exp) > 100_000
+    for chunk in (65_536, 100_003):
+        got = StreamPipeline(m, 16).scan_array(data, chunk_size=chunk, tuning=Tuning(checked=checked))
+        assert_same(as_list(got), exp, f"chunk {chunk} checked={checked}")
 
 
 def test_stream_boundary_kat():

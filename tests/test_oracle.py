@@ -212,6 +212,21 @@ def test_threaded_overlapping_scan_equals_single_thread():
         assert np.array_equal(out[:n], ref), threads
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_threaded_whole_set_scan_equals_single_thread(mode):
+    # or_scan_mt (literal shards or one literal job, one job per regex,
+    # merged and sorted) must equal or_scan on a mixed set in both modes.
+    pats, _, specs = W.mixed_set(60, 25, seed=21)
+    rng = np.random.default_rng(22)
+    data = W.printable_input(300_000, 23)
+    W.plant(data, [p for k, p in pats if k == 0] + [s.sample(rng) for s in specs], 300, 24)
+    o = Oracle(pats)
+    ref = o.scan(data, mode)
+    assert len(ref) > 500
+    for threads in (1, 3, 16):
+        assert np.array_equal(o.scan_mt(data, mode, threads), ref), threads
+
+
 # --- StreamScanner (core/src/stream.rs) -------------------------------------
 
 from oracle.oracle import StreamScannerOracle  # noqa: E402

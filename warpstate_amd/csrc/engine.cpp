@@ -39,7 +39,8 @@ hipError_t launch_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uin
                              uint64_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t launch_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
                               const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all,
-                              uint8_t* head, hipStream_t st);
+                              uint8_t* head, hipStream_t st, const uint32_t* carry_src = nullptr,
+                              uint32_t* carry_dst = nullptr, uint32_t ncarry = 0);
 hipError_t launch_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
                              const uint32_t* nd, uint32_t gshift, const uint8_t* head, const uint32_t* carried,
                              uint32_t seg_off, const uint32_t* rx_pid, uint32_t ties, Rec* out,
@@ -188,6 +189,10 @@ struct ws_ctx {
   std::map<GraphKey, GraphEntry> graphs;
   uint32_t* hcount = nullptr;  // pinned host mirror of the counters
   uint32_t last_hits = 0;      // filter hits of the last hash-variant scan
+  // Stream chunks on the checked pipeline: which half of `carried` holds the
+  // regex chains' state from the earlier chunks (the chain walk reads it and
+  // raises the other half, then the halves swap).
+  uint32_t carry_phase = 0;
 };
 
 struct ws_patterns {
@@ -868,8 +873,12 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       ws_status s2 = sort_into(ctx, nraw, cbits, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
                                ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>());
       if (s2 != WS_OK) return s2;
+      const uint32_t nrx = (uint32_t)h.rx.size();
+      uint32_t* cin = plan.use_carried ? ctx->carried.as<uint32_t>() + (size_t)ctx->carry_phase * nrx : nullptr;
+      uint32_t* cout = plan.use_carried ? ctx->carried.as<uint32_t>() + (size_t)(ctx->carry_phase ^ 1u) * nrx : nullptr;
       WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(),
-                                nraw, nullptr, res->sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st));
+                                nraw, nullptr, res->sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st, cin, cout,
+                                nrx));
       // The chain appends kept matches after the literal matches.
       if ((size_t)nlit + nraw > ctx->recs.cap / sizeof(Rec)) {
         // Grow while keeping the literal matches.
@@ -883,11 +892,10 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       }
       WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 1, nlit, st));
       WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(),
-                               nraw, nullptr, res->sbits, ctx->head.as<uint8_t>(),
-                               plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr,
+                               nraw, nullptr, res->sbits, ctx->head.as<uint8_t>(), cin,
                                plan.seg_off, p->rx_pid, 0u, ctx->recs.as<Rec>(),
-                               (uint32_t)(ctx->recs.cap / sizeof(Rec)), ctx->counters.as<uint32_t>(),
-                               plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, st));
+                               (uint32_t)(ctx->recs.cap / sizeof(Rec)), ctx->counters.as<uint32_t>(), cout, st));
+      if (plan.use_carried) ctx->carry_phase ^= 1u;
       s = read_counters(ctx);
       if (s != WS_OK) return s;
       nlit = ctx->hcount[0];
@@ -977,8 +985,9 @@ bool is_device_ptr(const void* p) {
 
 // Lays out one contiguous input (host or device) as a single-segment plan.
 ws_status plan_single(ws_ctx* ctx, const uint8_t* data, uint64_t len, uint32_t data_mem, Plan* plan) {
-  if (len > 0xFFFFFFFFull - 16)
-    return fail(WS_ERR_INPUT_TOO_LARGE, "scan input is too large (" + std::to_string(len) + " bytes)");
+  if (len > kMaxScanBytes)
+    return fail(WS_ERR_INPUT_TOO_LARGE, "scan input is too large (" + std::to_string(len) + " bytes; at most " +
+                                            std::to_string(kMaxScanBytes) + ")");
   const uint8_t* dptr = data;
   if (data_mem == WS_MEM_HOST) {
     WS_HIP(ctx->input.ensure(((len + 15) & ~15ull) + 16));
@@ -1217,8 +1226,12 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     WS_HIP(bucket_sort_pairs(ctx->tmp.p, &have, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
                              ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), rs.raw_cap, cnt + 2, rs.cbits - rs.cb,
                              1u << rs.cb, ccnt, ctx->bcur.as<uint32_t>(), cnt + 9, st, keyed));
+    // Stream chunks: run k reads half k % 2 of `carried` (the state the
+    // earlier chunks left) and writes the other half.
+    uint32_t* cin = plan.use_carried ? ctx->carried.as<uint32_t>() + (rs.items_off & 1) * nrx : nullptr;
+    uint32_t* cout = plan.use_carried ? ctx->carried.as<uint32_t>() + ((rs.items_off + 1) & 1) * nrx : nullptr;
     WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), rs.raw_cap,
-                              cnt + 2, rs.sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st));
+                              cnt + 2, rs.sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st, cin, cout, nrx));
     OrderOut oo{};
     if (keyed) {
       oo.keys = ctx->k2.as<uint64_t>();
@@ -1229,10 +1242,8 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
       oo.pbits = rs.pbits;
     }
     WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), rs.raw_cap,
-                             cnt + 2, rs.sbits, ctx->head.as<uint8_t>(),
-                             plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, plan.seg_off, p->rx_pid, 0u,
-                             ctx->recs.as<Rec>(), rs.rec_cap, cnt + 0,
-                             plan.use_carried ? ctx->carried.as<uint32_t>() : nullptr, st, keyed ? &oo : nullptr));
+                             cnt + 2, rs.sbits, ctx->head.as<uint8_t>(), cin, plan.seg_off, p->rx_pid, 0u,
+                             ctx->recs.as<Rec>(), rs.rec_cap, cnt + 0, cout, st, keyed ? &oo : nullptr));
   }
   if (ev) WS_HIP(hipEventRecord(ev[3], st));
 
@@ -1962,12 +1973,13 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
                 "standard-mode literal reductions are not chunk-stitched; use overlapping mode");
   *count = 0;
   if (len == 0) return WS_OK;
-  if (len > 0xFFFFFFFFull - 16) return fail(WS_ERR_INPUT_TOO_LARGE, "stream input is too large");
+  if (len > kMaxScanBytes) return fail(WS_ERR_INPUT_TOO_LARGE, "stream input is too large");
   std::lock_guard<std::mutex> lk(ctx->mu);
   WS_HIP(hipSetDevice(ctx->dev));
   ctx->last = ws_timings{};
   const uint32_t nrx = (uint32_t)p->h.rx.size();
-  if (nrx) WS_HIP(ctx->carried.ensure(nrx * sizeof(uint32_t)));
+  // Two halves: the chain state of earlier chunks, and the one being raised.
+  if (nrx) WS_HIP(ctx->carried.ensure(2 * (size_t)nrx * sizeof(uint32_t)));
   if (device_path_ok(mode, tuning)) {
     // Every chunk enqueued back to back, one synchronisation at the end.
     if (nrx) WS_HIP(launch_fill_u32(ctx->carried.as<uint32_t>(), nrx, 0, ctx->stream));
@@ -2010,6 +2022,7 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
     }
   }
   if (nrx) WS_HIP(launch_fill_u32(ctx->carried.as<uint32_t>(), nrx, 0, ctx->stream));
+  ctx->carry_phase = 0;
   uint64_t total = 0;
   for (uint64_t a = 0; a < len; a += chunk_size) {
     const uint64_t b = std::min(len, a + chunk_size);
@@ -2173,7 +2186,7 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
       const uint64_t len = item_at(k).len;
       plan.items[k] = make_uint2((uint32_t)tot, (uint32_t)(tot + len));
       tot += (len + 15) & ~15ull;
-      if (tot > 0xFFFFFFFFull - 16) return fail(WS_ERR_INPUT_TOO_LARGE, "coalesced batch exceeds 4 GiB");
+      if (tot > kMaxScanBytes) return fail(WS_ERR_INPUT_TOO_LARGE, "coalesced batch exceeds 4 GiB - 1 MiB");
     }
     std::vector<uint8_t> host(tot, 0);
     for (size_t k = 0; k < n_items; ++k) {
@@ -2190,8 +2203,8 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
   } else {
     if (lo == UINTPTR_MAX) lo = hi = 0;
     const uintptr_t base = lo & ~(uintptr_t)15;
-    if (hi - base > 0xFFFFFFFFull - 16)
-      return fail(WS_ERR_INPUT_TOO_LARGE, "device batch items span more than 4 GiB");
+    if (hi - base > kMaxScanBytes)
+      return fail(WS_ERR_INPUT_TOO_LARGE, "device batch items span more than 4 GiB - 1 MiB");
     for (size_t k = 0; k < n_items; ++k) {
       const ws_item& it = item_at(k);
       const uint64_t o = it.len ? reinterpret_cast<uintptr_t>(it.data) - base : 0;
@@ -2348,8 +2361,8 @@ ws_status ws_stream_scanner_feed(ws_stream_scanner* sc, const uint8_t* chunk, ui
   *count = 0;
   ws_ctx* ctx = sc->ctx;
   const uint64_t combined = sc->overlap + len;
-  if (combined > 0xFFFFFFFFull)
-    return fail(WS_ERR_INPUT_TOO_LARGE, "stream window of " + std::to_string(combined) + " bytes exceeds 4 GiB");
+  if (combined > kMaxScanBytes)
+    return fail(WS_ERR_INPUT_TOO_LARGE, "stream window of " + std::to_string(combined) + " bytes exceeds 4 GiB - 1 MiB");
   if (sc->processed + len > 0xFFFFFFFFull)
     return fail(WS_ERR_INPUT_TOO_LARGE, "stream offset " + std::to_string(sc->processed + len) + " exceeds 4 GiB");
   const int nxt = 1 - sc->cur;

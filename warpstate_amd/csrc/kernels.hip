@@ -497,6 +497,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 // behind their arithmetic: no hit list goes through HBM and the loop has no
 // block barriers. Literal matches and regex candidate starts go to the
 // block's output slices.
+// The filter's 32-bit bounds at the largest span (16-byte shift included):
+// the rounded segment end and the last half unit's end stay below 2^32.
+static_assert(kMaxScanBytes + 16 + 15 < (1ull << 32), "segment end wraps");
+static_assert(kMaxScanBytes + 16 + (uint64_t)kTPB * 64 + 16 < (1ull << 32), "half-unit bound wraps");
+
 template <int K, bool CI, int PL, bool RX>
 __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashParams h, SliceOut lit,
                                                              SliceOut cand) {
@@ -1033,9 +1038,17 @@ __device__ __forceinline__ uint64_t group_of(uint64_t k, uint32_t gshift) {
   return gshift >= 64 ? 0ull : (k >> gshift);
 }
 
+// With carry_dst set, the kernel also copies the ncarry words of carry_src
+// there (whatever the count): the chain walk that follows reads carry_src,
+// the previous chunk's final value, and raises carry_dst, so no lane sees a
+// value another lane of the same launch has already raised.
 __global__ void k_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw,
                               uint32_t n, const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen,
-                              uint32_t maxlen_all, uint8_t* head) {
+                              uint32_t maxlen_all, uint8_t* head, const uint32_t* carry_src, uint32_t* carry_dst,
+                              uint32_t ncarry) {
+  if (carry_dst)
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ncarry; k += gridDim.x * blockDim.x)
+      carry_dst[k] = carry_src[k];
   n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint64_t g = group_of(keys[i], gshift);
@@ -1064,6 +1077,11 @@ __global__ void k_chain_heads(const uint64_t* keys, const uint32_t* vals, const 
 // Output pattern id is rx_pid[rec.x] when rx_pid is set, else rec.x.
 // A head lane walks its chain twice: once to count the records it keeps,
 // then, after the block has reserved all its slots at once, to write them.
+// Stream chunks: carried[rx] is 1 + the absolute end of the regex's last
+// kept match in earlier chunks (0: none), read-only in this launch;
+// carried_out (a different buffer, holding a copy of carried) is raised to
+// this launch's kept ends. Both passes of a lane therefore see the same
+// carried value.
 // Launched with kTPB threads per block.
 __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw,
                                                      uint32_t n, const uint32_t* nd, uint32_t gshift,
@@ -1080,7 +1098,7 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
     uint32_t last_end = 0, last_start = 0xFFFFFFFFu, kept = 0;
     if (carried) {
       const uint32_t cv = carried[raw[vals[i]].x];
-      last_end = cv > seg_off ? cv - seg_off : 0u;
+      last_end = cv > seg_off + 1u ? cv - 1u - seg_off : 0u;
     }
     for (uint32_t j = i; j < n; ++j) {
       if (j > i && (head[j] || group_of(keys[j], gshift) != g)) break;
@@ -1101,7 +1119,7 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
             atomicAdd(&oo.cnt[k >> oo.shift], 1u);
           }
         }
-        if (carried_out) atomicMax(&carried_out[r.x], seg_off + r.z);
+        if (carried_out) atomicMax(&carried_out[r.x], seg_off + r.z + 1u);
       }
       ++kept;
     }
@@ -1396,9 +1414,10 @@ hipError_t launch_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uin
 
 hipError_t launch_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
                               const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all,
-                              uint8_t* head, hipStream_t st) {
+                              uint8_t* head, hipStream_t st, const uint32_t* carry_src, uint32_t* carry_dst,
+                              uint32_t ncarry) {
   hipLaunchKernelGGL(k_chain_heads, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, keys, vals, raw, n, nd, gshift,
-                     maxlen, maxlen_all, head);
+                     maxlen, maxlen_all, head, carry_src, carry_dst, ncarry);
   return hipGetLastError();
 }
 

@@ -10,6 +10,11 @@ constexpr int kTPB = 256;                 // threads per block for the scan kern
 constexpr uint32_t kHashUnit = kTPB * 16; // bytes per hash-kernel work unit at 16 positions per lane
 constexpr uint32_t kRecCap = 256;         // per-block LDS staging records (each list)
 constexpr int kFilterTPB = 1024;          // threads per block of the streaming hash filter
+// Largest span one scan covers (a single input, a stream, or a batch's items
+// in one allocation): 4 GiB - 1 MiB, so that the filter's 32-bit bounds --
+// a half unit's end (tile base + unit + 16) and a segment's rounded end --
+// cannot wrap.
+constexpr uint64_t kMaxScanBytes = (1ull << 32) - (1ull << 20);
 
 // Match / candidate record. Match: {pattern_id, start, end, item}.
 // Regex candidate: {regex index, start, 0, item}. Offsets are relative to the
