@@ -232,6 +232,11 @@ def main() -> int:
     # and 20 timed k-mer steps (8 ms) the first steps ran 10-15% slower.
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=100)
+    # Untimed steps for a fixed wall time before the warm-up: the GPU's
+    # clocks ramp for about a second under load, so a short run (say 5
+    # warm-up and 20 timed steps, 8 ms of k-mer work) would otherwise time
+    # steps that run 10-15% slower than settled ones.
+    ap.add_argument("--settle-s", type=float, default=3.0)
     ap.add_argument("--workload", default="kmer", choices=["kmer", "literal", "regex", "batch"])
     ap.add_argument("--size", type=int, default=0, help="input bytes (default: the workload's own)")
     ap.add_argument("--variant", default="auto", choices=["auto", "hash", "dfa_global", "dfa_lds"])
@@ -255,6 +260,13 @@ def main() -> int:
 
     w = Workload(args)
     w.ctx.set_profiling(True)
+    settle_steps = 0
+    t_settle = time.perf_counter()
+    while time.perf_counter() - t_settle < args.settle_s:
+        w.step()
+        settle_steps += 1
+    settle_s = time.perf_counter() - t_settle
+    log(f"clock settle: {settle_steps} untimed steps in {settle_s:.2f}s")
     for _ in range(args.warmup):
         count = w.step()
     synchronize()
@@ -323,6 +335,8 @@ def main() -> int:
         "cpu_baseline": cpu,
         "pct_hbm_peak_whole_step": round(100 * value / HBM_PEAK_GBS, 2),
         "graphs_replayed": w.ctx.graph_count,
+        "settle_s": round(settle_s, 2),
+        "settle_steps": settle_steps,
         "step_breakdown_ms": breakdown,
     }
     line["compiled_index"] = index_timing(w)
