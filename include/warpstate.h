@@ -208,6 +208,22 @@ ws_status ws_scan(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t* data,
                   uint32_t data_mem, uint32_t mode, const ws_tuning* tuning, ws_match* out,
                   uint64_t cap, uint32_t out_mem, uint64_t* count);
 
+/* ---- allocation-free scan. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). ws_scan in overlapping mode for
+ * device-resident data and a device-resident out array, with every workspace
+ * in one fixed array per GPU that the library sets up once per device and
+ * keeps until the process ends (20 MiB); the call itself allocates nothing.
+ * Limits: len <= 16 MiB (WS_ERR_INPUT_TOO_LARGE beyond); at most 65,536
+ * literal matches and 65,536 regex candidates per call; an overflow of these
+ * fixed buffers (also: matches too clustered for its buckets) is
+ * WS_ERR_MATCH_BUFFER_OVERFLOW with *count = 0 and no result (ws_scan then
+ * gives one). A full `out` is WS_ERR_MATCH_BUFFER_OVERFLOW with the required
+ * count. Every context on a GPU shares that GPU's array: calls that overlap
+ * on one GPU race and must be serialised by the caller. */
+ws_status ws_scan_noalloc(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t* data, uint64_t len,
+                          const ws_tuning* tuning, ws_match* out, uint64_t cap, uint64_t* count);
+
 /* ---- batch: batch::scan_batch / scan_batch_gpu / scan_batch_cpu (core/src/batch.rs:170-363).
  * Every item is scanned on its own (no match crosses items). Item data is host memory
  * when data_mem is WS_MEM_HOST, else device pointers. One persistent launch. */

@@ -159,6 +159,10 @@ SIGNATURES = {
         [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, POINTER(WsTuning), c_void_p, c_uint64,
          c_uint32, POINTER(c_uint64)],
     ),
+    "ws_scan_noalloc": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_uint64, POINTER(WsTuning), c_void_p, c_uint64, POINTER(c_uint64)],
+    ),
     "ws_scan_batch": (
         c_int,
         [c_void_p, c_void_p, POINTER(WsItem), c_size_t, c_uint32, c_uint32, POINTER(WsTuning), c_void_p,

@@ -110,11 +110,17 @@ uint32_t bits_for(uint64_t v) { return v == 0 ? 1u : 64u - (uint32_t)__builtin_c
 std::atomic<uint64_t> g_buf_epoch{1};
 std::atomic<uint64_t> g_pattern_serial{1};
 
+// Set while an allocation-free call runs on this thread: workspaces are then
+// views into the static scratch array, and a request beyond a view's size is
+// an error instead of a reallocation.
+thread_local bool g_noalloc = false;
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
   hipError_t ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
+    if (g_noalloc) return hipErrorOutOfMemory;
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -1039,8 +1045,15 @@ struct RunSetup {
   uint32_t publish_words = 0;
 };
 
+// Fixed capacities of an allocation-free call (instead of capacities that
+// follow the input size).
+struct RunLimits {
+  uint32_t lit_cap, cand_cap;
+  uint64_t slice_records;  // records of each output-slice buffer (slices + overflow area)
+};
+
 ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const ws_tuning* tuning,
-                      RunSetup* rs) {
+                      RunSetup* rs, const RunLimits* lim = nullptr) {
   const PatternsHost& h = p->h;
   if (plan.batch) return fail(WS_ERR_INVALID_ARGUMENT, "device-sized runs take single segments");
   rs->variant = pick_variant(p, tuning);
@@ -1054,6 +1067,10 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   rs->max_matches = (uint32_t)std::min<uint64_t>(ctx->cfg.max_matches, 0xFFFFFFF0ull);
   rs->lit_cap = (uint32_t)std::min<uint64_t>(rs->max_matches, std::max<uint64_t>(1u << 16, span / 16));
   rs->cand_cap = (uint32_t)std::min<uint64_t>(0x0FFFFFFFull, std::max<uint64_t>(1u << 16, span / 64));
+  if (lim) {
+    rs->lit_cap = std::min(rs->max_matches, lim->lit_cap);
+    rs->cand_cap = lim->cand_cap;
+  }
   const uint32_t nrx = (uint32_t)h.rx.size();
   rs->raw_cap = nrx ? rs->cand_cap : 0;
   rs->rec_cap = rs->lit_cap + rs->raw_cap;
@@ -1071,8 +1088,9 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   // capacity an earlier (checked) scan grew them to.
   rs->nvb = std::max(1u, std::min((rs->nunits + 7) / 8, ctx->cus * (per_cu > 4 ? 2u : 1u)));
   if (rs->variant == WS_VARIANT_HASH) {
-    WS_HIP(ctx->lslices.ensure(slice_records(rs->nvb) * sizeof(Rec)));
-    WS_HIP(ctx->cslices.ensure(slice_records(rs->nvb) * sizeof(Rec)));
+    const uint64_t srec = lim ? lim->slice_records : slice_records(rs->nvb);
+    WS_HIP(ctx->lslices.ensure(srec * sizeof(Rec)));
+    WS_HIP(ctx->cslices.ensure(srec * sizeof(Rec)));
     WS_HIP(ctx->scounts.ensure(((size_t)rs->nvb * 2 + 4) * 4 * 2));
     rs->lslice_total = ctx->lslices.cap / sizeof(Rec);
     rs->cslice_total = ctx->cslices.cap / sizeof(Rec);
@@ -1605,6 +1623,110 @@ uint32_t index_version(uint32_t index_flags) {
   return (index_flags & WS_INDEX_REFERENCE_V3) ? kIndexVersionRef : kIndexVersion;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free scans (ws_scan_noalloc). Every workspace of one device-sized
+// run is a fixed slice of `scratch`, one array per device for the life of the
+// process, shared by every context on that device: calls that overlap on one
+// GPU race on it. Capacities are fixed: inputs up to kNoAllocMaxBytes, up to
+// kNoAllocRecords literal matches and as many regex candidates per call.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNoAllocMaxBytes = 16ull << 20;
+constexpr uint32_t kNoAllocRecords = 1u << 16;
+constexpr uint64_t kNoAllocSliceRecords = 1u << 17;  // filter output slices (each list), overflow area included
+constexpr size_t kNoAllocTmpBytes = 1u << 20;        // the bucket sorts' scan storage
+constexpr uint32_t kNoAllocBuckets = 1u << 16;       // sort buckets: bucket_bits() of capacities <= 2^14 is <= 16
+constexpr uint32_t kNoAllocLitBucketBits = 10;       // literal-only buckets at lit_cap = 2^16 (17 - 7 bits)
+
+// Byte offsets of the workspaces inside `scratch` (each 256-byte aligned).
+struct NoAllocLayout {
+  size_t recs, cands, raw, k0, k1, k2, v0, v1, v2, head, lslices, cslices, scounts, counters, items, dstate,
+      lkeys, counts, lcnt, bcnt2, lofs, bcur, tmp, total;
+};
+constexpr size_t na_align(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr NoAllocLayout no_alloc_layout() {
+  NoAllocLayout L{};
+  size_t o = 0;
+  auto take = [&o](size_t bytes) {
+    const size_t at = o;
+    o = na_align(o + bytes);
+    return at;
+  };
+  const size_t rec_cap = 2ull * kNoAllocRecords;  // literal matches + regex matches
+  L.recs = take(rec_cap * sizeof(Rec));
+  L.cands = take(kNoAllocRecords * sizeof(Rec));
+  L.raw = take(kNoAllocRecords * sizeof(Rec));
+  L.k0 = take(rec_cap * 8);
+  L.k1 = take(rec_cap * 8);
+  L.k2 = take(rec_cap * 8);
+  L.v0 = take(rec_cap * 4);
+  L.v1 = take(rec_cap * 4);
+  L.v2 = take(rec_cap * 4);
+  L.head = take(kNoAllocRecords);
+  L.lslices = take(kNoAllocSliceRecords * sizeof(Rec));
+  L.cslices = take(kNoAllocSliceRecords * sizeof(Rec));
+  L.scounts = take(64 * 1024);
+  L.counters = take(64 * 4);
+  L.items = take(256);
+  L.dstate = take(256);
+  L.lkeys = take((size_t(1) << kNoAllocLitBucketBits) * kLitBucketCap * 8);
+  // The three count arrays that must start at zero, back to back: one fill.
+  L.counts = o;
+  L.lcnt = take((size_t)kLitBuckets * 4);
+  L.bcnt2 = take((size_t)kNoAllocBuckets * 4);
+  const size_t bcnt = take((size_t)kNoAllocBuckets * 4);
+  (void)bcnt;
+  L.lofs = take((size_t)kLitBuckets * 4);
+  L.bcur = take((size_t)kNoAllocBuckets * 4);
+  L.tmp = take(kNoAllocTmpBytes);
+  L.total = o;
+  return L;
+}
+constexpr NoAllocLayout kNoAllocLayout = no_alloc_layout();
+constexpr size_t kScratchBytes = 20ull << 20;
+static_assert(kNoAllocLayout.total <= kScratchBytes, "allocation-free workspaces exceed the scratch array");
+
+}  // namespace
+
+// The allocation-free path's workspaces (device memory of every GPU the
+// library runs on, held until the process ends). In host code `scratch` is
+// only a stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, once per device (scratch_init), and only that address
+// is passed to kernels and copies.
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+
+// With `dev` current: looks up this device's copy of `scratch` once.
+ws_status scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(WS_ERR_INVALID_ARGUMENT, "device ordinal above 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return WS_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(WS_ERR_GPU_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = reinterpret_cast<uint8_t*>(ptr);
+  return WS_OK;
+}
+
+// Points the context's workspaces at slices of the device's scratch array
+// for one call, and restores them afterwards (the context mutex is held).
+struct ScratchViews {
+  std::vector<std::pair<DevBuf*, DevBuf>> saved;
+  void view(DevBuf& b, uint8_t* base, size_t off, size_t bytes) {
+    saved.push_back({&b, b});
+    b.p = base + off;
+    b.cap = bytes;
+  }
+  ~ScratchViews() {
+    for (auto it = saved.rbegin(); it != saved.rend(); ++it) *it->first = it->second;
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -1663,6 +1785,14 @@ ws_status ws_ctx_create(int device_ordinal, const ws_config* cfg, ws_ctx** out) 
     e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_ordinal);
     if (e != hipSuccess) return bail("hipDeviceGetAttribute(wall clock rate)", e);
     ctx->wall_khz = khz > 0 ? (uint32_t)khz : 100000u;
+  }
+  {
+    ws_status ss = scratch_init(device_ordinal);
+    if (ss != WS_OK) {
+      const std::string m = g_err;
+      delete ctx;
+      return fail(ss, m);
+    }
   }
   e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hcount), 32 * sizeof(uint32_t), hipHostMallocDefault);
   if (e != hipSuccess) return bail("hipHostMalloc", e);
@@ -1954,6 +2084,109 @@ ws_status ws_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64
   if (r.count > cap)
     return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(r.count) +
                                                   " exceeds " + std::to_string(cap) + ")");
+  return WS_OK;
+}
+
+ws_status ws_scan_noalloc(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64_t len,
+                          const ws_tuning* tuning, ws_match* out, uint64_t cap, uint64_t* count) {
+  if (!ctx || !p || !count) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (p->dev != ctx->dev) return fail(WS_ERR_INVALID_ARGUMENT, "pattern set was compiled for another device");
+  if (len && !data) return fail(WS_ERR_INVALID_ARGUMENT, "data is null");
+  if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
+  *count = 0;
+  if (len > kNoAllocMaxBytes)
+    return fail(WS_ERR_INPUT_TOO_LARGE, "allocation-free scan input of " + std::to_string(len) +
+                                            " bytes exceeds its maximum of " + std::to_string(kNoAllocMaxBytes) +
+                                            "; use ws_scan");
+  if (tuning && (tuning->flags & WS_TUNING_CHECKED))
+    return fail(WS_ERR_INVALID_ARGUMENT, "the allocation-free scan has no checked pipeline");
+  if (len == 0) return WS_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  WS_HIP(hipSetDevice(ctx->dev));
+  uint8_t* base = g_scratch_ptr[ctx->dev];
+  if (!base) return fail(WS_ERR_GPU_DEVICE, "scratch array not set up for this device");
+  const hipStream_t st = ctx->stream;
+  Plan plan;
+  ws_status s = plan_single(ctx, data, len, WS_MEM_DEVICE, &plan);
+  if (s != WS_OK) return s;
+  ctx->init_clean = false;
+  const NoAllocLayout& L = kNoAllocLayout;
+  {
+    ScratchViews sv;
+    sv.view(ctx->recs, base, L.recs, L.cands - L.recs);
+    sv.view(ctx->cands, base, L.cands, L.raw - L.cands);
+    sv.view(ctx->raw, base, L.raw, L.k0 - L.raw);
+    sv.view(ctx->k0, base, L.k0, L.k1 - L.k0);
+    sv.view(ctx->k1, base, L.k1, L.k2 - L.k1);
+    sv.view(ctx->k2, base, L.k2, L.v0 - L.k2);
+    sv.view(ctx->v0, base, L.v0, L.v1 - L.v0);
+    sv.view(ctx->v1, base, L.v1, L.v2 - L.v1);
+    sv.view(ctx->v2, base, L.v2, L.head - L.v2);
+    sv.view(ctx->head, base, L.head, L.lslices - L.head);
+    sv.view(ctx->lslices, base, L.lslices, L.cslices - L.lslices);
+    sv.view(ctx->cslices, base, L.cslices, L.scounts - L.cslices);
+    sv.view(ctx->scounts, base, L.scounts, L.counters - L.scounts);
+    sv.view(ctx->counters, base, L.counters, L.items - L.counters);
+    sv.view(ctx->items, base, L.items, L.dstate - L.items);
+    sv.view(ctx->dstate, base, L.dstate, L.lkeys - L.dstate);
+    sv.view(ctx->lkeys, base, L.lkeys, L.lcnt - L.lkeys);
+    sv.view(ctx->lcnt, base, L.lcnt, L.bcnt2 - L.lcnt);
+    sv.view(ctx->bcnt2, base, L.bcnt2, (size_t)kNoAllocBuckets * 4);
+    sv.view(ctx->bcnt, base, L.bcnt2 + (size_t)kNoAllocBuckets * 4, (size_t)kNoAllocBuckets * 4);
+    sv.view(ctx->lofs, base, L.lofs, L.bcur - L.lofs);
+    sv.view(ctx->bcur, base, L.bcur, L.tmp - L.bcur);
+    sv.view(ctx->tmp, base, L.tmp, L.total - L.tmp);
+    struct Guard {
+      Guard() { g_noalloc = true; }
+      ~Guard() { g_noalloc = false; }
+    } guard;
+    const RunLimits lim{kNoAllocRecords, kNoAllocRecords, kNoAllocSliceRecords};
+    RunSetup rs;
+    s = prepare_run(ctx, p, plan, tuning, &rs, &lim);
+    if (s != WS_OK) {
+      if (std::string(g_err).find("out of memory") != std::string::npos)
+        return fail(WS_ERR_INPUT_TOO_LARGE, "the allocation-free scan's workspaces are too small for this pattern "
+                                            "set and input; use ws_scan");
+      return s;
+    }
+    // The call's first kernels: the count arrays (earlier calls on this GPU
+    // may have left them otherwise), then the state block, counters and segment.
+    const size_t counts_words = (L.lofs - L.counts) / 4;
+    WS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(base + L.counts), (uint32_t)counts_words, 0, st));
+    CallItems ci{};
+    ci.n = 1;
+    ci.v[0] = plan.items[0];
+    uint32_t* sticky = ctx->dstate.as<uint32_t>();
+    WS_HIP(launch_call_init(sticky, 10, ctx->counters.as<uint32_t>(), kCounterLitSnapshot + 1, nullptr, 0, ci,
+                            ctx->items.as<uint2>(), st));
+    DeviceRunState ds;
+    ds.out = out;
+    ds.out_cap = cap;
+    ds.sticky = sticky;
+    ds.total = reinterpret_cast<unsigned long long*>(sticky + 8);
+    rs.items_off = 0;
+    rs.stamps = nullptr;
+    rs.publish = nullptr;
+    s = launch_run(ctx, p, plan, rs, 0u, ds, nullptr);
+    if (s != WS_OK) return s;
+    uint32_t hs[16] = {};
+    WS_HIP(hipMemcpyAsync(hs, sticky, sizeof hs, hipMemcpyDeviceToHost, st));
+    WS_HIP(hipStreamSynchronize(st));
+    uint64_t tot = 0;
+    std::memcpy(&tot, hs + 8, 8);
+    if (hs[kStickyRedo])
+      return fail(WS_ERR_MATCH_BUFFER_OVERFLOW,
+                  "the allocation-free scan's fixed buffers overflowed (more than " + std::to_string(kNoAllocRecords) +
+                      " matches or regex candidates, or matches too clustered); use ws_scan");
+    *count = tot;
+    if (hs[kStickyError] & 1u)
+      return fail(WS_ERR_INPUT_TOO_LARGE,
+                  "a regex match may extend past the chunk overlap; raise the overlap to the longest possible match");
+    if ((hs[kStickyError] & 2u) || tot > cap)
+      return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(tot) + " exceeds " +
+                                                    std::to_string(std::min<uint64_t>(cap, ctx->cfg.max_matches)) +
+                                                    ")");
+  }
   return WS_OK;
 }
 
