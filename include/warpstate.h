@@ -286,14 +286,16 @@ uint64_t ws_stream_scanner_processed(const ws_stream_scanner* scanner);
  * from the literals and regex sources they store, as to_pattern_set does
  * (core/src/compiled_index/mod.rs:287-372); their regex-automata DFA sections
  * are checked for structure only. Version 3 is written for sets without
- * regexes (a file the reference loads). Version 4 is this library's default:
+ * regexes (a file the reference loads). Version 5 is this library's default:
  * the version-3 sections, then each regex's anchored DFA and the factor
  * automaton, so that loading skips the subset construction and the automaton
- * build. Versions 3 and 4 end in a CRC-32 (IEEE) of the rest. The builders
+ * build; version-4 files (an earlier layout of those sections) are read and
+ * rebuilt from their sources. Versions 3 to 5 end in a CRC-32 (IEEE) of the
+ * rest. The builders
  * write into `out` (host memory): with out == NULL they set *len to the size
  * and return WS_OK; with cap < the size they set *len and return
  * WS_ERR_INVALID_ARGUMENT. Malformed files are WS_ERR_PATTERN_COMPILATION_FAILED. */
-#define WS_INDEX_VERSION 4u
+#define WS_INDEX_VERSION 5u
 enum { WS_INDEX_REFERENCE_V3 = 1u }; /* index_flags: write version 3 */
 
 typedef struct ws_index_info {
@@ -303,7 +305,7 @@ typedef struct ws_index_info {
   uint32_t literal_count;
   uint32_t regex_count;
   uint32_t hash_window_len;
-  uint32_t prebuilt;          /* 1 for version 4: tables load without compilation */
+  uint32_t prebuilt;          /* 1 for version 5: tables load without compilation */
   uint32_t reserved;
 } ws_index_info;
 

@@ -33,7 +33,8 @@ from .oracle import fnv1a_window, prefilter_table
 
 MAGIC = b"WPSIDX01"   # mod.rs:17
 REF_VERSION = 3       # mod.rs:18
-OWN_VERSION = 4       # the library's version (one past the reference's)
+OWN_VERSION = 5       # the library's version
+OWN_VERSIONS = (4, 5)  # the library's layouts (4: an earlier one of its regex sections)
 HASH_WINDOW = 8       # compiler.rs:118
 NO_NAME = 0xFFFFFFFF  # builder.rs:128-140
 
@@ -140,7 +141,7 @@ def parse(data: bytes) -> dict:
     end = len(data)
     if len(data) >= 16:
         v = struct.unpack_from("<I", data, 8)[0]
-        if v in (REF_VERSION, OWN_VERSION):
+        if v == REF_VERSION or v in OWN_VERSIONS:
             stored = struct.unpack_from("<I", data, len(data) - 4)[0]
             if stored != crc32_ieee(data[:-4]):
                 raise IndexFormatError("compiled index integrity check failed (CRC mismatch)")
@@ -212,7 +213,7 @@ def parse(data: bytes) -> dict:
         if st + ln > len(packed):
             raise IndexFormatError("compiled index literal offset points outside the packed byte buffer")
     tail_start = c.off
-    if version < OWN_VERSION:
+    if version < min(OWN_VERSIONS):
         for _ in range(nrx):
             for _ in range(4):
                 c.vec()
@@ -227,7 +228,7 @@ def parse(data: bytes) -> dict:
         "regex_count": nrx, "hash_window_len": window, "packed": packed, "offsets": offsets,
         "names": names, "regexes": regexes, "literal_ids": ids, "prefix_meta": meta,
         "bucket_ranges": ranges, "entries": entries,
-        # Version 4: the library's own sections follow here (not restated).
+        # Versions 4 and 5: the library's own sections follow here (not restated).
         "tail": data[tail_start:end],
     }
 

@@ -133,7 +133,11 @@ static void bs_fold(bset* s) {
     if (bs_has(s, b) || bs_has(s, b - 32)) { bs_set(s, b); bs_set(s, b - 32); }
 }
 
-enum { N_EMPTY, N_SET, N_CAT, N_ALT, N_REP, N_GROUP };
+enum { N_EMPTY, N_SET, N_CAT, N_ALT, N_REP, N_GROUP, N_LOOK };
+/* Zero-width assertions of regex::bytes with multi_line(true) and
+ * unicode(false) (core/src/regex_engine.rs:7-17): ^ and $ at line starts and
+ * ends, \A and \z at the text's, \b and \B on the ASCII word class. */
+enum { L_BOL, L_EOL, L_BOT, L_EOT, L_WORDB, L_NWORDB };
 typedef struct {
   int kind;
   bset set;
@@ -371,7 +375,25 @@ static int parse_atom(parser* ps, int* flag_only) {
     return g;
   }
   if (c == ')') return pfail(ps, "unopened group");
-  if (c == '^' || c == '$') return pfail(ps, "anchors unsupported");
+  if (c == '^' || c == '$') {
+    ps->i++;
+    int x = new_node(ps, N_LOOK);
+    ps->nodes[x].rmin = c == '^' ? L_BOL : L_EOL;
+    return x;
+  }
+  if (c == '\\') {
+    int e = peekc(ps, 1), look = -1;
+    if (e == 'b') look = L_WORDB;
+    else if (e == 'B') look = L_NWORDB;
+    else if (e == 'A') look = L_BOT;
+    else if (e == 'z') look = L_EOT;
+    if (look >= 0) {
+      ps->i += 2;
+      int x = new_node(ps, N_LOOK);
+      ps->nodes[x].rmin = look;
+      return x;
+    }
+  }
   if (c == '*' || c == '+' || c == '?' || c == '{') return pfail(ps, "repetition missing expression");
   int x = new_node(ps, N_SET);
   bset s;
@@ -416,6 +438,8 @@ static int parse_concat(parser* ps) {
     if (fo) continue;
     if (a < 0) return -1;
     int c = peekc(ps, 0), mn = -1, mx = -1;
+    if (ps->nodes[a].kind == N_LOOK && (c == '*' || c == '+' || c == '?' || c == '{'))
+      return pfail(ps, "repetition of an assertion");
     if (c == '*') { mn = 0; mx = -1; ps->i++; }
     else if (c == '+') { mn = 1; mx = -1; ps->i++; }
     else if (c == '?') { mn = 0; mx = 1; ps->i++; }
@@ -523,27 +547,16 @@ static long literal_of(const node* nd, int x, uint8_t* buf, long pos, long cap) 
   return -1;
 }
 
-static long min_len(const node* nd, int x) {
-  const node* n = &nd[x];
-  switch (n->kind) {
-    case N_SET: return 1;
-    case N_CAT: { long s = 0; for (int k = n->kid; k >= 0; k = nd[k].next) s += min_len(nd, k); return s; }
-    case N_ALT: { long m = -1; for (int k = n->kid; k >= 0; k = nd[k].next) { long v = min_len(nd, k); if (m < 0 || v < m) m = v; } return m < 0 ? 0 : m; }
-    case N_REP: return min_len(nd, n->kid) * n->rmin;
-    case N_GROUP: return min_len(nd, n->kid);
-    default: return 0;
-  }
-}
-
 /* Program. */
-enum { I_SET, I_SPLIT, I_JMP, I_MATCH };
+enum { I_SET, I_SPLIT, I_JMP, I_MATCH, I_LOOK };
 typedef struct { int op, x, y, set; } instr;
 typedef struct {
   instr* code;
   int n, cap;
   bset* sets;
   int nsets, scap;
-  bset first; /* bytes that can start a match */
+  bset first;    /* bytes that can start a match */
+  int can_empty; /* a match may be empty (start anywhere, the text end included) */
 } prog;
 
 static int emit(prog* p, int op, int x, int y, int set) {
@@ -572,6 +585,7 @@ static int gen(prog* p, const node* nd, int x) {
   if (p->n > 2000000) return -1;
   switch (n->kind) {
     case N_EMPTY: return 0;
+    case N_LOOK: emit(p, I_LOOK, n->rmin, 0, 0); return 0;
     case N_SET: emit(p, I_SET, 0, 0, add_set(p, &n->set)); return 0;
     case N_GROUP: return gen(p, nd, n->kid);
     case N_CAT:
@@ -639,10 +653,29 @@ static void first_bytes(prog* p) {
     if (in->op == I_SET) bs_or(&p->first, &p->sets[in->set]);
     else if (in->op == I_JMP) st[sp++] = in->x;
     else if (in->op == I_SPLIT) { st[sp++] = in->x; st[sp++] = in->y; }
-    else { bs_not(&p->first); } /* MATCH reachable: any position */
+    else if (in->op == I_LOOK) st[sp++] = pc + 1; /* zero width: taken as satisfiable */
+    else { p->can_empty = 1; } /* MATCH reachable without a byte */
   }
   free(seen);
   free(st);
+}
+
+static int is_word(int b) {
+  return (b >= '0' && b <= '9') || (b >= 'A' && b <= 'Z') || (b >= 'a' && b <= 'z') || b == '_';
+}
+
+/* An assertion at position pos of d[0, len): the bytes on either side
+ * (none at the text's ends) decide it. */
+static int look_holds(int kind, const uint8_t* d, uint64_t len, uint64_t pos) {
+  const int prev = pos > 0 ? d[pos - 1] : -1, next = pos < len ? d[pos] : -1;
+  switch (kind) {
+    case L_BOL: return prev < 0 || prev == '\n';
+    case L_EOL: return next < 0 || next == '\n';
+    case L_BOT: return prev < 0;
+    case L_EOT: return next < 0;
+    case L_WORDB: return (prev >= 0 && is_word(prev)) != (next >= 0 && is_word(next));
+    default: return (prev >= 0 && is_word(prev)) == (next >= 0 && is_word(next));
+  }
 }
 
 /* Depth-first backtracking in priority order. */
@@ -658,7 +691,10 @@ static int64_t bt_run(const prog* p, const uint8_t* d, uint64_t len, uint64_t st
     for (;;) {
       const instr* in = &p->code[t.pc];
       if (in->op == I_MATCH) { res = (int64_t)t.pos; goto done; }
-      if (in->op == I_SET) {
+      if (in->op == I_LOOK) {
+        if (!look_holds(in->x, d, len, t.pos)) break;
+        t.pc++;
+      } else if (in->op == I_SET) {
         if (t.pos >= len || !bs_has(&p->sets[in->set], d[t.pos])) break;
         t.pos++;
         t.pc++;
@@ -821,13 +857,6 @@ or_set* or_compile(const or_pattern* pats, uint32_t n, int case_insensitive, int
       s->lit_pid[s->nlit++] = i;
       free(ps.nodes);
       continue;
-    }
-    if (min_len(ps.nodes, root) == 0) {
-      *status = 6;
-      snprintf(err, err_cap, "regex %u can match the empty string", i);
-      free(ps.nodes);
-      or_free(s);
-      return NULL;
     }
     prog* p = &s->rx[s->nrx];
     if (gen(p, ps.nodes, root) < 0) {
@@ -1004,46 +1033,59 @@ static void lits_hash_shortest(const or_set* s, const uint8_t* d, uint64_t len, 
   free(glen);
 }
 
-/* find_iter into a growable list */
-static void regex_find_all(const or_set* s, uint32_t r, const uint8_t* d, uint64_t len, mlist* out) {
-  const prog* p = &s->rx[r];
-  uint64_t pos = 0;
-  while (pos < len) {
-    int64_t e = -1;
-    uint64_t st;
-    for (st = pos; st < len; ++st) {
-      if (!bs_has(&p->first, d[st])) continue;
-      e = bt_run(p, d, len, st);
-      if (e >= 0) break;
-    }
-    if (e < 0) break;
-    ml_push(out, s->rx_pid[r], st, (uint64_t)e);
-    pos = (uint64_t)e > st ? (uint64_t)e : st + 1;
+/* Leftmost match start at or after `from`: the first st in [from, len] where
+ * the anchored leftmost-first run succeeds (positions whose byte cannot start
+ * a match are skipped; the text end only when a match may be empty). Returns
+ * the start or -1, the end in *e. */
+static int64_t leftmost_from(const prog* p, const uint8_t* d, uint64_t len, uint64_t from, int64_t* e) {
+  for (uint64_t st = from; st <= len; ++st) {
+    if (!p->can_empty && (st == len || !bs_has(&p->first, d[st]))) continue;
+    *e = bt_run(p, d, len, st);
+    if (*e >= 0) return (int64_t)st;
   }
+  return -1;
 }
 
-uint64_t or_regex_find_iter(const or_set* s, uint32_t r, const uint8_t* d, uint64_t len,
-                            or_match* out, uint64_t cap) {
+/* find_iter (core/src/dfa/scan.rs:433-457 calls regex::bytes::Regex::find_iter):
+ * search from the cursor; an empty match that ends where the previous match
+ * ended is skipped by searching again one byte further (the regex crate's
+ * iterator rule for overlapping empty matches); the cursor then moves to the
+ * match end. Returns the count, writing up to cap matches (out may be NULL). */
+static uint64_t find_iter_into(const or_set* s, uint32_t r, const uint8_t* d, uint64_t len, mlist* ml,
+                               or_match* out, uint64_t cap) {
   const prog* p = &s->rx[r];
-  uint64_t pos = 0, k = 0;
-  while (pos < len) {
+  uint64_t pos = 0, k = 0, last_end = 0;
+  int have_last = 0;
+  while (pos <= len) {
     int64_t e = -1;
-    uint64_t st;
-    for (st = pos; st < len; ++st) {
-      if (!bs_has(&p->first, d[st])) continue;
-      e = bt_run(p, d, len, st);
-      if (e >= 0) break;
+    int64_t st = leftmost_from(p, d, len, pos, &e);
+    if (st < 0) break;
+    if (e == st && have_last && (uint64_t)e == last_end) {
+      if (pos + 1 > len) break;
+      st = leftmost_from(p, d, len, pos + 1, &e);
+      if (st < 0) break;
     }
-    if (e < 0) break;
-    if (k < cap) {
+    if (ml) ml_push(ml, s->rx_pid[r], (uint64_t)st, (uint64_t)e);
+    if (out && k < cap) {
       out[k].pattern_id = s->rx_pid[r];
       out[k].start = (uint32_t)st;
       out[k].end = (uint32_t)e;
     }
     ++k;
-    pos = (uint64_t)e > st ? (uint64_t)e : st + 1;
+    pos = (uint64_t)e;
+    last_end = (uint64_t)e;
+    have_last = 1;
   }
   return k;
+}
+
+static void regex_find_all(const or_set* s, uint32_t r, const uint8_t* d, uint64_t len, mlist* out) {
+  find_iter_into(s, r, d, len, out, NULL, 0);
+}
+
+uint64_t or_regex_find_iter(const or_set* s, uint32_t r, const uint8_t* d, uint64_t len,
+                            or_match* out, uint64_t cap) {
+  return find_iter_into(s, r, d, len, NULL, out, cap);
 }
 
 int64_t or_regex_match_at(const or_set* s, uint32_t r, const uint8_t* d, uint64_t len,

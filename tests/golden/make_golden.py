@@ -6,11 +6,17 @@ kats.json — known-answer cases whose expected outputs are stated in the
   reference checkout). Inputs and outputs are data transcribed from those
   assertions.
 regex_vectors.json — leftmost-first find_iter vectors computed with Python's
-  `re` (bytes). For the supported syntax (literal bytes, classes, `.`, bounded
-  and unbounded greedy/lazy repeats, alternation, groups, no anchors) `re`
-  and the reference's regex crate with multi_line(true).unicode(false) both
-  implement backtracking-priority (leftmost-first) semantics with `.`
-  excluding only '\\n'.
+  `re` (bytes, re.MULTILINE). For this syntax (literal bytes, classes, `.`,
+  bounded and unbounded greedy/lazy repeats, alternation, groups, `^` `$`
+  `\\A` `\\z` `\\b` `\\B`) `re` and the reference's regex crate with
+  multi_line(true).unicode(false) both implement backtracking-priority
+  (leftmost-first) matching with `.` excluding only '\\n', multi-line `^`/`$`
+  and the ASCII word class. The two iterate differently over empty matches:
+  the regex crate skips an empty match that ends where the previous match
+  ended and searches again one byte further, where `re.finditer` keeps it
+  after a non-empty match. `rust_find_iter` therefore drives `re`'s search
+  (which sees the text before its start position, as the regex crate's
+  input span does) with the regex crate's rule.
 """
 import json
 import os
@@ -82,7 +88,96 @@ KATS = [
      [(R, "start")], False, "start here", "standard", [(0, 0, 5)]),
     ("single_char_repeated", "core/tests/unit/harness_regex.rs:298-305",
      [(R, "x")], False, "xxxx", "standard", [(0, 0, 1), (0, 1, 2), (0, 2, 3), (0, 3, 4)]),
+    # Anchors (multi-line), word boundaries, unselective first bytes and
+    # empty matches. Where a test asserts a count, the count fixes the list.
+    ("caret_multiline", "core/tests/unit/harness_regex.rs:128-136",
+     [(R, "^abc")], False, "abc\nxabc", "standard", [(0, 0, 3)]),
+    ("dollar_eoi", "core/tests/unit/harness_regex.rs:139-145",
+     [(R, "abc$")], False, "abc", "standard", [(0, 0, 3)]),
+    ("legendary_anchors", "core/tests/unit/test_legendary_suite.rs:749-757",
+     [(R, "^start"), (R, "end$")], False, "start and end", "standard", [(0, 0, 5), (1, 10, 13)]),
+    ("anchors_line_boundaries", "core/tests/adversarial/adversarial_audit_cpu.rs:108-120",
+     [(R, "^start"), (R, "end$")], False, "start middle end", "standard", [(0, 0, 5), (1, 13, 16)]),
+    ("caret_after_newline", "core/tests/adversarial/adversarial_audit_cpu.rs:124-130",
+     [(R, "^start"), (R, "end$")], False, "line\nstart here", "standard", [(0, 5, 10)]),
+    ("negated_class_count", "core/tests/unit/harness_regex.rs:70-74 (count >= 3 of single bytes)",
+     [(R, "[^a-z]")], False, "abc123", "standard", [(0, 3, 4), (0, 4, 5), (0, 5, 6)]),
+    ("negated_class_plus", "core/tests/adversarial/adversarial_jules.rs:326-337",
+     [(R, "[^a-z]+")], False, "abc 123 DEF", "standard", [(0, 3, 11)]),
+    ("anchored_start_yes", "core/tests/adversarial/adversarial_jules.rs:340-345",
+     [(R, "^start")], False, "start of text", "standard", [(0, 0, 5)]),
+    ("anchored_start_no", "core/tests/adversarial/adversarial_jules.rs:346-347",
+     [(R, "^start")], False, "the start of text", "standard", []),
+    ("anchored_end_yes", "core/tests/adversarial/adversarial_jules.rs:350-354",
+     [(R, "end$")], False, "this is the end", "standard", [(0, 12, 15)]),
+    ("anchored_end_no", "core/tests/adversarial/adversarial_jules.rs:355-356",
+     [(R, "end$")], False, "end of the world", "standard", []),
+    ("anchored_exact_yes", "core/tests/adversarial/adversarial_jules.rs:360-362",
+     [(R, "^exact$")], False, "exact", "standard", [(0, 0, 5)]),
+    ("anchored_exact_lead", "core/tests/adversarial/adversarial_jules.rs:363",
+     [(R, "^exact$")], False, " exact", "standard", []),
+    ("anchored_exact_trail", "core/tests/adversarial/adversarial_jules.rs:364",
+     [(R, "^exact$")], False, "exact ", "standard", []),
+    ("word_boundary_yes", "core/tests/adversarial/adversarial_jules.rs:368-371",
+     [(R, r"\bword\b")], False, "a word here", "standard", [(0, 2, 6)]),
+    ("word_boundary_prefix", "core/tests/adversarial/adversarial_jules.rs:372",
+     [(R, r"\bword\b")], False, "aword", "standard", []),
+    ("word_boundary_suffix", "core/tests/adversarial/adversarial_jules.rs:373",
+     [(R, r"\bword\b")], False, "worda", "standard", []),
+    ("caret_alternation", "core/tests/adversarial/adversarial_matching.rs:78-86",
+     [(R, "^(foo|bar)")], False, "foobaz", "standard", [(0, 0, 3)]),
+    ("alternation_dollar", "core/tests/adversarial/adversarial_matching.rs:88-96",
+     [(R, "(foo|bar)$")], False, "bazbar", "standard", [(0, 3, 6)]),
+    ("caret_plus_dollar", "core/tests/adversarial/adversarial_matching.rs:98-106",
+     [(R, "^a+$")], False, "aaaaa", "standard", [(0, 0, 5)]),
+    ("caret_plus_dollar_no", "core/tests/adversarial/adversarial_matching.rs:108-109",
+     [(R, "^a+$")], False, "baaaaa", "standard", []),
+    ("caret_plus_b", "core/tests/adversarial/adversarial_matching.rs:111-119",
+     [(R, "^a+b")], False, "aaabx", "standard", [(0, 0, 4)]),
+    ("b_plus_dollar", "core/tests/adversarial/adversarial_matching.rs:121-129",
+     [(R, "ba+$")], False, "xbaaa", "standard", [(0, 1, 5)]),
+    ("single_byte_anchored", "core/tests/adversarial/adversarial_matching.rs:284-292",
+     [(R, "^[a-z]$")], False, "z", "standard", [(0, 0, 1)]),
+    ("exact_length_anchored", "core/tests/adversarial/adversarial_matching.rs:294-302",
+     [(R, "^12345$")], False, "12345", "standard", [(0, 0, 5)]),
+    ("dot_star_empty_input", "core/tests/adversarial/break_it.rs:22-27",
+     [(R, ".*")], False, "", "standard", [(0, 0, 0)]),
+    ("a_star_empty_input", "core/tests/unit/mass_pattern_samples.rs:866-875",
+     [(R, "a*")], False, "", "standard", [(0, 0, 0)]),
 ]
+
+# Regexes with assertions, empty matches and unselective first bytes (the
+# regex crate's syntax; `\z` is written `\Z` for Python's `re`).
+LOOK_PATTERNS = [
+    "^abc", "abc$", "^[a-z]+$", r"\bfoo\b", r"\Bo\B", r"\b[0-9]+\b", "^", "$", "^$", r"\b", r"\B",
+    "a*", "x?", "q*?", "[^a-z]", "[^a-z]+", ".+foo", "(?s).{2}x", "[^\n]{1,5}end", r"\Aab", r"ab\z",
+    r"\A", r"\z", "(?:ab)*", "[a-z]*[0-9]", r"^\w+", r"\w+$", r"(?s)\b.{0,3}\b", "end$|^start",
+    r"[0-9]*\b", "(?:^|x)y", "o\b|\bf", ".*", r"\s*",
+]
+
+
+def py_pattern(p: str) -> bytes:
+    return p.replace(r"\z", r"\Z").encode()
+
+
+def rust_find_iter(rx, data: bytes):
+    """regex::bytes::Regex::find_iter over `re` searches: search from the
+    cursor; an empty match ending where the previous match ended is dropped
+    and the search repeated one byte further; the cursor moves to the end."""
+    out, pos, last = [], 0, None
+    while pos <= len(data):
+        m = rx.search(data, pos)
+        if not m:
+            break
+        if m.start() == m.end() and last is not None and m.end() == last:
+            if pos + 1 > len(data):
+                break
+            m = rx.search(data, pos + 1)
+            if not m:
+                break
+        out.append((m.start(), m.end()))
+        pos = last = m.end()
+    return out
 
 REGEX_PATTERNS = [
     "[0-9]+", "a[bc]*d", "x.{2,5}?y", "(foo|foobar)baz", "(?i:hello)", "[a-z]{3}[0-9]{1,3}",
@@ -124,6 +219,17 @@ def main() -> None:
         data = bytes(data)
         found = [(m.start(), m.end()) for m in re.finditer(s.source.encode(), data)]
         vectors.append({"pattern": s.source, "input": data.hex(), "expected": found})
+    look_alpha = np.frombuffer(b"abcfooendstartxyq0129_ \n\n-AB", dtype=np.uint8)
+    for i, pat in enumerate(LOOK_PATTERNS):
+        rx = re.compile(py_pattern(pat), re.M)
+        for k, size in enumerate((0, 1, 40, 700)):
+            data = look_alpha[r.integers(0, len(look_alpha), size=size)].tobytes()
+            # CPython's `re` never matches \B in an empty string; the regex
+            # crate's \B holds there (no word byte on either side), so `re`
+            # cannot stand in for that one case.
+            if size == 0 and r"\B" in pat:
+                continue
+            vectors.append({"pattern": pat, "input": data.hex(), "expected": rust_find_iter(rx, data)})
     with open(os.path.join(HERE, "regex_vectors.json"), "w") as f:
         json.dump(vectors, f, indent=0)
     print(f"wrote {len(kats)} KATs and {len(vectors)} regex vectors")

@@ -1,4 +1,4 @@
-"""Compiled pattern index on the device: a set loaded from a file (version 4
+"""Compiled pattern index on the device: a set loaded from a file (version 5
 with its stored tables, version 3 written here, versions 1-3 written by the
 oracle's restatement of the reference builder) scans exactly like the set
 compiled from its patterns, and like the oracle.
@@ -58,7 +58,7 @@ def test_reference_round_trip_cases():
     ix = w.CompiledPatternIndex.load(w.CompiledPatternIndex.build(ps))
     assert ix.scan(b"xxneedlexx") == ps.scan(b"xxneedlexx") == [(0, 2, 8)]
     ps = build([(0, f"pattern_{i:03}".encode(), None) for i in range(100)])
-    for version in (3, 4):
+    for version in (3, 5):
         ix = w.CompiledPatternIndex.load(w.CompiledPatternIndex.build(ps, version=version))
         got = ix.scan(b"xxpattern_042xxpattern_099xx")
         assert got == ps.scan(b"xxpattern_042xxpattern_099xx") == [(42, 2, 13), (99, 15, 26)]
@@ -166,7 +166,7 @@ def test_random_literal_sets_round_trip():
                 for _ in range(rng.randint(1, 50))]
         text = "".join(rng.choice(alpha + " ") for _ in range(rng.randint(0, 1000))).encode()
         ps = build(pats)
-        ix = w.CompiledPatternIndex.load(w.CompiledPatternIndex.build(ps, version=3 if case % 2 else 4))
+        ix = w.CompiledPatternIndex.load(w.CompiledPatternIndex.build(ps, version=3 if case % 2 else 5))
         got = ix.scan(text)
         assert got == ps.scan(text)
         assert [tuple(m) for m in got] == Oracle([(0, p) for _, p, _ in pats]).scan_list(text, 1)

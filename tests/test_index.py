@@ -34,7 +34,7 @@ def c_patterns(pats):
     return arr, bufs
 
 
-def lib_build(pats, ci=False, version=4):
+def lib_build(pats, ci=False, version=5):
     """ws_index_build: size query, then the write."""
     lib = N.load()
     arr, _keep = c_patterns(pats)
@@ -138,7 +138,7 @@ def _corruptions(valid):
     v[8:12] = struct.pack("<I", 0)
     out.append(("version_zero", bytes(v), ("version", "invalid")))
     v = bytearray(valid)
-    v[8:12] = struct.pack("<I", 5)
+    v[8:12] = struct.pack("<I", 6)  # one past the newest version either reader knows
     out.append(("future_version", bytes(v), ("version", "unsupported")))
     v = bytearray(valid)
     v[8:12] = struct.pack("<I", 0xFFFFFFFF)
@@ -225,11 +225,11 @@ def test_v3_refuses_regex_sets():
     assert e.value.variant == "InvalidArgument"
 
 
-def test_v4_shares_the_reference_sections():
+def test_v5_shares_the_reference_sections():
     for pats, ci in ((MIXED, False), (lits("Needle", "hay"), True), (lits(*[f"k{i}" for i in range(50)]), False)):
         mine = lib_build(pats, ci)
         p = F.parse(mine)
-        assert p["version"] == 4
+        assert p["version"] == 5
         assert F.to_pattern_list(p) == pats
         ref = F.build(pats, ci, version=3)
         rp = F.parse(ref)
@@ -274,7 +274,7 @@ def test_reads_own_files_with_the_same_set_facts():
     for pats, ci in ((MIXED, False), (lits("Needle", "hay"), True)):
         st, msg, hdr, info = inspect(lib_build(pats, ci))
         assert st == 0, msg
-        assert hdr["version"] == 4 and hdr["prebuilt"] == 1 and hdr["case_insensitive"] == int(ci)
+        assert hdr["version"] == 5 and hdr["prebuilt"] == 1 and hdr["case_insensitive"] == int(ci)
         assert info == analyze(pats, ci)
 
 
@@ -290,32 +290,38 @@ def test_reader_rejects_reference_corruptions(name, data, keys):
     assert any(k in msg for k in keys), msg
 
 
-def test_v4_corruptions():
+def test_v5_corruptions():
     valid = lib_build(MIXED)
     assert inspect(valid)[0] == 0
     v = bytearray(valid)
-    v[8:12] = struct.pack("<I", 3)  # a version-4 body is not a version-3 file
+    v[8:12] = struct.pack("<I", 3)  # a version-5 body is not a version-3 file
     st, msg, _, _ = inspect(recrc(v))
     assert st == 6
     p = F.parse(valid)
     tail0 = len(valid) - 4 - len(p["tail"])
-    # first regex section: nstates, nclass, start flag, cmap[256], table
+    # first regex section: nstates, nclass, start[4], cmap[256], table
     nst, ncls = struct.unpack_from("<II", valid, tail0)
     for field, value, key in [(0, 1, "state count"), (1, 0, "class count"), (1, 300, "class count"),
-                              (2, 7, "start flag")]:
+                              (2, 0, "start state"), (5, nst, "start state")]:
         v = bytearray(valid)
         v[tail0 + 4 * field: tail0 + 4 * field + 4] = struct.pack("<I", value)
         st, msg, _, _ = inspect(recrc(v))
         assert st == 6 and key in msg, msg
     # a transition to a state that does not exist
-    t0 = tail0 + 12 + 256 + 4
+    t0 = tail0 + 24 + 256 + 4
     v = bytearray(valid)
     v[t0:t0 + 4] = struct.pack("<I", nst)
     st, msg, _, _ = inspect(recrc(v))
     assert st == 6 and "state range" in msg, msg
+    # an end-of-text entry that continues to a state
+    e0 = t0 + 4 * ncls
+    v = bytearray(valid)
+    v[e0:e0 + 4] = struct.pack("<I", 1)
+    st, msg, _, _ = inspect(recrc(v))
+    assert st == 6 and "end-of-text" in msg, msg
     # a byte class outside the class range
     v = bytearray(valid)
-    v[tail0 + 12 + 65] = ncls
+    v[tail0 + 24 + 65] = ncls
     st, msg, _, _ = inspect(recrc(v))
     assert st == 6 and "class range" in msg, msg
     # one regex section too few
@@ -323,6 +329,33 @@ def test_v4_corruptions():
     v[24:28] = struct.pack("<I", 1)
     st, msg, _, _ = inspect(recrc(v))
     assert st == 6
+
+
+def to_version4(v5: bytes) -> bytes:
+    """The same set in the earlier version-4 layout of the library's regex
+    sections (one start flag, no end-of-text column)."""
+    p = F.parse(v5)
+    tail0 = len(v5) - 4 - len(p["tail"])
+    t, off, out = p["tail"], 0, bytearray()
+    for _ in range(p["regex_count"]):
+        nst, ncls = struct.unpack_from("<II", t, off)
+        cmap = t[off + 24: off + 280]
+        n = struct.unpack_from("<I", t, off + 280)[0]
+        table = np.frombuffer(t, dtype="<u4", count=n, offset=off + 284).reshape(nst, ncls + 1)[:, :ncls]
+        out += struct.pack("<III", nst, ncls, 0) + cmap + struct.pack("<I", nst * ncls) + table.astype("<u4").tobytes()
+        off += 284 + 4 * n
+    out += t[off:]  # the factor automaton section
+    body = bytearray(v5[:tail0]) + out
+    body[8:12] = struct.pack("<I", 4)
+    return bytes(body) + struct.pack("<I", zlib.crc32(bytes(body)) & 0xFFFFFFFF)
+
+
+def test_version4_files_are_rebuilt_from_their_sources():
+    v4 = to_version4(lib_build(MIXED))
+    st, msg, hdr, info = inspect(v4)
+    assert st == 0, msg
+    assert hdr["version"] == 4 and hdr["prebuilt"] == 0
+    assert info == analyze(MIXED)
 
 
 def test_every_truncation_is_an_error():
@@ -367,7 +400,7 @@ def test_load_from_file_size_limit(tmp_path):
 
 
 def test_load_is_faster_than_compiling():
-    # The point of version 4: rebuilding from the stored tables skips the
+    # The point of version 5: rebuilding from the stored tables skips the
     # subset construction and the automaton build.
     import time
 

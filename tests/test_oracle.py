@@ -52,6 +52,50 @@ def test_regex_grammar_vs_python_re(seed):
         assert o.find_iter(i, data) == exp, s.source
 
 
+def test_word_boundary_definitions_at_the_text_ends():
+    # The regex crate's ASCII \b / \B: a word byte on exactly one side / on
+    # both or neither side, the text's ends counting as non-word. In an empty
+    # text \B therefore holds at 0 and \b does not (CPython's `re` refuses
+    # \B there, so this case is pinned by the definition).
+    assert Oracle([(1, r"\B")]).find_iter(0, b"") == [(0, 0)]
+    assert Oracle([(1, r"\b")]).find_iter(0, b"") == []
+    assert Oracle([(1, r"\b")]).find_iter(0, b"ab") == [(0, 0), (2, 2)]
+    assert Oracle([(1, r"\B")]).find_iter(0, b"ab") == [(1, 1)]
+    assert Oracle([(1, r"\B")]).find_iter(0, b" ") == [(0, 0), (1, 1)]
+
+
+def _look_regex(r):
+    """A random regex over assertions, classes and (unnested) repeats."""
+    atoms = ["a", "b", "f", "o", "[ab]", "[^ab\n]", ".", r"\w", r"\d", "(?:fo|b)", "x"]
+    looks = ["^", "$", r"\b", r"\B", r"\A", r"\z"]
+    quants = ["", "", "*", "+", "?", "{0,2}", "{1,3}", "*?", "+?"]
+    branches = []
+    for _ in range(int(r.integers(1, 4))):
+        parts = []
+        for _ in range(int(r.integers(1, 5))):
+            if r.random() < 0.3:
+                parts.append(looks[int(r.integers(0, len(looks)))])
+            else:
+                parts.append(atoms[int(r.integers(0, len(atoms)))] + quants[int(r.integers(0, len(quants)))])
+        branches.append("".join(parts))
+    return "|".join(branches)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_look_and_empty_grammar_vs_python_re(seed):
+    # Seeded regexes with assertions and empty-capable repeats against `re`
+    # searches driven by the regex crate's iteration rule.
+    from golden.make_golden import py_pattern, rust_find_iter
+
+    r = np.random.default_rng(900 + seed)
+    alpha = np.frombuffer(b"abfox1 \n_-", dtype=np.uint8)
+    for k in range(60):
+        pat = _look_regex(r)
+        data = alpha[r.integers(0, len(alpha), size=int(r.integers(1, 300)))].tobytes()
+        rx = re.compile(py_pattern(pat), re.M)
+        assert Oracle([(1, pat)]).find_iter(0, data) == rust_find_iter(rx, data), (pat, data)
+
+
 def brute_overlapping(lits, data, ci=False):
     f = (lambda b: b.lower()) if ci else (lambda b: b)
     d = f(data)
@@ -179,7 +223,7 @@ def test_multi_memchr_differs_from_leftmost_longest_where_reference_does():
     assert o2.scan_list(b"baaa", 1) == [(1, 0, 2), (0, 2, 4)]
 
 
-@pytest.mark.parametrize("pat,status", [("(a+)+", 12), ("([0-9]+)*", 12), ("a(", 6), ("a{3", 6), ("a*", 6)])
+@pytest.mark.parametrize("pat,status", [("(a+)+", 12), ("([0-9]+)*", 12), ("a(", 6), ("a{3", 6), ("^*", 6)])
 def test_compile_errors(pat, status):
     with pytest.raises(OracleError) as e:
         Oracle([(1, pat)])

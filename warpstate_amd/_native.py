@@ -110,7 +110,7 @@ class WsTimings(Structure):
     ]
 
 
-INDEX_VERSION = 4
+INDEX_VERSION = 5
 INDEX_REFERENCE_V3 = 1  # ws_index_build / ws_patterns_serialize flag: write version 3
 
 

@@ -291,11 +291,11 @@ class PatternSet:
 
 class CompiledPatternIndex:
     """CompiledPatternIndex (core/src/compiled_index/mod.rs:22-845): a
-    serialized pattern set (WPSIDX01). `build` writes version 4 by default
+    serialized pattern set (WPSIDX01). `build` writes version 5 by default
     (the reference's sections plus the compiled DFA and automaton tables, so
     loading skips compilation) or version 3 (`version=3`, sets without
-    regexes: a file the reference crate loads). `load` reads versions 1-4;
-    files of versions 1-3 are rebuilt from their stored patterns as
+    regexes: a file the reference crate loads). `load` reads versions 1-5;
+    files of versions 1-4 are rebuilt from their stored patterns as
     to_pattern_set does. `scan` runs to_pattern_set().scan on the GPU: the
     reference's index scanner returns PatternSet::scan's results (its
     round-trip tests, core/src/compiled_index/tests.rs:5-128)."""

@@ -33,6 +33,7 @@ size_t ac_shmem_bytes(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCommon& c,
                      const AcParams& a);
 hipError_t launch_walk(uint32_t ncand_host, hipStream_t st, const WalkParams& w);
+hipError_t launch_rx_scan(hipStream_t st, const WalkParams& w);
 hipError_t launch_order_keys(const Rec* recs, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t pbits,
                              uint64_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t launch_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t rbits,
@@ -175,7 +176,8 @@ struct ws_ctx {
   hipEvent_t tev[2] = {};      // around a whole device-sized call (outside any graph)
   uint32_t wall_khz = 0;       // device wall-clock rate (kernel stamps)
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
-      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, k2, v2, bcnt2, bases, lkeys, lcnt, lofs;
+      scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, k2, v2, bcnt2, bases, lkeys, lcnt, lofs,
+      posb;
   std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
   uint8_t* hstage = nullptr;    // pinned staging read by captured copies
   size_t hstage_cap = 0;
@@ -219,7 +221,7 @@ struct ws_patterns {
   const uint16_t* ac_table16 = nullptr;
   const uint8_t* ac_cmap = nullptr;
   const uint32_t *rx_tables = nullptr, *rx_toff = nullptr, *rx_ncls = nullptr,
-                 *rx_pid = nullptr, *rx_maxlen = nullptr;
+                 *rx_pid = nullptr, *rx_maxlen = nullptr, *rx_starts = nullptr, *rx_scan = nullptr;
   const uint8_t* rx_cmaps = nullptr;
   uint32_t rx_states_total = 0;
   bool ac_lds_ok = false;
@@ -333,12 +335,13 @@ ws_status upload_patterns(ws_patterns* p) {
   size_t o_accmap = put(h.ac_cmap.data(), 256);
   size_t o_acos = put(h.ac_out_start.data(), h.ac_out_start.size() * 4);
   size_t o_acol = put(h.ac_out_list.data(), h.ac_out_list.size() * 4);
-  std::vector<uint32_t> rxt, toff, ncls, rxmax;
+  std::vector<uint32_t> rxt, toff, ncls, rxmax, starts;
   std::vector<uint8_t> cm;
   p->rx_states_total = 0;
   for (size_t r = 0; r < h.rx.size(); ++r) {
     toff.push_back((uint32_t)rxt.size());
-    ncls.push_back(h.rx[r].nclass);
+    ncls.push_back(h.rx[r].nclass + 1);  // row stride: the classes, then the end-of-text column
+    starts.insert(starts.end(), h.rx[r].start.begin(), h.rx[r].start.end());
     rxt.insert(rxt.end(), h.rx[r].table.begin(), h.rx[r].table.end());
     cm.insert(cm.end(), h.rx[r].cmap.begin(), h.rx[r].cmap.end());
     long mx = h.rx_maxlen[r];
@@ -351,6 +354,8 @@ ws_status upload_patterns(ws_patterns* p) {
   size_t o_cm = put(cm.data(), cm.size());
   size_t o_rxpid = put(h.rx_pid.data(), h.rx_pid.size() * 4);
   size_t o_rxmax = put(rxmax.data(), rxmax.size() * 4);
+  size_t o_rxstart = put(starts.data(), starts.size() * 4);
+  size_t o_rxscan = put(h.rx_scan.data(), h.rx_scan.size() * 4);
 
   WS_HIP(p->blob.ensure(blob.size()));
   WS_HIP(hipMemcpy(p->blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
@@ -377,11 +382,14 @@ ws_status upload_patterns(ws_patterns* p) {
   p->rx_cmaps = b + o_cm;
   p->rx_pid = reinterpret_cast<const uint32_t*>(b + o_rxpid);
   p->rx_maxlen = reinterpret_cast<const uint32_t*>(b + o_rxmax);
+  p->rx_starts = reinterpret_cast<const uint32_t*>(b + o_rxstart);
+  p->rx_scan = reinterpret_cast<const uint32_t*>(b + o_rxscan);
   return WS_OK;
 }
 
 uint32_t supported_variants(const ws_patterns* p) {
   uint32_t v = 1u << WS_VARIANT_HASH;
+  if (p->h.fac.empty()) return v;  // no factors: no prefilter pass at all
   if (p->h.ac_ok) v |= 1u << WS_VARIANT_DFA_GLOBAL;
   if (p->ac_lds_ok) v |= 1u << WS_VARIANT_DFA_LDS;
   return v;
@@ -391,8 +399,8 @@ uint32_t pick_variant(const ws_patterns* p, const ws_tuning* t) {
   uint32_t v = t ? t->variant : WS_VARIANT_AUTO;
   if (v != WS_VARIANT_AUTO) return v;
   // Without a tuned choice: the hash prefilter unless the window is tiny and a
-  // small factor DFA fits in LDS.
-  if (p->h.K <= 2 && p->ac_lds_ok) return WS_VARIANT_DFA_LDS;
+  // small factor DFA fits in LDS (a set without factors runs no prefilter).
+  if (p->h.K <= 2 && p->ac_lds_ok && !p->h.fac.empty()) return WS_VARIANT_DFA_LDS;
   return WS_VARIANT_HASH;
 }
 
@@ -406,8 +414,47 @@ struct Plan {
   bool window_truncated = false;
   uint32_t seg_off = 0;            // absolute offset of item 0 (stream chunks)
   bool use_carried = false;
+  bool ctx_before = false;         // base[items[0].x - 1] is the byte before the segment (a later stream chunk)
   uint64_t max_seg_len = 0;
 };
+
+// Candidate positions per regex of the every-position walk (k_rx_scan): each
+// item's bytes and its end (an empty match may start there); a stream chunk
+// takes the positions before its nominal end, and its end too when that is
+// the text's end.
+uint64_t scan_positions(const Plan& plan) {
+  uint64_t n = 0;
+  if (plan.batch) {
+    for (const auto& it : plan.items) n += (uint64_t)(it.y - it.x) + 1;
+    return n;
+  }
+  const uint64_t span = plan.items[0].y - plan.items[0].x;
+  return plan.emit_end >= span ? span + 1 : plan.emit_end;
+}
+
+// The walk parameters of a plan (candidates, raw output and chain pairs are
+// filled in by the caller).
+WalkParams walk_params(const ws_patterns* p, const Plan& plan, const uint2* items, uint32_t* counters,
+                       const uint64_t* pos_base) {
+  WalkParams w{};
+  w.base = plan.base;
+  w.items = items;
+  w.tables = p->rx_tables;
+  w.toff = p->rx_toff;
+  w.ncls = p->rx_ncls;
+  w.cmaps = p->rx_cmaps;
+  w.rx_pid = p->rx_pid;
+  w.counters = counters;
+  w.window_truncated = plan.window_truncated ? 1u : 0u;
+  w.starts = p->rx_starts;
+  w.ctx_before = plan.ctx_before ? 1u : 0u;
+  w.scan_rx = p->rx_scan;
+  w.nscan = (uint32_t)p->h.rx_scan.size();
+  w.nitems = (uint32_t)plan.items.size();
+  w.npos_total = w.nscan ? scan_positions(plan) : 0;
+  w.pos_base = plan.batch ? pos_base : nullptr;
+  return w;
+}
 
 struct Result {
   uint32_t count = 0;  // records in ctx->k1/v1 (sorted keys/values)
@@ -748,8 +795,10 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 9, 0, st));
     ScanCommon c = make_common(ctx, p, plan, nunits, lit_cap, cand_cap);
     if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[1], st));
-    if (nunits == 0 && ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
-    if (nunits > 0) {
+    // A set whose regexes all scan every position has no factors: no filter.
+    const bool filter = nunits > 0 && !h.fac.empty();
+    if (!filter && ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[6], st));
+    if (filter) {
       if (variant == WS_VARIANT_HASH) {
         const HashParams hp = hash_params(p, lane_bytes);
         uint32_t* sc = ctx->scounts.as<uint32_t>();
@@ -837,29 +886,40 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     nlit = ctx->hcount[0];
   }
 
-  // Regex candidates: anchored walks, then per-(item, regex) chains.
+  // Regex candidates: anchored walks (from prefilter hits, and from every
+  // position for regexes without a selective prefix), then per-(item, regex)
+  // chains.
   if (ctx->profiling) WS_HIP(hipEventRecord(ctx->ev[3], st));
   ctx->last.candidates = ncand;
-  if (ncand > 0) {
-    WS_HIP(ctx->raw.ensure((size_t)ncand * sizeof(Rec)));
-    WalkParams w{};
-    w.base = plan.base;
-    w.items = ctx->items.as<uint2>();
+  const uint32_t nscan = (uint32_t)h.rx_scan.size();
+  if (ncand > 0 || nscan > 0) {
+    if (nscan && plan.batch) {
+      std::vector<uint64_t> pb(plan.items.size() + 1, 0);
+      for (size_t k = 0; k < plan.items.size(); ++k) pb[k + 1] = pb[k] + (plan.items[k].y - plan.items[k].x) + 1;
+      WS_HIP(ctx->posb.ensure(pb.size() * 8));
+      WS_HIP(h2d_staged(ctx, ctx->posb.p, pb.data(), pb.size() * 8));
+    }
+    WalkParams w = walk_params(p, plan, ctx->items.as<uint2>(), ctx->counters.as<uint32_t>(), ctx->posb.as<uint64_t>());
     w.cands = ctx->cands.as<Rec>();
     w.ncand = ctx->counters.as<uint32_t>() + 1;
     w.cand_cap = cand_cap;
-    w.tables = p->rx_tables;
-    w.toff = p->rx_toff;
-    w.ncls = p->rx_ncls;
-    w.cmaps = p->rx_cmaps;
-    w.rx_pid = p->rx_pid;
-    w.raw = ctx->raw.as<Rec>();
-    w.raw_cap = ncand;
-    w.counters = ctx->counters.as<uint32_t>();
-    w.window_truncated = plan.window_truncated ? 1u : 0u;
-    WS_HIP(launch_walk(ncand, st, w));
-    ws_status s = read_counters(ctx);
-    if (s != WS_OK) return s;
+    // Raw matches: at most one per candidate, and per scanned position.
+    uint64_t raw_need = (uint64_t)ncand + std::min<uint64_t>(w.npos_total * w.nscan, 1u << 20);
+    for (int attempt = 0;; ++attempt) {
+      if (raw_need > 0xFFFFFFF0ull) return fail(WS_ERR_INPUT_TOO_LARGE, "too many raw regex matches");
+      WS_HIP(ctx->raw.ensure(std::max<uint64_t>(raw_need, 1) * sizeof(Rec)));
+      w.raw = ctx->raw.as<Rec>();
+      w.raw_cap = (uint32_t)raw_need;
+      WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>() + 2, 1, 0, st));
+      if (ncand) WS_HIP(launch_walk(ncand, st, w));
+      if (nscan) WS_HIP(launch_rx_scan(st, w));
+      ws_status s = read_counters(ctx);
+      if (s != WS_OK) return s;
+      if (ctx->hcount[2] <= raw_need) break;
+      if (attempt >= 2) return fail(WS_ERR_GPU_DEVICE, "raw regex buffer did not converge");
+      raw_need = ctx->hcount[2];
+    }
+    ws_status s = WS_OK;
     if (ctx->hcount[3] & kFlagTruncatedWalk)
       return fail(WS_ERR_INPUT_TOO_LARGE,
                   "a regex match may extend past the chunk overlap; raise the overlap to the "
@@ -925,12 +985,14 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     ws_status s = WS_OK;
     if (plan.batch) {
       // Items end to end in id order: dense (position, pid) keys, so the
-      // bucket sort applies (k_pack_tagged_dense maps positions back).
+      // bucket sort applies (k_pack_tagged_dense maps positions back). An
+      // item of length L takes L + 1 positions: a match may start at its end
+      // (an empty match), and an empty item's matches keep their own place.
       std::vector<uint64_t> bases(plan.items.size());
       uint64_t acc = 0;
       for (size_t i = 0; i < plan.items.size(); ++i) {
         bases[i] = acc;
-        acc += plan.items[i].y - plan.items[i].x;
+        acc += (uint64_t)(plan.items[i].y - plan.items[i].x) + 1;
       }
       const uint32_t gbits = bits_for(acc ? acc - 1 : 0);
       if (gbits + res->pbits > 63) return fail(WS_ERR_INPUT_TOO_LARGE, "batch key does not fit 63 bits");
@@ -990,16 +1052,25 @@ bool is_device_ptr(const void* p) {
 }
 
 // Lays out one contiguous input (host or device) as a single-segment plan.
-ws_status plan_single(ws_ctx* ctx, const uint8_t* data, uint64_t len, uint32_t data_mem, Plan* plan) {
+// with_prev: data[-1] exists and is the byte before the segment (a stream
+// chunk after the first); the regex walks read it for their start context.
+ws_status plan_single(ws_ctx* ctx, const uint8_t* data, uint64_t len, uint32_t data_mem, Plan* plan,
+                      bool with_prev = false) {
   if (len > kMaxScanBytes)
     return fail(WS_ERR_INPUT_TOO_LARGE, "scan input is too large (" + std::to_string(len) + " bytes; at most " +
                                             std::to_string(kMaxScanBytes) + ")");
   const uint8_t* dptr = data;
   if (data_mem == WS_MEM_HOST) {
-    WS_HIP(ctx->input.ensure(((len + 15) & ~15ull) + 16));
-    WS_HIP(hipMemcpyAsync(ctx->input.p, data, len, hipMemcpyHostToDevice, ctx->stream));
-    dptr = ctx->input.as<uint8_t>();
+    // Staged from offset 16, the byte before it (with_prev) at 15.
+    const size_t off = with_prev ? 16 : 0;
+    WS_HIP(ctx->input.ensure(off + ((len + 15) & ~15ull) + 16));
+    if (with_prev)
+      WS_HIP(hipMemcpyAsync(ctx->input.as<uint8_t>() + 15, data - 1, len + 1, hipMemcpyHostToDevice, ctx->stream));
+    else if (len)
+      WS_HIP(hipMemcpyAsync(ctx->input.p, data, len, hipMemcpyHostToDevice, ctx->stream));
+    dptr = ctx->input.as<uint8_t>() + off;
   }
+  plan->ctx_before = with_prev;
   const uintptr_t a = reinterpret_cast<uintptr_t>(dptr);
   plan->base = reinterpret_cast<const uint8_t*>(a & ~(uintptr_t)15);
   const uint32_t shift = (uint32_t)(a & 15);
@@ -1072,7 +1143,13 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
     rs->cand_cap = lim->cand_cap;
   }
   const uint32_t nrx = (uint32_t)h.rx.size();
-  rs->raw_cap = nrx ? rs->cand_cap : 0;
+  // Raw regex matches: one per candidate, plus (regexes without a selective
+  // prefix) one per scanned position, up to 4 M more; a denser run overflows
+  // to the checked pipeline, which sizes them exactly.
+  const uint64_t scan_raw = h.rx_scan.empty() ? 0 : scan_positions(plan) * h.rx_scan.size();
+  rs->raw_cap = nrx ? (uint32_t)std::min<uint64_t>(0x0FFFFFFFull, (uint64_t)rs->cand_cap +
+                                                                       (lim ? 0 : std::min<uint64_t>(scan_raw, 1u << 22)))
+                    : 0;
   rs->rec_cap = rs->lit_cap + rs->raw_cap;
   WS_HIP(ctx->recs.ensure((size_t)rs->rec_cap * sizeof(Rec)));
   WS_HIP(ctx->cands.ensure((size_t)rs->cand_cap * sizeof(Rec)));
@@ -1166,7 +1243,8 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   caps.max_matches = rs.max_matches;
   caps.lit_ovf_cap = caps.cand_ovf_cap = 0xFFFFFFFFu;
   bool keyed = false;  // ordering pairs already made (and counted) by the compaction
-  if (rs.nunits > 0 && rs.variant == WS_VARIANT_HASH) {
+  const bool filter = rs.nunits > 0 && !h.fac.empty();  // no factors: no filter pass
+  if (filter && rs.variant == WS_VARIANT_HASH) {
     const HashParams hp = hash_params(p, rs.lane_bytes);
     uint32_t* sc = ctx->scounts.as<uint32_t>();
     SliceOut lso = make_slices(ctx->lslices, rs.lslice_total, rs.nvb, sc, cnt + 5);
@@ -1196,7 +1274,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     keyed = true;
     caps.lit_ovf_cap = lso.ovf_cap;
     caps.cand_ovf_cap = cso.ovf_cap;
-  } else if (rs.nunits > 0) {
+  } else if (filter) {
     AcParams ap{p->ac_table32, p->ac_table16, p->ac_cmap, p->ac_out_start, p->ac_out_list,
                 p->ac_len, h.ac_states, h.ac_classes, h.ac_first_out, h.ac_maxlen};
     WS_HIP(launch_ac(rs.variant == WS_VARIANT_DFA_LDS, rs.grid, st, c, ap));
@@ -1210,21 +1288,12 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   // Regex candidates: anchored walks, chain ordering, greedy chains appended
   // after the literal matches.
   if (nrx) {
-    WalkParams w{};
-    w.base = plan.base;
-    w.items = items;
+    WalkParams w = walk_params(p, plan, items, cnt, nullptr);
     w.cands = ctx->cands.as<Rec>();
     w.ncand = cnt + 1;
     w.cand_cap = rs.cand_cap;
-    w.tables = p->rx_tables;
-    w.toff = p->rx_toff;
-    w.ncls = p->rx_ncls;
-    w.cmaps = p->rx_cmaps;
-    w.rx_pid = p->rx_pid;
     w.raw = ctx->raw.as<Rec>();
     w.raw_cap = rs.raw_cap;
-    w.counters = cnt;
-    w.window_truncated = plan.window_truncated ? 1u : 0u;
     // Hash variant: the walk writes the chain pairs and counts them (bcnt2),
     // the chain walk writes the kept matches' final pairs (k2/v2, bcnt).
     uint32_t* ccnt = keyed ? ctx->bcnt2.as<uint32_t>() : ctx->bcnt.as<uint32_t>();
@@ -1237,6 +1306,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
       w.rbits = rs.rbits;
     }
     WS_HIP(launch_walk(0, st, w));
+    if (w.nscan) WS_HIP(launch_rx_scan(st, w));
     if (!keyed)
       WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), rs.raw_cap, cnt + 2, rs.sbits, rs.rbits, ctx->k0.as<uint64_t>(),
                                ctx->v0.as<uint32_t>(), st));
@@ -1811,7 +1881,7 @@ void ws_ctx_destroy(ws_ctx* ctx) {
                     &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
                     &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
                     &ctx->bcur, &ctx->dstate, &ctx->k2, &ctx->v2, &ctx->bcnt2, &ctx->bases,
-                    &ctx->lkeys, &ctx->lcnt, &ctx->lofs})
+                    &ctx->lkeys, &ctx->lcnt, &ctx->lofs, &ctx->posb})
     b->release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -2044,7 +2114,8 @@ ws_status ws_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64
   if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
   if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");
   *count = 0;
-  if (len == 0) return WS_OK;
+  // An empty input matches only regexes that can match the empty string.
+  if (len == 0 && p->h.rx_scan.empty()) return WS_OK;
   std::lock_guard<std::mutex> lk(ctx->mu);
   WS_HIP(hipSetDevice(ctx->dev));
   ctx->last = ws_timings{};
@@ -2100,7 +2171,7 @@ ws_status ws_scan_noalloc(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data
                                             "; use ws_scan");
   if (tuning && (tuning->flags & WS_TUNING_CHECKED))
     return fail(WS_ERR_INVALID_ARGUMENT, "the allocation-free scan has no checked pipeline");
-  if (len == 0) return WS_OK;
+  if (len == 0 && p->h.rx_scan.empty()) return WS_OK;
   std::lock_guard<std::mutex> lk(ctx->mu);
   WS_HIP(hipSetDevice(ctx->dev));
   uint8_t* base = g_scratch_ptr[ctx->dev];
@@ -2205,7 +2276,8 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
     return fail(WS_ERR_INVALID_ARGUMENT,
                 "standard-mode literal reductions are not chunk-stitched; use overlapping mode");
   *count = 0;
-  if (len == 0) return WS_OK;
+  if (len == 0)  // no chunks: the empty input as one scan
+    return p->h.rx_scan.empty() ? WS_OK : ws_scan(ctx, p, data, 0, data_mem, mode, tuning, out, cap, out_mem, count);
   if (len > kMaxScanBytes) return fail(WS_ERR_INPUT_TOO_LARGE, "stream input is too large");
   std::lock_guard<std::mutex> lk(ctx->mu);
   WS_HIP(hipSetDevice(ctx->dev));
@@ -2222,7 +2294,7 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
       const uint64_t b = std::min(len, a + chunk_size);
       const uint64_t e = std::min(len, b + overlap);
       Plan plan;
-      ws_status s = plan_single(ctx, data + a, e - a, data_mem, &plan);
+      ws_status s = plan_single(ctx, data + a, e - a, data_mem, &plan, a > 0);
       if (s != WS_OK) return s;
       plan.emit_end = (uint32_t)(b - a);
       plan.window_truncated = e < len;
@@ -2261,7 +2333,7 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
     const uint64_t b = std::min(len, a + chunk_size);
     const uint64_t e = std::min(len, b + overlap);
     Plan plan;
-    ws_status s = plan_single(ctx, data + a, e - a, data_mem, &plan);
+    ws_status s = plan_single(ctx, data + a, e - a, data_mem, &plan, a > 0);
     if (s != WS_OK) return s;
     plan.emit_end = (uint32_t)(b - a);
     plan.window_truncated = e < len;
@@ -2429,7 +2501,7 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
       max_len = std::max<uint64_t>(max_len, it.len);
     }
     WS_HIP(ctx->input.ensure(tot + 16));
-    WS_HIP(hipMemcpyAsync(ctx->input.p, host.data(), tot, hipMemcpyHostToDevice, ctx->stream));
+    if (tot) WS_HIP(hipMemcpyAsync(ctx->input.p, host.data(), tot, hipMemcpyHostToDevice, ctx->stream));
     WS_HIP(hipStreamSynchronize(ctx->stream));
     plan.base = ctx->input.as<uint8_t>();
     plan.load_end = tot;

@@ -172,24 +172,42 @@ struct Cur {
   void cmap(std::array<uint8_t, 256>& m) { std::memcpy(m.data(), take(256), 256); }
 };
 
-// A version-4 regex section: the anchored DFA of one regex.
+// A version-5 regex section: the anchored DFA of one regex (states, classes,
+// four start states, class map, table of states x (classes + 1)).
 AnchoredDfa read_dfa(Cur& c) {
   AnchoredDfa d;
   d.nstates = c.u32();
   d.nclass = c.u32();
-  const uint32_t sm = c.u32();
+  for (auto& st : d.start) st = c.u32();
   c.cmap(d.cmap);
   d.table = c.vec("regex DFA table");
   if (d.nstates < 2 || d.nstates > DFA_STATE_MASK) corrupt("regex DFA state count is invalid");
   if (d.nclass < 1 || d.nclass > 256) corrupt("regex DFA class count is invalid");
-  if (sm > 1) corrupt("regex DFA start flag is invalid");
-  d.start_match = sm != 0;
+  for (uint32_t st : d.start)
+    if (st == 0 || st >= d.nstates) corrupt("regex DFA start state is out of range");
   for (int b = 0; b < 256; ++b)
     if (d.cmap[b] >= d.nclass) corrupt("regex DFA byte class points outside the class range");
-  if ((uint64_t)d.nstates * d.nclass != d.table.size()) corrupt("regex DFA table size is inconsistent");
+  const uint64_t W = (uint64_t)d.nclass + 1;
+  if ((uint64_t)d.nstates * W != d.table.size()) corrupt("regex DFA table size is inconsistent");
   for (uint32_t e : d.table)
     if ((e & DFA_STATE_MASK) >= d.nstates) corrupt("regex DFA transition points outside the state range");
+  for (uint64_t s = 0; s < d.nstates; ++s)
+    if (!(d.table[s * W + d.nclass] & DFA_DEAD)) corrupt("regex DFA end-of-text entry is not final");
+  for (uint32_t st : d.start)
+    for (uint64_t x = 0; x < W; ++x)
+      if (d.table[st * W + x] & DFA_MATCH) d.can_empty = true;
   return d;
+}
+
+// A version-4 regex section (one start flag, no end-of-text column): its
+// shape is checked and it is skipped; the set is rebuilt from its sources.
+void skip_dfa_v4(Cur& c) {
+  const uint32_t nstates = c.u32(), nclass = c.u32(), sm = c.u32();
+  std::array<uint8_t, 256> cmap{};
+  c.cmap(cmap);
+  const std::vector<uint32_t> table = c.vec("regex DFA table");
+  if (nstates < 2 || nclass < 1 || nclass > 256 || sm > 1 || (uint64_t)nstates * nclass != table.size())
+    corrupt("regex DFA section is inconsistent");
 }
 
 // The version-4 factor automaton section.
@@ -279,7 +297,7 @@ std::vector<uint8_t> index_serialize(const PatternsHost& h, uint32_t version) {
   if (version == kIndexVersionRef && !h.rx.empty())
     throw CompileError{WS_ERR_INVALID_ARGUMENT,
                        "a version-3 index stores regex-automata DFAs, which this library does not "
-                       "produce. Fix: write version 4, or keep the set to literals"};
+                       "produce. Fix: write version 5, or keep the set to literals"};
   for (uint32_t i = 0; i < h.npat; ++i)
     if (h.has_name[i] && !valid_utf8(h.names[i]))
       throw CompileError{WS_ERR_INVALID_ARGUMENT, "pattern name " + std::to_string(i) + " is not valid UTF-8"};
@@ -327,7 +345,7 @@ std::vector<uint8_t> index_serialize(const PatternsHost& h, uint32_t version) {
     for (const AnchoredDfa& d : h.rx) {
       o.u32(d.nstates);
       o.u32(d.nclass);
-      o.u32(d.start_match ? 1 : 0);
+      for (uint32_t st : d.start) o.u32(st);
       o.raw(d.cmap.data(), 256);
       o.vec(d.table, "regex DFA table");
     }
@@ -356,7 +374,7 @@ IndexContents index_parse(const uint8_t* data, size_t len) {
   if (len >= 16) {
     const uint32_t v = (uint32_t)data[8] | ((uint32_t)data[9] << 8) | ((uint32_t)data[10] << 16) |
                        ((uint32_t)data[11] << 24);
-    if (v == kIndexVersionRef || v == kIndexVersion) {
+    if (v == kIndexVersionRef || v == kIndexVersionOld || v == kIndexVersion) {
       const uint32_t stored = (uint32_t)data[len - 4] | ((uint32_t)data[len - 3] << 8) |
                               ((uint32_t)data[len - 2] << 16) | ((uint32_t)data[len - 1] << 24);
       const uint32_t computed = crc32_ieee(data, len - 4);
@@ -458,6 +476,10 @@ IndexContents index_parse(const uint8_t* data, size_t len) {
     for (uint32_t r = 0; r < hd.regex_count; ++r) ic.pre.rx.push_back(read_dfa(c));
     ic.pre.ac = read_ac(c);
     hd.prebuilt = true;
+  } else if (hd.version == kIndexVersionOld) {
+    c.fits(hd.regex_count, 16, "regex DFA sections");
+    for (uint32_t r = 0; r < hd.regex_count; ++r) skip_dfa_v4(c);
+    (void)read_ac(c);
   } else {
     for (uint32_t r = 0; r < hd.regex_count; ++r) skip_reference_dfa(c);
   }

@@ -7,11 +7,13 @@
 // for sets without regexes (the reference's regex sections hold
 // regex-automata DFAs, which this library does not produce).
 //
-// Version 4 is this library's: the same header and sections as version 3,
-// then one section per regex holding its anchored DFA and one holding the
-// factor automaton, so that loading skips the two expensive compilation
-// steps. Every length, count and state index in a file is checked before it
-// is used, and versions 3 and 4 carry a CRC-32 (IEEE) trailer.
+// Version 5 is this library's: the same header and sections as version 3,
+// then one section per regex holding its anchored DFA (four start states, an
+// end-of-text column) and one holding the factor automaton, so that loading
+// skips the two expensive compilation steps. Version 4, an earlier layout of
+// this library's regex sections, is read and rebuilt from its stored sources.
+// Every length, count and state index in a file is checked before it is used,
+// and versions 3 to 5 carry a CRC-32 (IEEE) trailer.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -22,7 +24,8 @@
 namespace ws {
 
 constexpr uint32_t kIndexVersionRef = 3;   // the reference's current version
-constexpr uint32_t kIndexVersion = 4;      // this library's version
+constexpr uint32_t kIndexVersion = 5;      // this library's version
+constexpr uint32_t kIndexVersionOld = 4;   // an earlier layout of this library's (rebuilt from sources)
 constexpr uint32_t kIndexHashWindow = 8;   // compiler.rs:118
 constexpr uint64_t kIndexMaxFileBytes = 512ull << 20;  // mod.rs:19
 
@@ -30,11 +33,11 @@ struct IndexHeader {
   uint32_t version = 0;
   bool ci = false;
   uint32_t pattern_count = 0, literal_count = 0, regex_count = 0, hash_window_len = 0;
-  bool prebuilt = false;  // version 4: DFA and automaton tables present
+  bool prebuilt = false;  // version 5: DFA and automaton tables present
 };
 
 // A parsed index: the pattern list as to_pattern_set rebuilds it and, for
-// version 4, the stored tables.
+// version 5, the stored tables.
 struct IndexContents {
   IndexHeader hdr;
   std::vector<PatternSrc> src;

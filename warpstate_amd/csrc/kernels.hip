@@ -970,8 +970,64 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t want, uint32_t* count
 }
 
 // ---------------------------------------------------------------------------
-// Anchored leftmost-first walk: one lane per candidate start.
+// Anchored leftmost-first walks.
 // ---------------------------------------------------------------------------
+// The context a byte leaves for the position after it: 1 '\n', 2 a word
+// byte [0-9A-Za-z_], 3 another byte (0 is the text start).
+__device__ __forceinline__ uint32_t byte_ctx(uint32_t b) {
+  const uint32_t lo = b | 0x20u;
+  const bool w = (b - '0' < 10u) || (lo - 'a' < 26u) || b == '_';
+  return b == '\n' ? 1u : (w ? 2u : 3u);
+}
+
+// The leftmost-first match of regex rx anchored at st (relative to segment
+// seg): its end (relative), or 0xFFFFFFFF. The walk starts in the state for
+// the byte before st (the text start at a segment's start, unless the buffer
+// holds the byte before it), records every position whose transition says a
+// match ends there, and stops when the DFA dies. At the segment's end it
+// reads the end-of-text column, or, in a window that ends before the input
+// does, sets *open (the match could still change).
+__device__ __forceinline__ uint32_t rx_match_at(const WalkParams& w, uint32_t rx, uint2 seg, uint32_t st,
+                                                bool* open) {
+  const uint32_t* T = w.tables + w.toff[rx];
+  const uint32_t C = w.ncls[rx];
+  const uint8_t* cm = w.cmaps + 256u * rx;
+  const uint64_t p0 = (uint64_t)seg.x + st;
+  const uint32_t ctx = (st > 0 || w.ctx_before) ? byte_ctx(w.base[p0 - 1]) : 0u;
+  uint32_t s = w.starts[4 * rx + ctx];
+  uint32_t last = 0xFFFFFFFFu;
+  for (uint64_t p = p0; p < seg.y; ++p) {
+    const uint32_t e = T[s * C + cm[w.base[p]]];
+    if (e & DFA_MATCH) last = (uint32_t)(p - seg.x);
+    if (e & DFA_DEAD) return last;
+    s = e & DFA_STATE_MASK;
+  }
+  if (w.window_truncated) {
+    *open = true;
+    return last;
+  }
+  if (T[s * C + C - 1] & DFA_MATCH) last = seg.y - seg.x;
+  return last;
+}
+
+// Appends a raw match {rx, start, end, item} (block-wide slot reservation;
+// every lane of the block calls it) and, in device-sized runs, its chain
+// ordering pair.
+__device__ __forceinline__ void emit_raw(const WalkParams& w, bool found, uint32_t rx, uint32_t st, uint32_t end,
+                                         uint32_t item, uint32_t* s_res) {
+  const uint32_t slot = block_reserve(found ? 1u : 0u, &w.counters[2], s_res);
+  if (found && slot < w.raw_cap) {
+    w.raw[slot] = make_uint4(rx, st, end, item);
+    if (w.ckeys) {
+      const uint64_t k = (((uint64_t)item << w.rbits | rx) << w.sbits) | st;
+      w.ckeys[slot] = k;
+      w.cvals[slot] = slot;
+      atomicAdd(&w.ccnt[k >> w.cshift], 1u);
+    }
+  }
+}
+
+// One lane per candidate start (a prefilter hit of a regex factor).
 __global__ __launch_bounds__(kTPB) void k_rx_walk(WalkParams w) {
   __shared__ uint32_t s_res[2];
   const uint32_t n = min(*w.ncand, w.cand_cap);
@@ -982,36 +1038,46 @@ __global__ __launch_bounds__(kTPB) void k_rx_walk(WalkParams w) {
     uint32_t last = 0xFFFFFFFFu;
     if (i < n) {
       cd = w.cands[i];
-      const uint32_t rx = cd.x;
-      const uint2 seg = w.items[cd.w];
-      const uint32_t* T = w.tables + w.toff[rx];
-      const uint32_t C = w.ncls[rx];
-      const uint8_t* cm = w.cmaps + 256u * rx;
-      uint32_t s = 1;
-      uint64_t p = (uint64_t)seg.x + cd.y;
-      bool dead = false;
-      for (; p < seg.y; ++p) {
-        const uint32_t e = T[s * C + cm[w.base[p]]];
-        if (e & DFA_DEAD) { dead = true; break; }
-        s = e & DFA_STATE_MASK;
-        if (e & DFA_MATCH) last = (uint32_t)(p + 1 - seg.x);
-      }
-      if (!dead && w.window_truncated) atomicOr(&w.counters[3], kFlagTruncatedWalk);
+      bool open = false;
+      last = rx_match_at(w, cd.x, w.items[cd.w], cd.y, &open);
+      if (open) atomicOr(&w.counters[3], kFlagTruncatedWalk);
     }
-    const bool found = last != 0xFFFFFFFFu;
-    const uint32_t slot = block_reserve(found ? 1u : 0u, &w.counters[2], s_res);
-    if (found) {
-      const uint32_t rx = cd.x, item = cd.w;
-      if (slot < w.raw_cap) {
-        w.raw[slot] = make_uint4(rx, cd.y, last, item);
-        if (w.ckeys) {
-          const uint64_t k = (((uint64_t)item << w.rbits | rx) << w.sbits) | cd.y;
-          w.ckeys[slot] = k;
-          w.cvals[slot] = slot;
-          atomicAdd(&w.ccnt[k >> w.cshift], 1u);
+    emit_raw(w, last != 0xFFFFFFFFu, cd.x, cd.y, last, cd.w, s_res);
+  }
+}
+
+// Every position of every segment, for each regex without a selective
+// prefix (w.scan_rx): position q of the flat position space lies in item i
+// when pos_base[i] <= q < pos_base[i + 1] (a single segment: item 0). An
+// item of length L has L + 1 positions (an empty match may start at its
+// end). Lanes take consecutive positions, so the first bytes load coalesced.
+__global__ __launch_bounds__(kTPB) void k_rx_scan(WalkParams w) {
+  __shared__ uint32_t s_res[2];
+  const uint64_t total = w.npos_total * w.nscan;
+  for (uint64_t b0 = (uint64_t)blockIdx.x * kTPB; b0 < total; b0 += (uint64_t)gridDim.x * kTPB) {
+    const uint64_t g = b0 + threadIdx.x;
+    uint32_t last = 0xFFFFFFFFu, rx = 0, item = 0, st = 0;
+    if (g < total) {
+      const uint32_t k = (uint32_t)(g / w.npos_total);
+      const uint64_t q = g - (uint64_t)k * w.npos_total;
+      rx = w.scan_rx[k];
+      uint64_t q0 = 0;
+      if (w.pos_base) {
+        uint32_t lo = 0, hi = w.nitems;  // pos_base[lo] <= q < pos_base[hi]
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) / 2;
+          if (w.pos_base[mid] <= q) lo = mid;
+          else hi = mid;
         }
+        item = lo;
+        q0 = w.pos_base[lo];
       }
+      st = (uint32_t)(q - q0);
+      bool open = false;
+      last = rx_match_at(w, rx, w.items[item], st, &open);
+      if (open) atomicOr(&w.counters[3], kFlagTruncatedWalk);
     }
+    emit_raw(w, last != 0xFFFFFFFFu, rx, st, last, item, s_res);
   }
 }
 
@@ -1055,14 +1121,18 @@ __global__ void k_chain_heads(const uint64_t* keys, const uint32_t* vals, const 
     const Rec ri = raw[vals[i]];
     bool h = (i == 0) || (group_of(keys[i - 1], gshift) != g);
     if (!h) {
+      // An empty record also depends on an earlier end equal to its start
+      // (find_iter skips an empty match where the previous one ended).
+      const bool empty = ri.y == ri.z;
       const uint32_t ml = maxlen ? maxlen[ri.x] : maxlen_all;
       if (ml != 0xFFFFFFFFu) {
         h = true;
         for (uint32_t j = i; j-- > 0;) {
           if (group_of(keys[j], gshift) != g) break;
           const Rec rj = raw[vals[j]];
-          if ((uint64_t)rj.y + ml <= ri.y) break;
-          if (rj.z > ri.y) { h = false; break; }
+          const uint64_t reach = (uint64_t)rj.y + ml;
+          if (reach < ri.y || (reach == ri.y && !empty)) break;
+          if (rj.z > ri.y || (empty && rj.z == ri.y)) { h = false; break; }
         }
       }
     }
@@ -1096,17 +1166,24 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
   auto walk = [&](uint32_t i, bool emit, uint32_t slot) -> uint32_t {
     const uint64_t g = group_of(keys[i], gshift);
     uint32_t last_end = 0, last_start = 0xFFFFFFFFu, kept = 0;
+    bool have_last = false;
     if (carried) {
       const uint32_t cv = carried[raw[vals[i]].x];
-      last_end = cv > seg_off + 1u ? cv - 1u - seg_off : 0u;
+      if (cv > seg_off) {  // the previous chunks' last kept match ends at or after this chunk's start
+        last_end = cv - 1u - seg_off;
+        have_last = true;
+      }
     }
     for (uint32_t j = i; j < n; ++j) {
       if (j > i && (head[j] || group_of(keys[j], gshift) != g)) break;
       const Rec r = raw[vals[j]];
       const bool tie = ties && r.y == last_start && r.z == last_end;
       if (r.y < last_end && !tie) continue;
+      // find_iter: an empty match where the previous match ended is skipped.
+      if (r.y == r.z && have_last && r.y == last_end && !tie) continue;
       last_end = r.z;
       last_start = r.y;
+      have_last = true;
       if (emit) {
         const uint32_t sl = slot + kept;
         const uint32_t pid = rx_pid ? rx_pid[r.x] : r.x;
@@ -1208,8 +1285,9 @@ __global__ void k_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32
 
 // Batch ordering keys over the items laid end to end in id order: item i's
 // match at start s sits at bases[i] + s (bases: exclusive prefix sums of the
-// item lengths), so keys (position << pbits | pid) are dense and the bucket
-// sort applies. Same order as (item, start, pid).
+// item lengths plus one, the end position of each item), so keys
+// (position << pbits | pid) are dense and the bucket sort applies. Same
+// order as (item, start, pid).
 __global__ void k_order_keys_dense(const Rec* recs, uint32_t n, uint32_t pbits, const uint64_t* bases,
                                    uint64_t* keys, uint32_t* vals) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1220,8 +1298,7 @@ __global__ void k_order_keys_dense(const Rec* recs, uint32_t n, uint32_t pbits, 
 }
 
 // Tagged output from dense batch keys: the item is the last one whose base is
-// <= the position (empty items share their successor's base and hold no
-// matches), found by binary search.
+// <= the position (bases increase strictly), found by binary search.
 __global__ void k_pack_tagged_dense(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t pbits,
                                     const uint64_t* bases, uint32_t nitems, const uint64_t* ids, TaggedOut* out) {
   const uint64_t pmask = (1ull << pbits) - 1;
@@ -1393,6 +1470,15 @@ static uint32_t grid_for(uint32_t n) {
 // (k_rx_walk reads it), so launch a fixed grid-stride grid.
 hipError_t launch_walk(uint32_t ncand_host, hipStream_t st, const WalkParams& w) {
   hipLaunchKernelGGL(k_rx_walk, dim3(ncand_host ? grid_for(ncand_host) : kDeviceGrid), dim3(kTPB), 0, st, w);
+  return hipGetLastError();
+}
+
+// Every position for the regexes of w.scan_rx (grid-stride over
+// npos_total * nscan lanes).
+hipError_t launch_rx_scan(hipStream_t st, const WalkParams& w) {
+  const uint64_t total = w.npos_total * w.nscan;
+  const uint64_t g = std::min<uint64_t>(4096, std::max<uint64_t>(1, (total + kTPB - 1) / kTPB));
+  hipLaunchKernelGGL(k_rx_scan, dim3((uint32_t)g), dim3(kTPB), 0, st, w);
   return hipGetLastError();
 }
 

@@ -106,6 +106,19 @@ struct WalkParams {
   uint32_t raw_cap;
   uint32_t* counters;
   uint32_t window_truncated;  // segment ends before the true input end
+  // Per regex: its four start states (preceded by the text start, '\n', a
+  // word byte, another byte); the table row stride is ncls (its classes plus
+  // the end-of-text column, the last).
+  const uint32_t* starts;
+  uint32_t ctx_before;  // the byte before a segment's start is in the buffer (a stream chunk after the first)
+  // k_rx_scan: regexes whose candidate starts are every position: their
+  // indices, and the positions per regex (npos_total, over all segments; a
+  // batch lists each item's first position in pos_base, nitems + 1 entries).
+  const uint32_t* scan_rx;
+  uint32_t nscan;
+  uint32_t nitems;
+  uint64_t npos_total;
+  const uint64_t* pos_base;
   // Optional (device-sized runs): chain ordering pairs written with each
   // raw match, key ((item << rbits | regex) << sbits) | start and value its
   // raw index, counted in bucket key >> cshift.

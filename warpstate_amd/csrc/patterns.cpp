@@ -283,10 +283,6 @@ PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_inse
     }
     long mn, mx;
     length_bounds(ast, &mn, &mx);
-    if (mn == 0)
-      throw CompileError{WS_ERR_PATTERN_COMPILATION_FAILED,
-                         "regex " + std::to_string(i) +
-                             " can match the empty string; not supported on the GPU path"};
     try {
       Nfa nfa = build_nfa(ast, kNfaMaxStates);
       const uint32_t r = (uint32_t)ps.rx.size();
@@ -300,16 +296,19 @@ PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_inse
         dfa = build_anchored_dfa(nfa, kDfaMaxStates);
       }
       std::vector<std::string> fs = extract_factors(nfa, kFactorMaxLen, kFactorFanout, kFactorMaxCount);
-      for (auto& s : fs)
-        if (s.empty())
-          throw ParseError{"no selective prefix (first byte matches too many values)"};
+      // An empty factor: a match may start with (almost) any byte, or be
+      // empty. Such a regex takes every position as a candidate start.
+      const bool scan_all = std::any_of(fs.begin(), fs.end(), [](const std::string& f) { return f.empty(); });
       ps.rx.push_back(std::move(dfa));
       ps.rx_src.push_back(p.bytes);
       ps.rx_pid.push_back(i);
       ps.rx_maxlen.push_back(mx);
       if (mx < 0) unbounded = true;
       else max_len = std::max<uint64_t>(max_len, (uint64_t)mx);
-      for (auto& s : fs) ps.fac.push_back({s, 1u, r, (uint32_t)s.size()});
+      if (scan_all)
+        ps.rx_scan.push_back(r);
+      else
+        for (auto& s : fs) ps.fac.push_back({s, 1u, r, (uint32_t)s.size()});
     } catch (const ParseError& e) {
       throw CompileError{WS_ERR_PATTERN_COMPILATION_FAILED,
                          "regex " + std::to_string(i) + ": " + e.msg};
@@ -321,7 +320,7 @@ PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_inse
     fac.push_back({ps.lits[l], 0u, l, std::min<uint32_t>((uint32_t)ps.lits[l].size(), kAcMaxLen)});
   for (auto& f : ps.fac) fac.push_back(f);
   ps.fac = std::move(fac);
-  ps.min_flen = 0xFFFFFFFFu;
+  ps.min_flen = ps.fac.empty() ? 0u : 0xFFFFFFFFu;
   ps.max_flen = 0;
   for (auto& f : ps.fac) {
     ps.min_flen = std::min<uint32_t>(ps.min_flen, (uint32_t)f.bytes.size());

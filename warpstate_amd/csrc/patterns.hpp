@@ -42,6 +42,11 @@ struct PatternsHost {
   std::vector<std::string> rx_src;  // regex source text, per regex
   std::vector<AnchoredDfa> rx;
   std::vector<long> rx_maxlen;  // -1 = unbounded
+  // Regexes without a selective prefix (their first byte admits too many
+  // values, or a match can be empty): every position of the input is a
+  // candidate start, walked by k_rx_scan instead of coming from the
+  // prefilter. Regex indices, ascending.
+  std::vector<uint32_t> rx_scan;
   std::vector<Factor> fac;
   uint32_t min_flen = 0, max_flen = 0, min_lit_len = 0;
   uint32_t max_match_len = 0;  // 0xFFFFFFFF when some regex is unbounded

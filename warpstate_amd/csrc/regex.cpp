@@ -63,6 +63,12 @@ struct Parser {
       bool flag_only = false;
       Ast atom = parse_atom(&flag_only);
       if (flag_only) continue;
+      if (atom.kind == Ast::LOOK) {
+        const int q = peek();
+        if (q == '*' || q == '+' || q == '?' || q == '{') fail("repetition of an assertion");
+        items.push_back(std::move(atom));
+        continue;
+      }
       items.push_back(parse_quant(std::move(atom)));
     }
     if (items.empty()) return Ast{};
@@ -179,10 +185,21 @@ struct Parser {
         if (!f.dotall) s.w['\n' >> 6] &= ~(1ull << ('\n' & 63));
         return bytes_node(s);
       }
-      case '^': case '$': fail("anchors are not supported on the GPU path");
+      case '^': case '$': {
+        ++i;
+        Ast a; a.kind = Ast::LOOK; a.look = c == '^' ? LOOK_BOL : LOOK_EOL;  // multi-line: line start / end
+        return a;
+      }
       case '*': case '+': case '?': case '{': fail("repetition operator missing expression");
       case '\\': {
         ++i;
+        const int e = peek();
+        if (e == 'b' || e == 'B' || e == 'A' || e == 'z') {
+          ++i;
+          Ast a; a.kind = Ast::LOOK;
+          a.look = e == 'b' ? LOOK_WORDB : e == 'B' ? LOOK_NWORDB : e == 'A' ? LOOK_BOT : LOOK_EOT;
+          return a;
+        }
         ByteSet s;
         if (parse_escape(&s, false)) return bytes_node(s);
         if (f.ci) fold_case(s);
@@ -247,9 +264,10 @@ struct Parser {
         s->set((int)v);
         return false;
       }
-      case 'b': case 'B': case 'A': case 'z': case '<': case '>':
-        if (!in_class) fail("word boundaries and anchors are not supported on the GPU path");
+      case 'b': case 'B': case 'A': case 'z':
         fail("invalid escape in class");
+      case '<': case '>':
+        fail("start / end of word assertions are not supported");
       case 'p': case 'P': fail("unicode classes are not supported in byte mode");
       default:
         if (c >= '0' && c <= '9') fail("backreferences and octal escapes are not supported");
@@ -368,6 +386,7 @@ struct NfaBuilder {
   int build(const Ast& a, int next) {
     switch (a.kind) {
       case Ast::EMPTY: return next;
+      case Ast::LOOK: return add(Nfa::LOOK, next, -1, a.look);
       case Ast::BYTES: return add(Nfa::BYTES, next, -1, set_id(a.set));
       case Ast::GROUP: return build(a.kids[0], next);
       case Ast::CONCAT: {
@@ -406,26 +425,17 @@ struct NfaBuilder {
   }
 };
 
-// Priority-ordered epsilon closure; stops after the first MATCH (leftmost-
-// first cut: lower-priority threads are dropped once a match is reached).
-void closure_ordered(const Nfa& n, const std::vector<int>& seeds, std::vector<int>* out,
-                     std::vector<uint32_t>& seen, uint32_t epoch) {
-  std::vector<int> stack;
-  for (int seed : seeds) {
-    stack.push_back(seed);
-    while (!stack.empty()) {
-      int s = stack.back();
-      stack.pop_back();
-      if (seen[s] == epoch) continue;
-      seen[s] = epoch;
-      const auto& st = n.st[s];
-      switch (st.t) {
-        case Nfa::SPLIT: stack.push_back(st.out1); stack.push_back(st.out); break;
-        case Nfa::EPS: stack.push_back(st.out); break;
-        case Nfa::MATCH: out->push_back(s); return;
-        case Nfa::BYTES: out->push_back(s); break;
-      }
-    }
+// Whether assertion `look` holds between a position's preceding context
+// (0 text start, 1 '\n', 2 word byte, 3 other byte) and its following one
+// (0 text end, else as dfa_ctx_of of the next byte).
+bool look_holds(int look, uint32_t prev, uint32_t next) {
+  switch (look) {
+    case LOOK_BOL: return prev == 0 || prev == 1;
+    case LOOK_EOL: return next == 0 || next == 1;
+    case LOOK_BOT: return prev == 0;
+    case LOOK_EOT: return next == 0;
+    case LOOK_WORDB: return (prev == 2) != (next == 2);
+    default: return (prev == 2) == (next == 2);
   }
 }
 
@@ -442,7 +452,7 @@ void closure_set(const Nfa& n, std::vector<int>& seeds, std::vector<int>* out, b
     const auto& st = n.st[s];
     switch (st.t) {
       case Nfa::SPLIT: stack.push_back(st.out1); stack.push_back(st.out); break;
-      case Nfa::EPS: stack.push_back(st.out); break;
+      case Nfa::EPS: case Nfa::LOOK: stack.push_back(st.out); break;
       case Nfa::MATCH: *match = true; break;
       case Nfa::BYTES: out->push_back(s); break;
     }
@@ -501,7 +511,7 @@ bool extract_literal(const Ast& a, std::string* out) {
 
 void length_bounds(const Ast& a, long* mn, long* mx) {
   switch (a.kind) {
-    case Ast::EMPTY: *mn = 0; *mx = 0; return;
+    case Ast::EMPTY: case Ast::LOOK: *mn = 0; *mx = 0; return;
     case Ast::BYTES: *mn = 1; *mx = 1; return;
     case Ast::GROUP: length_bounds(a.kids[0], mn, mx); return;
     case Ast::CONCAT: {
@@ -543,12 +553,31 @@ Nfa build_nfa(const Ast& a, size_t max_states) {
   return n;
 }
 
+bool has_look(const Nfa& n) {
+  for (const auto& st : n.st)
+    if (st.t == Nfa::LOOK) return true;
+  return false;
+}
+
 AnchoredDfa build_anchored_dfa(const Nfa& n, size_t max_states) {
   AnchoredDfa d;
-  // Byte classes: refine the 256 bytes by every byte set used in the NFA.
+  const bool look = has_look(n);
+  bool uses_prev = false;  // an assertion reads the byte before the position
+  for (const auto& st : n.st)
+    if (st.t == Nfa::LOOK && st.set != LOOK_EOL && st.set != LOOK_EOT) uses_prev = true;
+  // Byte classes: refine the 256 bytes by every byte set of the NFA and, when
+  // it has assertions, by '\n' and the word class (so a class fixes the
+  // context a byte gives its neighbours).
+  std::vector<ByteSet> refine(n.sets);
+  if (look) {
+    ByteSet nl;
+    nl.set('\n');
+    refine.push_back(nl);
+    refine.push_back(word_set());
+  }
   std::vector<int> cls(256, 0);
   int ncls = 1;
-  for (const auto& s : n.sets) {
+  for (const auto& s : refine) {
     std::map<std::pair<int, bool>, int> remap;
     int next = 0;
     std::vector<int> nc(256);
@@ -562,64 +591,86 @@ AnchoredDfa build_anchored_dfa(const Nfa& n, size_t max_states) {
     ncls = next;
   }
   d.nclass = (uint32_t)ncls;
+  const uint32_t W = d.nclass + 1;  // row width: the classes, then the end of the text
   std::vector<int> rep(ncls, -1);
   for (int b = 0; b < 256; ++b) {
     d.cmap[b] = (uint8_t)cls[b];
     if (rep[cls[b]] < 0) rep[cls[b]] = b;
   }
 
-  std::vector<uint32_t> seen(n.st.size(), 0);
-  uint32_t epoch = 0;
+  // A state is {context, ordered NFA seeds}: the NFA states reached by the
+  // bytes so far, before their epsilon closure, which depends on the bytes on
+  // both sides of the position. Without assertions that read the byte before,
+  // the context is always 0.
   std::unordered_map<std::vector<int>, uint32_t, VecHash> ids;
-  std::vector<std::vector<int>> states;
-  std::vector<char> is_match;
-  auto intern = [&](std::vector<int>&& v) -> uint32_t {
-    auto it = ids.find(v);
+  std::vector<std::vector<int>> keys(1);  // keys[0]: the dead state
+  auto intern = [&](uint32_t ctx, std::vector<int>&& seeds) -> uint32_t {
+    if (seeds.empty()) return 0;
+    seeds.insert(seeds.begin(), (int)(uses_prev ? ctx : 0u));
+    auto it = ids.find(seeds);
     if (it != ids.end()) return it->second;
-    if (states.size() >= max_states) throw ParseError{"regex DFA exceeds the state limit"};
-    uint32_t id = (uint32_t)states.size();
-    bool m = !v.empty() && n.st[v.back()].t == Nfa::MATCH;
-    ids.emplace(v, id);
-    states.push_back(std::move(v));
-    is_match.push_back(m);
+    if (keys.size() >= max_states) throw ParseError{"regex DFA exceeds the state limit"};
+    const uint32_t id = (uint32_t)keys.size();
+    ids.emplace(seeds, id);
+    keys.push_back(std::move(seeds));
     return id;
   };
-  intern(std::vector<int>{});  // 0 = dead
-  {
-    std::vector<int> start;
-    closure_ordered(n, {n.start}, &start, seen, ++epoch);
-    intern(std::move(start));  // 1 = start
-  }
-  std::vector<std::vector<uint32_t>> trans;
-  for (size_t s = 0; s < states.size(); ++s) {
-    std::vector<uint32_t> row(ncls, 0);
-    if (s != 0) {
-      for (int c = 0; c < ncls; ++c) {
-        int b = rep[c];
-        std::vector<int> seeds;
-        for (int ns : states[s]) {
-          const auto& st = n.st[ns];
-          if (st.t == Nfa::MATCH) break;
-          if (n.sets[st.set].has(b)) seeds.push_back(st.out);
+  for (uint32_t c = 0; c < 4; ++c) d.start[c] = intern(c, std::vector<int>{n.start});
+
+  // Priority-ordered epsilon closure of `seeds` at a position between
+  // contexts prev and next; stops at the first MATCH (leftmost-first: lower-
+  // priority threads are dropped once a match is reached) and reports it.
+  std::vector<uint32_t> seen(n.st.size(), 0);
+  uint32_t epoch = 0;
+  std::vector<int> stack;
+  auto closure = [&](const std::vector<int>& key, uint32_t prev, uint32_t next, std::vector<int>* out) -> bool {
+    ++epoch;
+    for (size_t k = 1; k < key.size(); ++k) {
+      stack.assign(1, key[k]);
+      while (!stack.empty()) {
+        const int s = stack.back();
+        stack.pop_back();
+        if (seen[s] == epoch) continue;
+        seen[s] = epoch;
+        const auto& st = n.st[s];
+        switch (st.t) {
+          case Nfa::SPLIT: stack.push_back(st.out1); stack.push_back(st.out); break;
+          case Nfa::EPS: stack.push_back(st.out); break;
+          case Nfa::LOOK: if (look_holds(st.set, prev, next)) stack.push_back(st.out); break;
+          case Nfa::MATCH: return true;
+          case Nfa::BYTES: out->push_back(s); break;
         }
-        std::vector<int> next;
-        closure_ordered(n, seeds, &next, seen, ++epoch);
-        row[c] = intern(std::move(next));
       }
     }
-    trans.push_back(std::move(row));
-  }
-  d.nstates = (uint32_t)states.size();
-  d.table.resize((size_t)d.nstates * ncls);
-  for (uint32_t s = 0; s < d.nstates; ++s)
-    for (int c = 0; c < ncls; ++c) {
-      uint32_t t = trans[s][c];
-      uint32_t e = t;
-      if (is_match[t]) e |= DFA_MATCH;
-      if (t == 0) e |= DFA_DEAD;
-      d.table[(size_t)s * ncls + c] = e;
+    return false;
+  };
+
+  std::vector<uint32_t> table(W, DFA_DEAD);  // the dead state's row
+  std::vector<int> cl;
+  for (size_t s = 1; s < keys.size(); ++s) {
+    const std::vector<int> key = keys[s];  // (keys grows below)
+    const uint32_t prev = (uint32_t)key[0];
+    const size_t row = table.size();
+    table.resize(row + W);
+    for (uint32_t c = 0; c < d.nclass; ++c) {
+      const int b = rep[c];
+      const uint32_t nb = dfa_ctx_of(b);
+      cl.clear();
+      const bool m = closure(key, prev, nb, &cl);
+      std::vector<int> seeds;
+      for (int x : cl)
+        if (n.sets[n.st[x].set].has(b)) seeds.push_back(n.st[x].out);
+      const uint32_t t = intern(nb, std::move(seeds));
+      table[row + c] = t | (m ? DFA_MATCH : 0u) | (t == 0 ? DFA_DEAD : 0u);
     }
-  d.start_match = is_match[1];
+    cl.clear();
+    table[row + d.nclass] = DFA_DEAD | (closure(key, prev, 0, &cl) ? DFA_MATCH : 0u);
+  }
+  d.nstates = (uint32_t)keys.size();
+  d.table = std::move(table);
+  for (uint32_t c = 0; c < 4; ++c)
+    for (uint32_t x = 0; x < W; ++x)
+      if (d.table[(size_t)d.start[c] * W + x] & DFA_MATCH) d.can_empty = true;
   return d;
 }
 

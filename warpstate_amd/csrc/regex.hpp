@@ -6,6 +6,11 @@
 // leftmost-first, greedy, non-overlapping per pattern
 // (core/src/dfa/scan.rs:433-457).
 //
+// Zero-width assertions: `^` and `$` at line starts and ends (multi-line),
+// `\A` and `\z` at the text's, `\b` and `\B` on the ASCII word class; and
+// regexes that can match the empty string, iterated as the regex crate's
+// find_iter does.
+//
 // Pipeline: parse -> AST -> Thompson NFA with priority-ordered splits ->
 //   * an anchored leftmost-first DFA (ordered-subset construction), walked on
 //     the device from each candidate start to find the match end;
@@ -35,8 +40,12 @@ struct ByteSet {
   bool operator==(const ByteSet& o) const { return w == o.w; }
 };
 
+// Assertion kinds (Ast::LOOK, Nfa::LOOK).
+enum Look : int { LOOK_BOL = 0, LOOK_EOL, LOOK_BOT, LOOK_EOT, LOOK_WORDB, LOOK_NWORDB };
+
 struct Ast {
-  enum Kind { EMPTY, BYTES, CONCAT, ALT, REPEAT, GROUP } kind = EMPTY;
+  enum Kind { EMPTY, BYTES, CONCAT, ALT, REPEAT, GROUP, LOOK } kind = EMPTY;
+  int look = 0;                // LOOK
   ByteSet set;                 // BYTES
   std::vector<Ast> kids;       // CONCAT / ALT / REPEAT(1) / GROUP(1)
   int rmin = 0, rmax = -1;     // REPEAT; rmax == -1 means unbounded
@@ -45,8 +54,9 @@ struct Ast {
 
 struct ParseError { std::string msg; };
 
-// Parses `pattern`; throws ParseError. Unsupported constructs (anchors, word
-// boundaries, class set operations, verbose mode) are reported as errors.
+// Parses `pattern`; throws ParseError. Unsupported constructs (look-around,
+// backreferences, class set operations, verbose mode, Unicode classes,
+// repeated assertions) are reported as errors.
 Ast parse_regex(const std::string& pattern, bool case_insensitive);
 
 // Nested repetition check of core/src/pattern/compiler.rs:381-405.
@@ -57,7 +67,8 @@ bool extract_literal(const Ast& a, std::string* out);
 void length_bounds(const Ast& a, long* mn, long* mx);
 
 struct Nfa {
-  enum Type : uint8_t { BYTES, SPLIT, EPS, MATCH };
+  enum Type : uint8_t { BYTES, SPLIT, EPS, MATCH, LOOK };
+  // BYTES: `set` indexes sets; LOOK: `set` is the Look kind.
   struct State { Type t; int out = -1, out1 = -1; int set = -1; };
   std::vector<State> st;
   std::vector<ByteSet> sets;
@@ -67,21 +78,39 @@ struct Nfa {
 // Builds the NFA; throws ParseError when it would exceed `max_states`.
 Nfa build_nfa(const Ast& a, size_t max_states);
 
-// Dense anchored leftmost-first DFA. State 0 is dead, state 1 is the start.
-// table[s * nclass + cls] = next | (match(next) << 31) | (dead(next) << 30).
+// Dense anchored leftmost-first DFA over byte classes, with the match
+// delayed by one byte (a state knows the byte before it; whether a match ends
+// at a position can depend on the byte after it). State 0 is dead. Rows have
+// nclass + 1 entries: column c < nclass is the transition on a byte of class
+// c, column nclass the end of the text. Entry = next | DFA_MATCH when a match
+// ends at the current position (before that byte) | DFA_DEAD when next is 0
+// (the end-of-text column is always DFA_DEAD).
+// A walk from position p starts in start[ctx], ctx being what precedes p:
+// 0 the text start, 1 '\n', 2 a word byte [0-9A-Za-z_], 3 any other byte.
 struct AnchoredDfa {
   uint32_t nstates = 0, nclass = 0;
   std::array<uint8_t, 256> cmap{};
-  std::vector<uint32_t> table;
-  bool start_match = false;
+  std::vector<uint32_t> table;  // nstates * (nclass + 1)
+  std::array<uint32_t, 4> start{};
+  bool can_empty = false;       // some start state matches before any byte (an empty match)
 };
 constexpr uint32_t DFA_MATCH = 0x80000000u;
 constexpr uint32_t DFA_DEAD = 0x40000000u;
 constexpr uint32_t DFA_STATE_MASK = 0x3FFFFFFFu;
+// The context a byte leaves for the position after it (see AnchoredDfa::start).
+inline uint32_t dfa_ctx_of(int b) {
+  if (b == '\n') return 1;
+  const bool w = (b >= '0' && b <= '9') || (b >= 'A' && b <= 'Z') || (b >= 'a' && b <= 'z') || b == '_';
+  return w ? 2 : 3;
+}
 
 AnchoredDfa build_anchored_dfa(const Nfa& n, size_t max_states);
 
-// Prefix-free factor strings (see file comment). Strings are extended while
+// Whether the NFA has assertions.
+bool has_look(const Nfa& n);
+
+// Prefix-free factor strings (see file comment). Assertions are taken as
+// satisfiable (the walk checks them). Strings are extended while
 // the next-byte fanout stays <= max_fanout, up to max_len bytes and
 // max_count strings in total.
 std::vector<std::string> extract_factors(const Nfa& n, int max_len, int max_fanout,
