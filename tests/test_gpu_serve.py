@@ -116,3 +116,61 @@ def test_admission_respects_the_byte_budget(world):
         nbytes += d * size[rid]
         assert live <= 3
         assert nbytes <= budget or live == 1
+
+
+def test_served_requests_equal_the_oracle_on_every_request_set():
+    # The request loop's own workload (core/src/router.rs:205-225 serves one
+    # AutoMatcher::scan per request): the 20 request pattern sets (10 to
+    # 10,000 patterns, literal fractions 1 to 0, ACGT k-mer sets among them),
+    # log-uniform request sizes over planted device-resident pools, Poisson
+    # arrivals, a random supported configuration per request. Every served
+    # list whose oracle work (bytes x (regexes + 1)) is under 2^28 is
+    # compared exactly with the oracle, and a seeded sample of 5 above it
+    # (up to 2^33).
+    from oracle.oracle import Oracle
+    from warpstate_amd.router import DevicePools, build_patternset, make_requests, request_pattern_sets
+
+    specs = request_pattern_sets()
+    sets = [build_patternset(sp) for sp in specs]
+    pools = DevicePools(specs, size=32 << 20, seed=31)
+    reqs = make_requests(300, specs, pools, seed=32)
+    infos = [GpuMatcher(s).info() for s in sets]
+    masks = [supported_mask(i) for i in infos]
+    rng = np.random.default_rng(33)
+
+    def route(batch):
+        out = []
+        for j in batch:
+            c = int(rng.choice(np.nonzero(masks[j.patterns])[0]))
+            out.append((c, config_tuning(c)))
+        return out
+
+    arrivals = poisson_arrivals(len(reqs), 3000.0, 34)
+    jobs = [Job(q.rid, q.set_index, pools.ptr(specs[q.set_index].kind, q.density, q.offset), q.nbytes, float(a),
+                density=pools.density[(specs[q.set_index].kind, q.density)]) for q, a in zip(reqs, arrivals)]
+    loop = RequestLoop(sets, route, slots=4, budget_bytes=64 << 20, keep_matches=True)
+    try:
+        outs = {o.rid: o for o in loop.run(jobs)}
+    finally:
+        loop.close()
+    assert len(outs) == len(jobs)
+
+    work = {q.rid: q.nbytes * (infos[q.set_index]["regex_count"] + 1) for q in reqs}
+    under = [q for q in reqs if work[q.rid] <= (1 << 28)]
+    over = [q for q in reqs if (1 << 28) < work[q.rid] <= (1 << 33)]
+    sample = [over[int(i)] for i in np.random.default_rng(35).choice(len(over), size=min(5, len(over)),
+                                                                      replace=False)]
+    assert len(under) > 150 and len({q.set_index for q in under}) == len(specs)
+    host = {key: buf.download(np.uint8, pools.size) for key, buf in pools.dev.items()}
+    oracles = {}
+    total = 0
+    for q in under + sample:
+        kind = specs[q.set_index].kind
+        data = host[(kind, q.density)][q.offset:q.offset + q.nbytes]
+        if q.set_index not in oracles:
+            oracles[q.set_index] = Oracle(specs[q.set_index].patterns)
+        exp = oracles[q.set_index].scan_mt(data, 0, 16)
+        got = outs[q.rid].matches
+        assert np.array_equal(got, exp), (q.rid, q.set_index, q.nbytes, outs[q.rid].config, len(got), len(exp))
+        total += len(exp)
+    assert total > 10_000
