@@ -226,11 +226,31 @@ ws_status ws_scan_noalloc(ws_ctx* ctx, const ws_patterns* patterns, const uint8_
 
 /* ---- batch: batch::scan_batch / scan_batch_gpu / scan_batch_cpu (core/src/batch.rs:170-363).
  * Every item is scanned on its own (no match crosses items). Item data is host memory
- * when data_mem is WS_MEM_HOST, else device pointers. One persistent launch. */
+ * when data_mem is WS_MEM_HOST, else device pointers. One persistent filter launch;
+ * overlapping-mode scans on the hash filter are device-sized (one synchronisation). */
 ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* patterns, const ws_item* items,
                         size_t n_items, uint32_t data_mem, uint32_t mode,
                         const ws_tuning* tuning, ws_tagged_match* out, uint64_t cap,
                         uint64_t* count);
+
+/* ---- batch plans: the host planning of scan_batch done once -- batch::coalesce's
+ * layout and planned_batches' grouping (core/src/batch.rs:65-104, 197-236) for
+ * items in device memory: id order, each item's segment, work units largest
+ * item first for every lane width, dense item positions and ids, all copied to
+ * the device. The items must stay in place, unchanged in length, while the plan
+ * is used. A plan may be scanned with any pattern set on its device, from any
+ * context on that device. */
+typedef struct ws_batch_plan ws_batch_plan;
+ws_status ws_batch_plan_create(ws_ctx* ctx, const ws_item* items, size_t n_items, ws_batch_plan** out);
+void ws_batch_plan_destroy(ws_batch_plan* plan);
+/* scan_batch over a plan (core/src/batch.rs:170-195): results as ws_scan_batch,
+ * into host or device memory (out_mem). Overlapping-mode scans on the hash
+ * filter are device-sized (one synchronisation) and a repeated call (same
+ * patterns, plan, output and tuning) replays a captured hipGraph; other modes
+ * and variants take the checked pipeline. */
+ws_status ws_scan_batch_planned(ws_ctx* ctx, const ws_patterns* patterns, const ws_batch_plan* plan,
+                                uint32_t mode, const ws_tuning* tuning, ws_tagged_match* out, uint64_t cap,
+                                uint32_t out_mem, uint64_t* count);
 
 /* ---- stream: StreamPipeline::scan (core/src/pipeline.rs:86-153).
  * Scans in chunks of chunk_size bytes; each chunk reads `overlap` extra bytes and keeps

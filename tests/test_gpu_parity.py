@@ -489,3 +489,59 @@ def test_repeat_calls_skip_init_only_when_state_is_clean():
         assert len(hits) == 1
     finally:
         ctx.set_profiling(False)
+
+
+@pytest.mark.parametrize("kind", ["mixed", "literals", "look"])
+def test_batch_plans_device_sized_replayed_and_device_output(kind):
+    # A batch plan scanned again and again (device-sized, then captured and
+    # replayed), into host and device memory, equals the unplanned batch and
+    # every item's own oracle scan; ids out of order and empty items.
+    from warpstate_amd import AutoMatcherConfig, BatchPlan, Context, scan_batch_device, scan_batch_planned
+    from warpstate_amd.device import DeviceBuffer
+
+    if kind == "mixed":
+        pats, _, specs = W.mixed_set(300, 100, seed=81)
+    elif kind == "literals":
+        pats, specs = [(0, l) for l in W.random_literals(500, 4, 20, W.LIT_ALPHABET, seed=82)], []
+    else:
+        pats, specs = [(1, r"^ab"), (1, r"b\b"), (1, r"^x*"), (0, b"abc")], []
+    rng = np.random.default_rng(83)
+    pieces = [p for k, p in pats if k == 0] + [s.sample(rng) for s in specs] + [b"ab", b"xab"]
+    ids, offs, sizes, buf = W.batch_items(2000, seed=84, pieces=pieces, plant_gap=500)
+    sizes = sizes.copy()
+    sizes[::53] = 0
+    ids = rng.permutation(np.arange(len(ids), dtype=np.uint64) * 5 + 3)
+    dev = DeviceBuffer.from_array(buf)
+    ptrs = (dev.ptr + offs).astype(np.uint64)
+    ctx = Context(0, AutoMatcherConfig())
+    m = GpuMatcher(build(pats), context=ctx)
+    o = Oracle(pats)
+    exp_l = []
+    for i in np.argsort(ids, kind="stable"):
+        chunk = buf[offs[i]:offs[i] + sizes[i]]
+        exp_l += [(int(ids[i]), int(p), int(s), int(e)) for p, s, e in
+                  zip(*[o.scan(chunk, 0)[f] for f in ("pattern_id", "start", "end")])]
+    assert len(exp_l) > 1000
+
+    def as_l(a):
+        return [(int(x), int(y), int(z), int(w)) for x, y, z, w in
+                zip(a["source_id"], a["pattern_id"], a["start"], a["end"])]
+
+    assert as_l(scan_batch_device(m, ids, ptrs, sizes)) == exp_l
+    plan = BatchPlan(ctx, ids, ptrs, sizes)
+    out = DeviceBuffer(24 * (len(exp_l) + 64))
+    for k in range(4):
+        assert as_l(scan_batch_planned(m, plan)) == exp_l, k
+        n = scan_batch_planned(m, plan, out_ptr=out.ptr, capacity=len(exp_l) + 64)
+        assert n == len(exp_l), k
+        assert as_l(out.download(np.uint8, 24 * n).view(N_TAGGED())) == exp_l, k
+    assert ctx.graph_count >= 2
+    # the checked pipeline over the same plan
+    assert as_l(scan_batch_planned(m, plan, tuning=Tuning(checked=True))) == exp_l
+    plan.close()
+
+
+def N_TAGGED():
+    from warpstate_amd import _native as N
+
+    return N.TAGGED_DTYPE

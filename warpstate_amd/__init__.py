@@ -17,6 +17,7 @@ from ._native import (  # noqa: F401
 )
 from .api import (  # noqa: F401
     AutoMatcherConfig,
+    BatchPlan,
     CompiledPatternIndex,
     Context,
     GpuMatcher,
@@ -36,6 +37,7 @@ from .api import (  # noqa: F401
     scan_batch,
     scan_batch_device,
     scan_batch_gpu,
+    scan_batch_planned,
 )
 
 Error = WarpstateError

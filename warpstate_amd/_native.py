@@ -163,6 +163,13 @@ SIGNATURES = {
         c_int,
         [c_void_p, c_void_p, c_void_p, c_uint64, POINTER(WsTuning), c_void_p, c_uint64, POINTER(c_uint64)],
     ),
+    "ws_batch_plan_create": (c_int, [c_void_p, POINTER(WsItem), c_size_t, POINTER(c_void_p)]),
+    "ws_batch_plan_destroy": (None, [c_void_p]),
+    "ws_scan_batch_planned": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_uint32, POINTER(WsTuning), c_void_p, c_uint64, c_uint32,
+         POINTER(c_uint64)],
+    ),
     "ws_scan_batch": (
         c_int,
         [c_void_p, c_void_p, POINTER(WsItem), c_size_t, c_uint32, c_uint32, POINTER(WsTuning), c_void_p,

@@ -703,6 +703,68 @@ def scan_batch_device(matcher: GpuMatcher, ids: np.ndarray, ptrs: np.ndarray, le
         return out[: count.value]
 
 
+class BatchPlan:
+    """A batch of device-resident items laid out once (batch::coalesce and
+    planned_batches, core/src/batch.rs:65-104,197-236): id order, segments,
+    work units and dense positions live on the device, so each
+    `scan_batch_planned` call over it does no host planning, is device-sized
+    and, when repeated with the same output, replays a captured hipGraph.
+    The items must stay where they are while the plan is used."""
+
+    def __init__(self, ctx: Context, ids: np.ndarray, ptrs: np.ndarray, lens: np.ndarray):
+        n = len(ids)
+        arr = (N.WsItem * n)()
+        view = np.frombuffer(arr, dtype=np.dtype([("id", "<u8"), ("data", "<u8"), ("len", "<u8")]), count=n)
+        view["id"] = ids
+        view["data"] = ptrs
+        view["len"] = lens
+        h = ctypes.c_void_p()
+        N.check(N.load().ws_batch_plan_create(ctx.handle, arr, n, ctypes.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.n_items = n
+        self.total = int(np.sum(lens))
+
+    def close(self) -> None:
+        if getattr(self, "h", None) and self.h.value:
+            N.load().ws_batch_plan_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scan_batch_planned(matcher: GpuMatcher, plan: BatchPlan, mode: str = "overlapping",
+                       tuning: Optional[Tuning] = None, out_ptr: Optional[int] = None,
+                       capacity: Optional[int] = None):
+    """scan_batch over a BatchPlan. With out_ptr (a device address holding
+    `capacity` TAGGED_DTYPE records) the matches stay on the device and the
+    count is returned; otherwise a TAGGED_DTYPE array comes back."""
+    lib = N.load()
+    t = tuning.to_c() if tuning else None
+    count = ctypes.c_uint64(0)
+    if out_ptr is not None:
+        st = lib.ws_scan_batch_planned(matcher.ctx.handle, matcher.handle, plan.h, _mode_code(mode),
+                                       ctypes.byref(t) if t else None, ctypes.c_void_p(out_ptr), capacity,
+                                       N.MEM_DEVICE, ctypes.byref(count))
+        N.check(st, int(count.value))
+        return int(count.value)
+    cap = capacity if capacity is not None else max(1024, min(matcher.config.max_matches, plan.total // 64 + 4096))
+    while True:
+        out = np.empty(cap, dtype=N.TAGGED_DTYPE)
+        st = lib.ws_scan_batch_planned(matcher.ctx.handle, matcher.handle, plan.h, _mode_code(mode),
+                                       ctypes.byref(t) if t else None, out.ctypes.data, cap, N.MEM_HOST,
+                                       ctypes.byref(count))
+        if st == 11 and capacity is None and count.value > cap and count.value <= matcher.config.max_matches:
+            cap = int(count.value)
+            continue
+        N.check(st, int(count.value))
+        return out[: count.value]
+
+
 def _tagged_list(arr: np.ndarray) -> List[TaggedMatch]:
     return [TaggedMatch(int(s), Match(int(p), int(a), int(b)))
             for s, p, a, b in zip(arr["source_id"], arr["pattern_id"], arr["start"], arr["end"])]

@@ -35,9 +35,9 @@ hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCo
 hipError_t launch_walk(uint32_t ncand_host, hipStream_t st, const WalkParams& w);
 hipError_t launch_rx_scan(hipStream_t st, const WalkParams& w);
 hipError_t launch_order_keys(const Rec* recs, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t pbits,
-                             uint64_t* keys, uint32_t* vals, hipStream_t st);
+                             uint64_t* keys, uint32_t* vals, hipStream_t st, const DenseOut& dense = DenseOut{});
 hipError_t launch_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t rbits,
-                             uint64_t* keys, uint32_t* vals, hipStream_t st);
+                             uint64_t* keys, uint32_t* vals, hipStream_t st, const uint64_t* dbase = nullptr);
 hipError_t launch_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
                               const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all,
                               uint8_t* head, hipStream_t st, const uint32_t* carry_src = nullptr,
@@ -53,22 +53,23 @@ hipError_t launch_pid_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint64_t
                            hipStream_t st);
 hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint32_t n, const uint32_t* nd,
                                uint32_t pbits, uint32_t start_off, void* out, uint64_t out_cap,
-                               const unsigned long long* base_dev, hipStream_t st);
+                               const unsigned long long* base_dev, hipStream_t st, const DenseOut& dense = DenseOut{});
 hipError_t launch_pack_tagged(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t sbits,
                               uint32_t pbits, const uint64_t* ids, void* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* lit_total,
                                      uint32_t* cand_total, uint32_t* lit_snap, uint32_t cap, uint32_t sbits,
                                      uint32_t pbits, uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift,
-                                     Rec* cout, uint32_t ccap, hipStream_t st);
+                                     Rec* cout, uint32_t ccap, hipStream_t st, const DenseOut& dense = DenseOut{});
 hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
                              uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift, uint32_t nb,
                              uint32_t* cnt, uint32_t* cursor, uint32_t* flag, uint32_t pbits, uint32_t start_off,
-                             void* out, uint64_t out_cap, const unsigned long long* base_dev, hipStream_t st);
+                             void* out, uint64_t out_cap, const unsigned long long* base_dev, hipStream_t st,
+                             const DenseOut& dense = DenseOut{});
 hipError_t launch_lit_order(const SliceOut& lit, uint32_t n, uint32_t pbits, uint32_t shift, uint32_t nb,
                             uint64_t* bkeys, uint32_t* bcnt, uint32_t* bofs, uint32_t* counters,
                             const uint32_t* flen, uint32_t start_off, void* out, uint64_t out_cap,
-                            const unsigned long long* base_dev, hipStream_t st);
+                            const unsigned long long* base_dev, hipStream_t st, const DenseOut& dense = DenseOut{});
 hipError_t launch_order_keys_dense(const Rec* recs, uint32_t n, uint32_t pbits, const uint64_t* bases,
                                    uint64_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t launch_pack_tagged_dense(const uint64_t* keys, const uint32_t* vals, uint32_t n, uint32_t pbits,
@@ -107,7 +108,10 @@ ws_status fail(ws_status s, const std::string& msg) {
 uint32_t bits_for(uint64_t v) { return v == 0 ? 1u : 64u - (uint32_t)__builtin_clzll(v); }
 
 // Bumped whenever a workspace is reallocated: captured graphs hold raw
-// device addresses, so a graph recorded under an older epoch is stale.
+// device addresses, so a graph recorded under an older epoch is stale. Each
+// context counts its own workspaces (DevBuf::epoch points at its counter),
+// so one context's growth does not make another's graphs stale; buffers of
+// no context (pattern blobs, batch plans) count here.
 std::atomic<uint64_t> g_buf_epoch{1};
 std::atomic<uint64_t> g_pattern_serial{1};
 
@@ -119,13 +123,14 @@ thread_local bool g_noalloc = false;
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  std::atomic<uint64_t>* epoch = &g_buf_epoch;
   hipError_t ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
     if (g_noalloc) return hipErrorOutOfMemory;
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    g_buf_epoch.fetch_add(1);
+    epoch->fetch_add(1);
     size_t want = std::max(bytes, (size_t)256);
     hipError_t e = hipMalloc(&p, want);
     if (e == hipSuccess) cap = want;
@@ -143,13 +148,14 @@ struct DevBuf {
 // from these, so an equal key can replay a captured graph.
 struct GraphKey {
   uint64_t pat_serial = 0, len = 0, chunk = 0, overlap = 0, cap = 0, epoch = 0;
+  uint64_t batch = 0;  // batch plan serial (0: a single input or a stream)
   const void* data = nullptr;
   const void* out = nullptr;
   uint32_t out_mem = 0, variant = 0, lane = 0, bpc = 0, profiling = 0;
   bool operator<(const GraphKey& o) const {
-    return std::tie(pat_serial, len, chunk, overlap, cap, epoch, data, out, out_mem, variant, lane, bpc, profiling) <
-           std::tie(o.pat_serial, o.len, o.chunk, o.overlap, o.cap, o.epoch, o.data, o.out, o.out_mem, o.variant,
-                    o.lane, o.bpc, o.profiling);
+    return std::tie(pat_serial, len, chunk, overlap, cap, epoch, batch, data, out, out_mem, variant, lane, bpc,
+                    profiling) < std::tie(o.pat_serial, o.len, o.chunk, o.overlap, o.cap, o.epoch, o.batch, o.data,
+                                          o.out, o.out_mem, o.variant, o.lane, o.bpc, o.profiling);
   }
 };
 
@@ -197,6 +203,8 @@ struct ws_ctx {
   std::map<GraphKey, GraphEntry> graphs;
   uint32_t* hcount = nullptr;  // pinned host mirror of the counters
   uint32_t last_hits = 0;      // filter hits of the last hash-variant scan
+  // Workspace generation of this context (its DevBufs count here).
+  std::atomic<uint64_t> epoch{1};
   // Stream chunks on the checked pipeline: which half of `carried` holds the
   // regex chains' state from the earlier chunks (the chain walk reads it and
   // raises the other half, then the halves swap).
@@ -416,7 +424,24 @@ struct Plan {
   bool use_carried = false;
   bool ctx_before = false;         // base[items[0].x - 1] is the byte before the segment (a later stream chunk)
   uint64_t max_seg_len = 0;
+  // A batch whose segments, work units (per lane width 16, 32, 64) and dense
+  // positions are already in device memory (a batch plan): device-sized runs
+  // take it as is.
+  const uint2* dev_items = nullptr;
+  const uint4* dev_units[3] = {};
+  uint32_t dev_nunits[3] = {};
+  DenseOut dense{};
+  uint64_t total_bytes = 0;  // bytes of all items
+  uint64_t total_pos = 0;    // dense positions: item bytes + one per item
 };
+
+uint32_t lane_index(uint32_t lane_bytes) { return lane_bytes == 64 ? 2u : lane_bytes == 32 ? 1u : 0u; }
+// Which of a batch plan's unit lists a variant uses: the hash filter's for
+// its lane width; the factor-DFA scan's 16 KiB units are the hash filter's at
+// 64 positions per lane.
+uint32_t unit_index(uint32_t variant, uint32_t lane_bytes) {
+  return variant == WS_VARIANT_HASH ? lane_index(lane_bytes) : 2u;
+}
 
 // Candidate positions per regex of the every-position walk (k_rx_scan): each
 // item's bytes and its end (an empty match may start there); a stream chunk
@@ -706,7 +731,7 @@ hipError_t ensure_hstate(ws_ctx* ctx, size_t bytes) {
   if (ctx->hstate) (void)hipHostFree(ctx->hstate);
   ctx->hstate = nullptr;
   ctx->hstate_cap = 0;
-  g_buf_epoch.fetch_add(1);
+  ctx->epoch.fetch_add(1);
   const size_t want = std::max<size_t>(bytes, 4096);
   hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hstate), want, hipHostMallocCoherent);
   if (e == hipSuccess) ctx->hstate_cap = want;
@@ -723,7 +748,7 @@ hipError_t ensure_stage(ws_ctx* ctx, size_t bytes) {
   }
   ctx->hstage = nullptr;
   ctx->hstage_cap = 0;
-  g_buf_epoch.fetch_add(1);
+  ctx->epoch.fetch_add(1);
   const size_t want = std::max<size_t>(bytes, 4096);
   hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hstage), want, hipHostMallocDefault);
   if (e == hipSuccess) ctx->hstage_cap = want;
@@ -949,6 +974,7 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       if ((size_t)nlit + nraw > ctx->recs.cap / sizeof(Rec)) {
         // Grow while keeping the literal matches.
         DevBuf bigger;
+        bigger.epoch = ctx->recs.epoch;
         WS_HIP(bigger.ensure(((size_t)nlit + nraw) * sizeof(Rec)));
         WS_HIP(hipMemcpyAsync(bigger.p, ctx->recs.p, (size_t)nlit * sizeof(Rec),
                               hipMemcpyDeviceToDevice, st));
@@ -1109,6 +1135,7 @@ struct RunSetup {
   uint32_t lit_cap = 0, cand_cap = 0, raw_cap = 0, rec_cap = 0, max_matches = 0;
   uint64_t lslice_total = 0, cslice_total = 0;
   uint32_t sbits = 0, pbits = 0, rbits = 0, obits = 0, cbits = 0, ob = 0, cb = 0;
+  uint32_t gbits = 0;  // a batch: bits of a dense position (chain and ordering keys)
   uint32_t lb = 0;       // literal-only ordering: 2^lb fixed-capacity buckets (0: not used)
   size_t items_off = 0;  // this run's index among the call's runs (its segment, its stamps)
   unsigned long long* stamps = nullptr;  // the call's kernel stamps (profiling), two per run
@@ -1126,15 +1153,21 @@ struct RunLimits {
 ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const ws_tuning* tuning,
                       RunSetup* rs, const RunLimits* lim = nullptr) {
   const PatternsHost& h = p->h;
-  if (plan.batch) return fail(WS_ERR_INVALID_ARGUMENT, "device-sized runs take single segments");
+  if (plan.batch && !plan.dev_items)
+    return fail(WS_ERR_INVALID_ARGUMENT, "device-sized runs take single segments or batch plans");
   rs->variant = pick_variant(p, tuning);
   if (!(supported_variants(p) & (1u << rs->variant)))
     return fail(WS_ERR_INVALID_ARGUMENT, "kernel variant not available for this pattern set");
   rs->lane_bytes = choose_lane_bytes(plan, tuning);
   const uint32_t unit = rs->variant == WS_VARIANT_HASH ? kTPB * rs->lane_bytes : ac_unit_bytes();
+  if (plan.batch && rs->variant != WS_VARIANT_HASH && unit != kTPB * 64u)
+    return fail(WS_ERR_GPU_DEVICE, "factor-DFA work units differ from the batch plan's");
+  const uint32_t ui = unit_index(rs->variant, rs->lane_bytes);
   std::vector<uint2> none;
-  rs->nunits = plan_units(plan, unit, &none);
-  const uint64_t span = plan.items[0].y - plan.items[0].x;
+  rs->nunits = plan.batch ? plan.dev_nunits[ui] : plan_units(plan, unit, &none);
+  if (plan.batch && !plan.dev_units[ui] && rs->nunits)
+    return fail(WS_ERR_INVALID_ARGUMENT, "batch plan has no work units for this lane width");
+  const uint64_t span = plan.batch ? plan.total_bytes : plan.items[0].y - plan.items[0].x;
   rs->max_matches = (uint32_t)std::min<uint64_t>(ctx->cfg.max_matches, 0xFFFFFFF0ull);
   rs->lit_cap = (uint32_t)std::min<uint64_t>(rs->max_matches, std::max<uint64_t>(1u << 16, span / 16));
   rs->cand_cap = (uint32_t)std::min<uint64_t>(0x0FFFFFFFull, std::max<uint64_t>(1u << 16, span / 64));
@@ -1175,8 +1208,13 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   rs->sbits = bits_for(plan.max_seg_len);
   rs->pbits = bits_for(h.npat ? h.npat - 1 : 0);
   rs->rbits = nrx ? bits_for(nrx - 1) : 0;
-  rs->obits = rs->sbits + rs->pbits;
-  rs->cbits = rs->sbits + rs->rbits;
+  // Ordering keys: start (one segment) or dense position (a batch), then the
+  // pattern id; chain keys: item, regex, start.
+  // A batch's keys are dense positions: ordering (position, pattern id),
+  // chains (regex, position), each (item, regex) group contiguous.
+  rs->gbits = plan.batch ? bits_for(plan.total_pos ? plan.total_pos - 1 : 0) : 0;
+  rs->obits = (plan.batch ? rs->gbits : rs->sbits) + rs->pbits;
+  rs->cbits = (plan.batch ? rs->gbits : rs->sbits) + rs->rbits;
   if (rs->obits > 63 || rs->cbits > 63) return fail(WS_ERR_INPUT_TOO_LARGE, "match key does not fit 63 bits");
   // Bucket counts follow the capacities up to 2^16 buckets (16 pairs per
   // bucket at a million matches); denser runs fall back through the
@@ -1226,14 +1264,16 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   const hipStream_t st = ctx->stream;
   const uint32_t nrx = (uint32_t)h.rx.size();
   uint32_t* cnt = ctx->counters.as<uint32_t>();
-  // This run's segment (all runs' segments were copied in one go).
-  const uint2* items = ctx->items.as<uint2>() + rs.items_off;
+  // This run's segment (all runs' segments were copied in one go), or a
+  // batch plan's segments.
+  const uint2* items = plan.dev_items ? plan.dev_items : ctx->items.as<uint2>() + rs.items_off;
   // Filter and verification.
   if (ev) WS_HIP(hipEventRecord(ev[0], st));
   // cnt[0..10] are zero here: cleared before the first run, and by
   // k_run_close after each one.
   ScanCommon c = make_common(ctx, p, plan, rs.nunits, rs.lit_cap, rs.cand_cap);
   c.items = items;
+  if (plan.batch) c.units = plan.dev_units[unit_index(rs.variant, rs.lane_bytes)];
   if (ev && rs.stamps) c.stamp = rs.stamps + 2 * rs.items_off;
   RunCaps caps{};
   caps.lit_cap = rs.lit_cap;
@@ -1259,7 +1299,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
       caps.cand_ovf_cap = cso.ovf_cap;
       WS_HIP(launch_lit_order(lso, rs.nvb, rs.pbits, rs.obits - rs.lb, 1u << rs.lb, ctx->lkeys.as<uint64_t>(),
                               ctx->lcnt.as<uint32_t>(), ctx->lofs.as<uint32_t>(), cnt, p->flen, start_off, ds.out,
-                              ds.out_cap, ds.total, st));
+                              ds.out_cap, ds.total, st, plan.dense));
       WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
       if (ev) WS_HIP(hipEventRecord(ev[4], st));
       return WS_OK;
@@ -1270,7 +1310,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     WS_HIP(launch_slice_compact_keys(lso, cso, rs.nvb, cnt + 0, cnt + 1, cnt + kCounterLitSnapshot, c.lit_cap,
                                      rs.sbits, rs.pbits, (nrx ? ctx->k2 : ctx->k0).as<uint64_t>(),
                                      (nrx ? ctx->v2 : ctx->v0).as<uint32_t>(), ctx->bcnt.as<uint32_t>(),
-                                     rs.obits - rs.ob, nrx ? c.cand_out : nullptr, c.cand_cap, st));
+                                     rs.obits - rs.ob, nrx ? c.cand_out : nullptr, c.cand_cap, st, plan.dense));
     keyed = true;
     caps.lit_ovf_cap = lso.ovf_cap;
     caps.cand_ovf_cap = cso.ovf_cap;
@@ -1288,7 +1328,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   // Regex candidates: anchored walks, chain ordering, greedy chains appended
   // after the literal matches.
   if (nrx) {
-    WalkParams w = walk_params(p, plan, items, cnt, nullptr);
+    WalkParams w = walk_params(p, plan, items, cnt, plan.dense.bases);
     w.cands = ctx->cands.as<Rec>();
     w.ncand = cnt + 1;
     w.cand_cap = rs.cand_cap;
@@ -1297,19 +1337,22 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     // Hash variant: the walk writes the chain pairs and counts them (bcnt2),
     // the chain walk writes the kept matches' final pairs (k2/v2, bcnt).
     uint32_t* ccnt = keyed ? ctx->bcnt2.as<uint32_t>() : ctx->bcnt.as<uint32_t>();
+    const uint32_t ksbits = plan.batch ? rs.gbits : rs.sbits;  // chain key position bits
+    const uint32_t gshift = plan.batch ? kGroupByRecord : rs.sbits;
     if (keyed) {
       w.ckeys = ctx->k0.as<uint64_t>();
       w.cvals = ctx->v0.as<uint32_t>();
       w.ccnt = ccnt;
       w.cshift = rs.cbits - rs.cb;
-      w.sbits = rs.sbits;
+      w.sbits = ksbits;
       w.rbits = rs.rbits;
+      w.cbase = plan.batch ? plan.dense.bases : nullptr;
     }
     WS_HIP(launch_walk(0, st, w));
     if (w.nscan) WS_HIP(launch_rx_scan(st, w));
     if (!keyed)
-      WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), rs.raw_cap, cnt + 2, rs.sbits, rs.rbits, ctx->k0.as<uint64_t>(),
-                               ctx->v0.as<uint32_t>(), st));
+      WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), rs.raw_cap, cnt + 2, ksbits, rs.rbits, ctx->k0.as<uint64_t>(),
+                               ctx->v0.as<uint32_t>(), st, plan.batch ? plan.dense.bases : nullptr));
     size_t have = ctx->tmp.cap;
     WS_HIP(bucket_sort_pairs(ctx->tmp.p, &have, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
                              ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), rs.raw_cap, cnt + 2, rs.cbits - rs.cb,
@@ -1319,7 +1362,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     uint32_t* cin = plan.use_carried ? ctx->carried.as<uint32_t>() + (rs.items_off & 1) * nrx : nullptr;
     uint32_t* cout = plan.use_carried ? ctx->carried.as<uint32_t>() + ((rs.items_off + 1) & 1) * nrx : nullptr;
     WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), rs.raw_cap,
-                              cnt + 2, rs.sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st, cin, cout, nrx));
+                              cnt + 2, gshift, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st, cin, cout, nrx));
     OrderOut oo{};
     if (keyed) {
       oo.keys = ctx->k2.as<uint64_t>();
@@ -1328,9 +1371,10 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
       oo.shift = rs.obits - rs.ob;
       oo.sbits = rs.sbits;
       oo.pbits = rs.pbits;
+      oo.dense = plan.dense;
     }
     WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), rs.raw_cap,
-                             cnt + 2, rs.sbits, ctx->head.as<uint8_t>(), cin, plan.seg_off, p->rx_pid, 0u,
+                             cnt + 2, gshift, ctx->head.as<uint8_t>(), cin, plan.seg_off, p->rx_pid, 0u,
                              ctx->recs.as<Rec>(), rs.rec_cap, cnt + 0, cout, st, keyed ? &oo : nullptr));
   }
   if (ev) WS_HIP(hipEventRecord(ev[3], st));
@@ -1345,19 +1389,19 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
                              fv, nrx ? ctx->v0.as<uint32_t>() : ctx->v1.as<uint32_t>(), rs.rec_cap, cnt + 0,
                              rs.obits - rs.ob,
                              1u << rs.ob, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, rs.pbits,
-                             start_off, ds.out, ds.out_cap, ds.total, st));
+                             start_off, ds.out, ds.out_cap, ds.total, st, plan.dense));
     WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
     if (ev) WS_HIP(hipEventRecord(ev[4], st));
     return WS_OK;
   }
   WS_HIP(launch_order_keys(ctx->recs.as<Rec>(), rs.rec_cap, cnt + 0, rs.sbits, rs.pbits, ctx->k0.as<uint64_t>(),
-                           ctx->v0.as<uint32_t>(), st));
+                           ctx->v0.as<uint32_t>(), st, plan.dense));
   size_t have = ctx->tmp.cap;
   WS_HIP(bucket_sort_pairs(ctx->tmp.p, &have, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
                            ctx->v0.as<uint32_t>(), ctx->v1.as<uint32_t>(), rs.rec_cap, cnt + 0, rs.obits - rs.ob,
                            1u << rs.ob, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, st));
   WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), rs.rec_cap, cnt + 0, rs.pbits,
-                             start_off, ds.out, ds.out_cap, ds.total, st));
+                             start_off, ds.out, ds.out_cap, ds.total, st, plan.dense));
   WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
   if (ev) WS_HIP(hipEventRecord(ev[4], st));
   return WS_OK;
@@ -1376,13 +1420,17 @@ bool device_path_ok(uint32_t mode, const ws_tuning* tuning) {
 // and replayed from then on (the steady state of repeated scans and
 // streams). *outcome = kDeviceScanRedo when a buffer overflowed; nothing was
 // then written to a host `out`.
-ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>& plans,
-                      const std::vector<uint32_t>& offs, const ws_tuning* tuning, ws_match* out, uint64_t cap,
-                      uint32_t out_mem, uint64_t* count, DeviceScanResult* outcome, const GraphKey* key_in) {
+// A batch plan (one run, plans[0].batch) writes ws_tagged_match records
+// (rec_bytes = sizeof(ws_tagged_match)); its segments are already on the
+// device.
+ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size_t nplans, const uint32_t* offs,
+                      const ws_tuning* tuning, void* out, uint64_t cap, uint32_t out_mem, uint64_t* count,
+                      DeviceScanResult* outcome, const GraphKey* key_in, size_t rec_bytes = sizeof(ws_match)) {
   const hipStream_t st = ctx->stream;
   *outcome = kDeviceScanDone;
+  const bool batch = nplans == 1 && plans[0].batch;
   // Everything sized and staged before the first launch.
-  std::vector<RunSetup> setups(plans.size());
+  std::vector<RunSetup> setups(nplans);
   const uint64_t max_matches = ctx->cfg.max_matches;
   DeviceRunState ds;
   if (out_mem == WS_MEM_DEVICE) {
@@ -1390,12 +1438,12 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     ds.out_cap = cap;
   } else {
     ds.out_cap = std::min<uint64_t>(cap, max_matches);
-    WS_HIP(ctx->outbuf.ensure(std::max<uint64_t>(ds.out_cap, 1) * sizeof(ws_match)));
+    WS_HIP(ctx->outbuf.ensure(std::max<uint64_t>(ds.out_cap, 1) * rec_bytes));
     ds.out = ctx->outbuf.p;
   }
   // The device state block: sticky flags and the running total (64 bytes),
   // then (profiling) two kernel stamps per run; one copy reads it back.
-  const size_t nstamp = ctx->profiling ? plans.size() : 0;
+  const size_t nstamp = ctx->profiling ? nplans : 0;
   const size_t state_bytes = 64 + nstamp * 2 * sizeof(unsigned long long);
   WS_HIP(ctx->dstate.ensure(state_bytes));
   ds.sticky = ctx->dstate.as<uint32_t>();
@@ -1403,14 +1451,14 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
   unsigned long long* dstamps = ctx->profiling ? reinterpret_cast<unsigned long long*>(ds.sticky + 16) : nullptr;
   // Pinned staging for the runs' items when they do not fit the init
   // kernel's arguments; the state block comes back through ctx->hstate.
-  const bool items_by_arg = plans.size() <= kCallItemsMax;
-  if (!items_by_arg) WS_HIP(ensure_stage(ctx, plans.size() * sizeof(uint2)));
+  const bool items_by_arg = nplans <= kCallItemsMax;
+  if (!items_by_arg) WS_HIP(ensure_stage(ctx, nplans * sizeof(uint2)));
   WS_HIP(ensure_hstate(ctx, state_bytes + 4));
   // The completion flag after the state block (set by the last kernel).
   volatile uint32_t* done_flag = ctx->hstate + state_bytes / 4;
   *done_flag = 0u;
   std::atomic_thread_fence(std::memory_order_seq_cst);
-  WS_HIP(ctx->items.ensure(plans.size() * sizeof(uint2)));
+  if (!batch) WS_HIP(ctx->items.ensure(nplans * sizeof(uint2)));
   // A call whose launch sequence is already captured skips the per-run setup:
   // nothing it sizes has changed since the capture (a reallocation bumps
   // the buffer epoch, which is part of the key).
@@ -1419,33 +1467,35 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
   auto lookup = [&]() {
     if (!key_in) return;
     key = *key_in;
-    key.epoch = g_buf_epoch.load();
+    key.epoch = ctx->epoch.load();
     ge = &ctx->graphs[key];
   };
   lookup();
   const bool captured = ge && ge->exec;
   CallItems ci{};
-  ci.n = items_by_arg ? (uint32_t)plans.size() : 0u;
-  for (size_t k = 0; k < plans.size(); ++k) {
+  ci.n = items_by_arg && !batch ? (uint32_t)nplans : 0u;
+  for (size_t k = 0; k < nplans; ++k) {
     if (!captured) {
       ws_status s = prepare_run(ctx, p, plans[k], tuning, &setups[k]);
       if (s != WS_OK) return s;
     }
     setups[k].items_off = k;
     setups[k].stamps = dstamps;
-    if (k + 1 == plans.size()) {
+    if (k + 1 == nplans) {
       setups[k].publish = ctx->hstate;
       setups[k].publish_words = (uint32_t)(state_bytes / 4);
     }
+    if (batch)
+      continue;  // the plan's segments are on the device
     if (items_by_arg)
       ci.v[k] = plans[k].items[0];
     else
       std::memcpy(ctx->hstage + k * sizeof(uint2), plans[k].items.data(), sizeof(uint2));
   }
   if (!captured) lookup();  // the setup may have grown buffers
-  if (ctx->profiling && ctx->pev.size() < 5 * plans.size()) {
+  if (ctx->profiling && ctx->pev.size() < 5 * nplans) {
     const size_t had = ctx->pev.size();
-    ctx->pev.resize(5 * plans.size(), nullptr);
+    ctx->pev.resize(5 * nplans, nullptr);
     for (size_t k = had; k < ctx->pev.size(); ++k) WS_HIP(hipEventCreate(&ctx->pev[k]));
   }
   // The call's init (state reset, and the runs' segments when they fit its
@@ -1459,8 +1509,8 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
   };
   auto launch_all = [&]() -> ws_status {
     if (!items_by_arg)
-      WS_HIP(hipMemcpyAsync(ctx->items.p, ctx->hstage, plans.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
-    for (size_t k = 0; k < plans.size(); ++k) {
+      WS_HIP(hipMemcpyAsync(ctx->items.p, ctx->hstage, nplans * sizeof(uint2), hipMemcpyHostToDevice, st));
+    for (size_t k = 0; k < nplans; ++k) {
       ws_status s = launch_run(ctx, p, plans[k], setups[k], offs[k], ds,
                                ctx->profiling ? &ctx->pev[5 * k] : nullptr);
       if (s != WS_OK) return s;
@@ -1556,13 +1606,13 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     // (then -1: unknown).
     ws_timings t{};
     const unsigned long long* stamps = reinterpret_cast<const unsigned long long*>(hs + 16);
-    for (size_t k = 0; k < plans.size(); ++k)
+    for (size_t k = 0; k < nplans; ++k)
       if (stamps[2 * k + 1] > stamps[2 * k])
         t.prefilter_ms += (float)((double)(stamps[2 * k + 1] - stamps[2 * k]) / ctx->wall_khz);
     if (replayed) {
       t.verify_ms = t.regex_ms = t.order_ms = -1.0f;
     } else {
-      for (size_t k = 0; k < plans.size(); ++k) {
+      for (size_t k = 0; k < nplans; ++k) {
         hipEvent_t* ev = &ctx->pev[5 * k];
         float b = 0, c2 = 0, d = 0;
         (void)hipEventElapsedTime(&b, ev[1], ev[2]);
@@ -1592,7 +1642,7 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const std::vector<Plan>
     return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(tot) + " exceeds " +
                                                   std::to_string(std::min<uint64_t>(cap, max_matches)) + ")");
   if (out_mem == WS_MEM_HOST && tot && out) {  // (no host array: timing runs)
-    WS_HIP(hipMemcpyAsync(out, ds.out, tot * sizeof(ws_match), hipMemcpyDeviceToHost, st));
+    WS_HIP(hipMemcpyAsync(out, ds.out, tot * rec_bytes, hipMemcpyDeviceToHost, st));
     WS_HIP(hipStreamSynchronize(st));
   }
   return WS_OK;
@@ -1799,6 +1849,192 @@ struct ScratchViews {
 
 }  // namespace
 
+// A batch laid out for the device: items in source-id order (stable; batch
+// output order is (source_id, start, pattern_id, end), core/src/batch.rs:159-165),
+// each item's segment relative to one 16-aligned base (host items coalesced
+// into the context's input buffer, each 16-byte aligned), the ids in that
+// order and the dense positions (item length + 1 each).
+struct BatchLayout {
+  Plan plan;
+  std::vector<uint64_t> ids;
+  std::vector<uint64_t> bases;  // nitems + 1
+};
+
+ws_status layout_batch(ws_ctx* ctx, const ws_item* items, size_t n_items, uint32_t data_mem, BatchLayout* L) {
+  // One pass checks the id order and, for device items, finds their span;
+  // `order` is built only when the ids are out of order.
+  bool sorted = true;
+  uintptr_t lo = UINTPTR_MAX, hi = 0;
+  for (size_t k = 0; k < n_items; ++k) {
+    const ws_item& it = items[k];
+    if (k && items[k - 1].id > it.id) sorted = false;
+    if (!it.len) continue;
+    if (!it.data) return fail(WS_ERR_INVALID_ARGUMENT, "item data is null");
+    lo = std::min(lo, reinterpret_cast<uintptr_t>(it.data));
+    hi = std::max(hi, reinterpret_cast<uintptr_t>(it.data) + (uintptr_t)it.len);
+  }
+  std::vector<uint32_t> order;
+  if (!sorted) {
+    // (id, index) pairs sort contiguously; equal ids keep their input order.
+    std::vector<std::pair<uint64_t, uint32_t>> key(n_items);
+    for (size_t k = 0; k < n_items; ++k) key[k] = {items[k].id, (uint32_t)k};
+    std::sort(key.begin(), key.end());
+    order.resize(n_items);
+    for (size_t k = 0; k < n_items; ++k) order[k] = key[k].second;
+  }
+  auto item_at = [&](size_t k) -> const ws_item& { return items[sorted ? k : order[k]]; };
+  Plan& plan = L->plan;
+  plan.batch = true;
+  plan.items.resize(n_items);
+  L->ids.resize(n_items);
+  uint64_t max_len = 0;
+  if (data_mem == WS_MEM_HOST) {
+    uint64_t tot = 0;
+    for (size_t k = 0; k < n_items; ++k) {
+      const uint64_t len = item_at(k).len;
+      plan.items[k] = make_uint2((uint32_t)tot, (uint32_t)(tot + len));
+      tot += (len + 15) & ~15ull;
+      if (tot > kMaxScanBytes) return fail(WS_ERR_INPUT_TOO_LARGE, "coalesced batch exceeds 4 GiB - 1 MiB");
+    }
+    std::vector<uint8_t> host(tot, 0);
+    for (size_t k = 0; k < n_items; ++k) {
+      const ws_item& it = item_at(k);
+      if (it.len) std::memcpy(host.data() + plan.items[k].x, it.data, it.len);
+      L->ids[k] = it.id;
+      max_len = std::max<uint64_t>(max_len, it.len);
+    }
+    WS_HIP(ctx->input.ensure(tot + 16));
+    if (tot) WS_HIP(hipMemcpyAsync(ctx->input.p, host.data(), tot, hipMemcpyHostToDevice, ctx->stream));
+    WS_HIP(hipStreamSynchronize(ctx->stream));
+    plan.base = ctx->input.as<uint8_t>();
+    plan.load_end = tot;
+  } else {
+    if (lo == UINTPTR_MAX) lo = hi = 0;
+    const uintptr_t base = lo & ~(uintptr_t)15;
+    if (hi - base > kMaxScanBytes)
+      return fail(WS_ERR_INPUT_TOO_LARGE, "device batch items span more than 4 GiB - 1 MiB");
+    for (size_t k = 0; k < n_items; ++k) {
+      const ws_item& it = item_at(k);
+      const uint64_t o = it.len ? reinterpret_cast<uintptr_t>(it.data) - base : 0;
+      plan.items[k] = make_uint2((uint32_t)o, (uint32_t)(o + it.len));
+      L->ids[k] = it.id;
+      max_len = std::max<uint64_t>(max_len, it.len);
+    }
+    plan.base = reinterpret_cast<const uint8_t*>(base);
+    plan.load_end = (hi - base + 15) & ~15ull;
+  }
+  plan.max_seg_len = max_len;
+  L->bases.resize(n_items + 1);
+  uint64_t acc = 0, bytes = 0;
+  for (size_t k = 0; k < n_items; ++k) {
+    L->bases[k] = acc;
+    const uint64_t len = plan.items[k].y - plan.items[k].x;
+    acc += len + 1;
+    bytes += len;
+  }
+  L->bases[n_items] = acc;
+  plan.total_bytes = bytes;
+  plan.total_pos = acc;
+  return WS_OK;
+}
+
+// Copies a layout's segments, dense positions, ids and the work units of
+// the lane widths in `lanes` (bit k: 16 << k positions per lane) into the
+// given buffers on the context's stream (through pinned staging), and points
+// the layout's plan at them.
+ws_status upload_batch(ws_ctx* ctx, BatchLayout& L, DevBuf& items, DevBuf& bases, DevBuf& ids, DevBuf* const* units,
+                       uint32_t lanes) {
+  Plan& plan = L.plan;
+  const size_t n = plan.items.size();
+  WS_HIP(items.ensure(n * sizeof(uint2)));
+  WS_HIP(h2d_staged(ctx, items.p, plan.items.data(), n * sizeof(uint2)));
+  WS_HIP(bases.ensure((n + 1) * 8));
+  WS_HIP(h2d_staged(ctx, bases.p, L.bases.data(), (n + 1) * 8));
+  WS_HIP(ids.ensure(n * 8));
+  WS_HIP(h2d_staged(ctx, ids.p, L.ids.data(), n * 8));
+  plan.dev_items = items.as<uint2>();
+  plan.dense = DenseOut{bases.as<uint64_t>(), ids.as<uint64_t>(), (uint32_t)n};
+  for (uint32_t k = 0; k < 3; ++k) {
+    if (!(lanes & (1u << k))) continue;
+    const uint32_t unit = kTPB * (16u << k);
+    std::vector<uint2> runs;
+    const uint32_t nunits = plan_units(plan, unit, &runs);
+    DevBuf& u = *units[k];
+    WS_HIP(u.ensure((size_t)nunits * sizeof(uint4) + runs.size() * sizeof(uint2) + 16));
+    uint2* druns = reinterpret_cast<uint2*>(u.as<uint4>() + nunits);
+    WS_HIP(h2d_staged(ctx, druns, runs.data(), runs.size() * sizeof(uint2)));
+    if (nunits)
+      WS_HIP(launch_expand_units(druns, (uint32_t)runs.size(), items.as<uint2>(), unit, u.as<uint4>(), nunits,
+                                 ctx->stream));
+    plan.dev_units[k] = u.as<uint4>();
+    plan.dev_nunits[k] = nunits;
+  }
+  return WS_OK;
+}
+
+// Batches take the device-sized pipeline in overlapping mode (the standard
+// mode's reductions run on the checked pipeline).
+bool batch_device_ok(const ws_patterns* p, uint32_t mode, const ws_tuning* tuning) {
+  return device_path_ok(mode, tuning) && (supported_variants(p) & (1u << pick_variant(p, tuning)));
+}
+
+// The checked pipeline over a batch layout; the tagged matches go to `out`
+// (host or device memory).
+ws_status batch_checked(ws_ctx* ctx, const ws_patterns* p, const BatchLayout& L, uint32_t mode,
+                        const ws_tuning* tuning, ws_tagged_match* out, uint64_t cap, uint32_t out_mem,
+                        uint64_t* count) {
+  Result r;
+  ws_status s = run_plan(ctx, p, L.plan, mode, tuning, &r);
+  if (s != WS_OK) {
+    if (s == WS_ERR_MATCH_BUFFER_OVERFLOW) *count = r.count;
+    return s;
+  }
+  *count = r.count;
+  const uint32_t n = (uint32_t)std::min<uint64_t>(r.count, cap);
+  if (r.bucketed) {
+    // The bucket sort's overflow flag came back with this synchronisation.
+    WS_HIP(hipStreamSynchronize(ctx->stream));
+    bool redone = false;
+    s = confirm_order(ctx, r, &redone);
+    if (s != WS_OK) return s;
+  }
+  if (n) {
+    WS_HIP(ctx->ids.ensure(L.ids.size() * 8));
+    WS_HIP(h2d_staged(ctx, ctx->ids.p, L.ids.data(), L.ids.size() * 8));
+    void* dst = out;
+    if (out_mem != WS_MEM_DEVICE) {
+      WS_HIP(ctx->outbuf.ensure((size_t)n * sizeof(ws_tagged_match)));
+      dst = ctx->outbuf.p;
+    }
+    if (r.dense_batch)
+      WS_HIP(launch_pack_tagged_dense(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.pbits,
+                                      ctx->bases.as<uint64_t>(), (uint32_t)L.plan.items.size(),
+                                      ctx->ids.as<uint64_t>(), dst, ctx->stream));
+    else
+      WS_HIP(launch_pack_tagged(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.sbits, r.pbits,
+                                ctx->ids.as<uint64_t>(), dst, ctx->stream));
+    if (out_mem != WS_MEM_DEVICE)
+      WS_HIP(hipMemcpyAsync(out, ctx->outbuf.p, (size_t)n * sizeof(ws_tagged_match), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    WS_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  finish_timings(ctx);
+  if (r.count > cap)
+    return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(r.count) +
+                                                  " exceeds " + std::to_string(cap) + ")");
+  return WS_OK;
+}
+
+// Every workspace of a context.
+std::vector<DevBuf*> ctx_buffers(ws_ctx* ctx) {
+  return {&ctx->recs, &ctx->cands, &ctx->raw, &ctx->k0, &ctx->v0, &ctx->k1, &ctx->v1,
+          &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
+          &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
+          &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
+          &ctx->bcur, &ctx->dstate, &ctx->k2, &ctx->v2, &ctx->bcnt2, &ctx->bases,
+          &ctx->lkeys, &ctx->lcnt, &ctx->lofs, &ctx->posb};
+}
+
 extern "C" {
 
 uint32_t ws_abi_version(void) { return WS_ABI_VERSION; }
@@ -1825,6 +2061,7 @@ ws_status ws_ctx_create(int device_ordinal, const ws_config* cfg, ws_ctx** out) 
     return fail(WS_ERR_NO_GPU_ADAPTER, "device ordinal out of range");
   ws_ctx* ctx = new ws_ctx();
   ctx->dev = device_ordinal;
+  for (DevBuf* b : ctx_buffers(ctx)) b->epoch = &ctx->epoch;
   if (cfg) ctx->cfg = *cfg;
   if (!ctx->cfg.max_matches) ctx->cfg.max_matches = 1000000;
   if (!ctx->cfg.chunk_size) ctx->cfg.chunk_size = 128ull << 20;
@@ -1876,13 +2113,7 @@ void ws_ctx_destroy(ws_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->dev);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (DevBuf* b : {&ctx->recs, &ctx->cands, &ctx->raw, &ctx->k0, &ctx->v0, &ctx->k1, &ctx->v1,
-                    &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
-                    &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
-                    &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
-                    &ctx->bcur, &ctx->dstate, &ctx->k2, &ctx->v2, &ctx->bcnt2, &ctx->bases,
-                    &ctx->lkeys, &ctx->lcnt, &ctx->lofs, &ctx->posb})
-    b->release();
+  for (DevBuf* b : ctx_buffers(ctx)) b->release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto& ev : ctx->tev)
@@ -2137,7 +2368,8 @@ ws_status ws_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64
       key.lane = tuning->bytes_per_lane;
       key.bpc = tuning->blocks_per_cu;
     }
-    s = device_scan(ctx, p, {plan}, {0u}, tuning, out, cap, out_mem, count, &oc,
+    const uint32_t off0 = 0;
+    s = device_scan(ctx, p, &plan, 1, &off0, tuning, out, cap, out_mem, count, &oc,
                     data_mem == WS_MEM_DEVICE ? &key : nullptr);
     if (s != WS_OK || oc == kDeviceScanDone) return s;
     ctx->last = ws_timings{};  // a buffer overflowed: redo on the checked path
@@ -2321,7 +2553,8 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
         key.lane = tuning->bytes_per_lane;
         key.bpc = tuning->blocks_per_cu;
       }
-      ws_status s = device_scan(ctx, p, plans, offs, tuning, out, cap, out_mem, count, &oc, &key);
+      ws_status s = device_scan(ctx, p, plans.data(), plans.size(), offs.data(), tuning, out, cap, out_mem, count,
+                                &oc, &key);
       if (s != WS_OK || oc == kDeviceScanDone) return s;
       ctx->last = ws_timings{};
     }
@@ -2409,7 +2642,8 @@ ws_status ws_autotune(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, ui
         // The path a scan of this request takes: device-sized, eager.
         DeviceScanResult oc;
         uint64_t n = 0;
-        s = device_scan(ctx, p, {plan}, {0u}, &kConfigTable[i], nullptr, ctx->cfg.max_matches, WS_MEM_HOST, &n,
+        const uint32_t off0 = 0;
+        s = device_scan(ctx, p, &plan, 1, &off0, &kConfigTable[i], nullptr, ctx->cfg.max_matches, WS_MEM_HOST, &n,
                         &oc, nullptr);
         if (s != WS_OK && s != WS_ERR_MATCH_BUFFER_OVERFLOW) return s;
         if (oc == kDeviceScanDone) {
@@ -2455,107 +2689,112 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
   std::lock_guard<std::mutex> lk(ctx->mu);
   WS_HIP(hipSetDevice(ctx->dev));
   ctx->last = ws_timings{};
-  // Items are ordered by source id so the ordering key carries the id rank
-  // (batch output order is (source_id, start, pattern_id, end)). One pass
-  // checks the order and, for device items, finds their span; `order` is
-  // built only when the ids are out of order.
-  bool sorted = true;
-  uintptr_t lo = UINTPTR_MAX, hi = 0;
-  for (size_t k = 0; k < n_items; ++k) {
-    const ws_item& it = items[k];
-    if (k && items[k - 1].id > it.id) sorted = false;
-    if (!it.len) continue;
-    if (!it.data) return fail(WS_ERR_INVALID_ARGUMENT, "item data is null");
-    lo = std::min(lo, reinterpret_cast<uintptr_t>(it.data));
-    hi = std::max(hi, reinterpret_cast<uintptr_t>(it.data) + (uintptr_t)it.len);
+  BatchLayout L;
+  ws_status s = layout_batch(ctx, items, n_items, data_mem, &L);
+  if (s != WS_OK) return s;
+  if (batch_device_ok(p, mode, tuning)) {
+    // Device-sized: segments, positions, ids and this lane width's work
+    // units staged into the context's buffers, one synchronisation.
+    ctx->stage_used = 0;
+    ctx->init_clean = false;
+    DevBuf* units[3] = {&ctx->units, &ctx->units, &ctx->units};
+    s = upload_batch(ctx, L, ctx->items, ctx->bases, ctx->ids, units,
+                     1u << unit_index(pick_variant(p, tuning), choose_lane_bytes(L.plan, tuning)));
+    if (s != WS_OK) return s;
+    DeviceScanResult oc;
+    const uint32_t off0 = 0;
+    s = device_scan(ctx, p, &L.plan, 1, &off0, tuning, out, cap, WS_MEM_HOST, count, &oc, nullptr,
+                    sizeof(ws_tagged_match));
+    if (s != WS_OK || oc == kDeviceScanDone) return s;
+    ctx->last = ws_timings{};  // a buffer overflowed: redo on the checked path
   }
-  std::vector<uint32_t> order;
-  if (!sorted) {
-    // (id, index) pairs sort contiguously; equal ids keep their input order.
-    std::vector<std::pair<uint64_t, uint32_t>> key(n_items);
-    for (size_t k = 0; k < n_items; ++k) key[k] = {items[k].id, (uint32_t)k};
-    std::sort(key.begin(), key.end());
-    order.resize(n_items);
-    for (size_t k = 0; k < n_items; ++k) order[k] = key[k].second;
+  return batch_checked(ctx, p, L, mode, tuning, out, cap, WS_MEM_HOST, count);
+}
+
+// ---------------------------------------------------------------------------
+// Batch plans: a batch laid out once, scanned many times.
+// ---------------------------------------------------------------------------
+struct ws_batch_plan {
+  int dev = 0;
+  uint64_t serial = 0;
+  BatchLayout L;
+  DevBuf items, bases, ids, units[3];
+};
+
+ws_status ws_batch_plan_create(ws_ctx* ctx, const ws_item* items, size_t n_items, ws_batch_plan** out) {
+  if (!ctx || !out) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (n_items && !items) return fail(WS_ERR_INVALID_ARGUMENT, "items is null");
+  if (n_items == 0) return fail(WS_ERR_INVALID_ARGUMENT, "a batch plan needs at least one item");
+  if (n_items > 0x7FFFFFFFull) return fail(WS_ERR_INPUT_TOO_LARGE, "too many batch items");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  WS_HIP(hipSetDevice(ctx->dev));
+  auto* bp = new ws_batch_plan();
+  bp->dev = ctx->dev;
+  bp->serial = g_pattern_serial.fetch_add(1);
+  ws_status s = layout_batch(ctx, items, n_items, WS_MEM_DEVICE, &bp->L);
+  if (s == WS_OK) {
+    ctx->stage_used = 0;
+    ctx->init_clean = false;
+    DevBuf* units[3] = {&bp->units[0], &bp->units[1], &bp->units[2]};
+    s = upload_batch(ctx, bp->L, bp->items, bp->bases, bp->ids, units, 7u);
   }
-  auto item_at = [&](size_t k) -> const ws_item& { return items[sorted ? k : order[k]]; };
-  Plan plan;
-  plan.batch = true;
-  plan.items.resize(n_items);
-  std::vector<uint64_t> ids(n_items);
-  uint64_t max_len = 0;
-  if (data_mem == WS_MEM_HOST) {
-    // Coalesce into the device input buffer, each item 16-byte aligned.
-    uint64_t tot = 0;
-    for (size_t k = 0; k < n_items; ++k) {
-      const uint64_t len = item_at(k).len;
-      plan.items[k] = make_uint2((uint32_t)tot, (uint32_t)(tot + len));
-      tot += (len + 15) & ~15ull;
-      if (tot > kMaxScanBytes) return fail(WS_ERR_INPUT_TOO_LARGE, "coalesced batch exceeds 4 GiB - 1 MiB");
-    }
-    std::vector<uint8_t> host(tot, 0);
-    for (size_t k = 0; k < n_items; ++k) {
-      const ws_item& it = item_at(k);
-      if (it.len) std::memcpy(host.data() + plan.items[k].x, it.data, it.len);
-      ids[k] = it.id;
-      max_len = std::max<uint64_t>(max_len, it.len);
-    }
-    WS_HIP(ctx->input.ensure(tot + 16));
-    if (tot) WS_HIP(hipMemcpyAsync(ctx->input.p, host.data(), tot, hipMemcpyHostToDevice, ctx->stream));
-    WS_HIP(hipStreamSynchronize(ctx->stream));
-    plan.base = ctx->input.as<uint8_t>();
-    plan.load_end = tot;
-  } else {
-    if (lo == UINTPTR_MAX) lo = hi = 0;
-    const uintptr_t base = lo & ~(uintptr_t)15;
-    if (hi - base > kMaxScanBytes)
-      return fail(WS_ERR_INPUT_TOO_LARGE, "device batch items span more than 4 GiB - 1 MiB");
-    for (size_t k = 0; k < n_items; ++k) {
-      const ws_item& it = item_at(k);
-      const uint64_t o = it.len ? reinterpret_cast<uintptr_t>(it.data) - base : 0;
-      plan.items[k] = make_uint2((uint32_t)o, (uint32_t)(o + it.len));
-      ids[k] = it.id;
-      max_len = std::max<uint64_t>(max_len, it.len);
-    }
-    plan.base = reinterpret_cast<const uint8_t*>(base);
-    plan.load_end = (hi - base + 15) & ~15ull;
+  if (s == WS_OK) {
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) s = fail(WS_ERR_GPU_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
   }
-  plan.max_seg_len = max_len;
-  Result r;
-  ws_status s = run_plan(ctx, p, plan, mode, tuning, &r);
   if (s != WS_OK) {
-    if (s == WS_ERR_MATCH_BUFFER_OVERFLOW) *count = r.count;
+    for (DevBuf* b : {&bp->items, &bp->bases, &bp->ids, &bp->units[0], &bp->units[1], &bp->units[2]}) b->release();
+    delete bp;
     return s;
   }
-  *count = r.count;
-  uint32_t n = (uint32_t)std::min<uint64_t>(r.count, cap);
-  if (r.bucketed) {
-    // The bucket sort's overflow flag came back with this synchronisation.
-    WS_HIP(hipStreamSynchronize(ctx->stream));
-    bool redone = false;
-    s = confirm_order(ctx, r, &redone);
-    if (s != WS_OK) return s;
-  }
-  if (n) {
-    WS_HIP(ctx->ids.ensure(ids.size() * 8));
-    WS_HIP(h2d_staged(ctx, ctx->ids.p, ids.data(), ids.size() * 8));
-    WS_HIP(ctx->outbuf.ensure((size_t)n * sizeof(ws_tagged_match)));
-    if (r.dense_batch)
-      WS_HIP(launch_pack_tagged_dense(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.pbits,
-                                      ctx->bases.as<uint64_t>(), (uint32_t)plan.items.size(), ctx->ids.as<uint64_t>(),
-                                      ctx->outbuf.p, ctx->stream));
-    else
-      WS_HIP(launch_pack_tagged(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), n, r.sbits, r.pbits,
-                                ctx->ids.as<uint64_t>(), ctx->outbuf.p, ctx->stream));
-    WS_HIP(hipMemcpyAsync(out, ctx->outbuf.p, (size_t)n * sizeof(ws_tagged_match),
-                          hipMemcpyDeviceToHost, ctx->stream));
-    WS_HIP(hipStreamSynchronize(ctx->stream));
-  }
-  finish_timings(ctx);
-  if (r.count > cap)
-    return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(r.count) +
-                                                  " exceeds " + std::to_string(cap) + ")");
+  *out = bp;
   return WS_OK;
+}
+
+void ws_batch_plan_destroy(ws_batch_plan* bp) {
+  if (!bp) return;
+  (void)hipSetDevice(bp->dev);
+  (void)hipDeviceSynchronize();  // scans of other contexts may still read the plan's buffers
+  for (DevBuf* b : {&bp->items, &bp->bases, &bp->ids, &bp->units[0], &bp->units[1], &bp->units[2]}) b->release();
+  delete bp;
+}
+
+ws_status ws_scan_batch_planned(ws_ctx* ctx, const ws_patterns* p, const ws_batch_plan* bp, uint32_t mode,
+                                const ws_tuning* tuning, ws_tagged_match* out, uint64_t cap, uint32_t out_mem,
+                                uint64_t* count) {
+  if (!ctx || !p || !bp || !count) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (p->dev != ctx->dev || bp->dev != ctx->dev)
+    return fail(WS_ERR_INVALID_ARGUMENT, "pattern set or batch plan belongs to another device");
+  if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
+  if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");
+  *count = 0;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  WS_HIP(hipSetDevice(ctx->dev));
+  ctx->last = ws_timings{};
+  if (batch_device_ok(p, mode, tuning)) {
+    // A repeated call (same patterns, plan, output and tuning) replays a
+    // captured graph.
+    GraphKey key;
+    key.pat_serial = p->serial;
+    key.batch = bp->serial;
+    key.cap = cap;
+    key.out = out_mem == WS_MEM_DEVICE ? out : nullptr;
+    key.out_mem = out_mem;
+    key.profiling = ctx->profiling ? 1u : 0u;
+    if (tuning) {
+      key.variant = tuning->variant;
+      key.lane = tuning->bytes_per_lane;
+      key.bpc = tuning->blocks_per_cu;
+    }
+    DeviceScanResult oc;
+    const uint32_t off0 = 0;
+    ws_status s = device_scan(ctx, p, &bp->L.plan, 1, &off0, tuning, out, cap, out_mem, count, &oc, &key,
+                              sizeof(ws_tagged_match));
+    if (s != WS_OK || oc == kDeviceScanDone) return s;
+    ctx->last = ws_timings{};
+  }
+  return batch_checked(ctx, p, bp->L, mode, tuning, out, cap, out_mem, count);
 }
 
 // ---------------------------------------------------------------------------

@@ -815,7 +815,7 @@ __global__ __launch_bounds__(kTPB) void k_slice_gather_keys(SliceOut lit, uint32
                                                              uint32_t* vals, uint32_t* cnt, uint32_t shift,
                                                              SliceOut cand, Rec* cout, uint32_t ccap,
                                                              uint32_t* lit_total, uint32_t* cand_total,
-                                                             uint32_t* lit_snap) {
+                                                             uint32_t* lit_snap, DenseOut dense) {
   if (blockIdx.x >= n + 64) {
     const uint32_t bx = blockIdx.x - (n + 64);
     const uint32_t o = block_slice_sum(cand, bx < n ? bx : n);
@@ -832,7 +832,7 @@ __global__ __launch_bounds__(kTPB) void k_slice_gather_keys(SliceOut lit, uint32
     return;
   }
   auto put = [&](uint32_t i, const Rec& r) {
-    const uint64_t k = ((((uint64_t)r.w << sbits) | r.y) << pbits) | r.x;
+    const uint64_t k = (order_pos(dense, r.w, r.y, sbits) << pbits) | r.x;
     keys[i] = k;
     vals[i] = r.z;
     atomicAdd(&cnt[k >> shift], 1u);
@@ -1019,7 +1019,8 @@ __device__ __forceinline__ void emit_raw(const WalkParams& w, bool found, uint32
   if (found && slot < w.raw_cap) {
     w.raw[slot] = make_uint4(rx, st, end, item);
     if (w.ckeys) {
-      const uint64_t k = (((uint64_t)item << w.rbits | rx) << w.sbits) | st;
+      const uint64_t k = w.cbase ? (((uint64_t)rx << w.sbits) | (w.cbase[item] + st))
+                                 : ((((uint64_t)item << w.rbits | rx) << w.sbits) | st);
       w.ckeys[slot] = k;
       w.cvals[slot] = slot;
       atomicAdd(&w.ccnt[k >> w.cshift], 1u);
@@ -1085,23 +1086,33 @@ __global__ __launch_bounds__(kTPB) void k_rx_scan(WalkParams w) {
 // a previous kernel left in device memory, capped at n.
 __device__ __forceinline__ uint32_t count_of(uint32_t n, const uint32_t* nd) { return nd ? min(*nd, n) : n; }
 
-// Chain keys: group (item, regex) in the high bits, start in the low bits.
+// Chain keys: group (item, regex) in the high bits, start in the low bits;
+// with dbase (a batch) regex, then the dense position dbase[item] + start.
 __global__ void k_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t rbits,
-                             uint64_t* keys, uint32_t* vals) {
+                             uint64_t* keys, uint32_t* vals, const uint64_t* dbase) {
   n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const Rec r = raw[i];
-    keys[i] = (((uint64_t)r.w << rbits | r.x) << sbits) | r.y;
+    keys[i] = dbase ? (((uint64_t)r.x << sbits) | (dbase[r.w] + r.y)) : ((((uint64_t)r.w << rbits | r.x) << sbits) | r.y);
     vals[i] = i;
   }
 }
 
 // A record starts a chain segment when it opens its group or when no earlier
 // record of the group can overlap it (none within max length ends after it).
-// group = key >> gshift (all one group when gshift >= 64); max length comes
-// from maxlen[rec.x] or, when maxlen is null, from maxlen_all.
+// group = key >> gshift (all one group when gshift is 64), or with
+// kGroupByRecord the record's (regex, item); max length comes from
+// maxlen[rec.x] or, when maxlen is null, from maxlen_all.
 __device__ __forceinline__ uint64_t group_of(uint64_t k, uint32_t gshift) {
   return gshift >= 64 ? 0ull : (k >> gshift);
+}
+__device__ __forceinline__ uint64_t group_at(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t j,
+                                             uint32_t gshift) {
+  if (gshift == kGroupByRecord) {
+    const Rec r = raw[vals[j]];
+    return ((uint64_t)r.x << 32) | r.w;
+  }
+  return group_of(keys[j], gshift);
 }
 
 // With carry_dst set, the kernel also copies the ncarry words of carry_src
@@ -1117,9 +1128,9 @@ __global__ void k_chain_heads(const uint64_t* keys, const uint32_t* vals, const 
       carry_dst[k] = carry_src[k];
   n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint64_t g = group_of(keys[i], gshift);
+    const uint64_t g = group_at(keys, vals, raw, i, gshift);
     const Rec ri = raw[vals[i]];
-    bool h = (i == 0) || (group_of(keys[i - 1], gshift) != g);
+    bool h = (i == 0) || (group_at(keys, vals, raw, i - 1, gshift) != g);
     if (!h) {
       // An empty record also depends on an earlier end equal to its start
       // (find_iter skips an empty match where the previous one ended).
@@ -1128,7 +1139,7 @@ __global__ void k_chain_heads(const uint64_t* keys, const uint32_t* vals, const 
       if (ml != 0xFFFFFFFFu) {
         h = true;
         for (uint32_t j = i; j-- > 0;) {
-          if (group_of(keys[j], gshift) != g) break;
+          if (group_at(keys, vals, raw, j, gshift) != g) break;
           const Rec rj = raw[vals[j]];
           const uint64_t reach = (uint64_t)rj.y + ml;
           if (reach < ri.y || (reach == ri.y && !empty)) break;
@@ -1164,7 +1175,7 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
   // The chain from head i: with `emit`, its kept records go to slots
   // slot, slot + 1, ...; returns how many it keeps.
   auto walk = [&](uint32_t i, bool emit, uint32_t slot) -> uint32_t {
-    const uint64_t g = group_of(keys[i], gshift);
+    const uint64_t g = group_at(keys, vals, raw, i, gshift);
     uint32_t last_end = 0, last_start = 0xFFFFFFFFu, kept = 0;
     bool have_last = false;
     if (carried) {
@@ -1175,7 +1186,7 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
       }
     }
     for (uint32_t j = i; j < n; ++j) {
-      if (j > i && (head[j] || group_of(keys[j], gshift) != g)) break;
+      if (j > i && (head[j] || group_at(keys, vals, raw, j, gshift) != g)) break;
       const Rec r = raw[vals[j]];
       const bool tie = ties && r.y == last_start && r.z == last_end;
       if (r.y < last_end && !tie) continue;
@@ -1190,7 +1201,7 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
         if (sl < out_cap) {
           out[sl] = make_uint4(pid, r.y, r.z, r.w);
           if (oo.keys) {
-            const uint64_t k = ((((uint64_t)r.w << oo.sbits) | r.y) << oo.pbits) | pid;
+            const uint64_t k = (order_pos(oo.dense, r.w, r.y, oo.sbits) << oo.pbits) | pid;
             oo.keys[sl] = k;
             oo.vals[sl] = r.z;
             atomicAdd(&oo.cnt[k >> oo.shift], 1u);
@@ -1234,13 +1245,14 @@ __global__ void k_pid_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint64_t
   }
 }
 
-// Final ordering keys: (item, start, pattern_id); the end rides as value.
+// Final ordering keys: (position, pattern_id), the position (item, start)
+// or a dense batch position (order_pos); the end rides as value.
 __global__ void k_order_keys(const Rec* recs, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t pbits,
-                             uint64_t* keys, uint32_t* vals) {
+                             uint64_t* keys, uint32_t* vals, DenseOut dense) {
   n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const Rec r = recs[i];
-    keys[i] = ((((uint64_t)r.w << sbits) | r.y) << pbits) | r.x;
+    keys[i] = (order_pos(dense, r.w, r.y, sbits) << pbits) | r.x;
     vals[i] = r.z;
   }
 }
@@ -1248,22 +1260,49 @@ __global__ void k_order_keys(const Rec* recs, uint32_t n, const uint32_t* nd, ui
 struct MatchOut { uint32_t pid, start, end; };
 struct TaggedOut { uint64_t id; uint32_t pid, start, end, pad; };
 
+// Writes output record `slot` of a match whose ordering key position is pos
+// and whose end is `end` (relative to its segment), or, with `len`, whose
+// length is `end`: a Match at start_off + pos, or with dense batch positions
+// a TaggedMatch (the item is the last one whose base is <= pos).
+__device__ __forceinline__ void put_match(void* out, uint64_t slot, uint64_t pos, uint32_t pid, uint32_t end,
+                                          uint32_t start_off, const DenseOut& d, bool len = false) {
+  if (!d.bases) {
+    MatchOut m;
+    m.pid = pid;
+    m.start = (uint32_t)pos + start_off;
+    m.end = len ? m.start + end : end + start_off;
+    reinterpret_cast<MatchOut*>(out)[slot] = m;
+    return;
+  }
+  uint32_t lo = 0, hi = d.nitems;  // bases[lo] <= pos < bases[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (d.bases[mid] <= pos) lo = mid;
+    else hi = mid;
+  }
+  TaggedOut t;
+  t.id = d.ids[lo];
+  t.pid = pid;
+  t.start = (uint32_t)(pos - d.bases[lo]);
+  t.end = len ? t.start + end : end;
+  t.pad = 0;
+  reinterpret_cast<TaggedOut*>(out)[slot] = t;
+}
+
+
 // Writes out[base + i] for i < n (base = *base_dev for device-sized runs,
-// else 0), dropping records at or past out_cap.
+// else 0), dropping records at or past out_cap: Matches, or TaggedMatches
+// from dense batch positions.
 __global__ void k_pack_matches(const uint64_t* keys, const uint32_t* vals, uint32_t n, const uint32_t* nd,
-                               uint32_t pbits, uint32_t start_off, MatchOut* out, uint64_t out_cap,
-                               const unsigned long long* base_dev) {
+                               uint32_t pbits, uint32_t start_off, void* out, uint64_t out_cap,
+                               const unsigned long long* base_dev, DenseOut dense) {
   n = count_of(n, nd);
   const uint64_t base = base_dev ? *base_dev : 0ull;
   const uint64_t pmask = (1ull << pbits) - 1;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (base + i >= out_cap) break;
     const uint64_t k = keys[i];
-    MatchOut m;
-    m.pid = (uint32_t)(k & pmask);
-    m.start = (uint32_t)(k >> pbits) + start_off;
-    m.end = vals[i] + start_off;
-    out[base + i] = m;
+    put_match(out, base + i, k >> pbits, (uint32_t)(k & pmask), vals[i], start_off, dense);
   }
 }
 
@@ -1487,14 +1526,16 @@ hipError_t launch_rx_scan(hipStream_t st, const WalkParams& w) {
 static uint32_t grid_of(uint32_t n, const uint32_t* nd) { return nd ? std::min(kDeviceGrid, grid_for(n)) : grid_for(n); }
 
 hipError_t launch_order_keys(const Rec* recs, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t pbits,
-                             uint64_t* keys, uint32_t* vals, hipStream_t st) {
-  hipLaunchKernelGGL(k_order_keys, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, recs, n, nd, sbits, pbits, keys, vals);
+                             uint64_t* keys, uint32_t* vals, hipStream_t st, const DenseOut& dense) {
+  hipLaunchKernelGGL(k_order_keys, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, recs, n, nd, sbits, pbits, keys, vals,
+                     dense);
   return hipGetLastError();
 }
 
 hipError_t launch_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t rbits,
-                             uint64_t* keys, uint32_t* vals, hipStream_t st) {
-  hipLaunchKernelGGL(k_chain_keys, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, raw, n, nd, sbits, rbits, keys, vals);
+                             uint64_t* keys, uint32_t* vals, hipStream_t st, const uint64_t* dbase) {
+  hipLaunchKernelGGL(k_chain_keys, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, raw, n, nd, sbits, rbits, keys, vals,
+                     dbase);
   return hipGetLastError();
 }
 
@@ -1532,9 +1573,9 @@ hipError_t launch_pid_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint64_t
 
 hipError_t launch_pack_matches(const uint64_t* keys, const uint32_t* vals, uint32_t n, const uint32_t* nd,
                                uint32_t pbits, uint32_t start_off, void* out, uint64_t out_cap,
-                               const unsigned long long* base_dev, hipStream_t st) {
+                               const unsigned long long* base_dev, hipStream_t st, const DenseOut& dense) {
   hipLaunchKernelGGL(k_pack_matches, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, keys, vals, n, nd, pbits, start_off,
-                     reinterpret_cast<MatchOut*>(out), out_cap, base_dev);
+                     out, out_cap, base_dev, dense);
   return hipGetLastError();
 }
 
@@ -1564,9 +1605,9 @@ hipError_t launch_slice_compact2(const SliceOut& lit, const SliceOut& cand, uint
 hipError_t launch_slice_compact_keys(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* lit_total,
                                      uint32_t* cand_total, uint32_t* lit_snap, uint32_t cap, uint32_t sbits,
                                      uint32_t pbits, uint64_t* keys, uint32_t* vals, uint32_t* cnt, uint32_t shift,
-                                     Rec* cout, uint32_t ccap, hipStream_t st) {
+                                     Rec* cout, uint32_t ccap, hipStream_t st, const DenseOut& dense) {
   hipLaunchKernelGGL(k_slice_gather_keys, dim3((cout ? 2 : 1) * (n + 64)), dim3(kTPB), 0, st, lit, n, cap, sbits,
-                     pbits, keys, vals, cnt, shift, cand, cout, ccap, lit_total, cand_total, lit_snap);
+                     pbits, keys, vals, cnt, shift, cand, cout, ccap, lit_total, cand_total, lit_snap, dense);
   return hipGetLastError();
 }
 
@@ -1744,10 +1785,11 @@ __global__ void k_bucket_sort(uint64_t* keys, uint32_t* vals, const uint32_t* en
 
 // Final ordering step fused with packing: one lane per bucket sorts it in
 // place (as k_bucket_sort) and writes its matches at the running output
-// offset (as k_pack_matches). Leaves the counts zero.
+// offset (as k_pack_matches, or tagged with dense batch positions). Leaves
+// the counts zero.
 __global__ void k_bucket_sort_pack(uint64_t* keys, uint32_t* vals, const uint32_t* end, uint32_t* cnt, uint32_t nb,
-                                   uint32_t* flag, uint32_t pbits, uint32_t start_off, MatchOut* out,
-                                   uint64_t out_cap, const unsigned long long* base_dev) {
+                                   uint32_t* flag, uint32_t pbits, uint32_t start_off, void* out,
+                                   uint64_t out_cap, const unsigned long long* base_dev, DenseOut dense) {
   const uint64_t base = *base_dev;
   const uint64_t pmask = (1ull << pbits) - 1;
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
@@ -1774,11 +1816,7 @@ __global__ void k_bucket_sort_pack(uint64_t* keys, uint32_t* vals, const uint32_
     for (uint32_t i = lo; i < lo + c; ++i) {
       if (base + i >= out_cap) break;
       const uint64_t k = keys[i];
-      MatchOut m;
-      m.pid = (uint32_t)(k & pmask);
-      m.start = (uint32_t)(k >> pbits) + start_off;
-      m.end = vals[i] + start_off;
-      out[base + i] = m;
+      put_match(out, base + i, k >> pbits, (uint32_t)(k & pmask), vals[i], start_off, dense);
     }
   }
 }
@@ -1799,12 +1837,12 @@ constexpr int kGatherTPB = 1024;
 
 __global__ __launch_bounds__(kGatherTPB) void k_gather_buckets(SliceOut lit, uint32_t n, uint32_t pbits,
                                                                uint32_t shift, uint64_t* bkeys, uint32_t* bcnt,
-                                                               uint32_t* counters) {
+                                                               uint32_t* counters, DenseOut dense) {
   // Called by whole waves (`live` false for lanes without a record). Lanes
   // of a wave holding the same bucket share one atomic: consecutive slice
   // records mostly fall in one or two buckets.
   auto put = [&](bool live, const Rec& r) {
-    const uint64_t k = ((uint64_t)r.y << pbits) | r.x;
+    const uint64_t k = ((dense.bases && live ? dense.bases[r.w] + r.y : (uint64_t)r.y) << pbits) | r.x;
     const uint32_t b = (uint32_t)(k >> shift);
     uint64_t todo = __ballot(live);
     while (todo) {
@@ -1889,8 +1927,8 @@ constexpr int kRankWaves = 4;  // buckets (one per wave) per block of k_rank_pac
 __global__ __launch_bounds__(64 * kRankWaves) void k_rank_pack(const uint64_t* bkeys, uint32_t* bcnt,
                                                                const uint32_t* bofs, uint32_t nb, uint32_t pbits,
                                                                const uint32_t* flen, uint32_t start_off,
-                                                               MatchOut* out, uint64_t out_cap,
-                                                               const unsigned long long* base_dev) {
+                                                               void* out, uint64_t out_cap,
+                                                               const unsigned long long* base_dev, DenseOut dense) {
   __shared__ uint64_t s_keys[kRankWaves][kLitBucketCap];
   const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t b = blockIdx.x * kRankWaves + w;
@@ -1911,11 +1949,8 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rank_pack(const uint64_t* b
     uint32_t r = 0;
     for (uint32_t j = 0; j < c; ++j) r += s_keys[w][j] < k;
     if (base + r < out_cap) {
-      MatchOut m;
-      m.pid = (uint32_t)(k & pmask);
-      m.start = (uint32_t)(k >> pbits) + start_off;
-      m.end = m.start + flen[m.pid];
-      out[base + r] = m;
+      const uint32_t pid = (uint32_t)(k & pmask);
+      put_match(out, base + r, k >> pbits, pid, flen[pid], start_off, dense, true);
     }
   }
   if (lane == 0) bcnt[b] = 0;
@@ -1924,16 +1959,16 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rank_pack(const uint64_t* b
 hipError_t launch_lit_order(const SliceOut& lit, uint32_t n, uint32_t pbits, uint32_t shift, uint32_t nb,
                             uint64_t* bkeys, uint32_t* bcnt, uint32_t* bofs, uint32_t* counters,
                             const uint32_t* flen, uint32_t start_off, void* out, uint64_t out_cap,
-                            const unsigned long long* base_dev, hipStream_t st) {
+                            const unsigned long long* base_dev, hipStream_t st, const DenseOut& dense) {
   hipLaunchKernelGGL(k_gather_buckets, dim3(n + 64), dim3(kGatherTPB), 0, st, lit, n, pbits, shift, bkeys, bcnt,
-                     counters);
+                     counters, dense);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_scan_buckets, dim3(1), dim3(kGatherTPB), 0, st, bcnt, bofs, counters);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_rank_pack, dim3((nb + kRankWaves - 1) / kRankWaves), dim3(64 * kRankWaves), 0, st, bkeys,
-                     bcnt, bofs, nb, pbits, flen, start_off, reinterpret_cast<MatchOut*>(out), out_cap, base_dev);
+                     bcnt, bofs, nb, pbits, flen, start_off, out, out_cap, base_dev, dense);
   return hipGetLastError();
 }
 
@@ -1942,14 +1977,15 @@ hipError_t launch_lit_order(const SliceOut& lit, uint32_t n, uint32_t pbits, uin
 hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
                              uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift, uint32_t nb,
                              uint32_t* cnt, uint32_t* cursor, uint32_t* flag, uint32_t pbits, uint32_t start_off,
-                             void* out, uint64_t out_cap, const unsigned long long* base_dev, hipStream_t st) {
+                             void* out, uint64_t out_cap, const unsigned long long* base_dev, hipStream_t st,
+                             const DenseOut& dense) {
   hipError_t e = rocprim::exclusive_scan(tmp, tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(), st);
   if (e != hipSuccess) return e;
   const uint32_t g = std::max(1u, std::min(nd ? kDeviceGrid : 4096u, (n + 255) / 256));
   hipLaunchKernelGGL(k_bucket_scatter, dim3(g), dim3(256), 0, st, kin, vin, n, nd, shift, cursor, kout, vout);
   const uint32_t gb = std::min(4096u, (nb + 255) / 256);
   hipLaunchKernelGGL(k_bucket_sort_pack, dim3(gb), dim3(256), 0, st, kout, vout, cursor, cnt, nb, flag, pbits,
-                     start_off, reinterpret_cast<MatchOut*>(out), out_cap, base_dev);
+                     start_off, out, out_cap, base_dev, dense);
   return hipGetLastError();
 }
 

@@ -126,16 +126,38 @@ struct WalkParams {
   uint32_t* cvals;
   uint32_t* ccnt;
   uint32_t cshift, sbits, rbits;
+  // A batch: chain keys (regex << sbits) | (cbase[item] + start), the dense
+  // position (a group (item, regex) stays contiguous and the buckets follow
+  // positions, not items).
+  const uint64_t* cbase;
+};
+// Chain kernels: groups given by the records' (regex, item) instead of key bits.
+constexpr uint32_t kGroupByRecord = 65;
+
+// Batch ordering over the items laid end to end in id order: item i's start
+// s is position bases[i] + s (bases: exclusive prefix sums of item length + 1,
+// nitems + 1 entries), so ordering keys (position << pbits | pattern id) are
+// dense, and tagged output finds the item again by binary search. bases null:
+// one segment, position (item << sbits) | start as before.
+struct DenseOut {
+  const uint64_t* bases;
+  const uint64_t* ids;  // source id per item (tagged output)
+  uint32_t nitems;
 };
 
+__device__ __forceinline__ uint64_t order_pos(const DenseOut& d, uint32_t item, uint32_t start, uint32_t sbits) {
+  return d.bases ? d.bases[item] + start : (((uint64_t)item << sbits) | start);
+}
+
 // Optional output of k_chain_walk: final ordering pairs of the kept matches,
-// key ((item << sbits | start) << pbits) | pattern id and value end, counted
-// in bucket key >> shift.
+// key (position << pbits) | pattern id (position as order_pos) and value
+// end, counted in bucket key >> shift.
 struct OrderOut {
   uint64_t* keys;
   uint32_t* vals;
   uint32_t* cnt;
   uint32_t shift, sbits, pbits;
+  DenseOut dense;
 };
 
 enum : uint32_t { kFlagTruncatedWalk = 1u };
