@@ -512,7 +512,12 @@ hipError_t ensure_zeroed_counts(ws_ctx* ctx, DevBuf& b, uint32_t nb) {
   if (e != hipSuccess) return e;
   return hipMemsetAsync(b.p, 0, b.cap, ctx->stream);
 }
-hipError_t ensure_bucket_counts(ws_ctx* ctx, uint32_t nb) { return ensure_zeroed_counts(ctx, ctx->bcnt, nb); }
+// A bucket sort's counts: at least kScanOneBlock of them (the one-block
+// count scan reads that many).
+hipError_t ensure_bucket_counts(ws_ctx* ctx, DevBuf& b, uint32_t nb) {
+  return ensure_zeroed_counts(ctx, b, std::max(nb, kScanOneBlock));
+}
+hipError_t ensure_bucket_counts(ws_ctx* ctx, uint32_t nb) { return ensure_bucket_counts(ctx, ctx->bcnt, nb); }
 
 ws_status order_pairs(ws_ctx* ctx, uint32_t n, uint32_t bits, bool dense, Result* res) {
   res->bucketed = false;
@@ -523,7 +528,7 @@ ws_status order_pairs(ws_ctx* ctx, uint32_t n, uint32_t bits, bool dense, Result
   const uint32_t bb = std::min<uint32_t>(bits, std::min(22u, std::max(10u, bits_for(n) + 1)));
   const uint32_t shift = bits - bb, nb = 1u << bb;
   WS_HIP(ensure_bucket_counts(ctx, nb));
-  WS_HIP(ctx->bcur.ensure((size_t)nb * 4));
+  WS_HIP(ctx->bcur.ensure((size_t)std::max(nb, kScanOneBlock) * 4));
   size_t need = 0;
   WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, n, nullptr, shift, nb,
                            ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), nullptr, ctx->stream));
@@ -1236,11 +1241,11 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   if (nrx && rs->variant == WS_VARIANT_HASH) {
     // Regex sets on the hash variant: final ordering pairs (k2, v2) and their
     // counts (bcnt) are made while the chain sort uses k0..v1 and bcnt2.
-    WS_HIP(ensure_zeroed_counts(ctx, ctx->bcnt2, 1u << rs->cb));
+    WS_HIP(ensure_bucket_counts(ctx, ctx->bcnt2, 1u << rs->cb));
     WS_HIP(ctx->k2.ensure((size_t)rs->rec_cap * 8));
     WS_HIP(ctx->v2.ensure((size_t)rs->rec_cap * 4));
   }
-  WS_HIP(ctx->bcur.ensure((size_t)nb * 4));
+  WS_HIP(ctx->bcur.ensure((size_t)std::max(nb, kScanOneBlock) * 4));
   size_t need = 0, need2 = 0;
   WS_HIP(bucket_sort_pairs(nullptr, &need, nullptr, nullptr, nullptr, nullptr, rs->rec_cap, nullptr, 0,
                            1u << rs->ob, nullptr, nullptr, nullptr, ctx->stream));
@@ -1755,6 +1760,7 @@ constexpr uint32_t kNoAllocRecords = 1u << 16;
 constexpr uint64_t kNoAllocSliceRecords = 1u << 17;  // filter output slices (each list), overflow area included
 constexpr size_t kNoAllocTmpBytes = 1u << 20;        // the bucket sorts' scan storage
 constexpr uint32_t kNoAllocBuckets = 1u << 16;       // sort buckets: bucket_bits() of capacities <= 2^14 is <= 16
+static_assert(kNoAllocBuckets >= kScanOneBlock, "the one-block count scan reads kScanOneBlock counts");
 constexpr uint32_t kNoAllocLitBucketBits = 10;       // literal-only buckets at lit_cap = 2^16 (17 - 7 bits)
 
 // Byte offsets of the workspaces inside `scratch` (each 256-byte aligned).

@@ -1733,17 +1733,68 @@ hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Bucket sort of (key, value) pairs whose keys spread over their range (match
 // keys: position-major). Bucket = key >> shift, about two buckets per pair:
-// count, exclusive scan (rocPRIM), scatter, then one lane sorts each bucket
-// in place. A bucket over kBucketMax pairs is left unsorted and raises *flag;
-// the caller then falls back to the radix sort.
+// count, exclusive scan (one block up to 2^16 buckets, else rocPRIM),
+// scatter, then each bucket's pairs are ranked in LDS. A bucket over
+// kBucketMax pairs is left unsorted and raises *flag; the caller then falls
+// back to the radix sort.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kBucketMax = 64;
+constexpr uint32_t kBucketMax = 256;
+constexpr int kSortWaves = 4;                     // buckets (one per wave) per block of k_bucket_rank
 
 __global__ void k_bucket_count(const uint64_t* keys, uint32_t n, const uint32_t* nd, uint32_t shift,
                                uint32_t* cnt) {
   n = count_of(n, nd);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     atomicAdd(&cnt[keys[i] >> shift], 1u);
+}
+
+// Exclusive scan of nb <= kScanOneBlock counts into cursor in one 1024-lane
+// block: tiles of 4096 counts, four per lane, all loaded up front (cnt and
+// cursor hold kScanOneBlock entries, zero beyond nb); a wave scan per tile,
+// the 16 x 16 wave totals scanned by wave 0 (tile-major).
+__global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* cnt, uint32_t* cursor, uint32_t nb) {
+  constexpr int T = kScanOneBlock / 4096;
+  __shared__ uint32_t s_tot[T * 16];
+  const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const uint4* c4 = reinterpret_cast<const uint4*>(cnt);
+  uint4 v[T];
+  uint32_t own[T], inc[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) v[t] = c4[t * 1024 + tid];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    own[t] = v[t].x + v[t].y + v[t].z + v[t].w;
+    uint32_t x = own[t];
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= (uint32_t)d) x += y;
+    }
+    inc[t] = x;
+    if (lane == 63) s_tot[t * 16 + w] = x;
+  }
+  __syncthreads();
+  if (w == 0) {
+    const uint32_t a0 = s_tot[lane * 4], a1 = s_tot[lane * 4 + 1], a2 = s_tot[lane * 4 + 2], a3 = s_tot[lane * 4 + 3];
+    const uint32_t sum = a0 + a1 + a2 + a3;
+    uint32_t x = sum;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= (uint32_t)d) x += y;
+    }
+    const uint32_t ex = x - sum;
+    s_tot[lane * 4] = ex;
+    s_tot[lane * 4 + 1] = ex + a0;
+    s_tot[lane * 4 + 2] = ex + a0 + a1;
+    s_tot[lane * 4 + 3] = ex + a0 + a1 + a2;
+  }
+  __syncthreads();
+  uint4* o4 = reinterpret_cast<uint4*>(cursor);
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    if (t * 4096u + tid * 4u >= nb) continue;
+    const uint32_t b = s_tot[t * 16 + w] + inc[t] - own[t];
+    o4[t * 1024 + tid] = make_uint4(b, b + v[t].x, b + v[t].x + v[t].y, b + v[t].x + v[t].y + v[t].z);
+  }
 }
 
 __global__ void k_bucket_scatter(const uint64_t* kin, const uint32_t* vin, uint32_t n, const uint32_t* nd,
@@ -1757,70 +1808,128 @@ __global__ void k_bucket_scatter(const uint64_t* kin, const uint32_t* vin, uint3
   }
 }
 
-__global__ void k_bucket_sort(uint64_t* keys, uint32_t* vals, const uint32_t* end, uint32_t* cnt,
-                              uint32_t nb, uint32_t* flag) {
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
-    const uint32_t c = cnt[b];
-    if (c) cnt[b] = 0;  // the counts start at zero again for the next sort
-    if (c < 2) continue;
-    if (c > kBucketMax) {
-      atomicOr(flag, 1u);
-      continue;
-    }
-    const uint32_t lo = end[b] - c;
-    for (uint32_t i = lo + 1; i < lo + c; ++i) {
-      const uint64_t k = keys[i];
-      const uint32_t v = vals[i];
-      uint32_t j = i;
-      while (j > lo && keys[j - 1] > k) {
-        keys[j] = keys[j - 1];
-        vals[j] = vals[j - 1];
-        --j;
+// Ranks each bucket's pairs (end[b] is its end after the scatter, cnt[b]
+// its size): a pair's rank is the number of smaller keys, equal keys by
+// index, so the order is stable. Two parts of one grid:
+//  * blocks below `split`: a wave takes 64 consecutive buckets, one lane
+//    each for those of 1..kLaneSort pairs, in the lane's own column of the
+//    wave's LDS;
+//  * the rest: wave k of W takes buckets k, k + W, ... (so runs of large
+//    buckets spread over the waves) and ranks those over kLaneSort pairs
+//    with the whole wave.
+// Each part zeroes the counts it sorted (the counts start at zero for the
+// next sort); a count one part reads is either its own or left alone by the
+// other. Pairs are written back in place or, with PACK, as matches at the
+// running output offset (put_match).
+constexpr uint32_t kLaneSort = 8;
+constexpr uint32_t kRankSlots = kLaneSort * 64 > kBucketMax ? kLaneSort * 64 : kBucketMax;
+
+template <bool PACK>
+__device__ __forceinline__ void rank_out(uint64_t* keys, uint32_t* vals, uint64_t slot, uint64_t k, uint32_t v,
+                                         uint32_t pbits, uint32_t start_off, void* out, uint64_t out_cap,
+                                         const DenseOut& dense) {
+  if (PACK) {
+    if (slot < out_cap) put_match(out, slot, k >> pbits, (uint32_t)(k & ((1ull << pbits) - 1)), v, start_off, dense);
+  } else {
+    keys[slot] = k;
+    vals[slot] = v;
+  }
+}
+
+template <bool PACK>
+__global__ __launch_bounds__(64 * kSortWaves) void k_bucket_rank(uint64_t* keys, uint32_t* vals, const uint32_t* end,
+                                                                  uint32_t* cnt, uint32_t nb, uint32_t split,
+                                                                  uint32_t* flag, uint32_t pbits, uint32_t start_off,
+                                                                  void* out, uint64_t out_cap,
+                                                                  const unsigned long long* base_dev, DenseOut dense) {
+  __shared__ uint64_t sk[kSortWaves][kRankSlots];
+  __shared__ uint32_t sv[kSortWaves][kRankSlots];
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t base = PACK ? *base_dev : 0ull;
+  uint64_t* k_ = sk[w];
+  uint32_t* v_ = sv[w];
+  if (blockIdx.x < split) {
+    for (uint32_t g = blockIdx.x * kSortWaves + w; g * 64u < nb; g += split * kSortWaves) {
+      const uint32_t b = g * 64u + lane;
+      const uint32_t c = b < nb ? cnt[b] : 0u;
+      if (c == 0 || c > kLaneSort) continue;
+      cnt[b] = 0;
+      const uint32_t lo = end[b] - c;
+      if (c == 1) {
+        if (PACK) rank_out<PACK>(keys, vals, base + lo, keys[lo], vals[lo], pbits, start_off, out, out_cap, dense);
+        continue;
       }
-      keys[j] = k;
-      vals[j] = v;
+      // Column `lane`: entry t at t * 64 + lane.
+      for (uint32_t t = 0; t < c; ++t) {
+        k_[t * 64 + lane] = keys[lo + t];
+        v_[t * 64 + lane] = vals[lo + t];
+      }
+      for (uint32_t t = 0; t < c; ++t) {
+        const uint64_t k = k_[t * 64 + lane];
+        uint32_t r = 0;
+        for (uint32_t u = 0; u < c; ++u) {
+          const uint64_t ku = k_[u * 64 + lane];
+          r += (ku < k) || (ku == k && u < t);
+        }
+        rank_out<PACK>(keys, vals, base + lo + r, k, v_[t * 64 + lane], pbits, start_off, out, out_cap, dense);
+      }
+    }
+    return;
+  }
+  const uint32_t W = (gridDim.x - split) * kSortWaves;
+  const uint32_t k0 = (blockIdx.x - split) * kSortWaves + w;
+  // Wave-uniform loop: 64 of the wave's buckets per round, one per lane.
+  for (uint32_t m = 0; (uint64_t)k0 + (uint64_t)W * 64u * m < nb; ++m) {
+    const uint64_t b = (uint64_t)k0 + (uint64_t)W * (64u * m + lane);
+    const uint32_t c = b < nb ? cnt[b] : 0u;
+    uint64_t big = __ballot(c > kLaneSort);
+    while (big) {
+      const uint32_t l = __builtin_ctzll(big);
+      big &= big - 1;
+      const uint32_t cb = __shfl(c, l, 64);
+      const uint32_t bb = (uint32_t)__shfl((uint32_t)b, l, 64);
+      if (lane == 0) cnt[bb] = 0;
+      if (cb > kBucketMax) {
+        if (lane == 0) atomicOr(flag, 1u);
+        continue;
+      }
+      const uint32_t lob = end[bb] - cb;
+      for (uint32_t i = lane; i < cb; i += 64) {
+        k_[i] = keys[lob + i];
+        v_[i] = vals[lob + i];
+      }
+      wave_lds_sync();
+      for (uint32_t i = lane; i < cb; i += 64) {
+        const uint64_t k = k_[i];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < cb; ++j) {
+          const uint64_t kj = k_[j];
+          r += (kj < k) || (kj == k && j < i);
+        }
+        rank_out<PACK>(keys, vals, base + lob + r, k, v_[i], pbits, start_off, out, out_cap, dense);
+      }
+      wave_lds_sync();  // the next bucket reuses the slots
     }
   }
 }
 
-// Final ordering step fused with packing: one lane per bucket sorts it in
-// place (as k_bucket_sort) and writes its matches at the running output
-// offset (as k_pack_matches, or tagged with dense batch positions). Leaves
-// the counts zero.
-__global__ void k_bucket_sort_pack(uint64_t* keys, uint32_t* vals, const uint32_t* end, uint32_t* cnt, uint32_t nb,
-                                   uint32_t* flag, uint32_t pbits, uint32_t start_off, void* out,
-                                   uint64_t out_cap, const unsigned long long* base_dev, DenseOut dense) {
-  const uint64_t base = *base_dev;
-  const uint64_t pmask = (1ull << pbits) - 1;
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
-    const uint32_t c = cnt[b];
-    if (c == 0) continue;
-    cnt[b] = 0;
-    if (c > kBucketMax) {
-      atomicOr(flag, 1u);
-      continue;
-    }
-    const uint32_t lo = end[b] - c;
-    for (uint32_t i = lo + 1; i < lo + c; ++i) {
-      const uint64_t k = keys[i];
-      const uint32_t v = vals[i];
-      uint32_t j = i;
-      while (j > lo && keys[j - 1] > k) {
-        keys[j] = keys[j - 1];
-        vals[j] = vals[j - 1];
-        --j;
-      }
-      keys[j] = k;
-      vals[j] = v;
-    }
-    for (uint32_t i = lo; i < lo + c; ++i) {
-      if (base + i >= out_cap) break;
-      const uint64_t k = keys[i];
-      put_match(out, base + i, k >> pbits, (uint32_t)(k & pmask), vals[i], start_off, dense);
-    }
+// cursor = exclusive scan of cnt (nb counts).
+static hipError_t scan_counts(void* tmp, size_t tmp_bytes, uint32_t* cnt, uint32_t* cursor, uint32_t nb,
+                              hipStream_t st) {
+  if (nb <= kScanOneBlock) {
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, cnt, cursor, nb);
+    return hipGetLastError();
   }
+  return rocprim::exclusive_scan(tmp, tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(), st);
 }
 
+// Blocks of k_bucket_rank: `split` of lane-sorting blocks, then the
+// wave-sorting ones (one wave per 16 buckets, at most 1024 blocks).
+static void rank_grid(uint32_t nb, uint32_t* split, uint32_t* grid) {
+  const uint32_t groups = (nb + 63) / 64;
+  *split = std::max(1u, std::min(2048u, (groups + kSortWaves - 1) / kSortWaves));
+  *grid = *split + std::max(1u, std::min(1024u, (nb / 16 + kSortWaves - 1) / kSortWaves));
+}
 // ---------------------------------------------------------------------------
 // Ordering of a literal-only set's matches in three launches. A match's key is
 // start << pbits | pattern id (a literal-only set's pattern ids are its
@@ -1973,19 +2082,20 @@ hipError_t launch_lit_order(const SliceOut& lit, uint32_t n, uint32_t pbits, uin
 }
 
 // Pairs already counted into cnt (k_slice_gather_keys): scan -> scatter ->
-// per-bucket sort fused with packing into `out` at *base_dev.
+// per-bucket rank fused with packing into `out` at *base_dev.
 hipError_t bucket_order_pack(void* tmp, size_t tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
                              uint32_t* vout, uint32_t n, const uint32_t* nd, uint32_t shift, uint32_t nb,
                              uint32_t* cnt, uint32_t* cursor, uint32_t* flag, uint32_t pbits, uint32_t start_off,
                              void* out, uint64_t out_cap, const unsigned long long* base_dev, hipStream_t st,
                              const DenseOut& dense) {
-  hipError_t e = rocprim::exclusive_scan(tmp, tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(), st);
+  hipError_t e = scan_counts(tmp, tmp_bytes, cnt, cursor, nb, st);
   if (e != hipSuccess) return e;
   const uint32_t g = std::max(1u, std::min(nd ? kDeviceGrid : 4096u, (n + 255) / 256));
   hipLaunchKernelGGL(k_bucket_scatter, dim3(g), dim3(256), 0, st, kin, vin, n, nd, shift, cursor, kout, vout);
-  const uint32_t gb = std::min(4096u, (nb + 255) / 256);
-  hipLaunchKernelGGL(k_bucket_sort_pack, dim3(gb), dim3(256), 0, st, kout, vout, cursor, cnt, nb, flag, pbits,
-                     start_off, out, out_cap, base_dev, dense);
+  uint32_t split, grid;
+  rank_grid(nb, &split, &grid);
+  hipLaunchKernelGGL(k_bucket_rank<true>, dim3(grid), dim3(64 * kSortWaves), 0, st, kout, vout, cursor, cnt, nb,
+                     split, flag, pbits, start_off, out, out_cap, base_dev, dense);
   return hipGetLastError();
 }
 
@@ -2003,11 +2113,13 @@ hipError_t bucket_sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, 
                                    st);
   const uint32_t g = std::max(1u, std::min(nd ? kDeviceGrid : 4096u, (n + 255) / 256));
   if (!counted) hipLaunchKernelGGL(k_bucket_count, dim3(g), dim3(256), 0, st, kin, n, nd, shift, cnt);
-  hipError_t e = rocprim::exclusive_scan(tmp, *tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(), st);
+  hipError_t e = scan_counts(tmp, *tmp_bytes, cnt, cursor, nb, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_bucket_scatter, dim3(g), dim3(256), 0, st, kin, vin, n, nd, shift, cursor, kout, vout);
-  const uint32_t gb = std::min(4096u, (nb + 255) / 256);
-  hipLaunchKernelGGL(k_bucket_sort, dim3(gb), dim3(256), 0, st, kout, vout, cursor, cnt, nb, flag);
+  uint32_t split, grid;
+  rank_grid(nb, &split, &grid);
+  hipLaunchKernelGGL(k_bucket_rank<false>, dim3(grid), dim3(64 * kSortWaves), 0, st, kout, vout, cursor, cnt, nb,
+                     split, flag, 0u, 0u, nullptr, 0ull, nullptr, DenseOut{});
   return hipGetLastError();
 }
 

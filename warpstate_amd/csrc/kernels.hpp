@@ -182,6 +182,10 @@ constexpr uint32_t kCounterLitSnapshot = 10;
 // and the most buckets (the count array always has this many entries).
 constexpr uint32_t kLitBucketCap = 256;
 constexpr uint32_t kLitBuckets = 8192;
+// Bucket sorts of up to this many buckets scan their counts in one block;
+// their count and cursor arrays always hold this many entries (the counts
+// beyond the buckets in use zero), so the scan loads them unconditionally.
+constexpr uint32_t kScanOneBlock = 1u << 16;
 // sticky[0]: a buffer overflowed (redo on the checked path); sticky[1]: error
 // bits (1 = regex window truncated, 2 = more than max_matches);
 // sticky[2..4]: running sums of filter hits, candidates, raw regex matches.
