@@ -79,7 +79,13 @@ def test_analyze_reports_set_facts():
         b.literal(p)
     info = b.build().analyze()
     assert info["literal_count"] == 2000 and info["regex_count"] == 0
+    # DNA grams of 11 bytes at every second position would be too frequent
+    # for 2000 literals over 4 letters: 12-byte windows at every position
     assert info["min_literal_len"] >= 12 and info["hash_window"] == 12
+    b = PatternSet.builder()
+    for p in W.kmer_set(200, seed=1):
+        b.literal(p)
+    assert b.build().analyze()["hash_window"] == 11
     assert info["variants_supported"] & (1 << N.VARIANT_HASH)
     ps = PatternSet.builder().regex("abc[a-z0-9]{1,8}|de[A-F0-9]{1,8}").literal("abcd").build()
     info = ps.analyze()

@@ -280,17 +280,18 @@ ws_status upload_patterns(ws_patterns* p) {
     fmeta.push_back(fc.kind ? fc.owner : h.lit_pid[fc.owner]);
   }
   // Verification table (see HashParams): two 32-byte slots per bucket
-  // carrying the hash, factor id, length and first 16 (folded) bytes, so a
-  // filter hit is checked with one 64-byte read; crowded buckets continue in
-  // `vmore`.
+  // carrying the gram hash, factor byte offset | gram offset, length and
+  // first 16 (folded) bytes, so a filter hit is checked with one 64-byte
+  // read; crowded buckets continue in `vmore`.
   std::vector<uint32_t> vtab, vmore;
   if (h.hash_ok) {
     const uint32_t nb = 1u << h.nbits;
     vtab.assign((size_t)nb * 16, 0);
-    auto fill = [&](uint32_t* e, uint32_t f) {
+    auto fill = [&](uint32_t* e, uint32_t ent) {
+      const uint32_t f = h.ent_fac[ent];
       const Factor& fc = h.fac[f];
-      e[0] = h.fhash[f];
-      e[1] = foff[f];
+      e[0] = h.ent_hash[ent];
+      e[1] = foff[f] | h.ent_off[ent];
       e[2] = flen[f] | (fc.kind << 31);
       uint8_t pre[16] = {0};
       std::memcpy(pre, fb.data() + foff[f], std::min<uint32_t>(16, flen[f]));
@@ -722,6 +723,7 @@ HashParams hash_params(const ws_patterns* p, uint32_t lane_bytes) {
   hp.s2 = h.s2;
   hp.lane_bytes = lane_bytes;
   hp.BB = h.BB;
+  hp.BB2 = h.BB2;
   hp.NBKK = h.NBKK;
   hp.amask = (hp.nwords - 1u) << 2;
   hp.vtab = p->vtab;

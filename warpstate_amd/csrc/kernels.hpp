@@ -52,11 +52,13 @@ struct HashParams {
   uint32_t nwords, wshift, wbits, B, NBK, nbits;
   uint32_t s2;  // shift of the second bloom bit (0 = single-bit exact filter)
   uint32_t lane_bytes;  // positions per lane per work unit: 16, 32 or 64
-  // Bloom hash for K >= 3 (two 16-bit lanes): multipliers, -multiplier^K,
-  // and the bloom byte-address mask ((nwords - 1) << 2).
-  uint32_t BB, NBKK, amask;
+  // Bloom hash for K >= 3 (two 16-bit lanes): multipliers, their squares
+  // (windows every second position), -multiplier^K, and the bloom
+  // byte-address mask ((nwords - 1) << 2).
+  uint32_t BB, BB2, NBKK, amask;
   // Verification table: 2^nbits buckets of two 32-byte slots. A slot is
-  // {hash, factor byte offset, length | kind << 31, bytes 0-3},
+  // {gram hash, factor byte offset (16-aligned) | gram offset in the factor,
+  // length | kind << 31, bytes 0-3},
   // {bytes 4-15, output id} (hash 0xFFFFFFFF = empty); a bucket with more than
   // two factors keeps one in slot 0 and turns slot 1 into
   // {kVtabMore, first, count, 0} over vmore (same slot format).

@@ -3,6 +3,8 @@
 #include "patterns.hpp"
 
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <deque>
 #include <map>
 
@@ -29,10 +31,28 @@ uint32_t ceil_log2(uint64_t x) {
 
 void build_hash(PatternsHost& ps) {
   const uint32_t F = (uint32_t)ps.fac.size();
+  // Window lengths with a kernel build: every position for these...
   static const uint32_t kAllowed[] = {1, 2, 3, 4, 6, 8, 12, 16};
+  // ...and every second position for these (a factor needs K + 1 bytes).
+  static const uint32_t kAllowed2[] = {5, 7, 11, 15};
   uint32_t K = 1;
   for (uint32_t k : kAllowed) if (k <= ps.min_flen) K = k;
+  // Half the windows at one byte less: half the filter work per byte, with
+  // two grams per factor (factors of 6+ bytes) -- when the shorter grams
+  // stay selective. Over text drawn from the factors' own A distinct bytes,
+  // about 2F / A^K of the stride-2 windows hold a gram; above 1/2000 (DNA:
+  // A = 4, 10,000 factors, K = 11: 1/210) the extra verification costs more
+  // than the halved filter work saves.
+  std::array<bool, 256> seen{};
+  for (const Factor& f : ps.fac)
+    for (char ch : f.bytes) seen[fold((uint8_t)ch, ps.ci)] = true;
+  const double A = (double)std::max<size_t>(1, std::count(seen.begin(), seen.end(), true));
+  for (uint32_t k : kAllowed2) {
+    if (k + 1 > ps.min_flen || k + 1 < K) continue;
+    if (2.0 * (double)F <= std::pow(A, (double)k) / 2000.0) K = k;
+  }
   ps.K = K;
+  ps.S = hash_stride(K);
   ps.B = K <= 2 ? 256u : 0x3779B1u;
   uint64_t bk = 1;
   for (uint32_t t = 0; t < K; ++t) bk = (bk * ps.B) & 0xFFFFFFu;
@@ -46,8 +66,10 @@ void build_hash(PatternsHost& ps) {
       ph = (ph * Bh) & 0xFFFFu;
     }
     ps.BB = Bl | (Bh << 16);
+    ps.BB2 = ((Bl * Bl) & 0xFFFFu) | (((Bh * Bh) & 0xFFFFu) << 16);
     ps.NBKK = ((0x10000u - pl) & 0xFFFFu) | (((0x10000u - ph) & 0xFFFFu) << 16);
   }
+  const uint32_t S = ps.S, E = F * S;  // verification entries: S grams per factor
   if (K <= 2) {
     // Exact filter over the raw 1- or 2-byte key: one bit per key value.
     ps.wbits = 8 * K - 5;
@@ -56,38 +78,46 @@ void build_hash(PatternsHost& ps) {
   } else {
     // Blocked bloom filter over the 32-bit dual hash D: the word is
     // D[wbits+1:2], its two bits D[20:16] and D[28:24]; about 256 bits per
-    // factor, 1 KiB .. 128 KiB.
-    uint32_t lb = std::min(20u, std::max(13u, ceil_log2(F) + 8));
+    // gram, 1 KiB .. 128 KiB.
+    uint32_t lb = std::min(20u, std::max(13u, ceil_log2(E) + 8));
     ps.wbits = lb - 5;
     ps.wshift = 2;
     ps.s2 = 5;
   }
   ps.bloom.assign(1u << ps.wbits, 0);
-  ps.nbits = std::max(1u, ceil_log2(2ull * F));
+  ps.nbits = std::max(1u, ceil_log2(2ull * E));
   const uint32_t nb = 1u << ps.nbits;
-  ps.fhash.resize(F);
-  std::vector<uint32_t> bucket(F), cnt(nb + 1, 0);
+  ps.ent_fac.resize(E);
+  ps.ent_off.resize(E);
+  ps.ent_hash.resize(E);
+  std::vector<uint32_t> bucket(E), cnt(nb + 1, 0);
   for (uint32_t f = 0; f < F; ++f) {
-    uint32_t h = rolling_hash((const uint8_t*)ps.fac[f].bytes.data(), K, ps.B, ps.ci);
-    ps.fhash[f] = h;
-    if (ps.dual) {
-      const uint32_t d = dual_hash((const uint8_t*)ps.fac[f].bytes.data(), K, ps.BB, ps.ci);
-      const uint32_t w = (d >> 2) & ((1u << ps.wbits) - 1);
-      ps.bloom[w] |= 1u << ((d >> 16) & 31);
-      ps.bloom[w] |= 1u << ((d >> 24) & 31);
-    } else {
-      const uint32_t w = (h >> ps.wshift) & ((1u << ps.wbits) - 1);
-      ps.bloom[w] |= 1u << (h & 31);
-      ps.bloom[w] |= 1u << ((h >> ps.s2) & 31);
+    for (uint32_t o = 0; o < S; ++o) {
+      const uint32_t e = f * S + o;
+      const uint8_t* g = (const uint8_t*)ps.fac[f].bytes.data() + o;
+      const uint32_t h = rolling_hash(g, K, ps.B, ps.ci);
+      ps.ent_fac[e] = f;
+      ps.ent_off[e] = o;
+      ps.ent_hash[e] = h;
+      if (ps.dual) {
+        const uint32_t d = dual_hash(g, K, ps.BB, ps.ci);
+        const uint32_t w = (d >> 2) & ((1u << ps.wbits) - 1);
+        ps.bloom[w] |= 1u << ((d >> 16) & 31);
+        ps.bloom[w] |= 1u << ((d >> 24) & 31);
+      } else {
+        const uint32_t w = (h >> ps.wshift) & ((1u << ps.wbits) - 1);
+        ps.bloom[w] |= 1u << (h & 31);
+        ps.bloom[w] |= 1u << ((h >> ps.s2) & 31);
+      }
+      bucket[e] = (uint32_t)((h * 0x9E3779B1u) >> (32 - ps.nbits));
+      cnt[bucket[e] + 1]++;
     }
-    bucket[f] = (uint32_t)((h * 0x9E3779B1u) >> (32 - ps.nbits));
-    cnt[bucket[f] + 1]++;
   }
   for (uint32_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
   ps.bstart = cnt;
-  ps.bent.resize(F);
+  ps.bent.resize(E);
   std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
-  for (uint32_t f = 0; f < F; ++f) ps.bent[fill[bucket[f]]++] = f;
+  for (uint32_t e = 0; e < E; ++e) ps.bent[fill[bucket[e]]++] = e;
   ps.hash_ok = true;
 }
 

@@ -52,14 +52,20 @@ struct PatternsHost {
   uint32_t max_match_len = 0;  // 0xFFFFFFFF when some regex is unbounded
   uint32_t strategy = STRAT_NONE;
 
-  // Rolling-hash prefilter.
+  // Rolling-hash prefilter over K-byte windows at every S-th position
+  // (hash_stride(K)); with S = 2 every factor has at least K + 1 bytes and
+  // both its K-grams at offsets 0 and 1 are in the filter.
   bool hash_ok = false;
-  uint32_t K = 0, B = 0, NBK = 0, wshift = 0, wbits = 0, nbits = 0, s2 = 0;
+  uint32_t K = 0, S = 1, B = 0, NBK = 0, wshift = 0, wbits = 0, nbits = 0, s2 = 0;
   // Bloom hash for K >= 3: two 16-bit polynomial hashes side by side
-  // (multipliers packed low | high << 16, and -multiplier^K likewise).
+  // (multipliers packed low | high << 16, their squares, and -multiplier^K
+  // likewise).
   bool dual = false;
-  uint32_t BB = 0, NBKK = 0;
-  std::vector<uint32_t> bloom, bstart, bent, fhash;
+  uint32_t BB = 0, BB2 = 0, NBKK = 0;
+  std::vector<uint32_t> bloom;
+  // Verification entries (factor, gram offset < S, 24-bit rolling hash of the
+  // gram), bucketed by hash: bucket b holds bent[bstart[b] .. bstart[b+1]).
+  std::vector<uint32_t> ent_fac, ent_off, ent_hash, bstart, bent;
 
   // Factor DFA (dense Aho-Corasick automaton over byte classes).
   bool ac_ok = false;
@@ -101,6 +107,11 @@ struct Prebuilt {
 // instead of being built.
 PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_insensitive,
                               Prebuilt* pre = nullptr);
+
+// Window stride of the rolling-hash filter for window length K: the odd K
+// (5, 7, 11, 15) test every second position, the others every position.
+// Shared by the set compiler (which picks K) and the filter kernel.
+constexpr uint32_t hash_stride(uint32_t K) { return (K >= 5 && (K & 1u)) ? 2u : 1u; }
 
 // Rolling hash of the first K bytes (mod 2^24), shared by host and kernel.
 uint32_t rolling_hash(const uint8_t* p, uint32_t K, uint32_t B, bool ci);
