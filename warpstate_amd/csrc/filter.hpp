@@ -174,45 +174,44 @@ __device__ __forceinline__ uint32_t pk2(u16x2 v) { return __builtin_bit_cast(uin
 __device__ __forceinline__ u16x2 upk2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
 
 // K >= 3: the same filter over the 32-bit dual hash D (two 16-bit
-// polynomial hashes rolled together by v_pk_mad_u16). The bloom word's byte
-// address is D & amask, its two bits D[20:16] and D[28:24], which the shifts
-// take straight from the upper half and top byte of D (SDWA operand
-// selects). The window bytes are first spread into 16-bit pairs (one v_perm
-// per two bytes), so a multiply-add takes its byte as either half of a pair
-// (VOP3P operand selects): per position about half a perm, two packed
-// multiply-adds, one address op, the LDS read, two shifts, an and and the
-// mask update.
+// polynomial hashes rolled together by v_pk_mad_u16) of the window's K - 1
+// little-endian byte pairs x[t] = b[t] | b[t+1] << 8. The pairs at even t
+// are the halves of the loaded dwords, those at odd t the halves of one
+// v_alignbyte per dword, and a multiply-add takes either half of a
+// register (VOP3P operand selects). The bloom word's byte address is
+// D & amask, its two bits D[20:16] and D[28:24], which the shifts take
+// straight from the upper half and top byte of D (SDWA operand selects):
+// per position two packed multiply-adds, a quarter of an align, one
+// address op, the LDS read, two shifts, an and and the mask update.
 template <int K, bool CI, int PL>
 __device__ __forceinline__ void filter_lane_dual(const HashParams& h, const uint32_t* bloom, uint32_t* d,
                                                  uint32_t* out) {
   constexpr int NW = PL / 4 + 4;
+  static_assert(PL + K - 1 <= 4 * NW, "pair window bytes");
   if (CI) {
 #pragma unroll
     for (int j = 0; j < NW; ++j) d[j] = fold_word(d[j]);
   }
-  // pr[2m] = {byte 4m, byte 4m+1}, pr[2m+1] = {byte 4m+2, byte 4m+3}.
-  u16x2 pr[2 * NW];
+  // od[m]: bytes 4m+1 .. 4m+4, the pairs at 4m+1 (low half) and 4m+3.
+  uint32_t od[NW];
 #pragma unroll
-  for (int m = 0; m < NW; ++m) {
-    pr[2 * m] = upk2(__builtin_amdgcn_perm(0u, d[m], 0x0c010c00u));
-    pr[2 * m + 1] = upk2(__builtin_amdgcn_perm(0u, d[m], 0x0c030c02u));
-  }
-  auto at = [&](int t) -> u16x2 {
-    const u16x2 v = pr[t >> 1];
-    return (t & 1) ? __builtin_shufflevector(v, v, 1, 1) : __builtin_shufflevector(v, v, 0, 0);
+  for (int m = 0; m < NW; ++m) od[m] = __builtin_amdgcn_alignbyte(m + 1 < NW ? d[m + 1] : 0u, d[m], 1);
+  auto xt = [&](int t) -> u16x2 {
+    const u16x2 v = upk2((t & 1) ? od[t >> 2] : d[t >> 2]);
+    return ((t >> 1) & 1) ? __builtin_shufflevector(v, v, 1, 1) : __builtin_shufflevector(v, v, 0, 0);
   };
   const u16x2 bb = upk2(h.BB), nbk = upk2(h.NBKK);
-  u16x2 D = at(0);
+  u16x2 D = xt(0);
   uint32_t mask = 0;
 #pragma unroll
-  for (int t = 1; t < K; ++t) D = D * bb + at(t);
+  for (int t = 1; t + 1 < K; ++t) D = D * bb + xt(t);
   // The bloom words sit at LDS address 0 (k_hash_filter has no static LDS),
   // so the masked hash is the ds_read address as it stands.
   (void)bloom;
   const __attribute__((address_space(3))) char* lds = nullptr;
 #pragma unroll
   for (int i = 0; i < PL; ++i) {
-    if (i > 0) D = D * bb + (at(i - 1) * nbk + at(i + K - 1));
+    if (i > 0) D = D * bb + (xt(i - 1) * nbk + xt(i + K - 2));
     const uint32_t Dw = pk2(D);
     const uint32_t w = *reinterpret_cast<const __attribute__((address_space(3))) uint32_t*>(lds + (Dw & h.amask));
     const uint32_t hit = (w >> ((Dw >> 16) & 31u)) & (w >> ((Dw >> 24) & 31u));

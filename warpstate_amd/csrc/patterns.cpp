@@ -59,9 +59,11 @@ void build_hash(PatternsHost& ps) {
   ps.NBK = (uint32_t)((0x1000000u - bk) & 0xFFFFFFu);
   ps.dual = K >= 3;
   if (ps.dual) {
+    // -B^(K-1) rolls the pair hash (every position: K - 1 pair terms),
+    // -B^K the byte hash (every second position).
     const uint32_t Bl = 0xA3B1u, Bh = 0xC6A5u;
     uint32_t pl = 1, ph = 1;
-    for (uint32_t t = 0; t < K; ++t) {
+    for (uint32_t t = 0; t < (hash_stride(K) == 1 ? K - 1 : K); ++t) {
       pl = (pl * Bl) & 0xFFFFu;
       ph = (ph * Bh) & 0xFFFFu;
     }
@@ -100,7 +102,7 @@ void build_hash(PatternsHost& ps) {
       ps.ent_off[e] = o;
       ps.ent_hash[e] = h;
       if (ps.dual) {
-        const uint32_t d = dual_hash(g, K, ps.BB, ps.ci);
+        const uint32_t d = S == 1 ? pair_hash(g, K, ps.BB, ps.ci) : dual_hash(g, K, ps.BB, ps.ci);
         const uint32_t w = (d >> 2) & ((1u << ps.wbits) - 1);
         ps.bloom[w] |= 1u << ((d >> 16) & 31);
         ps.bloom[w] |= 1u << ((d >> 24) & 31);
@@ -253,6 +255,17 @@ uint32_t dual_hash(const uint8_t* p, uint32_t K, uint32_t BB, bool ci) {
     const uint32_t b = (ci && p[t] >= 'A' && p[t] <= 'Z') ? p[t] + 32u : p[t];
     hl = (hl * Bl + b) & 0xFFFFu;
     hh = (hh * Bh + b) & 0xFFFFu;
+  }
+  return hl | (hh << 16);
+}
+
+uint32_t pair_hash(const uint8_t* p, uint32_t K, uint32_t BB, bool ci) {
+  const uint32_t Bl = BB & 0xFFFFu, Bh = BB >> 16;
+  uint32_t hl = 0, hh = 0;
+  for (uint32_t t = 0; t + 1 < K; ++t) {
+    const uint32_t x = fold(p[t], ci) | ((uint32_t)fold(p[t + 1], ci) << 8);
+    hl = (hl * Bl + x) & 0xFFFFu;
+    hh = (hh * Bh + x) & 0xFFFFu;
   }
   return hl | (hh << 16);
 }

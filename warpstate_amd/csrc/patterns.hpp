@@ -115,8 +115,12 @@ constexpr uint32_t hash_stride(uint32_t K) { return (K >= 5 && (K & 1u)) ? 2u : 
 
 // Rolling hash of the first K bytes (mod 2^24), shared by host and kernel.
 uint32_t rolling_hash(const uint8_t* p, uint32_t K, uint32_t B, bool ci);
-// The 32-bit bloom hash of a K-byte window: low half sum b[t] Bl^(K-1-t),
-// high half with Bh, each mod 2^16 (BB = Bl | Bh << 16).
+// The 32-bit bloom hash of a K-byte window at every second position: low
+// half sum b[t] Bl^(K-1-t), high half with Bh, each mod 2^16
+// (BB = Bl | Bh << 16).
 uint32_t dual_hash(const uint8_t* p, uint32_t K, uint32_t BB, bool ci);
+// The same over the K - 1 little-endian byte pairs x[t] = b[t] | b[t+1] << 8
+// (windows at every position): sum x[t] B^(K-2-t) per half.
+uint32_t pair_hash(const uint8_t* p, uint32_t K, uint32_t BB, bool ci);
 
 }  // namespace ws
