@@ -221,7 +221,6 @@ struct ws_patterns {
   const uint8_t* fbytes = nullptr;
   const uint4* fmeta = nullptr;
   const uint4 *vtab = nullptr, *vmore = nullptr;
-  const uint2* vtag = nullptr;
   const uint32_t *foff = nullptr, *flen = nullptr, *fown = nullptr, *lit_pid = nullptr;
   const uint32_t* bloom = nullptr;
   const uint32_t *ac_table32 = nullptr, *ac_out_start = nullptr, *ac_out_list = nullptr,
@@ -318,12 +317,6 @@ ws_status upload_patterns(ws_patterns* p) {
   }
   size_t o_fmeta = put(fmeta.data(), fmeta.size() * 4);
   size_t o_vtab = put(vtab.data(), vtab.size() * 4);
-  std::vector<uint32_t> vtag(vtab.size() / 8);
-  for (size_t b = 0; b < vtab.size() / 16; ++b) {
-    vtag[2 * b] = vtab[16 * b];
-    vtag[2 * b + 1] = vtab[16 * b + 8];
-  }
-  size_t o_vtag = put(vtag.data(), vtag.size() * 4);
   size_t o_vmore = put(vmore.data(), vmore.size() * 4);
   size_t o_fbytes = put(fb.data(), fb.size());
   size_t o_foff = put(foff.data(), foff.size() * 4);
@@ -373,7 +366,6 @@ ws_status upload_patterns(ws_patterns* p) {
   p->fmeta = reinterpret_cast<const uint4*>(b + o_fmeta);
   p->vtab = reinterpret_cast<const uint4*>(b + o_vtab);
   p->vmore = reinterpret_cast<const uint4*>(b + o_vmore);
-  p->vtag = reinterpret_cast<const uint2*>(b + o_vtag);
   p->foff = reinterpret_cast<const uint32_t*>(b + o_foff);
   p->flen = reinterpret_cast<const uint32_t*>(b + o_flen);
   p->fown = reinterpret_cast<const uint32_t*>(b + o_fown);
@@ -728,7 +720,6 @@ HashParams hash_params(const ws_patterns* p, uint32_t lane_bytes) {
   hp.amask = (hp.nwords - 1u) << 2;
   hp.vtab = p->vtab;
   hp.vmore = p->vmore;
-  hp.vtag = p->vtag;
   hp.fmeta = p->fmeta;
   return hp;
 }

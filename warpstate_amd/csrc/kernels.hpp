@@ -64,9 +64,6 @@ struct HashParams {
   // {kVtabMore, first, count, 0} over vmore (same slot format).
   const uint4* vtab;
   const uint4* vmore;
-  // The two slots' hash fields per bucket ({slot 0, slot 1}, 8 bytes): most
-  // prefilter hits are rejected on these alone, which stay cache-resident.
-  const uint2* vtag;
   const uint4* fmeta;   // per factor {byte offset (4-aligned), length, kind << 31 | owner, output id}
 };
 constexpr uint32_t kVtabEmpty = 0xFFFFFFFFu;
