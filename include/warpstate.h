@@ -361,7 +361,7 @@ ws_status ws_patterns_name(const ws_patterns* patterns, uint32_t id, char* buf, 
  * trained on the GPU from the autotuner's timings) predicts the fastest one.
  * Routing never changes results: every configuration returns the same matches. */
 #define WS_ROUTER_FEATURES 7
-#define WS_ROUTER_CONFIGS 8
+#define WS_ROUTER_CONFIGS 16
 
 typedef struct ws_router ws_router;
 
@@ -372,12 +372,13 @@ ws_status ws_config_tuning(uint32_t index, ws_tuning* out);
  * out[7] is padding (0). */
 ws_status ws_router_features(const ws_patterns_info* info, uint64_t bytes, float matches_per_mib,
                              float out[8]);
-/* Exhaustive autotune: times each supported configuration `repeats` times on
- * the request (device or host data) and writes the best time per configuration
- * (ms; 0 where unsupported), mask[i] = 1 for supported ones, and the fastest index. */
+/* Exhaustive autotune: times each supported configuration of the routing table
+ * whose bit is set in `configs` (0 = every one) `repeats` times on the request
+ * (device or host data) and writes the best time per configuration (ms; 0 where
+ * not timed), mask[i] = 1 for timed ones, and the fastest index. */
 ws_status ws_autotune(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t* data, uint64_t len,
-                      uint32_t data_mem, uint32_t mode, uint32_t repeats, float* times_ms,
-                      float* mask, uint32_t* best);
+                      uint32_t data_mem, uint32_t mode, uint32_t repeats, uint32_t configs,
+                      float* times_ms, float* mask, uint32_t* best);
 
 ws_status ws_router_create(int device_ordinal, uint32_t seed, ws_router** out);
 void ws_router_destroy(ws_router* router);
@@ -389,11 +390,13 @@ ws_status ws_router_set_normalization(ws_router* router, const float* xmean, con
                                       float ymean, float ystd);
 ws_status ws_router_get_normalization(ws_router* router, float* xmean, float* xstd, float* ymean,
                                       float* ystd);
-/* Trains on raw features x[n][8] and log-ms targets y[n][8] (mask[n][8] = 1 where measured). */
+/* Trains on raw features x[n][8] and log-ms targets y[n][WS_ROUTER_CONFIGS] (mask, same shape: 1
+ * where measured; a request may be timed on part of the table). */
 ws_status ws_router_train(ws_router* router, const float* x, const float* y, const float* mask,
                           uint32_t n, uint32_t epochs, uint32_t batch, float lr, uint32_t seed,
                           float* final_loss);
-/* Predicted ms per configuration (optional) and the fastest allowed configuration. */
+/* Predicted ms per configuration (pred_ms[n][WS_ROUTER_CONFIGS], optional) and the fastest
+ * configuration allowed by mask[n][WS_ROUTER_CONFIGS]. */
 ws_status ws_router_predict(ws_router* router, const float* x, uint32_t n, const float* mask,
                             float* pred_ms, uint32_t* choice);
 /* Raw outputs / loss gradient on standardized inputs (for checking against a CPU reference). */

@@ -1,13 +1,13 @@
 """CPU (numpy, float64) reference of the router MLP — TEST INFRASTRUCTURE ONLY.
 
 Same parameter layout as the HIP kernels: W1[64][8], b1[64], W2[64][64], b2[64],
-W3[64][64], b3[64], W4[8][64], b4[8]; ReLU hidden layers; loss
+W3[64][64], b3[64], W4[16][64], b4[16]; ReLU hidden layers; loss
 L = scale * sum(mask * (out - y)^2). Used to check the GPU forward pass and
 gradients; its own gradient is pinned against central finite differences.
 """
 import numpy as np
 
-IN, HID, OUT = 8, 64, 8
+IN, HID, OUT = 8, 64, 16
 SHAPES = [("W1", (HID, IN)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)),
           ("W3", (HID, HID)), ("b3", (HID,)), ("W4", (OUT, HID)), ("b4", (OUT,))]
 N_PARAMS = sum(int(np.prod(s)) for _, s in SHAPES)

@@ -25,19 +25,20 @@ def test_mlp_oracle_gradient_matches_finite_differences():
 
 
 def test_param_count_matches_layout():
-    assert N.load().ws_router_param_count() == M.N_PARAMS == 9416
+    # 8 -> 64 -> 64 -> 64 -> 16
+    assert N.load().ws_router_param_count() == M.N_PARAMS == 64 * 9 + 2 * 64 * 65 + 16 * 65 == 9936
 
 
 def test_config_table():
     seen = set()
-    for i in range(8):
+    for i in range(16):
         t = N.WsTuning()
         N.check(N.load().ws_config_tuning(i, ctypes.byref(t)))
-        assert t.variant in (1, 2, 3)
-        seen.add((t.variant, t.bytes_per_lane, t.blocks_per_cu))
-    assert len(seen) == 8
+        assert t.variant in (1, 2, 3) and t.flags in (0, N.TUNING_CHECKED)
+        seen.add((t.variant, t.bytes_per_lane, t.blocks_per_cu, t.flags))
+    assert len(seen) == 16
     t = N.WsTuning()
-    assert N.load().ws_config_tuning(8, ctypes.byref(t)) == 13
+    assert N.load().ws_config_tuning(16, ctypes.byref(t)) == 13
 
 
 def test_features():

@@ -177,8 +177,8 @@ SIGNATURES = {
     ),
     "ws_config_tuning": (c_int, [c_uint32, POINTER(WsTuning)]),
     "ws_router_features": (c_int, [POINTER(WsPatternsInfo), c_uint64, c_float, c_void_p]),
-    "ws_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p,
-                            POINTER(c_uint32)]),
+    "ws_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, c_uint32, c_uint32,
+                            c_void_p, c_void_p, POINTER(c_uint32)]),
     "ws_router_create": (c_int, [c_int, c_uint32, POINTER(c_void_p)]),
     "ws_router_destroy": (None, [c_void_p]),
     "ws_router_last_error": (c_char_p, []),

@@ -3,7 +3,7 @@
 // from (it stands in for AutoMatcher's size threshold, core/src/router.rs:205-225,
 // and for an exhaustive tuning sweep on every request).
 //
-// Network: 8 inputs (7 request features + 1 zero pad) -> 64 -> 64 -> 64 -> 8
+// Network: 8 inputs (7 request features + 1 zero pad) -> 64 -> 64 -> 64 -> 16
 // outputs (predicted standardized log time of each kernel configuration),
 // ReLU between layers, masked mean-squared-error loss, Adam.
 //
@@ -29,7 +29,8 @@
 
 namespace ws {
 
-constexpr int kIn = 8, kHid = 64, kOut = 8;
+constexpr int kIn = 8, kHid = 64, kOut = WS_ROUTER_CONFIGS;
+static_assert(kOut <= 64, "one lane per output");
 // Parameter layout (row-major W[out][in]).
 constexpr int kW1 = 0, kB1 = kW1 + kHid * kIn;
 constexpr int kW2 = kB1 + kHid, kB2 = kW2 + kHid * kHid;
@@ -413,8 +414,8 @@ ws_status ws_router_forward(ws_router* r, const float* x, uint32_t n, float* out
   return WS_OK;
 }
 
-// Trains on raw features x[n][8] (7 used) and log-time targets y[n][8] with
-// mask[n][8] (1 = measured). Standardization statistics are computed here.
+// Trains on raw features x[n][8] (7 used) and log-time targets y[n][kOut] with
+// mask[n][kOut] (1 = measured). Standardization statistics are computed here.
 // Mini-batch Adam for `epochs` passes; *final_loss receives the last epoch's
 // masked MSE in standardized units.
 ws_status ws_router_train(ws_router* r, const float* x, const float* y, const float* mask, uint32_t n,

@@ -4,7 +4,7 @@ AutoMatcher, and the continuous-batching request loop.
 The reference routes every request by one size threshold
 (AutoMatcher::scan, core/src/router.rs:205-225; threshold default
 core/src/config.rs:4-35). Here each request picks one of WS_ROUTER_CONFIGS
-kernel configurations (variant x lane width x occupancy). The exhaustive
+kernel configurations (variant x lane width x occupancy x pipeline). The exhaustive
 autotuner (ws_autotune) times all of them; the learned router is a small FP32
 MLP trained on those timings by the library's own HIP kernels
 (ws_router_train) and predicts the fastest configuration from 7 request
@@ -26,14 +26,14 @@ from . import _native as N
 from .api import AutoMatcherConfig, Context, GpuMatcher, PatternSet, Tuning, default_context
 from . import workloads as W
 
-NCFG = 8
+NCFG = 16
 NFEAT = 7
 
 
 def config_tuning(i: int) -> Tuning:
     t = N.WsTuning()
     N.check(N.load().ws_config_tuning(i, ctypes.byref(t)))
-    return Tuning(t.variant, t.bytes_per_lane, t.blocks_per_cu)
+    return Tuning(t.variant, t.bytes_per_lane, t.blocks_per_cu, bool(t.flags & N.TUNING_CHECKED))
 
 
 def features(info: dict, nbytes: int, matches_per_mib: float) -> np.ndarray:
@@ -114,14 +114,16 @@ class Router:
         return g
 
 
-def autotune(matcher: GpuMatcher, data_ptr: int, nbytes: int, repeats: int = 2) -> Tuple[np.ndarray, np.ndarray, int]:
-    """Exhaustive sweep over the routing table on device-resident data."""
+def autotune(matcher: GpuMatcher, data_ptr: int, nbytes: int, repeats: int = 2,
+             configs: int = 0) -> Tuple[np.ndarray, np.ndarray, int]:
+    """Exhaustive sweep over the routing table on device-resident data
+    (`configs`: bit mask of the entries to time, 0 = all)."""
     times = np.zeros(NCFG, dtype=np.float32)
     mask = np.zeros(NCFG, dtype=np.float32)
     best = ctypes.c_uint32(0)
     N.check(N.load().ws_autotune(matcher.ctx.handle, matcher.handle, ctypes.c_void_p(data_ptr), nbytes,
-                                 N.MEM_DEVICE, N.MODE_OVERLAPPING, repeats, times.ctypes.data, mask.ctypes.data,
-                                 ctypes.byref(best)))
+                                 N.MEM_DEVICE, N.MODE_OVERLAPPING, repeats, configs, times.ctypes.data,
+                                 mask.ctypes.data, ctypes.byref(best)))
     return times, mask, int(best.value)
 
 
@@ -250,24 +252,33 @@ def make_requests(n: int, specs, pools: DevicePools, seed: int) -> List[Request]
 
 def default_config_index(info: dict, nbytes: int) -> int:
     """The configuration the library picks without a router (WS_VARIANT_AUTO on
-    a single scan): the LDS factor DFA when the hash window is at most 2
-    bytes, else the hash filter at 64 positions per lane from 256 MiB up and
-    32 below."""
+    a single scan, the device-sized pipeline): the LDS factor DFA at 4 blocks
+    per CU when the hash window is at most 2 bytes, else the hash filter at 16
+    positions per lane up to 16 MiB, 64 from 256 MiB and 32 between."""
     if info["hash_window"] <= 2 and info["variants_supported"] & (1 << N.VARIANT_DFA_LDS):
-        return 6
+        return 13
+    if nbytes <= (16 << 20):
+        return 0
     return 2 if nbytes >= (256 << 20) else 1
 
 
-def harvest(reqs, specs, pools, matchers, repeats: int = 2):
-    """Exhaustive autotune of each request: features, log-ms targets, masks,
-    times and the default configuration."""
+def harvest(reqs, specs, pools, matchers, repeats: int = 2, subset: int = 0, seed: int = 0):
+    """Autotune of each request: features, log-ms targets, masks, times and
+    the default configuration. subset = 0 times the whole table (exhaustive);
+    subset = k times k random entries per request (training data: the loss
+    is masked, so each request teaches the configurations it timed)."""
     X, Y, M, T, D = [], [], [], [], []
     infos = {i: m.info() for i, m in matchers.items()} if isinstance(matchers, dict) else \
         {i: m.info() for i, m in enumerate(matchers)}
+    rng = np.random.default_rng(seed)
     for q in reqs:
         spec = specs[q.set_index]
         m = matchers[q.set_index]
-        times, mask, best = autotune(m, pools.ptr(spec.kind, q.density, q.offset), q.nbytes, repeats)
+        configs = 0
+        if subset:
+            for c in rng.choice(NCFG, size=min(subset, NCFG), replace=False):
+                configs |= 1 << int(c)
+        times, mask, best = autotune(m, pools.ptr(spec.kind, q.density, q.offset), q.nbytes, repeats, configs)
         info = infos[q.set_index]
         X.append(features(info, q.nbytes, pools.density[(spec.kind, q.density)]))
         Y.append(np.where(mask > 0, np.log(np.maximum(times, 1e-3)), 0.0))
@@ -324,7 +335,7 @@ def _route_fn(router: Optional[Router], specs, pools, infos, masks):
 
 def run_study(n_requests: int = 50_000, n_train: int = 1500, n_test: int = 600, pool_size: int = 256 << 20,
               slots: int = 4, load: float = 0.6, seed: int = 5, epochs: int = 300, n_sets: int = 20,
-              max_patterns: int = 10_000, log=None) -> dict:
+              max_patterns: int = 10_000, train_subset: int = NCFG // 2, log=None) -> dict:
     """Harvest autotuner timings, train the router on the GPU, measure its
     regret on held-out requests against the exhaustive sweep, and serve a
     Poisson stream of `n_requests` through the continuous-batching loop with
@@ -350,9 +361,9 @@ def run_study(n_requests: int = 50_000, n_train: int = 1500, n_test: int = 600, 
     train = make_requests(n_train, specs, pools, seed + 11)
     test = make_requests(n_test, specs, pools, seed + 12)
     t = time.perf_counter()
-    Xtr, Ytr, Mtr, Ttr, Dtr = harvest(train, specs, pools, matchers)
+    Xtr, Ytr, Mtr, Ttr, Dtr = harvest(train, specs, pools, matchers, subset=train_subset, seed=seed + 15)
     harvest_s = time.perf_counter() - t
-    say(f"harvested {n_train} training requests x {NCFG} configs in {harvest_s:.1f}s")
+    say(f"harvested {n_train} training requests x {train_subset or NCFG} of {NCFG} configs in {harvest_s:.1f}s")
     router = Router(seed=seed)
     t = time.perf_counter()
     loss = router.train(Xtr, Ytr, Mtr, epochs=epochs, batch=256, lr=2e-3, seed=seed)
@@ -426,7 +437,7 @@ def run_study(n_requests: int = 50_000, n_train: int = 1500, n_test: int = 600, 
             "tuning_cost_ratio_amortized": round(amortized, 1),
             "train_requests": n_train, "test_requests": n_test, "train_loss": round(loss, 5),
             "harvest_s": round(harvest_s, 2), "train_s": round(train_s, 2),
-            "configs": NCFG, "pattern_sets": len(sets),
+            "configs": NCFG, "train_configs_per_request": train_subset or NCFG, "pattern_sets": len(sets),
         },
         "serving": serving,
     }
