@@ -453,6 +453,13 @@ class GpuMatcher:
         N.check(st, int(count.value))
         return int(count.value)
 
+    def prepare_device(self, data_ptr: int, length: int, out_ptr: int, capacity: int, mode="overlapping",
+                       tuning: Optional[Tuning] = None) -> "PreparedScan":
+        """scan_device with its arguments converted once: a repeated call on
+        the same buffers costs one foreign call (about 10 us less per call
+        from Python than scan_device)."""
+        return PreparedScan(self, data_ptr, length, out_ptr, capacity, mode, tuning)
+
     def scan_device_array(self, data_ptr: int, length: int, mode="overlapping", tuning: Optional[Tuning] = None,
                           capacity: Optional[int] = None) -> np.ndarray:
         """Device-resident input, match list returned to the host (MATCH_DTYPE)."""
@@ -476,6 +483,29 @@ class GpuMatcher:
 
 
 WarpStateer = GpuMatcher
+
+
+class PreparedScan:
+    """A device-resident scan (ws_scan on device input and output) whose
+    foreign-call arguments are built once; calling it returns the match count
+    and raises as scan_device does."""
+
+    def __init__(self, matcher: GpuMatcher, data_ptr: int, length: int, out_ptr: int, capacity: int,
+                 mode="overlapping", tuning: Optional[Tuning] = None):
+        self._fn = N.load().ws_scan
+        self._count = ctypes.c_uint64(0)
+        self._tuning = tuning.to_c() if tuning else None
+        self._args = (matcher.ctx.handle, matcher.handle, ctypes.c_void_p(data_ptr), ctypes.c_uint64(length),
+                      ctypes.c_uint32(N.MEM_DEVICE), ctypes.c_uint32(_mode_code(mode)),
+                      ctypes.byref(self._tuning) if self._tuning is not None else None, ctypes.c_void_p(out_ptr),
+                      ctypes.c_uint64(capacity), ctypes.c_uint32(N.MEM_DEVICE), ctypes.byref(self._count))
+        self._keep = matcher  # the compiled set and context outlive the prepared call
+
+    def __call__(self) -> int:
+        st = self._fn(*self._args)
+        if st:
+            N.check(st, int(self._count.value))
+        return self._count.value
 
 
 class StreamPipeline:

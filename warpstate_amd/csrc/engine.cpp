@@ -82,6 +82,10 @@ hipError_t launch_call_init(uint32_t* sticky, uint32_t nsticky, uint32_t* counte
                             hipStream_t st);
 hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nwords, bool rx);
 hipError_t prepare_ac(bool lds_table, uint32_t states, uint32_t classes);
+hipError_t launch_lit_order_small(const SliceOut& lit, uint32_t n, uint32_t pbits, uint32_t sbits,
+                                  const uint32_t* flen, uint32_t start_off, void* out, uint64_t out_cap,
+                                  uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
+                                  unsigned long long* total, uint32_t* publish, uint32_t pwords, hipStream_t st);
 hipError_t launch_run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
                             unsigned long long* total, uint32_t* publish, uint32_t pwords, hipStream_t st);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t st);
@@ -152,10 +156,12 @@ struct GraphKey {
   const void* data = nullptr;
   const void* out = nullptr;
   uint32_t out_mem = 0, variant = 0, lane = 0, bpc = 0, profiling = 0;
+  uint32_t small = 0;  // literal-only runs ordered by the one-block kernel
   bool operator<(const GraphKey& o) const {
     return std::tie(pat_serial, len, chunk, overlap, cap, epoch, batch, data, out, out_mem, variant, lane, bpc,
-                    profiling) < std::tie(o.pat_serial, o.len, o.chunk, o.overlap, o.cap, o.epoch, o.batch, o.data,
-                                          o.out, o.out_mem, o.variant, o.lane, o.bpc, o.profiling);
+                    profiling, small) < std::tie(o.pat_serial, o.len, o.chunk, o.overlap, o.cap, o.epoch, o.batch,
+                                                 o.data, o.out, o.out_mem, o.variant, o.lane, o.bpc, o.profiling,
+                                                 o.small);
   }
 };
 
@@ -196,6 +202,11 @@ struct ws_ctx {
   // call with the same segments skips k_call_init. Every other user of the
   // counters or the segment list clears it.
   bool init_clean = false;
+  // Literal-only device-sized runs order their matches with the one-block
+  // kernel (k_lit_order_small) while the last such call had at most a
+  // quarter of its capacity per run; a run with more is redone on the
+  // checked path and turns it off.
+  bool lit_small = false;
   CallItems clean_items{};
   const void* clean_state = nullptr;
   size_t clean_state_bytes = 0;
@@ -1135,6 +1146,7 @@ struct RunSetup {
   uint32_t sbits = 0, pbits = 0, rbits = 0, obits = 0, cbits = 0, ob = 0, cb = 0;
   uint32_t gbits = 0;  // a batch: bits of a dense position (chain and ordering keys)
   uint32_t lb = 0;       // literal-only ordering: 2^lb fixed-capacity buckets (0: not used)
+  bool small_order = false;  // literal-only ordering by the one-block kernel instead
   size_t items_off = 0;  // this run's index among the call's runs (its segment, its stamps)
   unsigned long long* stamps = nullptr;  // the call's kernel stamps (profiling), two per run
   uint32_t* publish = nullptr;           // the call's last run: pinned host copy of the state block
@@ -1230,6 +1242,7 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
     WS_HIP(ctx->lkeys.ensure((size_t)lnb * kLitBucketCap * 8));
     WS_HIP(ensure_zeroed_counts(ctx, ctx->lcnt, kLitBuckets));
     WS_HIP(ctx->lofs.ensure((size_t)kLitBuckets * 4));
+    rs->small_order = ctx->lit_small && !plan.batch && rs->nvb <= kSmallOrderSlices;
   }
   if (nrx && rs->variant == WS_VARIANT_HASH) {
     // Regex sets on the hash variant: final ordering pairs (k2, v2) and their
@@ -1295,10 +1308,15 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
       if (ev) WS_HIP(hipEventRecord(ev[3], st));
       caps.lit_ovf_cap = lso.ovf_cap;
       caps.cand_ovf_cap = cso.ovf_cap;
-      WS_HIP(launch_lit_order(lso, rs.nvb, rs.pbits, rs.obits - rs.lb, 1u << rs.lb, ctx->lkeys.as<uint64_t>(),
-                              ctx->lcnt.as<uint32_t>(), ctx->lofs.as<uint32_t>(), cnt, p->flen, start_off, ds.out,
-                              ds.out_cap, ds.total, st, plan.dense));
-      WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
+      if (rs.small_order) {
+        WS_HIP(launch_lit_order_small(lso, rs.nvb, rs.pbits, rs.sbits, p->flen, start_off, ds.out, ds.out_cap, cnt,
+                                      caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
+      } else {
+        WS_HIP(launch_lit_order(lso, rs.nvb, rs.pbits, rs.obits - rs.lb, 1u << rs.lb, ctx->lkeys.as<uint64_t>(),
+                                ctx->lcnt.as<uint32_t>(), ctx->lofs.as<uint32_t>(), cnt, p->flen, start_off,
+                                ds.out, ds.out_cap, ds.total, st, plan.dense));
+        WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
+      }
       if (ev) WS_HIP(hipEventRecord(ev[4], st));
       return WS_OK;
     }
@@ -1462,10 +1480,16 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
   // the buffer epoch, which is part of the key).
   GraphEntry* ge = nullptr;
   GraphKey key;
+  // Literal-only single inputs and streams may order on the one-block path
+  // (prepare_run decides per run; the flag is part of the graph key).
+  const bool lit_only = !batch && p->h.rx.empty() && !p->h.fac.empty() &&
+                        pick_variant(p, tuning) == WS_VARIANT_HASH;
+  const bool small = lit_only && ctx->lit_small;
   auto lookup = [&]() {
     if (!key_in) return;
     key = *key_in;
     key.epoch = ctx->epoch.load();
+    key.small = small ? 1u : 0u;
     ge = &ctx->graphs[key];
   };
   lookup();
@@ -1593,11 +1617,13 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
   }
   const uint32_t* hs = ctx->hstate;
   if (hs[kStickyRedo]) {
+    if (small) ctx->lit_small = false;
     *outcome = kDeviceScanRedo;
     return WS_OK;
   }
   uint64_t tot = 0;
   std::memcpy(&tot, hs + 8, 8);
+  if (lit_only) ctx->lit_small = tot <= (uint64_t)nplans * (kSmallOrder / 4);
   if (ctx->profiling) {
     // The prefilter from its in-kernel stamps; the other phases from the
     // events between launches, which a replayed graph does not re-record

@@ -933,9 +933,9 @@ hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_
 
 // End of a device-sized run: record overflows and errors, accumulate
 // statistics, advance the output offset by this run's match count, and zero
-// the run's counters for the next run.
-__global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, unsigned long long* total,
-                            uint32_t* publish, uint32_t pwords) {
+// the run's counters for the next run. Called by a whole block.
+__device__ void run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky, unsigned long long* total,
+                          uint32_t* publish, uint32_t pwords) {
   if (threadIdx.x == 0) {
     uint32_t redo = 0;
     if (counters[5] > caps.lit_ovf_cap || counters[6] > caps.cand_ovf_cap) redo |= 2u;
@@ -972,6 +972,11 @@ __global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, 
     unsigned long long* stamps = reinterpret_cast<unsigned long long*>(sticky + 16);
     for (uint32_t k = threadIdx.x; k < (pwords - 16) / 2; k += blockDim.x) stamps[k] = (k & 1) ? 0ull : ~0ull;
   }
+}
+
+__global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, unsigned long long* total,
+                            uint32_t* publish, uint32_t pwords) {
+  run_close(counters, caps, sticky, total, publish, pwords);
 }
 
 __global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) {
@@ -1337,6 +1342,198 @@ hipError_t launch_lit_order(const SliceOut& lit, uint32_t n, uint32_t pbits, uin
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_rank_pack, dim3((nb + kRankWaves - 1) / kRankWaves), dim3(64 * kRankWaves), 0, st, bkeys,
                      bcnt, bofs, nb, pbits, flen, start_off, out, out_cap, base_dev, dense);
+  return hipGetLastError();
+}
+
+// The same ordering for a run with at most kSmallOrder matches, and the
+// run's close, in one block: the slices' records (and the overflow area's)
+// are counted into kSmallBuckets position buckets in LDS, scattered by
+// bucket, ranked within their bucket and packed in order. More matches, or
+// a bucket over kLitBucketCap, set the bucket-overflow flag: the call is
+// redone on the checked path, and the host returns to the three-launch
+// ordering. One launch instead of four: on a 16 MiB input with a few
+// hundred matches the kernels after the filter took 16 us, most of it the
+// fixed cost of each launch.
+constexpr uint32_t kSmallOrderTPB = 1024;
+constexpr uint32_t kSmallBuckets = 1024;
+constexpr uint32_t kSmallPer = kSmallOrder / kSmallOrderTPB;  // records per lane
+
+// Exclusive prefix sum over the block (one value per lane, 1024 lanes);
+// *total gets the sum. s_w: 16 words of scratch.
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= (uint32_t)d) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kSmallOrderTPB / 64; ++k) {
+    const uint32_t t = s_w[k];
+    before += k < w ? t : 0u;
+    all += t;
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+__global__ __launch_bounds__(kSmallOrderTPB) void k_lit_order_small(
+    SliceOut lit, uint32_t n, uint32_t pbits, uint32_t bshift, uint32_t start_off, void* out, uint64_t out_cap,
+    uint32_t* counters, RunCaps caps, uint32_t* sticky, unsigned long long* total, uint32_t* publish,
+    uint32_t pwords) {
+  static_assert(kSmallOrderSlices + 1 <= kSmallOrderTPB && kSmallBuckets == kSmallOrderTPB, "one per lane");
+  __shared__ uint64_t s_keys[kSmallOrder];
+  __shared__ uint32_t s_end[kSmallOrder];
+  __shared__ uint32_t s_off[kSmallOrderTPB + 1];
+  __shared__ uint32_t s_bcnt[kSmallBuckets];
+  __shared__ uint32_t s_bofs[kSmallBuckets];
+  __shared__ uint32_t s_w[kSmallOrderTPB / 64];
+  __shared__ uint32_t s_state[kStateResetWords];
+  __shared__ unsigned long long s_base;
+  const uint32_t tid = threadIdx.x;
+  // The close's inputs, read first so that their latency overlaps the
+  // ordering: the filter's counters, the state block (and, for the host
+  // copy, its kernel stamps) and the running total.
+  uint32_t cnt_in[kCounterLitSnapshot + 1] = {};
+  uint32_t st_in[5] = {};
+  if (tid == 0) {
+#pragma unroll
+    for (uint32_t k = 0; k <= kCounterLitSnapshot; ++k) cnt_in[k] = counters[k];
+#pragma unroll
+    for (uint32_t k = 0; k < 5; ++k) st_in[k] = sticky[k];
+    s_base = *total;
+  }
+  const uint32_t pub_in = publish && tid < pwords ? sticky[tid] : 0u;
+  // Slice record counts to offsets (slice `tid`; lane n: the overflow area).
+  uint32_t c = 0;
+  if (tid < n) c = min(lit.counts[tid], lit.cap);
+  else if (tid == n) c = min(*lit.ovf_count, lit.ovf_cap);
+  uint32_t T = 0;
+  s_off[tid] = block_excl_sum(c, s_w, &T);
+  s_bcnt[tid] = 0;
+  if (tid == 0) s_off[kSmallOrderTPB] = T;
+  __syncthreads();
+  bool ok = T <= kSmallOrder;
+  uint64_t key[kSmallPer];
+  uint32_t slot[kSmallPer], endv[kSmallPer];
+  if (ok) {
+#pragma unroll
+    for (uint32_t q = 0; q < kSmallPer; ++q) {
+      const uint32_t i = tid + q * kSmallOrderTPB;
+      if (i < T) {
+        uint32_t lo = 0, hi = n + 1;  // s_off[lo] <= i < s_off[hi]
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) / 2;
+          if (s_off[mid] <= i) lo = mid;
+          else hi = mid;
+        }
+        const Rec r = lo < n ? lit.buf[(size_t)lo * lit.cap + (i - s_off[lo])] : lit.ovf[i - s_off[n]];
+        key[q] = ((uint64_t)r.y << pbits) | r.x;
+        endv[q] = r.z;  // a literal match record carries its end
+        slot[q] = atomicAdd(&s_bcnt[min((uint32_t)(r.y >> bshift), kSmallBuckets - 1)], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t bc = s_bcnt[tid];
+  uint32_t unused = 0;
+  s_bofs[tid] = block_excl_sum(bc, s_w, &unused);
+  // A crowded bucket would cost a quadratic rank: the checked path instead.
+  if (bc > kLitBucketCap) ok = false;
+  const bool fail = __syncthreads_or(!ok);
+  if (!fail) {
+#pragma unroll
+    for (uint32_t q = 0; q < kSmallPer; ++q) {
+      const uint32_t i = tid + q * kSmallOrderTPB;
+      if (i < T) {
+        const uint32_t b = min((uint32_t)((key[q] >> pbits) >> bshift), kSmallBuckets - 1);
+        s_keys[s_bofs[b] + slot[q]] = key[q];
+        s_end[s_bofs[b] + slot[q]] = endv[q];
+      }
+    }
+    __syncthreads();
+    // Keys are distinct (one match per pattern and start): a key's place is
+    // its bucket's offset plus the number of smaller keys in its bucket.
+    const uint64_t pmask = (1ull << pbits) - 1, base = s_base;
+#pragma unroll
+    for (uint32_t q = 0; q < kSmallPer; ++q) {
+      const uint32_t i = tid + q * kSmallOrderTPB;
+      if (i < T) {
+        const uint64_t k = s_keys[i];
+        const uint32_t b = min((uint32_t)((k >> pbits) >> bshift), kSmallBuckets - 1);
+        const uint32_t b0 = s_bofs[b], b1 = b0 + s_bcnt[b];
+        uint32_t r = 0;
+        for (uint32_t j = b0; j < b1; ++j) r += s_keys[j] < k;
+        if (base + b0 + r < out_cap) {
+          MatchOut mo;
+          mo.pid = (uint32_t)(k & pmask);
+          mo.start = (uint32_t)(k >> pbits) + start_off;
+          mo.end = s_end[i] + start_off;
+          reinterpret_cast<MatchOut*>(out)[base + b0 + r] = mo;
+        }
+      }
+    }
+  }
+  // The close (what run_close does, from the values read at the start):
+  // this run's count is T, or it is redone when it did not fit.
+  if (tid == 0) {
+    const uint32_t c0 = fail ? 0u : T;
+    uint32_t redo = 0;
+    if (cnt_in[5] > caps.lit_ovf_cap || cnt_in[6] > caps.cand_ovf_cap) redo |= 2u;
+    if (c0 > caps.lit_cap) redo |= 4u;
+    if (cnt_in[1] > caps.cand_cap) redo |= 8u;
+    if (cnt_in[2] > caps.raw_cap) redo |= 16u;
+    if (c0 > caps.rec_cap) redo |= 32u;
+    if (fail || cnt_in[9]) redo |= 64u;
+    uint32_t e = st_in[kStickyError];
+    if (cnt_in[3] & kFlagTruncatedWalk) e |= 1u;
+    if (c0 > caps.max_matches) e |= 2u;
+    const unsigned long long tot = s_base + c0;
+    s_state[kStickyRedo] = st_in[kStickyRedo] | redo;
+    s_state[kStickyError] = e;
+    s_state[kStickyHits] = st_in[kStickyHits] + cnt_in[8];
+    s_state[kStickyCands] = st_in[kStickyCands] + cnt_in[1];
+    s_state[kStickyRaw] = st_in[kStickyRaw] + cnt_in[2];
+    s_state[5] = s_state[6] = s_state[7] = 0;
+    s_state[8] = (uint32_t)tot;
+    s_state[9] = (uint32_t)(tot >> 32);
+#pragma unroll
+    for (uint32_t k = 0; k < 5; ++k) sticky[k] = s_state[k];
+    *total = tot;
+#pragma unroll
+    for (uint32_t k = 0; k <= kCounterLitSnapshot; ++k) counters[k] = 0;
+  }
+  if (publish) {
+    // The state block to pinned host memory (words 0-9 as just closed, the
+    // stamps as read), then the completion flag; then the block is left as
+    // the call's init kernel sets it.
+    __syncthreads();
+    if (tid < pwords) publish[tid] = tid < kStateResetWords ? s_state[tid] : pub_in;
+    for (uint32_t k = tid + kSmallOrderTPB; k < pwords; k += kSmallOrderTPB) publish[k] = sticky[k];
+    __threadfence_system();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&publish[pwords], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (tid < kStateResetWords) sticky[tid] = 0;
+    unsigned long long* stamps = reinterpret_cast<unsigned long long*>(sticky + 16);
+    for (uint32_t k = tid; k < (pwords - 16) / 2; k += kSmallOrderTPB) stamps[k] = (k & 1) ? 0ull : ~0ull;
+  }
+}
+
+// sbits: bits of the largest start (the run's segment length).
+hipError_t launch_lit_order_small(const SliceOut& lit, uint32_t n, uint32_t pbits, uint32_t sbits,
+                                  const uint32_t* flen, uint32_t start_off, void* out, uint64_t out_cap,
+                                  uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
+                                  unsigned long long* total, uint32_t* publish, uint32_t pwords, hipStream_t st) {
+  if (n > kSmallOrderSlices) return hipErrorInvalidValue;
+  const uint32_t bshift = sbits > 10 ? sbits - 10 : 0;  // kSmallBuckets = 2^10
+  (void)flen;  // a literal match record carries its end
+  hipLaunchKernelGGL(k_lit_order_small, dim3(1), dim3(kSmallOrderTPB), 0, st, lit, n, pbits, bshift, start_off,
+                     out, out_cap, counters, caps, sticky, total, publish, pwords);
   return hipGetLastError();
 }
 

@@ -192,5 +192,9 @@ enum : uint32_t { kStickyRedo = 0, kStickyError = 1, kStickyHits = 2, kStickyCan
 // A call's state block: sticky words, the 64-bit match total at words 8-9
 // (both zeroed by k_call_init), kernel stamp pairs from word 16.
 constexpr uint32_t kStateResetWords = 10;
+// One-block ordering of a literal-only run (k_lit_order_small): at most this
+// many matches, from at most kSmallOrderSlices filter blocks.
+constexpr uint32_t kSmallOrder = 4096;
+constexpr uint32_t kSmallOrderSlices = 1023;
 
 }  // namespace ws
