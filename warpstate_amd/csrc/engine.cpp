@@ -157,11 +157,12 @@ struct GraphKey {
   const void* out = nullptr;
   uint32_t out_mem = 0, variant = 0, lane = 0, bpc = 0, profiling = 0;
   uint32_t small = 0;  // literal-only runs ordered by the one-block kernel
+  uint32_t wide = 0;   // sort buckets for dense runs (RunSetup ob / cb)
   bool operator<(const GraphKey& o) const {
     return std::tie(pat_serial, len, chunk, overlap, cap, epoch, batch, data, out, out_mem, variant, lane, bpc,
-                    profiling, small) < std::tie(o.pat_serial, o.len, o.chunk, o.overlap, o.cap, o.epoch, o.batch,
-                                                 o.data, o.out, o.out_mem, o.variant, o.lane, o.bpc, o.profiling,
-                                                 o.small);
+                    profiling, small, wide) < std::tie(o.pat_serial, o.len, o.chunk, o.overlap, o.cap, o.epoch,
+                                                       o.batch, o.data, o.out, o.out_mem, o.variant, o.lane, o.bpc,
+                                                       o.profiling, o.small, o.wide);
   }
 };
 
@@ -207,6 +208,10 @@ struct ws_ctx {
   // quarter of its capacity per run; a run with more is redone on the
   // checked path and turns it off.
   bool lit_small = false;
+  // Device-sized sorts use up to 2^14 buckets (a scan of their counts and
+  // a rank grid cost a few us) unless the last call had more than 2^18
+  // records per run or overflowed a bucket: then up to 2^16.
+  bool sort_wide = false;
   CallItems clean_items{};
   const void* clean_state = nullptr;
   size_t clean_state_bytes = 0;
@@ -467,6 +472,7 @@ WalkParams walk_params(const ws_patterns* p, const Plan& plan, const uint2* item
                        const uint64_t* pos_base) {
   WalkParams w{};
   w.base = plan.base;
+  w.load_end = plan.load_end;
   w.items = items;
   w.tables = p->rx_tables;
   w.toff = p->rx_toff;
@@ -1226,11 +1232,12 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   rs->obits = (plan.batch ? rs->gbits : rs->sbits) + rs->pbits;
   rs->cbits = (plan.batch ? rs->gbits : rs->sbits) + rs->rbits;
   if (rs->obits > 63 || rs->cbits > 63) return fail(WS_ERR_INPUT_TOO_LARGE, "match key does not fit 63 bits");
-  // Bucket counts follow the capacities up to 2^16 buckets (16 pairs per
-  // bucket at a million matches); denser runs fall back through the
-  // overflow flag.
-  rs->ob = bucket_bits(rs->obits, std::min(rs->rec_cap, 1u << 14));
-  rs->cb = nrx ? bucket_bits(rs->cbits, std::min(rs->raw_cap, 1u << 14)) : 0;
+  // Bucket counts follow the capacities up to 2^14 buckets (16 pairs per
+  // bucket at 2^18 records), or 2^16 after a dense call (16 pairs per
+  // bucket at a million); denser runs fall back through the overflow flag.
+  const uint32_t bcap = ctx->sort_wide ? (1u << 14) : (1u << 12);
+  rs->ob = bucket_bits(rs->obits, std::min(rs->rec_cap, bcap));
+  rs->cb = nrx ? bucket_bits(rs->cbits, std::min(rs->raw_cap, bcap)) : 0;
   const uint32_t nb = 1u << std::max(rs->ob, rs->cb);
   WS_HIP(ensure_bucket_counts(ctx, nb));
   if (!nrx && rs->variant == WS_VARIANT_HASH) {
@@ -1490,6 +1497,7 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
     key = *key_in;
     key.epoch = ctx->epoch.load();
     key.small = small ? 1u : 0u;
+    key.wide = ctx->sort_wide ? 1u : 0u;
     ge = &ctx->graphs[key];
   };
   lookup();
@@ -1618,12 +1626,14 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
   const uint32_t* hs = ctx->hstate;
   if (hs[kStickyRedo]) {
     if (small) ctx->lit_small = false;
+    if (hs[kStickyRedo] & 64u) ctx->sort_wide = true;  // a sort bucket overflowed
     *outcome = kDeviceScanRedo;
     return WS_OK;
   }
   uint64_t tot = 0;
   std::memcpy(&tot, hs + 8, 8);
   if (lit_only) ctx->lit_small = tot <= (uint64_t)nplans * (kSmallOrder / 4);
+  ctx->sort_wide = std::max<uint64_t>(tot, hs[kStickyRaw]) > ((uint64_t)nplans << 18);
   if (ctx->profiling) {
     // The prefilter from its in-kernel stamps; the other phases from the
     // events between launches, which a replayed graph does not re-record

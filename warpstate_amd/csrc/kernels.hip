@@ -293,6 +293,15 @@ __device__ __forceinline__ uint32_t byte_ctx(uint32_t b) {
 // match ends there, and stops when the DFA dies. At the segment's end it
 // reads the end-of-text column, or, in a window that ends before the input
 // does, sets *open (the match could still change).
+// The 16 bytes at a (16-aligned) of the walk's buffer; bytes at or past
+// load_end read as zero.
+__device__ __forceinline__ uint4 load16(const WalkParams& w, uint64_t a) {
+  if (a + 16 <= w.load_end) return *reinterpret_cast<const uint4*>(w.base + a);
+  uint32_t v[4] = {0, 0, 0, 0};
+  for (uint32_t k = 0; k < 16 && a + k < w.load_end; ++k) v[k >> 2] |= (uint32_t)w.base[a + k] << (8 * (k & 3));
+  return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
 __device__ __forceinline__ uint32_t rx_match_at(const WalkParams& w, uint32_t rx, uint2 seg, uint32_t st,
                                                 bool* open) {
   const uint32_t* T = w.tables + w.toff[rx];
@@ -302,8 +311,39 @@ __device__ __forceinline__ uint32_t rx_match_at(const WalkParams& w, uint32_t rx
   const uint32_t ctx = (st > 0 || w.ctx_before) ? byte_ctx(w.base[p0 - 1]) : 0u;
   uint32_t s = w.starts[4 * rx + ctx];
   uint32_t last = 0xFFFFFFFFu;
+  // The first 48 bytes from p0 & ~15 in three loads issued together, and
+  // their classes looked up at once (independent reads, packed one per
+  // byte): inside this window the walk waits on one dependent read per
+  // byte, the transition. (A wave's lanes step through their bytes
+  // together but cross 16-byte boundaries at different steps, so per-block
+  // loads would queue one round trip per crossing.) Past the window, byte by
+  // byte.
+  const uint64_t a0 = p0 & ~15ull;
+  uint32_t cls[12];
+  {
+    const uint4 b0 = load16(w, a0), b1 = load16(w, a0 + 16), b2 = load16(w, a0 + 32);
+    const uint32_t d[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+      cls[k] = (uint32_t)cm[d[k] & 0xFFu] | (uint32_t)cm[(d[k] >> 8) & 0xFFu] << 8 |
+               (uint32_t)cm[(d[k] >> 16) & 0xFFu] << 16 | (uint32_t)cm[d[k] >> 24] << 24;
+  }
   for (uint64_t p = p0; p < seg.y; ++p) {
-    const uint32_t e = T[s * C + cm[w.base[p]]];
+    const uint64_t o = p - a0;
+    uint32_t c;
+    if (o < 48) {
+      // cls[o / 4] by selects on the index bits (no indexed register array).
+      const uint32_t i = (uint32_t)o >> 2;
+      const uint32_t l0 = (i & 1) ? cls[1] : cls[0], l1 = (i & 1) ? cls[3] : cls[2];
+      const uint32_t l2 = (i & 1) ? cls[5] : cls[4], l3 = (i & 1) ? cls[7] : cls[6];
+      const uint32_t l4 = (i & 1) ? cls[9] : cls[8], l5 = (i & 1) ? cls[11] : cls[10];
+      const uint32_t m0 = (i & 2) ? l1 : l0, m1 = (i & 2) ? l3 : l2, m2 = (i & 2) ? l5 : l4;
+      const uint32_t v = (i & 8) ? m2 : ((i & 4) ? m1 : m0);
+      c = (v >> (8 * ((uint32_t)o & 3))) & 0xFFu;
+    } else {
+      c = cm[w.base[p]];
+    }
+    const uint32_t e = T[s * C + c];
     if (e & DFA_MATCH) last = (uint32_t)(p - seg.x);
     if (e & DFA_DEAD) return last;
     s = e & DFA_STATE_MASK;
@@ -478,9 +518,25 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
                                                      OrderOut oo) {
   __shared__ uint32_t s_res[2];
   n = count_of(n, nd);
+  // Writes kept record r to output slot sl (and its ordering pair, and the
+  // regex's carried end).
+  auto emit_rec = [&](const Rec& r, uint32_t sl) {
+    const uint32_t pid = rx_pid ? rx_pid[r.x] : r.x;
+    if (sl < out_cap) {
+      out[sl] = make_uint4(pid, r.y, r.z, r.w);
+      if (oo.keys) {
+        const uint64_t k = (order_pos(oo.dense, r.w, r.y, oo.sbits) << oo.pbits) | pid;
+        oo.keys[sl] = k;
+        oo.vals[sl] = r.z;
+        atomicAdd(&oo.cnt[k >> oo.shift], 1u);
+      }
+    }
+    if (carried_out) atomicMax(&carried_out[r.x], seg_off + r.z + 1u);
+  };
   // The chain from head i: with `emit`, its kept records go to slots
-  // slot, slot + 1, ...; returns how many it keeps.
-  auto walk = [&](uint32_t i, bool emit, uint32_t slot) -> uint32_t {
+  // slot, slot + 1, ...; returns how many it keeps, and (counting) the last
+  // kept record in *kept_rec.
+  auto walk = [&](uint32_t i, bool emit, uint32_t slot, Rec* kept_rec) -> uint32_t {
     const uint64_t g = group_at(keys, vals, raw, i, gshift);
     uint32_t last_end = 0, last_start = 0xFFFFFFFFu, kept = 0;
     bool have_last = false;
@@ -501,31 +557,25 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
       last_end = r.z;
       last_start = r.y;
       have_last = true;
-      if (emit) {
-        const uint32_t sl = slot + kept;
-        const uint32_t pid = rx_pid ? rx_pid[r.x] : r.x;
-        if (sl < out_cap) {
-          out[sl] = make_uint4(pid, r.y, r.z, r.w);
-          if (oo.keys) {
-            const uint64_t k = (order_pos(oo.dense, r.w, r.y, oo.sbits) << oo.pbits) | pid;
-            oo.keys[sl] = k;
-            oo.vals[sl] = r.z;
-            atomicAdd(&oo.cnt[k >> oo.shift], 1u);
-          }
-        }
-        if (carried_out) atomicMax(&carried_out[r.x], seg_off + r.z + 1u);
-      }
+      if (emit) emit_rec(r, slot + kept);
+      else *kept_rec = r;
       ++kept;
     }
     return kept;
   };
-  // Block-uniform loop: every lane reaches the slot reservation.
+  // Block-uniform loop: every lane reaches the slot reservation. A chain
+  // that keeps one record (most) emits it from registers; longer ones are
+  // walked again.
   for (uint32_t b0 = blockIdx.x * kTPB; b0 < n; b0 += gridDim.x * kTPB) {
     const uint32_t i = b0 + threadIdx.x;
     const bool on = i < n && head[i];
-    const uint32_t kept = on ? walk(i, false, 0) : 0u;
+    Rec one = make_uint4(0, 0, 0, 0);
+    const uint32_t kept = on ? walk(i, false, 0, &one) : 0u;
     const uint32_t slot = block_reserve(kept, out_count, s_res);
-    if (kept) walk(i, true, slot);
+    if (kept == 1)
+      emit_rec(one, slot);
+    else if (kept)
+      walk(i, true, slot, &one);
   }
 }
 
@@ -1016,8 +1066,11 @@ __global__ void k_bucket_count(const uint64_t* keys, uint32_t n, const uint32_t*
 // block: tiles of 4096 counts, four per lane, all loaded up front (cnt and
 // cursor hold kScanOneBlock entries, zero beyond nb); a wave scan per tile,
 // the 16 x 16 wave totals scanned by wave 0 (tile-major).
+// T tiles of 4096 counts (T * 4096 >= nb, T a power of two <= 16): the
+// counts past nb are zero and not written.
+template <int T>
 __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* cnt, uint32_t* cursor, uint32_t nb) {
-  constexpr int T = kScanOneBlock / 4096;
+  static_assert(T >= 1 && T <= kScanOneBlock / 4096 && 64 % T == 0, "tiles");
   __shared__ uint32_t s_tot[T * 16];
   const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const uint4* c4 = reinterpret_cast<const uint4*>(cnt);
@@ -1038,18 +1091,29 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* cnt, uint3
   }
   __syncthreads();
   if (w == 0) {
-    const uint32_t a0 = s_tot[lane * 4], a1 = s_tot[lane * 4 + 1], a2 = s_tot[lane * 4 + 2], a3 = s_tot[lane * 4 + 3];
-    const uint32_t sum = a0 + a1 + a2 + a3;
+    // The T * 16 wave totals (tile-major), T / 4 per lane: lane l holds
+    // entries [l * T / 4, (l + 1) * T / 4) when T >= 4, else lanes below
+    // T * 16 hold one each.
+    constexpr int per = T >= 4 ? T / 4 : 1;
+    uint32_t a[per];
+    uint32_t sum = 0;
+    const bool mine = T >= 4 || lane < (uint32_t)(T * 16);
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+      a[k] = mine ? s_tot[lane * per + k] : 0u;
+      sum += a[k];
+    }
     uint32_t x = sum;
     for (int d = 1; d < 64; d <<= 1) {
       const uint32_t y = __shfl_up(x, d, 64);
       if (lane >= (uint32_t)d) x += y;
     }
-    const uint32_t ex = x - sum;
-    s_tot[lane * 4] = ex;
-    s_tot[lane * 4 + 1] = ex + a0;
-    s_tot[lane * 4 + 2] = ex + a0 + a1;
-    s_tot[lane * 4 + 3] = ex + a0 + a1 + a2;
+    uint32_t ex = x - sum;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+      if (mine) s_tot[lane * per + k] = ex;
+      ex += a[k];
+    }
   }
   __syncthreads();
   uint4* o4 = reinterpret_cast<uint4*>(cursor);
@@ -1181,7 +1245,12 @@ __global__ __launch_bounds__(64 * kSortWaves) void k_bucket_rank(uint64_t* keys,
 static hipError_t scan_counts(void* tmp, size_t tmp_bytes, uint32_t* cnt, uint32_t* cursor, uint32_t nb,
                               hipStream_t st) {
   if (nb <= kScanOneBlock) {
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, cnt, cursor, nb);
+    const uint32_t tiles = (nb + 4095) / 4096;
+    if (tiles <= 1) hipLaunchKernelGGL(k_scan_counts<1>, dim3(1), dim3(1024), 0, st, cnt, cursor, nb);
+    else if (tiles <= 2) hipLaunchKernelGGL(k_scan_counts<2>, dim3(1), dim3(1024), 0, st, cnt, cursor, nb);
+    else if (tiles <= 4) hipLaunchKernelGGL(k_scan_counts<4>, dim3(1), dim3(1024), 0, st, cnt, cursor, nb);
+    else if (tiles <= 8) hipLaunchKernelGGL(k_scan_counts<8>, dim3(1), dim3(1024), 0, st, cnt, cursor, nb);
+    else hipLaunchKernelGGL(k_scan_counts<16>, dim3(1), dim3(1024), 0, st, cnt, cursor, nb);
     return hipGetLastError();
   }
   return rocprim::exclusive_scan(tmp, tmp_bytes, cnt, cursor, 0u, (size_t)nb, rocprim::plus<uint32_t>(), st);

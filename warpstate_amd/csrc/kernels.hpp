@@ -92,6 +92,7 @@ struct AcParams {
 
 struct WalkParams {
   const uint8_t* base;
+  uint64_t load_end;  // bytes [0, load_end) of base may be loaded
   const uint2* items;
   const Rec* cands;
   const uint32_t* ncand;
