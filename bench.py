@@ -279,6 +279,8 @@ def main() -> int:
     ap.add_argument("--no-router", dest="router", action="store_false")
     ap.add_argument("--requests", type=int, default=50_000, help="requests in the serving loop")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every call eagerly (no hipGraph capture), e.g. under a profiler")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
@@ -289,6 +291,8 @@ def main() -> int:
     from warpstate_amd.device import synchronize
 
     w = Workload(args)
+    if args.no_graphs:
+        w.ctx.set_graphs(False)
     settle_steps = 0
     t_settle = time.perf_counter()
     while time.perf_counter() - t_settle < args.settle_s:

@@ -179,6 +179,9 @@ uint32_t ws_ctx_cu_count(ws_ctx* ctx);
  * device memory that repeats (same patterns, input, output and tuning) is
  * captured into a hipGraph on its second call and replayed from then on. */
 uint32_t ws_ctx_graph_count(ws_ctx* ctx);
+/* Turn that capture and replay off (every call launched eagerly; results are
+ * the same) or back on (the default). For profilers and debugging. */
+void ws_ctx_set_graphs(ws_ctx* ctx, int on);
 /* Enable per-call HIP-event timing (ws_ctx_last_timings). */
 void ws_ctx_set_profiling(ws_ctx* ctx, int on);
 ws_status ws_ctx_last_timings(ws_ctx* ctx, ws_timings* out);

@@ -140,6 +140,7 @@ SIGNATURES = {
     "ws_ctx_cu_count": (c_uint32, [c_void_p]),
     "ws_ctx_graph_count": (c_uint32, [c_void_p]),
     "ws_ctx_set_profiling": (None, [c_void_p, c_int]),
+    "ws_ctx_set_graphs": (None, [c_void_p, c_int]),
     "ws_ctx_last_timings": (c_int, [c_void_p, POINTER(WsTimings)]),
     "ws_patterns_compile": (c_int, [c_void_p, POINTER(WsPattern), c_size_t, c_uint32, POINTER(c_void_p)]),
     "ws_patterns_free": (None, [c_void_p]),

@@ -125,6 +125,10 @@ class Context:
     def set_profiling(self, on: bool) -> None:
         N.load().ws_ctx_set_profiling(self.handle, 1 if on else 0)
 
+    def set_graphs(self, on: bool) -> None:
+        """Capture and replay repeated device-sized calls (default on)."""
+        N.load().ws_ctx_set_graphs(self.handle, 1 if on else 0)
+
     def last_timings(self) -> N.WsTimings:
         t = N.WsTimings()
         N.check(N.load().ws_ctx_last_timings(self.handle, ctypes.byref(t)))

@@ -208,9 +208,11 @@ struct ws_ctx {
   // quarter of its capacity per run; a run with more is redone on the
   // checked path and turns it off.
   bool lit_small = false;
+  bool graphs_on = true;  // capture repeated device-sized calls (ws_ctx_set_graphs)
   // Device-sized sorts use up to 2^14 buckets (a scan of their counts and
-  // a rank grid cost a few us) unless the last call had more than 2^18
-  // records per run or overflowed a bucket: then up to 2^16.
+  // a rank grid cost a few us) unless the last call had more than 2^16
+  // records per run or overflowed a bucket: then up to 2^16 (a bucket's
+  // rank is quadratic in its size).
   bool sort_wide = false;
   CallItems clean_items{};
   const void* clean_state = nullptr;
@@ -1232,8 +1234,8 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   rs->obits = (plan.batch ? rs->gbits : rs->sbits) + rs->pbits;
   rs->cbits = (plan.batch ? rs->gbits : rs->sbits) + rs->rbits;
   if (rs->obits > 63 || rs->cbits > 63) return fail(WS_ERR_INPUT_TOO_LARGE, "match key does not fit 63 bits");
-  // Bucket counts follow the capacities up to 2^14 buckets (16 pairs per
-  // bucket at 2^18 records), or 2^16 after a dense call (16 pairs per
+  // Bucket counts follow the capacities up to 2^14 buckets (4 pairs per
+  // bucket at 2^16 records), or 2^16 after a dense call (16 pairs per
   // bucket at a million); denser runs fall back through the overflow flag.
   const uint32_t bcap = ctx->sort_wide ? (1u << 14) : (1u << 12);
   rs->ob = bucket_bits(rs->obits, std::min(rs->rec_cap, bcap));
@@ -1493,7 +1495,7 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
                         pick_variant(p, tuning) == WS_VARIANT_HASH;
   const bool small = lit_only && ctx->lit_small;
   auto lookup = [&]() {
-    if (!key_in) return;
+    if (!key_in || !ctx->graphs_on) return;
     key = *key_in;
     key.epoch = ctx->epoch.load();
     key.small = small ? 1u : 0u;
@@ -1633,7 +1635,7 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
   uint64_t tot = 0;
   std::memcpy(&tot, hs + 8, 8);
   if (lit_only) ctx->lit_small = tot <= (uint64_t)nplans * (kSmallOrder / 4);
-  ctx->sort_wide = std::max<uint64_t>(tot, hs[kStickyRaw]) > ((uint64_t)nplans << 18);
+  ctx->sort_wide = std::max<uint64_t>(tot, hs[kStickyRaw]) > ((uint64_t)nplans << 16);
   if (ctx->profiling) {
     // The prefilter from its in-kernel stamps; the other phases from the
     // events between launches, which a replayed graph does not re-record
@@ -2170,6 +2172,10 @@ uint32_t ws_ctx_cu_count(ws_ctx* ctx) { return ctx ? ctx->cus : 0; }
 
 void ws_ctx_set_profiling(ws_ctx* ctx, int on) {
   if (ctx) ctx->profiling = on != 0;
+}
+
+void ws_ctx_set_graphs(ws_ctx* ctx, int on) {
+  if (ctx) ctx->graphs_on = on != 0;
 }
 
 ws_status ws_ctx_last_timings(ws_ctx* ctx, ws_timings* out) {
