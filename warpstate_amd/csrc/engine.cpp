@@ -84,9 +84,9 @@ hipError_t prepare_hash(uint32_t K, bool ci, uint32_t lane_bytes, uint32_t nword
 hipError_t prepare_ac(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_lit_order_small(const SliceOut& lit, uint32_t n, uint32_t pbits, uint32_t sbits,
                                   const uint32_t* flen, uint32_t start_off, void* out, uint64_t out_cap,
-                                  uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
+                                  uint32_t* counters, uint32_t* fcnt, const RunCaps& caps, uint32_t* sticky,
                                   unsigned long long* total, uint32_t* publish, uint32_t pwords, hipStream_t st);
-hipError_t launch_run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
+hipError_t launch_run_close(uint32_t* counters, uint32_t* fcnt, const RunCaps& caps, uint32_t* sticky,
                             unsigned long long* total, uint32_t* publish, uint32_t pwords, hipStream_t st);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t st);
 hipError_t launch_slice_compact2(const SliceOut& lit, const SliceOut& cand, uint32_t n, uint32_t* loffs,
@@ -190,7 +190,13 @@ struct ws_ctx {
   uint32_t wall_khz = 0;       // device wall-clock rate (kernel stamps)
   DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
       scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, k2, v2, bcnt2, bases, lkeys, lcnt, lofs,
-      posb;
+      posb, lslices2, cslices2, scounts2;
+  // Multi-run calls (stream chunks) on the hash variant filter run k + 1 on
+  // stream2 while run k's later passes run on `stream`: its own slices
+  // (lslices2 / cslices2 / scounts2 for odd runs) and counter block; xev:
+  // fork, filter done (by parity), run done (by parity).
+  hipStream_t stream2 = nullptr;
+  hipEvent_t xev[5] = {};
   std::vector<hipEvent_t> pev;  // per-chunk profiling events of device-sized runs
   uint8_t* hstage = nullptr;    // pinned staging read by captured copies
   size_t hstage_cap = 0;
@@ -1155,6 +1161,10 @@ struct RunSetup {
   uint32_t gbits = 0;  // a batch: bits of a dense position (chain and ordering keys)
   uint32_t lb = 0;       // literal-only ordering: 2^lb fixed-capacity buckets (0: not used)
   bool small_order = false;  // literal-only ordering by the one-block kernel instead
+  // Two-stream runs: the slice buffers' parity and the filter's counter
+  // block (null: the run counters).
+  uint32_t parity = 0;
+  uint32_t* fcnt = nullptr;
   size_t items_off = 0;  // this run's index among the call's runs (its segment, its stamps)
   unsigned long long* stamps = nullptr;  // the call's kernel stamps (profiling), two per run
   uint32_t* publish = nullptr;           // the call's last run: pinned host copy of the state block
@@ -1278,8 +1288,12 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
 }
 
 // Launches one prepared run: stream operations only (capturable).
+// part: kRunAll, or (two-stream runs, hash variant) kRunFilter (the filter
+// alone, on stream2) and kRunRest (everything after it, on the stream).
+enum RunPart { kRunAll = 0, kRunFilter = 1, kRunRest = 2 };
+
 ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const RunSetup& rs, uint32_t start_off,
-                     const DeviceRunState& ds, hipEvent_t* ev) {
+                     const DeviceRunState& ds, hipEvent_t* ev, RunPart part = kRunAll) {
   const PatternsHost& h = p->h;
   const hipStream_t st = ctx->stream;
   const uint32_t nrx = (uint32_t)h.rx.size();
@@ -1304,12 +1318,22 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   caps.lit_ovf_cap = caps.cand_ovf_cap = 0xFFFFFFFFu;
   bool keyed = false;  // ordering pairs already made (and counted) by the compaction
   const bool filter = rs.nunits > 0 && !h.fac.empty();  // no factors: no filter pass
+  if (part == kRunFilter && !(filter && rs.variant == WS_VARIANT_HASH)) return WS_OK;  // nothing to filter
   if (filter && rs.variant == WS_VARIANT_HASH) {
     const HashParams hp = hash_params(p, rs.lane_bytes);
-    uint32_t* sc = ctx->scounts.as<uint32_t>();
-    SliceOut lso = make_slices(ctx->lslices, rs.lslice_total, rs.nvb, sc, cnt + 5);
-    SliceOut cso = make_slices(ctx->cslices, rs.cslice_total, rs.nvb, sc + rs.nvb, cnt + 6);
-    WS_HIP(launch_hash(h.K, h.ci, rs.nvb, st, c, hp, lso, cso, nrx > 0));
+    // The filter's slices and its counters (hits, slice overflows): the
+    // run's parity's own with two streams.
+    uint32_t* fc = rs.fcnt ? rs.fcnt : cnt;
+    uint32_t* sc = (rs.parity ? ctx->scounts2 : ctx->scounts).as<uint32_t>();
+    SliceOut lso = make_slices(rs.parity ? ctx->lslices2 : ctx->lslices, rs.lslice_total, rs.nvb, sc, fc + 5);
+    SliceOut cso = make_slices(rs.parity ? ctx->cslices2 : ctx->cslices, rs.cslice_total, rs.nvb, sc + rs.nvb,
+                               fc + 6);
+    if (part != kRunRest) {
+      ScanCommon cf = c;
+      cf.counters = fc;
+      WS_HIP(launch_hash(h.K, h.ci, rs.nvb, part == kRunFilter ? ctx->stream2 : st, cf, hp, lso, cso, nrx > 0));
+      if (part == kRunFilter) return WS_OK;
+    }
     if (ev) WS_HIP(hipEventRecord(ev[1], st));
     if (rs.lb) {
       // Literal-only: slices -> fixed-capacity buckets -> ranked and packed.
@@ -1319,12 +1343,12 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
       caps.cand_ovf_cap = cso.ovf_cap;
       if (rs.small_order) {
         WS_HIP(launch_lit_order_small(lso, rs.nvb, rs.pbits, rs.sbits, p->flen, start_off, ds.out, ds.out_cap, cnt,
-                                      caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
+                                      rs.fcnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
       } else {
         WS_HIP(launch_lit_order(lso, rs.nvb, rs.pbits, rs.obits - rs.lb, 1u << rs.lb, ctx->lkeys.as<uint64_t>(),
                                 ctx->lcnt.as<uint32_t>(), ctx->lofs.as<uint32_t>(), cnt, p->flen, start_off,
                                 ds.out, ds.out_cap, ds.total, st, plan.dense));
-        WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
+        WS_HIP(launch_run_close(cnt, rs.fcnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
       }
       if (ev) WS_HIP(hipEventRecord(ev[4], st));
       return WS_OK;
@@ -1415,7 +1439,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
                              rs.obits - rs.ob,
                              1u << rs.ob, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, rs.pbits,
                              start_off, ds.out, ds.out_cap, ds.total, st, plan.dense));
-    WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
+    WS_HIP(launch_run_close(cnt, rs.fcnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
     if (ev) WS_HIP(hipEventRecord(ev[4], st));
     return WS_OK;
   }
@@ -1427,7 +1451,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
                            1u << rs.ob, ctx->bcnt.as<uint32_t>(), ctx->bcur.as<uint32_t>(), cnt + 9, st));
   WS_HIP(launch_pack_matches(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), rs.rec_cap, cnt + 0, rs.pbits,
                              start_off, ds.out, ds.out_cap, ds.total, st, plan.dense));
-  WS_HIP(launch_run_close(cnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
+  WS_HIP(launch_run_close(cnt, rs.fcnt, caps, ds.sticky, ds.total, rs.publish, rs.publish_words, st));
   if (ev) WS_HIP(hipEventRecord(ev[4], st));
   return WS_OK;
 }
@@ -1494,6 +1518,10 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
   const bool lit_only = !batch && p->h.rx.empty() && !p->h.fac.empty() &&
                         pick_variant(p, tuning) == WS_VARIANT_HASH;
   const bool small = lit_only && ctx->lit_small;
+  // Several runs on the hash variant (stream chunks): filters on a second
+  // stream, one run ahead (not with per-phase profiling events).
+  const bool two = nplans >= 2 && !batch && !ctx->profiling && !p->h.fac.empty() &&
+                   pick_variant(p, tuning) == WS_VARIANT_HASH;
   auto lookup = [&]() {
     if (!key_in || !ctx->graphs_on) return;
     key = *key_in;
@@ -1513,6 +1541,10 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
     }
     setups[k].items_off = k;
     setups[k].stamps = dstamps;
+    if (two) {
+      setups[k].parity = (uint32_t)(k & 1);
+      setups[k].fcnt = ctx->counters.as<uint32_t>() + kFilterCounterBlock * (1 + (k & 1));
+    }
     if (k + 1 == nplans) {
       setups[k].publish = ctx->hstate;
       setups[k].publish_words = (uint32_t)(state_bytes / 4);
@@ -1523,6 +1555,12 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
       ci.v[k] = plans[k].items[0];
     else
       std::memcpy(ctx->hstage + k * sizeof(uint2), plans[k].items.data(), sizeof(uint2));
+  }
+  if (two && !captured) {
+    // Odd runs' slice buffers, as large as the even runs'.
+    WS_HIP(ctx->lslices2.ensure(ctx->lslices.cap));
+    WS_HIP(ctx->cslices2.ensure(ctx->cslices.cap));
+    WS_HIP(ctx->scounts2.ensure(ctx->scounts.cap));
   }
   if (!captured) lookup();  // the setup may have grown buffers
   if (ctx->profiling && ctx->pev.size() < 5 * nplans) {
@@ -1535,13 +1573,34 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
   // graph then holds only the runs, and each replay gets this call's
   // segments without a copy node.
   auto launch_init = [&]() -> ws_status {
-    WS_HIP(launch_call_init(ds.sticky, 10, ctx->counters.as<uint32_t>(), kCounterLitSnapshot + 1, dstamps,
+    WS_HIP(launch_call_init(ds.sticky, 10, ctx->counters.as<uint32_t>(), kCounterWords, dstamps,
                             (uint32_t)nstamp, ci, ctx->items.as<uint2>(), st));
     return WS_OK;
   };
   auto launch_all = [&]() -> ws_status {
     if (!items_by_arg)
       WS_HIP(hipMemcpyAsync(ctx->items.p, ctx->hstage, nplans * sizeof(uint2), hipMemcpyHostToDevice, st));
+    if (two) {
+      // Run k's filter on stream2 once run k - 2 is done (its slices and
+      // counter block are free again); run k's later passes on the stream
+      // once that filter is done. Run k + 1's filter then overlaps run k's
+      // regex passes and ordering. (Events: fork, filter done by parity,
+      // run done by parity; a wait binds to the latest record before it.)
+      const hipStream_t sb = ctx->stream2;
+      WS_HIP(hipEventRecord(ctx->xev[0], st));
+      WS_HIP(hipStreamWaitEvent(sb, ctx->xev[0], 0));
+      for (size_t k = 0; k < nplans; ++k) {
+        if (k >= 2) WS_HIP(hipStreamWaitEvent(sb, ctx->xev[3 + (k & 1)], 0));
+        ws_status s = launch_run(ctx, p, plans[k], setups[k], offs[k], ds, nullptr, kRunFilter);
+        if (s != WS_OK) return s;
+        WS_HIP(hipEventRecord(ctx->xev[1 + (k & 1)], sb));
+        WS_HIP(hipStreamWaitEvent(st, ctx->xev[1 + (k & 1)], 0));
+        s = launch_run(ctx, p, plans[k], setups[k], offs[k], ds, nullptr, kRunRest);
+        if (s != WS_OK) return s;
+        WS_HIP(hipEventRecord(ctx->xev[3 + (k & 1)], st));
+      }
+      return WS_OK;
+    }
     for (size_t k = 0; k < nplans; ++k) {
       ws_status s = launch_run(ctx, p, plans[k], setups[k], offs[k], ds,
                                ctx->profiling ? &ctx->pev[5 * k] : nullptr);
@@ -2069,7 +2128,8 @@ std::vector<DevBuf*> ctx_buffers(ws_ctx* ctx) {
           &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
           &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
           &ctx->bcur, &ctx->dstate, &ctx->k2, &ctx->v2, &ctx->bcnt2, &ctx->bases,
-          &ctx->lkeys, &ctx->lcnt, &ctx->lofs, &ctx->posb};
+          &ctx->lkeys, &ctx->lcnt, &ctx->lofs, &ctx->posb, &ctx->lslices2, &ctx->cslices2,
+          &ctx->scounts2};
 }
 
 extern "C" {
@@ -2116,6 +2176,12 @@ ws_status ws_ctx_create(int device_ordinal, const ws_config* cfg, ws_ctx** out) 
   ctx->cus = (uint32_t)prop.multiProcessorCount;
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return bail("hipStreamCreate", e);
+  e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail("hipStreamCreate", e);
+  for (auto& ev : ctx->xev) {
+    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return bail("hipEventCreate", e);
+  }
   for (auto& ev : ctx->ev) {
     e = hipEventCreate(&ev);
     if (e != hipSuccess) return bail("hipEventCreate", e);
@@ -2140,7 +2206,7 @@ ws_status ws_ctx_create(int device_ordinal, const ws_config* cfg, ws_ctx** out) 
   }
   e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hcount), 32 * sizeof(uint32_t), hipHostMallocDefault);
   if (e != hipSuccess) return bail("hipHostMalloc", e);
-  e = ctx->counters.ensure(16 * sizeof(uint32_t));
+  e = ctx->counters.ensure(kCounterWords * sizeof(uint32_t));
   if (e != hipSuccess) return bail("hipMalloc", e);
   *out = ctx;
   return WS_OK;
@@ -2150,7 +2216,10 @@ void ws_ctx_destroy(ws_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->dev);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   for (DevBuf* b : ctx_buffers(ctx)) b->release();
+  for (auto& ev : ctx->xev)
+    if (ev) (void)hipEventDestroy(ev);
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto& ev : ctx->tev)
@@ -2163,6 +2232,7 @@ void ws_ctx_destroy(ws_ctx* ctx) {
   if (ctx->hstate) (void)hipHostFree(ctx->hstate);
   if (ctx->hcount) (void)hipHostFree(ctx->hcount);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   delete ctx;
 }
 

@@ -984,11 +984,14 @@ hipError_t sort_pairs(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_
 // End of a device-sized run: record overflows and errors, accumulate
 // statistics, advance the output offset by this run's match count, and zero
 // the run's counters for the next run. Called by a whole block.
-__device__ void run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky, unsigned long long* total,
-                          uint32_t* publish, uint32_t pwords) {
+// fcnt: the filter's counter block (words 5, 6 and 8; zeroed here), or
+// null when the filter counted in `counters`.
+__device__ void run_close(uint32_t* counters, uint32_t* fcnt, const RunCaps& caps, uint32_t* sticky,
+                          unsigned long long* total, uint32_t* publish, uint32_t pwords) {
   if (threadIdx.x == 0) {
+    const uint32_t* f = fcnt ? fcnt : counters;
     uint32_t redo = 0;
-    if (counters[5] > caps.lit_ovf_cap || counters[6] > caps.cand_ovf_cap) redo |= 2u;
+    if (f[5] > caps.lit_ovf_cap || f[6] > caps.cand_ovf_cap) redo |= 2u;
     if (counters[kCounterLitSnapshot] > caps.lit_cap) redo |= 4u;
     if (counters[1] > caps.cand_cap) redo |= 8u;
     if (counters[2] > caps.raw_cap) redo |= 16u;
@@ -997,11 +1000,12 @@ __device__ void run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sti
     sticky[kStickyRedo] |= redo;
     if (counters[3] & kFlagTruncatedWalk) sticky[kStickyError] |= 1u;
     if (counters[0] > caps.max_matches) sticky[kStickyError] |= 2u;
-    sticky[kStickyHits] += counters[8];
+    sticky[kStickyHits] += f[8];
     sticky[kStickyCands] += counters[1];
     sticky[kStickyRaw] += counters[2];
     *total += counters[0];
     for (uint32_t k = 0; k <= kCounterLitSnapshot; ++k) counters[k] = 0;
+    if (fcnt) fcnt[5] = fcnt[6] = fcnt[8] = 0;
   }
   // The call's last run hands the state block (flags, total, kernel stamps)
   // to pinned host memory, so the host reads it without a copy.
@@ -1024,18 +1028,18 @@ __device__ void run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sti
   }
 }
 
-__global__ void k_run_close(uint32_t* counters, RunCaps caps, uint32_t* sticky, unsigned long long* total,
-                            uint32_t* publish, uint32_t pwords) {
-  run_close(counters, caps, sticky, total, publish, pwords);
+__global__ void k_run_close(uint32_t* counters, uint32_t* fcnt, RunCaps caps, uint32_t* sticky,
+                            unsigned long long* total, uint32_t* publish, uint32_t pwords) {
+  run_close(counters, fcnt, caps, sticky, total, publish, pwords);
 }
 
 __global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *dst = *src;
 }
 
-hipError_t launch_run_close(uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
+hipError_t launch_run_close(uint32_t* counters, uint32_t* fcnt, const RunCaps& caps, uint32_t* sticky,
                             unsigned long long* total, uint32_t* publish, uint32_t pwords, hipStream_t st) {
-  hipLaunchKernelGGL(k_run_close, dim3(1), dim3(64), 0, st, counters, caps, sticky, total, publish, pwords);
+  hipLaunchKernelGGL(k_run_close, dim3(1), dim3(64), 0, st, counters, fcnt, caps, sticky, total, publish, pwords);
   return hipGetLastError();
 }
 
@@ -1453,8 +1457,8 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, ui
 
 __global__ __launch_bounds__(kSmallOrderTPB) void k_lit_order_small(
     SliceOut lit, uint32_t n, uint32_t pbits, uint32_t bshift, uint32_t start_off, void* out, uint64_t out_cap,
-    uint32_t* counters, RunCaps caps, uint32_t* sticky, unsigned long long* total, uint32_t* publish,
-    uint32_t pwords) {
+    uint32_t* counters, uint32_t* fcnt, RunCaps caps, uint32_t* sticky, unsigned long long* total,
+    uint32_t* publish, uint32_t pwords) {
   static_assert(kSmallOrderSlices + 1 <= kSmallOrderTPB && kSmallBuckets == kSmallOrderTPB, "one per lane");
   __shared__ uint64_t s_keys[kSmallOrder];
   __shared__ uint32_t s_end[kSmallOrder];
@@ -1473,6 +1477,11 @@ __global__ __launch_bounds__(kSmallOrderTPB) void k_lit_order_small(
   if (tid == 0) {
 #pragma unroll
     for (uint32_t k = 0; k <= kCounterLitSnapshot; ++k) cnt_in[k] = counters[k];
+    if (fcnt) {  // the filter counted its hits and overflows in its own block
+      cnt_in[5] = fcnt[5];
+      cnt_in[6] = fcnt[6];
+      cnt_in[8] = fcnt[8];
+    }
 #pragma unroll
     for (uint32_t k = 0; k < 5; ++k) st_in[k] = sticky[k];
     s_base = *total;
@@ -1576,6 +1585,7 @@ __global__ __launch_bounds__(kSmallOrderTPB) void k_lit_order_small(
     *total = tot;
 #pragma unroll
     for (uint32_t k = 0; k <= kCounterLitSnapshot; ++k) counters[k] = 0;
+    if (fcnt) fcnt[5] = fcnt[6] = fcnt[8] = 0;
   }
   if (publish) {
     // The state block to pinned host memory (words 0-9 as just closed, the
@@ -1596,13 +1606,13 @@ __global__ __launch_bounds__(kSmallOrderTPB) void k_lit_order_small(
 // sbits: bits of the largest start (the run's segment length).
 hipError_t launch_lit_order_small(const SliceOut& lit, uint32_t n, uint32_t pbits, uint32_t sbits,
                                   const uint32_t* flen, uint32_t start_off, void* out, uint64_t out_cap,
-                                  uint32_t* counters, const RunCaps& caps, uint32_t* sticky,
+                                  uint32_t* counters, uint32_t* fcnt, const RunCaps& caps, uint32_t* sticky,
                                   unsigned long long* total, uint32_t* publish, uint32_t pwords, hipStream_t st) {
   if (n > kSmallOrderSlices) return hipErrorInvalidValue;
   const uint32_t bshift = sbits > 10 ? sbits - 10 : 0;  // kSmallBuckets = 2^10
   (void)flen;  // a literal match record carries its end
   hipLaunchKernelGGL(k_lit_order_small, dim3(1), dim3(kSmallOrderTPB), 0, st, lit, n, pbits, bshift, start_off,
-                     out, out_cap, counters, caps, sticky, total, publish, pwords);
+                     out, out_cap, counters, fcnt, caps, sticky, total, publish, pwords);
   return hipGetLastError();
 }
 

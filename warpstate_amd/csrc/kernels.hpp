@@ -178,6 +178,11 @@ struct RunCaps {
   uint32_t lit_cap, cand_cap, raw_cap, rec_cap, max_matches;
 };
 constexpr uint32_t kCounterLitSnapshot = 10;
+// The counter buffer: the run counters (words 0-15), then two filter
+// counter blocks of 16 words (two-stream runs: the filter of run k counts
+// its hits and slice overflows in block k % 2, words 5, 6 and 8).
+constexpr uint32_t kCounterWords = 48;
+constexpr uint32_t kFilterCounterBlock = 16;
 // Literal-only ordering: records per bucket of the fixed-capacity buckets,
 // and the most buckets (the count array always has this many entries).
 constexpr uint32_t kLitBucketCap = 256;
