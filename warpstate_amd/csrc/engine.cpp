@@ -254,6 +254,7 @@ struct ws_patterns {
   const uint32_t *rx_tables = nullptr, *rx_toff = nullptr, *rx_ncls = nullptr,
                  *rx_pid = nullptr, *rx_maxlen = nullptr, *rx_starts = nullptr, *rx_scan = nullptr;
   const uint8_t* rx_cmaps = nullptr;
+  const uint2* rx_resume = nullptr;  // per 16-byte slot of fbytes x start context (regex factors; null: none)
   uint32_t rx_states_total = 0;
   bool ac_lds_ok = false;
 };
@@ -382,6 +383,38 @@ ws_status upload_patterns(ws_patterns* p) {
   size_t o_rxmax = put(rxmax.data(), rxmax.size() * 4);
   size_t o_rxstart = put(starts.data(), starts.size() * 4);
   size_t o_rxscan = put(h.rx_scan.data(), h.rx_scan.size() * 4);
+  // Where an anchored walk stands once it has read a regex factor that the
+  // prefilter verified at the walk's start: per factor (16-byte slot of its
+  // bytes) and start context, {state (bit 31: dead inside the factor),
+  // length << 16 | 1 + the offset of the last match flag inside it (0:
+  // none)}. The walk then starts after the factor. Case-folding sets
+  // verify folded bytes, so they walk from the start.
+  std::vector<uint2> resume;
+  if (!h.ci && !h.rx.empty()) {
+    resume.assign((fb.size() / 16) * 4, make_uint2(0, 0));
+    for (size_t f = 0; f < h.fac.size(); ++f) {
+      const Factor& fc = h.fac[f];
+      if (fc.kind != 1 || fc.bytes.size() > 0xFFFFu) continue;
+      const AnchoredDfa& d = h.rx[fc.owner];
+      const uint32_t C = d.nclass + 1;
+      for (uint32_t cx = 0; cx < 4; ++cx) {
+        uint32_t s = d.start[cx], last1 = 0;
+        bool dead = false;
+        for (size_t t = 0; t < fc.bytes.size(); ++t) {
+          const uint32_t e = d.table[(size_t)s * C + d.cmap[(uint8_t)fc.bytes[t]]];
+          if (e & DFA_MATCH) last1 = (uint32_t)t + 1;
+          if (e & DFA_DEAD) {
+            dead = true;
+            break;
+          }
+          s = e & DFA_STATE_MASK;
+        }
+        resume[(foff[f] / 16) * 4 + cx] =
+            make_uint2(dead ? 0x80000000u : s, ((uint32_t)fc.bytes.size() << 16) | last1);
+      }
+    }
+  }
+  size_t o_resume = put(resume.data(), resume.size() * sizeof(uint2));
 
   WS_HIP(p->blob.ensure(blob.size()));
   WS_HIP(hipMemcpy(p->blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
@@ -409,6 +442,7 @@ ws_status upload_patterns(ws_patterns* p) {
   p->rx_maxlen = reinterpret_cast<const uint32_t*>(b + o_rxmax);
   p->rx_starts = reinterpret_cast<const uint32_t*>(b + o_rxstart);
   p->rx_scan = reinterpret_cast<const uint32_t*>(b + o_rxscan);
+  p->rx_resume = resume.empty() ? nullptr : reinterpret_cast<const uint2*>(b + o_resume);
   return WS_OK;
 }
 
@@ -486,6 +520,7 @@ WalkParams walk_params(const ws_patterns* p, const Plan& plan, const uint2* item
   w.toff = p->rx_toff;
   w.ncls = p->rx_ncls;
   w.cmaps = p->rx_cmaps;
+  w.resume = p->rx_resume;
   w.rx_pid = p->rx_pid;
   w.counters = counters;
   w.window_truncated = plan.window_truncated ? 1u : 0u;

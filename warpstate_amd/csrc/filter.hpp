@@ -355,7 +355,9 @@ __device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, 
   }
   const uint32_t start = (uint32_t)(q - seg.x);
   const uint32_t kind = (e0.z >> 31) ? 2u : 1u;
-  *rec = make_uint4(e1.w, start, kind == 2u ? 0u : start + len, item);
+  // A literal match carries its end; a regex candidate its factor's slot
+  // (byte offset / 16) + 1, where the walk can resume after the factor.
+  *rec = make_uint4(e1.w, start, kind == 2u ? (e0.y >> 4) + 1u : start + len, item);
   if (len > 16) {
     // Bytes 16..31 of the factor in one load.
     const uint4 p1 = *reinterpret_cast<const uint4*>(c.fbytes + e0.y + 16);

@@ -302,8 +302,9 @@ __device__ __forceinline__ uint4 load16(const WalkParams& w, uint64_t a) {
   return make_uint4(v[0], v[1], v[2], v[3]);
 }
 
+// fz: the candidate's factor slot + 1 (0: none), see WalkParams::resume.
 __device__ __forceinline__ uint32_t rx_match_at(const WalkParams& w, uint32_t rx, uint2 seg, uint32_t st,
-                                                bool* open) {
+                                                bool* open, uint32_t fz = 0) {
   const uint32_t* T = w.tables + w.toff[rx];
   const uint32_t C = w.ncls[rx];
   const uint8_t* cm = w.cmaps + 256u * rx;
@@ -311,6 +312,14 @@ __device__ __forceinline__ uint32_t rx_match_at(const WalkParams& w, uint32_t rx
   const uint32_t ctx = (st > 0 || w.ctx_before) ? byte_ctx(w.base[p0 - 1]) : 0u;
   uint32_t s = w.starts[4 * rx + ctx];
   uint32_t last = 0xFFFFFFFFu;
+  uint64_t pb = p0;  // first byte the walk reads
+  if (fz && w.resume) {
+    const uint2 r = w.resume[4 * (fz - 1) + ctx];
+    if (r.y & 0xFFFFu) last = st + (r.y & 0xFFFFu) - 1u;
+    if (r.x & 0x80000000u) return last;
+    s = r.x;
+    pb = p0 + (r.y >> 16);
+  }
   // The first 48 bytes from p0 & ~15 in three loads issued together, and
   // their classes looked up at once (independent reads, packed one per
   // byte): inside this window the walk waits on one dependent read per
@@ -328,7 +337,7 @@ __device__ __forceinline__ uint32_t rx_match_at(const WalkParams& w, uint32_t rx
       cls[k] = (uint32_t)cm[d[k] & 0xFFu] | (uint32_t)cm[(d[k] >> 8) & 0xFFu] << 8 |
                (uint32_t)cm[(d[k] >> 16) & 0xFFu] << 16 | (uint32_t)cm[d[k] >> 24] << 24;
   }
-  for (uint64_t p = p0; p < seg.y; ++p) {
+  for (uint64_t p = pb; p < seg.y; ++p) {
     const uint64_t o = p - a0;
     uint32_t c;
     if (o < 48) {
@@ -386,7 +395,7 @@ __global__ __launch_bounds__(kTPB) void k_rx_walk(WalkParams w) {
     if (i < n) {
       cd = w.cands[i];
       bool open = false;
-      last = rx_match_at(w, cd.x, w.items[cd.w], cd.y, &open);
+      last = rx_match_at(w, cd.x, w.items[cd.w], cd.y, &open, cd.z);
       if (open) atomicOr(&w.counters[3], kFlagTruncatedWalk);
     }
     emit_raw(w, last != 0xFFFFFFFFu, cd.x, cd.y, last, cd.w, s_res);

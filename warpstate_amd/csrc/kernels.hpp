@@ -101,6 +101,10 @@ struct WalkParams {
   const uint32_t* toff;
   const uint32_t* ncls;
   const uint8_t* cmaps;
+  // Per factor slot (byte offset / 16) and start context: the state after
+  // the factor's bytes and where matches ended inside it (a candidate whose
+  // z field is slot + 1 starts its walk after the factor); null: no resume.
+  const uint2* resume;
   const uint32_t* rx_pid;
   Rec* raw;
   uint32_t raw_cap;
