@@ -211,6 +211,25 @@ ws_status ws_scan(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t* data,
                   uint32_t data_mem, uint32_t mode, const ws_tuning* tuning, ws_match* out,
                   uint64_t cap, uint32_t out_mem, uint64_t* count);
 
+/* ---- async scan: GpuMatcher::scan's async contract (core/src/gpu/mod.rs:223-260: the
+ * future resolves when the device's results are read back).
+ * ws_scan_async starts ws_scan with the same arguments and returns once the call is
+ * enqueued (overlapping mode on the device-sized pipeline; other modes and pipelines
+ * finish before it returns). The input, output and pattern set must stay valid and
+ * unchanged until ws_wait. One call per context may be outstanding: a second
+ * ws_scan_async before ws_wait is WS_ERR_INVALID_ARGUMENT. Any other call on the
+ * context first completes the outstanding one, whose result ws_wait still returns.
+ * A non-OK status from ws_scan_async means nothing is outstanding. */
+ws_status ws_scan_async(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t* data, uint64_t len,
+                        uint32_t data_mem, uint32_t mode, const ws_tuning* tuning, ws_match* out,
+                        uint64_t cap, uint32_t out_mem);
+/* 1 when ws_wait would return without waiting for the device (or nothing is outstanding),
+ * 0 while the device is still working on the call. */
+int ws_poll(ws_ctx* ctx);
+/* Completes the outstanding ws_scan_async call: its status and *count exactly as ws_scan
+ * would have returned them. WS_ERR_INVALID_ARGUMENT when nothing is outstanding. */
+ws_status ws_wait(ws_ctx* ctx, uint64_t* count);
+
 /* ---- allocation-free scan. No counterpart in the reference: for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc). ws_scan in overlapping mode for
@@ -258,7 +277,10 @@ ws_status ws_scan_batch_planned(ws_ctx* ctx, const ws_patterns* patterns, const 
 /* ---- stream: StreamPipeline::scan (core/src/pipeline.rs:86-153).
  * Scans in chunks of chunk_size bytes; each chunk reads `overlap` extra bytes and keeps
  * matches that start inside it. Regex non-overlap chains carry across chunks, so the
- * result equals one ws_scan over the whole input when overlap >= the longest match. */
+ * result equals one ws_scan over the whole input when overlap >= the longest match.
+ * WS_MODE_STANDARD with literal reductions over more than one chunk needs overlap >= the
+ * longest literal - 1 (WS_ERR_INVALID_ARGUMENT otherwise): the reductions are one greedy
+ * pass over the whole input, so such a call returns ws_scan's whole-input result. */
 ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* patterns, const uint8_t* data,
                          uint64_t len, uint32_t data_mem, uint64_t chunk_size, uint64_t overlap,
                          uint32_t mode, const ws_tuning* tuning, ws_match* out, uint64_t cap,

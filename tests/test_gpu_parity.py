@@ -187,6 +187,27 @@ def test_stream_chains_spanning_many_blocks(checked):
         assert_same(as_list(got), exp, f"chunk {chunk} checked={checked}")
 
 
+@pytest.mark.parametrize("npat", [4, 64, 6000])
+def test_stream_standard_mode_equals_whole(npat):
+    # PatternSet::scan semantics through StreamPipeline over many chunks: the
+    # literal reductions (MultiMemchr, LeftmostLongest, HashScanner) span the
+    # chunk edges, so the result is the whole input's.
+    alpha = np.frombuffer(b"ab", dtype=np.uint8)
+    lmin, lmax = {4: (2, 12), 64: (4, 12), 6000: (12, 14)}[npat]
+    lits = W.random_literals(npat, lmin, lmax, alpha, seed=npat + 1)
+    data = alpha[np.random.default_rng(npat + 1).integers(0, 2, size=150_000)].tobytes()
+    pats = [(0, l) for l in lits] + ([(1, b"ab{3,5}a")] if npat == 64 else [])
+    ps = build(pats)
+    m = GpuMatcher(ps)
+    exp = [tuple(x) for x in Oracle(pats).scan_list(data, mode=1)]
+    overlap = max(len(l) for l in lits) + 8
+    for chunk in (4096, 65_537):
+        got = StreamPipeline(m, overlap).scan_array(data, mode="standard", chunk_size=chunk)
+        assert_same(as_list(got), exp, f"chunk {chunk}")
+    with pytest.raises(WarpstateError):
+        StreamPipeline(m, 0).scan_array(data, mode="standard", chunk_size=4096)
+
+
 def test_stream_boundary_kat():
     # tests/unit/mod.rs:11-54: "legendary" over a 1024-byte chunk edge
     ps = PatternSet.builder().regex("legendary").build()

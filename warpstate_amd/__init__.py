@@ -21,6 +21,7 @@ from .api import (  # noqa: F401
     CompiledPatternIndex,
     Context,
     GpuMatcher,
+    PendingScan,
     PreparedScan,
     HotSwapPatternSet,
     Match,

@@ -160,6 +160,13 @@ SIGNATURES = {
         [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, POINTER(WsTuning), c_void_p, c_uint64,
          c_uint32, POINTER(c_uint64)],
     ),
+    "ws_scan_async": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, POINTER(WsTuning), c_void_p, c_uint64,
+         c_uint32],
+    ),
+    "ws_poll": (c_int, [c_void_p]),
+    "ws_wait": (c_int, [c_void_p, POINTER(c_uint64)]),
     "ws_scan_noalloc": (
         c_int,
         [c_void_p, c_void_p, c_void_p, c_uint64, POINTER(WsTuning), c_void_p, c_uint64, POINTER(c_uint64)],

@@ -457,6 +457,19 @@ class GpuMatcher:
         N.check(st, int(count.value))
         return int(count.value)
 
+    def scan_device_async(self, data_ptr: int, length: int, out_ptr: int, capacity: int, mode="overlapping",
+                          tuning: Optional[Tuning] = None, out_on_device: bool = True) -> "PendingScan":
+        """scan_device without waiting: the scan is enqueued and a PendingScan
+        returned (GpuMatcher::scan's future, core/src/gpu/mod.rs:223-260).
+        The buffers must stay valid until wait(); one outstanding call per
+        context."""
+        t = tuning.to_c() if tuning else None
+        st = N.load().ws_scan_async(self.ctx.handle, self.handle, ctypes.c_void_p(data_ptr), length, N.MEM_DEVICE,
+                                    _mode_code(mode), ctypes.byref(t) if t else None, ctypes.c_void_p(out_ptr),
+                                    capacity, N.MEM_DEVICE if out_on_device else N.MEM_HOST)
+        N.check(st)
+        return PendingScan(self)
+
     def prepare_device(self, data_ptr: int, length: int, out_ptr: int, capacity: int, mode="overlapping",
                        tuning: Optional[Tuning] = None) -> "PreparedScan":
         """scan_device with its arguments converted once: a repeated call on
@@ -510,6 +523,34 @@ class PreparedScan:
         if st:
             N.check(st, int(self._count.value))
         return self._count.value
+
+
+class PendingScan:
+    """An outstanding GpuMatcher.scan_device_async call: ready() polls,
+    wait() returns the match count (raising as scan_device would)."""
+
+    def __init__(self, matcher: GpuMatcher):
+        self._m = matcher  # keeps the context and the compiled set alive
+        self._done = False
+        self._count = 0
+        self._error = None
+
+    def ready(self) -> bool:
+        return self._done or N.load().ws_poll(self._m.ctx.handle) == 1
+
+    def wait(self) -> int:
+        if not self._done:
+            count = ctypes.c_uint64(0)
+            st = N.load().ws_wait(self._m.ctx.handle, ctypes.byref(count))
+            self._done = True
+            self._count = int(count.value)
+            try:
+                N.check(st, self._count)
+            except N.WarpstateError as e:
+                self._error = e
+        if self._error is not None:
+            raise self._error
+        return self._count
 
 
 class StreamPipeline:

@@ -233,6 +233,8 @@ struct ws_ctx {
   // regex chains' state from the earlier chunks (the chain walk reads it and
   // raises the other half, then the halves swap).
   uint32_t carry_phase = 0;
+  // The ws_scan_async call not yet collected by ws_wait, or null.
+  struct ws_async_call* pending = nullptr;
 };
 
 struct ws_patterns {
@@ -1507,11 +1509,24 @@ bool device_path_ok(uint32_t mode, const ws_tuning* tuning) {
 // A batch plan (one run, plans[0].batch) writes ws_tagged_match records
 // (rec_bytes = sizeof(ws_tagged_match)); its segments are already on the
 // device.
-ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size_t nplans, const uint32_t* offs,
-                      const ws_tuning* tuning, void* out, uint64_t cap, uint32_t out_mem, uint64_t* count,
-                      DeviceScanResult* outcome, const GraphKey* key_in, size_t rec_bytes = sizeof(ws_match)) {
+// Two halves: device_scan_launch enqueues the call, device_scan_complete
+// waits for it and reads the outcome (ws_scan_async returns between them).
+struct DevicePending {
+  size_t nplans = 0;
+  bool lit_only = false, small = false, items_by_arg = false, replayed = false, profiling = false;
+  CallItems ci{};
+  size_t state_bytes = 0;
+  void* dev_out = nullptr;  // where the kernels write the records
+  void* out = nullptr;      // the caller's output
+  uint64_t cap = 0;
+  uint32_t out_mem = 0;
+  size_t rec_bytes = sizeof(ws_match);
+};
+
+ws_status device_scan_launch(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size_t nplans,
+                             const uint32_t* offs, const ws_tuning* tuning, void* out, uint64_t cap,
+                             uint32_t out_mem, const GraphKey* key_in, size_t rec_bytes, DevicePending* dp) {
   const hipStream_t st = ctx->stream;
-  *outcome = kDeviceScanDone;
   const bool batch = nplans == 1 && plans[0].batch;
   // Everything sized and staged before the first launch.
   std::vector<RunSetup> setups(nplans);
@@ -1693,11 +1708,47 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
     if (s != WS_OK) return s;
   }
   if (ctx->profiling) WS_HIP(hipEventRecord(ctx->tev[1], st));
+  dp->nplans = nplans;
+  dp->lit_only = lit_only;
+  dp->small = small;
+  dp->items_by_arg = items_by_arg;
+  dp->replayed = replayed;
+  dp->profiling = ctx->profiling;
+  dp->ci = ci;
+  dp->state_bytes = state_bytes;
+  dp->dev_out = ds.out;
+  dp->out = out;
+  dp->cap = cap;
+  dp->out_mem = out_mem;
+  dp->rec_bytes = rec_bytes;
+  return WS_OK;
+}
+
+// Whether the launched call has finished on the device (without waiting).
+bool device_scan_ready(ws_ctx* ctx, const DevicePending& dp) {
+  volatile uint32_t* done_flag = ctx->hstate + dp.state_bytes / 4;
+  if (*done_flag == 1u) return true;
+  const hipError_t e = hipStreamQuery(ctx->stream);
+  if (e == hipErrorNotReady) (void)hipGetLastError();  // not an error of a later call
+  return e == hipSuccess;
+}
+
+ws_status device_scan_complete(ws_ctx* ctx, const DevicePending& dp, uint64_t* count,
+                               DeviceScanResult* outcome) {
+  const hipStream_t st = ctx->stream;
+  *outcome = kDeviceScanDone;
+  const size_t nplans = dp.nplans, state_bytes = dp.state_bytes, rec_bytes = dp.rec_bytes;
+  const bool lit_only = dp.lit_only, small = dp.small, items_by_arg = dp.items_by_arg, replayed = dp.replayed;
+  const CallItems& ci = dp.ci;
+  const uint64_t cap = dp.cap, max_matches = ctx->cfg.max_matches;
+  const uint32_t out_mem = dp.out_mem;
+  void* const out = dp.out;
+  volatile uint32_t* done_flag = ctx->hstate + state_bytes / 4;
   // Completion: spin briefly on the flag the last kernel sets (a shorter
   // wake-up than a blocking wait), then block on the stream. Per-phase
   // events of an eager call need the stream finished.
   bool flagged = false;
-  if (!(ctx->profiling && !replayed)) {
+  if (!(dp.profiling && !replayed)) {
     const auto t_spin = std::chrono::steady_clock::now();
     for (uint32_t it = 0;; ++it) {
       if (*done_flag == 1u) {
@@ -1730,7 +1781,7 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
   std::memcpy(&tot, hs + 8, 8);
   if (lit_only) ctx->lit_small = tot <= (uint64_t)nplans * (kSmallOrder / 4);
   ctx->sort_wide = std::max<uint64_t>(tot, hs[kStickyRaw]) > ((uint64_t)nplans << 16);
-  if (ctx->profiling) {
+  if (dp.profiling) {
     // The prefilter from its in-kernel stamps; the other phases from the
     // events between launches, which a replayed graph does not re-record
     // (then -1: unknown).
@@ -1772,10 +1823,20 @@ ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size
     return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(tot) + " exceeds " +
                                                   std::to_string(std::min<uint64_t>(cap, max_matches)) + ")");
   if (out_mem == WS_MEM_HOST && tot && out) {  // (no host array: timing runs)
-    WS_HIP(hipMemcpyAsync(out, ds.out, tot * rec_bytes, hipMemcpyDeviceToHost, st));
+    WS_HIP(hipMemcpyAsync(out, dp.dev_out, tot * rec_bytes, hipMemcpyDeviceToHost, st));
     WS_HIP(hipStreamSynchronize(st));
   }
   return WS_OK;
+}
+
+ws_status device_scan(ws_ctx* ctx, const ws_patterns* p, const Plan* plans, size_t nplans, const uint32_t* offs,
+                      const ws_tuning* tuning, void* out, uint64_t cap, uint32_t out_mem, uint64_t* count,
+                      DeviceScanResult* outcome, const GraphKey* key_in, size_t rec_bytes = sizeof(ws_match)) {
+  *outcome = kDeviceScanDone;
+  DevicePending dp;
+  ws_status s = device_scan_launch(ctx, p, plans, nplans, offs, tuning, out, cap, out_mem, key_in, rec_bytes, &dp);
+  if (s != WS_OK) return s;
+  return device_scan_complete(ctx, dp, count, outcome);
 }
 
 ws_status pack_out(ws_ctx* ctx, const Result& r, uint32_t start_off, ws_match* out, uint64_t cap,
@@ -1804,6 +1865,113 @@ ws_status pack_out(ws_ctx* ctx, const Result& r, uint32_t start_off, ws_match* o
     if (!redone) break;
   }
   return WS_OK;
+}
+
+// A single scan after its plan: the device-sized pipeline, and on an
+// overflow (or for the checked pipeline and standard mode) the host-checked
+// one. Called with ctx->mu held.
+ws_status scan_checked(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t mode, const ws_tuning* tuning,
+                       ws_match* out, uint64_t cap, uint32_t out_mem, uint64_t* count) {
+  Result r;
+  ws_status s = run_plan(ctx, p, plan, mode, tuning, &r);
+  if (s != WS_OK) {
+    if (s == WS_ERR_MATCH_BUFFER_OVERFLOW) *count = r.count;
+    return s;
+  }
+  *count = r.count;
+  s = pack_out(ctx, r, 0, out, cap, out_mem, 0);
+  finish_timings(ctx);
+  if (s != WS_OK) return s;
+  if (r.count > cap)
+    return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(r.count) +
+                                                  " exceeds " + std::to_string(cap) + ")");
+  return WS_OK;
+}
+
+GraphKey single_key(const ws_patterns* p, const uint8_t* data, uint64_t len, const ws_tuning* tuning,
+                    ws_match* out, uint64_t cap, uint32_t out_mem, bool profiling) {
+  GraphKey key;
+  key.pat_serial = p->serial;
+  key.data = data;
+  key.len = len;
+  key.cap = cap;
+  key.out = out_mem == WS_MEM_DEVICE ? out : nullptr;  // host output is copied after the launches
+  key.out_mem = out_mem;
+  key.profiling = profiling ? 1u : 0u;
+  if (tuning) {
+    key.variant = tuning->variant;
+    key.lane = tuning->bytes_per_lane;
+    key.bpc = tuning->blocks_per_cu;
+  }
+  return key;
+}
+
+ws_status scan_locked(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64_t len, uint32_t data_mem,
+                      uint32_t mode, const ws_tuning* tuning, ws_match* out, uint64_t cap, uint32_t out_mem,
+                      uint64_t* count) {
+  WS_HIP(hipSetDevice(ctx->dev));
+  ctx->last = ws_timings{};
+  Plan plan;
+  ws_status s = plan_single(ctx, data, len, data_mem, &plan);
+  if (s != WS_OK) return s;
+  if (device_path_ok(mode, tuning)) {
+    DeviceScanResult oc;
+    const GraphKey key = single_key(p, data, len, tuning, out, cap, out_mem, ctx->profiling);
+    const uint32_t off0 = 0;
+    s = device_scan(ctx, p, &plan, 1, &off0, tuning, out, cap, out_mem, count, &oc,
+                    data_mem == WS_MEM_DEVICE ? &key : nullptr);
+    if (s != WS_OK || oc == kDeviceScanDone) return s;
+    ctx->last = ws_timings{};  // a buffer overflowed: redo on the checked path
+  }
+  return scan_checked(ctx, p, plan, mode, tuning, out, cap, out_mem, count);
+}
+
+}  // namespace
+
+// The one ws_scan_async call of a context until ws_wait collects it: either
+// launched on the device-sized pipeline (launched = true until completed), or
+// already finished when ws_scan_async returned.
+struct ws_async_call {
+  bool launched = false;
+  DevicePending dp;
+  // To redo the call on the checked pipeline after an overflow.
+  ws_patterns* p = nullptr;  // a reference held while launched
+  Plan plan;
+  uint32_t mode = 0;
+  bool has_tuning = false;
+  ws_tuning tuning{};
+  ws_match* out = nullptr;
+  uint64_t cap = 0;
+  uint32_t out_mem = 0;
+  // The result, once known.
+  ws_status status = WS_OK;
+  uint64_t count = 0;
+  std::string err;
+};
+
+namespace {
+
+// Completes a launched ws_scan_async call (ctx->mu held): every other call
+// on the context does this first, so its buffers are free; the result stays
+// for ws_wait.
+void settle_pending(ws_ctx* ctx) {
+  ws_async_call* a = ctx->pending;
+  if (!a || !a->launched) return;
+  a->launched = false;
+  (void)hipSetDevice(ctx->dev);
+  DeviceScanResult oc = kDeviceScanDone;
+  uint64_t n = 0;
+  ws_status s = device_scan_complete(ctx, a->dp, &n, &oc);
+  if (s == WS_OK && oc == kDeviceScanRedo) {
+    ctx->last = ws_timings{};
+    s = scan_checked(ctx, a->p, a->plan, a->mode, a->has_tuning ? &a->tuning : nullptr, a->out, a->cap,
+                     a->out_mem, &n);
+  }
+  a->status = s;
+  a->count = n;
+  if (s != WS_OK) a->err = g_err;
+  ws_patterns_free(a->p);
+  a->p = nullptr;
 }
 
 // Pattern sources from the C ABI, names included.
@@ -1843,6 +2011,7 @@ ws_status make_patterns(ws_ctx* ctx, PatternsHost&& h, ws_patterns** out) {
   p->serial = g_pattern_serial.fetch_add(1);
   p->h = std::move(h);
   std::lock_guard<std::mutex> lk(ctx->mu);
+  settle_pending(ctx);
   p->dev = ctx->dev;
   if (hipSetDevice(ctx->dev) != hipSuccess) {
     delete p;
@@ -2250,6 +2419,11 @@ ws_status ws_ctx_create(int device_ordinal, const ws_config* cfg, ws_ctx** out) 
 void ws_ctx_destroy(ws_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->dev);
+  if (ctx->pending) {
+    settle_pending(ctx);
+    delete ctx->pending;
+    ctx->pending = nullptr;
+  }
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   for (DevBuf* b : ctx_buffers(ctx)) b->release();
@@ -2494,46 +2668,77 @@ ws_status ws_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64
   // An empty input matches only regexes that can match the empty string.
   if (len == 0 && p->h.rx_scan.empty()) return WS_OK;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  WS_HIP(hipSetDevice(ctx->dev));
-  ctx->last = ws_timings{};
-  Plan plan;
-  ws_status s = plan_single(ctx, data, len, data_mem, &plan);
-  if (s != WS_OK) return s;
+  settle_pending(ctx);
+  return scan_locked(ctx, p, data, len, data_mem, mode, tuning, out, cap, out_mem, count);
+}
+
+ws_status ws_scan_async(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64_t len,
+                        uint32_t data_mem, uint32_t mode, const ws_tuning* tuning, ws_match* out,
+                        uint64_t cap, uint32_t out_mem) {
+  if (!ctx || !p) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  if (p->dev != ctx->dev) return fail(WS_ERR_INVALID_ARGUMENT, "pattern set was compiled for another device");
+  if (len && !data) return fail(WS_ERR_INVALID_ARGUMENT, "data is null");
+  if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
+  if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (ctx->pending)
+    return fail(WS_ERR_INVALID_ARGUMENT, "this context's last ws_scan_async call has not been collected by ws_wait");
+  auto* a = new ws_async_call();
+  a->mode = mode;
+  a->has_tuning = tuning != nullptr;
+  if (tuning) a->tuning = *tuning;
+  a->out = out;
+  a->cap = cap;
+  a->out_mem = out_mem;
+  ctx->pending = a;
+  if (len == 0 && p->h.rx_scan.empty()) return WS_OK;  // no matches
   if (device_path_ok(mode, tuning)) {
-    DeviceScanResult oc;
-    GraphKey key;
-    key.pat_serial = p->serial;
-    key.data = data;
-    key.len = len;
-    key.cap = cap;
-    key.out = out_mem == WS_MEM_DEVICE ? out : nullptr;  // host output is copied after the launches
-    key.out_mem = out_mem;
-    key.profiling = ctx->profiling ? 1u : 0u;
-    if (tuning) {
-      key.variant = tuning->variant;
-      key.lane = tuning->bytes_per_lane;
-      key.bpc = tuning->blocks_per_cu;
+    WS_HIP(hipSetDevice(ctx->dev));
+    ctx->last = ws_timings{};
+    ws_status s = plan_single(ctx, data, len, data_mem, &a->plan);
+    if (s == WS_OK) {
+      const GraphKey key = single_key(p, data, len, tuning, out, cap, out_mem, ctx->profiling);
+      const uint32_t off0 = 0;
+      s = device_scan_launch(ctx, p, &a->plan, 1, &off0, tuning, out, cap, out_mem,
+                             data_mem == WS_MEM_DEVICE ? &key : nullptr, sizeof(ws_match), &a->dp);
     }
-    const uint32_t off0 = 0;
-    s = device_scan(ctx, p, &plan, 1, &off0, tuning, out, cap, out_mem, count, &oc,
-                    data_mem == WS_MEM_DEVICE ? &key : nullptr);
-    if (s != WS_OK || oc == kDeviceScanDone) return s;
-    ctx->last = ws_timings{};  // a buffer overflowed: redo on the checked path
+    if (s != WS_OK) {
+      ctx->pending = nullptr;
+      delete a;
+      return s;  // nothing is pending
+    }
+    ws_patterns_retain(const_cast<ws_patterns*>(p));
+    a->p = const_cast<ws_patterns*>(p);
+    a->launched = true;
+    return WS_OK;
   }
-  Result r;
-  s = run_plan(ctx, p, plan, mode, tuning, &r);
-  if (s != WS_OK) {
-    if (s == WS_ERR_MATCH_BUFFER_OVERFLOW) *count = r.count;
-    return s;
-  }
-  *count = r.count;
-  s = pack_out(ctx, r, 0, out, cap, out_mem, 0);
-  finish_timings(ctx);
-  if (s != WS_OK) return s;
-  if (r.count > cap)
-    return fail(WS_ERR_MATCH_BUFFER_OVERFLOW, "too many matches (" + std::to_string(r.count) +
-                                                  " exceeds " + std::to_string(cap) + ")");
+  // Other paths synchronise between their passes: finished here.
+  a->status = scan_locked(ctx, p, data, len, data_mem, mode, tuning, out, cap, out_mem, &a->count);
+  if (a->status != WS_OK) a->err = g_err;
   return WS_OK;
+}
+
+int ws_poll(ws_ctx* ctx) {
+  if (!ctx) return 1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ws_async_call* a = ctx->pending;
+  if (!a || !a->launched) return 1;
+  (void)hipSetDevice(ctx->dev);
+  return device_scan_ready(ctx, a->dp) ? 1 : 0;
+}
+
+ws_status ws_wait(ws_ctx* ctx, uint64_t* count) {
+  if (!ctx || !count) return fail(WS_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ws_async_call* a = ctx->pending;
+  if (!a) return fail(WS_ERR_INVALID_ARGUMENT, "no ws_scan_async call is pending on this context");
+  settle_pending(ctx);
+  ctx->pending = nullptr;
+  *count = a->count;
+  const ws_status s = a->status;
+  if (s != WS_OK) fail(s, a->err);
+  delete a;
+  return s;
 }
 
 ws_status ws_scan_noalloc(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, uint64_t len,
@@ -2551,6 +2756,7 @@ ws_status ws_scan_noalloc(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data
     return fail(WS_ERR_INVALID_ARGUMENT, "the allocation-free scan has no checked pipeline");
   if (len == 0 && p->h.rx_scan.empty()) return WS_OK;
   std::lock_guard<std::mutex> lk(ctx->mu);
+  settle_pending(ctx);
   WS_HIP(hipSetDevice(ctx->dev));
   uint8_t* base = g_scratch_ptr[ctx->dev];
   if (!base) return fail(WS_ERR_GPU_DEVICE, "scratch array not set up for this device");
@@ -2649,15 +2855,28 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
   if (cap && !out) return fail(WS_ERR_INVALID_ARGUMENT, "out is null");
   if (chunk_size == 0) return fail(WS_ERR_INVALID_ARGUMENT, "chunk_size must be positive");
   if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");
-  if (mode == WS_MODE_STANDARD && !p->h.lits.empty() && p->h.strategy != STRAT_NONE &&
-      len > chunk_size)
-    return fail(WS_ERR_INVALID_ARGUMENT,
-                "standard-mode literal reductions are not chunk-stitched; use overlapping mode");
   *count = 0;
+  if (mode == WS_MODE_STANDARD && !p->h.lits.empty() && p->h.strategy != STRAT_NONE &&
+      len > chunk_size) {
+    // PatternSet::scan's literal reductions are one greedy pass over the
+    // whole input's overlapping literal matches, so chunks cannot be reduced
+    // one at a time. When every literal fits a chunk's window (overlap >=
+    // longest literal - 1) the chunks' literal matches are the whole input's,
+    // and the chunked result is the whole-input scan; one ws_scan covers the
+    // input (up to 4 GiB - 1 MiB), so that is how it runs.
+    uint64_t maxlit = 0;
+    for (auto& l : p->h.lits) maxlit = std::max<uint64_t>(maxlit, l.size());
+    if (overlap + 1 < maxlit)
+      return fail(WS_ERR_INVALID_ARGUMENT,
+                  "standard mode needs overlap >= the longest literal - 1 (" + std::to_string(maxlit - 1) +
+                      ") to reduce literal matches across chunks");
+    return ws_scan(ctx, p, data, len, data_mem, mode, tuning, out, cap, out_mem, count);
+  }
   if (len == 0)  // no chunks: the empty input as one scan
     return p->h.rx_scan.empty() ? WS_OK : ws_scan(ctx, p, data, 0, data_mem, mode, tuning, out, cap, out_mem, count);
   if (len > kMaxScanBytes) return fail(WS_ERR_INPUT_TOO_LARGE, "stream input is too large");
   std::lock_guard<std::mutex> lk(ctx->mu);
+  settle_pending(ctx);
   WS_HIP(hipSetDevice(ctx->dev));
   ctx->last = ws_timings{};
   const uint32_t nrx = (uint32_t)p->h.rx.size();
@@ -2783,6 +3002,7 @@ ws_status ws_autotune(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data, ui
     return fail(WS_ERR_INVALID_ARGUMENT, "bad arguments");
   if (p->dev != ctx->dev) return fail(WS_ERR_INVALID_ARGUMENT, "pattern set was compiled for another device");
   std::lock_guard<std::mutex> lk(ctx->mu);
+  settle_pending(ctx);
   WS_HIP(hipSetDevice(ctx->dev));
   const uint32_t sup = supported_variants(p);
   int bi = -1;
@@ -2847,6 +3067,7 @@ ws_status ws_scan_batch(ws_ctx* ctx, const ws_patterns* p, const ws_item* items,
   if (n_items == 0) return WS_OK;
   if (n_items > 0x7FFFFFFFull) return fail(WS_ERR_INPUT_TOO_LARGE, "too many batch items");
   std::lock_guard<std::mutex> lk(ctx->mu);
+  settle_pending(ctx);
   WS_HIP(hipSetDevice(ctx->dev));
   ctx->last = ws_timings{};
   BatchLayout L;
@@ -2888,6 +3109,7 @@ ws_status ws_batch_plan_create(ws_ctx* ctx, const ws_item* items, size_t n_items
   if (n_items == 0) return fail(WS_ERR_INVALID_ARGUMENT, "a batch plan needs at least one item");
   if (n_items > 0x7FFFFFFFull) return fail(WS_ERR_INPUT_TOO_LARGE, "too many batch items");
   std::lock_guard<std::mutex> lk(ctx->mu);
+  settle_pending(ctx);
   WS_HIP(hipSetDevice(ctx->dev));
   auto* bp = new ws_batch_plan();
   bp->dev = ctx->dev;
@@ -2930,6 +3152,7 @@ ws_status ws_scan_batch_planned(ws_ctx* ctx, const ws_patterns* p, const ws_batc
   if (mode > WS_MODE_STANDARD) return fail(WS_ERR_INVALID_ARGUMENT, "unknown mode");
   *count = 0;
   std::lock_guard<std::mutex> lk(ctx->mu);
+  settle_pending(ctx);
   WS_HIP(hipSetDevice(ctx->dev));
   ctx->last = ws_timings{};
   if (batch_device_ok(p, mode, tuning)) {
@@ -3046,6 +3269,7 @@ void ws_stream_scanner_destroy(ws_stream_scanner* sc) {
   if (!sc) return;
   {
     std::lock_guard<std::mutex> lk(sc->ctx->mu);
+    settle_pending(sc->ctx);
     (void)hipSetDevice(sc->ctx->dev);
     (void)hipStreamSynchronize(sc->ctx->stream);
     sc->win[0].release();
@@ -3073,6 +3297,7 @@ ws_status ws_stream_scanner_feed(ws_stream_scanner* sc, const uint8_t* chunk, ui
   {
     // The next window: the current one's carried tail, then the chunk.
     std::lock_guard<std::mutex> lk(ctx->mu);
+    settle_pending(ctx);
     WS_HIP(hipSetDevice(ctx->dev));
     WS_HIP(sc->win[nxt].ensure(((combined + 15) & ~15ull) + 16));
     if (sc->overlap)
