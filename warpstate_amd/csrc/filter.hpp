@@ -415,6 +415,35 @@ struct HitWork {
   bool live;             // false for a lane without a hit this round
 };
 
+// Bytes 3j .. 3j + 2 of the window in W (bytes at K and beyond zero) as a
+// 24-bit value: one v_perm_b32.
+template <int K>
+__device__ __forceinline__ uint32_t key_chunk(const uint32_t* W, int j) {
+  const int o = 3 * j, w0 = o >> 2;
+  uint32_t sel = 0x0C000000u;  // byte 3: zero
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int b = o + i;
+    // v_perm_b32 byte selectors: 0-3 the second operand (W[w0]), 4-7 the
+    // first (W[w0 + 1]), 0x0C zero.
+    const uint32_t si = b < K ? (uint32_t)(b - 4 * w0) : 0x0Cu;
+    sel |= si << (8 * i);
+  }
+  const int last = o + 2 < K ? o + 2 : K - 1;  // the chunk's last window byte
+  const bool hi = (last >> 2) > w0;
+  return __builtin_amdgcn_perm(hi ? W[w0 + 1] : 0u, W[w0], sel);
+}
+
+// window_key (patterns.hpp) of the K bytes in W, already folded when CI.
+template <int K>
+__device__ __forceinline__ uint32_t window_key_dev(const uint32_t* W) {
+  constexpr int NCH = (K + 2) / 3;
+  uint32_t H = NCH == 1 ? key_chunk<K>(W, 0) : W[0];  // v_mad_u32_u24 reads the low 24 bits
+#pragma unroll
+  for (int j = 1; j < NCH; ++j) H = mad24_s(H, kWindowKeyMul, key_chunk<K>(W, j));
+  return H & 0x7FFFFFFFu;
+}
+
 // raw: the 48-byte aligned span from q & ~15 (zeros past the loadable end).
 // The window hash picks the bucket, whose two slots are read at once: one
 // dependent read per hit. (Checking an 8-byte tag pair first and reading the
@@ -429,25 +458,18 @@ __device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParam
   hw.seg = seg;
   const uint32_t sh = (uint32_t)(hw.q & 15);
   realign32(raw, sh, hw.W);
-  // The verification table's 24-bit window hash (v_mad_u32_u24 reads the
-  // low 24 bits of the running value; the window lies in the first 16 bytes).
-  uint32_t H = 0;
+  if (CI) {
 #pragma unroll
-  for (int t = 0; t < K; ++t) {
-    const uint32_t b = (hw.W[t >> 2] >> ((t & 3) * 8)) & 0xFFu;
-    H = mad24_s(H, h.B, CI ? fold_byte((uint8_t)b) : b);
+    for (int k = 0; k < 8; ++k) hw.W[k] = fold_word(hw.W[k]);
   }
-  H &= 0xFFFFFFu;
+  // The verification table's key of the window (its first 16 bytes).
+  const uint32_t H = window_key_dev<K>(hw.W);
   hw.H = H;
   const uint4* slot = h.vtab + 4 * ((H * 0x9E3779B1u) >> (32 - h.nbits));
   hw.s0 = slot[0];
   hw.s1 = slot[1];
   hw.s2 = slot[2];
   hw.s3 = slot[3];
-  if (CI) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) hw.W[k] = fold_word(hw.W[k]);
-  }
 }
 
 // Stride-2 windows: S = 2 hits are at window positions p, and an entry for
@@ -512,7 +534,7 @@ constexpr uint32_t kHitRing = 128;
 template <int K, bool CI, bool RX>
 __device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashParams& h, const uint2* ring,
                                               uint32_t head, uint32_t n, uint32_t* s_cnt, const SliceOut& lit,
-                                              const SliceOut& cand, uint2 seg0) {
+                                              const SliceOut& cand, uint2 seg0, __amdgpu_buffer_rsrc_t rsrc) {
   const uint32_t lane = lane_id();
   HitWork hw;
   hw.live = false;
@@ -521,15 +543,13 @@ __device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashPar
   if (lane < n) {
     const uint2 hr = ring[(head + lane) & (kHitRing - 1)];
     const uint2 seg = c.units ? c.items[hr.y] : seg0;
-    uint32_t lend = (seg.y + 15) & ~15u;
-    if (lend > (uint32_t)c.load_end) lend = (uint32_t)c.load_end;
     const uint32_t a = hr.x & ~15u;
     uint32_t raw[12];
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
-      uint4 x = make_uint4(0, 0, 0, 0);
-      if (a + 16 * v + 16 <= lend) x = *reinterpret_cast<const uint4*>(c.base + a + 16 * v);
-      raw[4 * v] = x.x; raw[4 * v + 1] = x.y; raw[4 * v + 2] = x.z; raw[4 * v + 3] = x.w;
+      typedef int i4 __attribute__((ext_vector_type(4)));
+      const i4 x = __builtin_amdgcn_raw_buffer_load_b128(rsrc, a + 16 * v, 0, 0);
+      raw[4 * v] = (uint32_t)x.x; raw[4 * v + 1] = (uint32_t)x.y; raw[4 * v + 2] = (uint32_t)x.z; raw[4 * v + 3] = (uint32_t)x.w;
     }
     hit_prepare<K, CI>(c, h, hr, seg, raw, hw);
   }
@@ -551,19 +571,6 @@ __device__ __forceinline__ void hit_prepare_s2(const ScanCommon& c, const HashPa
   realign36(raw, sh, Wb);
 #pragma unroll
   for (int i = 0; i < 8; ++i) hw.W[i] = __builtin_amdgcn_alignbyte(Wb[i + 1], Wb[i], o1);
-  uint32_t H = 0;
-#pragma unroll
-  for (int t = 0; t < K; ++t) {
-    const uint32_t b = (hw.W[t >> 2] >> ((t & 3) * 8)) & 0xFFu;
-    H = mad24_s(H, h.B, CI ? fold_byte((uint8_t)b) : b);
-  }
-  H &= 0xFFFFFFu;
-  hw.H = H;
-  const uint4* slot = h.vtab + 4 * ((H * 0x9E3779B1u) >> (32 - h.nbits));
-  hw.s0 = slot[0];
-  hw.s1 = slot[1];
-  hw.s2 = slot[2];
-  hw.s3 = slot[3];
 #pragma unroll
   for (int i = 0; i < 8; ++i) hw.W1[i] = Wb[i];
   if (CI) {
@@ -573,12 +580,19 @@ __device__ __forceinline__ void hit_prepare_s2(const ScanCommon& c, const HashPa
       hw.W1[k] = fold_word(hw.W1[k]);
     }
   }
+  const uint32_t H = window_key_dev<K>(hw.W);
+  hw.H = H;
+  const uint4* slot = h.vtab + 4 * ((H * 0x9E3779B1u) >> (32 - h.nbits));
+  hw.s0 = slot[0];
+  hw.s1 = slot[1];
+  hw.s2 = slot[2];
+  hw.s3 = slot[3];
 }
 
 template <int K, bool CI, bool RX>
 __device__ __forceinline__ void verify_queued_s2(const ScanCommon& c, const HashParams& h, const uint2* ring,
                                                  uint32_t head, uint32_t n, uint32_t* s_cnt, const SliceOut& lit,
-                                                 const SliceOut& cand, uint2 seg0) {
+                                                 const SliceOut& cand, uint2 seg0, __amdgpu_buffer_rsrc_t rsrc) {
   const uint32_t lane = lane_id();
   HitWork hw;
   hw.live = false;
@@ -587,16 +601,14 @@ __device__ __forceinline__ void verify_queued_s2(const ScanCommon& c, const Hash
   if (lane < n) {
     const uint2 hr = ring[(head + lane) & (kHitRing - 1)];
     const uint2 seg = c.units ? c.items[hr.y] : seg0;
-    uint32_t lend = (seg.y + 15) & ~15u;
-    if (lend > (uint32_t)c.load_end) lend = (uint32_t)c.load_end;
     const uint32_t o1 = hr.x > 0 ? 1u : 0u;
     const uint32_t a = (hr.x - o1) & ~15u;
     uint32_t raw[16];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      uint4 x = make_uint4(0, 0, 0, 0);
-      if (a + 16 * v + 16 <= lend) x = *reinterpret_cast<const uint4*>(c.base + a + 16 * v);
-      raw[4 * v] = x.x; raw[4 * v + 1] = x.y; raw[4 * v + 2] = x.z; raw[4 * v + 3] = x.w;
+      typedef int i4 __attribute__((ext_vector_type(4)));
+      const i4 x = __builtin_amdgcn_raw_buffer_load_b128(rsrc, a + 16 * v, 0, 0);
+      raw[4 * v] = (uint32_t)x.x; raw[4 * v + 1] = (uint32_t)x.y; raw[4 * v + 2] = (uint32_t)x.z; raw[4 * v + 3] = (uint32_t)x.w;
     }
     hit_prepare_s2<K, CI>(c, h, hr, seg, raw, o1, hw);
   }
@@ -614,17 +626,18 @@ __device__ __forceinline__ void stamp_end(const ScanCommon& c) {
 }
 
 // Inclusive prefix sum over the wave: row_shr 1, 2, 4, 8 within each row of
-// 16 lanes (DPP, lanes without a source add 0), then the totals of the rows
-// below (lanes 15, 31, 47).
+// 16 lanes (DPP, lanes without a source add 0), then row_bcast:15 adds
+// lane 15's sum to row 1 and lane 47's to row 3, and row_bcast:31 lane 31's
+// to rows 2 and 3 (rows outside the row mask add 0): six DPP adds.
 __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x, uint32_t lane) {
+  (void)lane;
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);
-  const uint32_t r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31);
-  const uint32_t r2 = __builtin_amdgcn_readlane(x, 47);
-  const uint32_t row = lane >> 4;
-  return x + (row == 0 ? 0u : row == 1 ? r0 : row == 2 ? r0 + r1 : r0 + r1 + r2);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);
+  return x;
 }
 
 // Orders a wave's LDS ring writes before its lanes read each other's entries.
@@ -746,14 +759,15 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
   // load per granule on every path (so the waits before each use can count
   // them), and positions whose window leaves the segment are masked out
   // afterwards, whatever bytes they hashed.
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(c.base), 0, (int)load_end, 0x00020000);
   auto load_quarter = [&](const Half& hf, int j, uint32_t* dj) {
     const uint32_t p0 = quarter_p0(hf, j);
-    const uint32_t amin = hf.lo & ~15u, amax = hf.lend - 16;
 #pragma unroll
     for (int v = 0; v < NW / 4; ++v) {
-      const uint32_t a = min(max(p0 + 16 * v, amin), amax);  // v_med3_u32
-      const uint4 x = *reinterpret_cast<const uint4*>(c.base + a);
-      dj[4 * v] = x.x; dj[4 * v + 1] = x.y; dj[4 * v + 2] = x.z; dj[4 * v + 3] = x.w;
+      typedef int i4 __attribute__((ext_vector_type(4)));
+      const i4 x = __builtin_amdgcn_raw_buffer_load_b128(rsrc, p0 + 16 * v, 0, 0);
+      dj[4 * v] = (uint32_t)x.x; dj[4 * v + 1] = (uint32_t)x.y; dj[4 * v + 2] = (uint32_t)x.z; dj[4 * v + 3] = (uint32_t)x.w;
     }
   };
   // Window positions: every S-th byte of the lane's PL (S = 2 for odd K).
@@ -785,9 +799,9 @@ __global__ __launch_bounds__(kFilterTPB) void k_hash_filter(ScanCommon c, HashPa
   };
   auto verify = [&](uint32_t head, uint32_t n) {
     if constexpr (S == 2)
-      verify_queued_s2<K, CI, RX>(c, h, ring, head, n, s_cnt, lit, cand, seg0);
+      verify_queued_s2<K, CI, RX>(c, h, ring, head, n, s_cnt, lit, cand, seg0, rsrc);
     else
-      verify_queued<K, CI, RX>(c, h, ring, head, n, s_cnt, lit, cand, seg0);
+      verify_queued<K, CI, RX>(c, h, ring, head, n, s_cnt, lit, cand, seg0, rsrc);
   };
 
   // Software pipeline over the wave's half units: a quarter's window is

@@ -105,9 +105,10 @@ void build_hash(PatternsHost& ps) {
       const uint32_t e = f * S + o;
       const uint8_t* g = (const uint8_t*)ps.fac[f].bytes.data() + o;
       const uint32_t h = rolling_hash(g, K, ps.B, ps.ci);
+      const uint32_t key = window_key(g, K, ps.ci);
       ps.ent_fac[e] = f;
       ps.ent_off[e] = o;
-      ps.ent_hash[e] = h;
+      ps.ent_hash[e] = key;
       if (ps.dual) {
         const uint32_t d = S == 1 ? pair_hash(g, K, ps.BB, ps.ci) : dual_hash(g, K, ps.BB, ps.ci);
         const uint32_t w = (d >> 2) & ((1u << ps.wbits) - 1);
@@ -118,7 +119,7 @@ void build_hash(PatternsHost& ps) {
         ps.bloom[w] |= 1u << (h & 31);
         ps.bloom[w] |= 1u << ((h >> ps.s2) & 31);
       }
-      bucket[e] = (uint32_t)((h * 0x9E3779B1u) >> (32 - ps.nbits));
+      bucket[e] = (uint32_t)((key * 0x9E3779B1u) >> (32 - ps.nbits));
       cnt[bucket[e] + 1]++;
     }
   }
@@ -275,6 +276,16 @@ uint32_t pair_hash(const uint8_t* p, uint32_t K, uint32_t BB, bool ci) {
     hh = (hh * Bh + x) & 0xFFFFu;
   }
   return hl | (hh << 16);
+}
+
+uint32_t window_key(const uint8_t* p, uint32_t K, bool ci) {
+  uint32_t h = 0;
+  for (uint32_t j = 0; 3 * j < K; ++j) {
+    uint32_t c = 0;
+    for (uint32_t i = 0; i < 3 && 3 * j + i < K; ++i) c |= (uint32_t)fold(p[3 * j + i], ci) << (8 * i);
+    h = j == 0 ? c : (h & 0xFFFFFFu) * kWindowKeyMul + c;
+  }
+  return h & 0x7FFFFFFFu;
 }
 
 uint32_t rolling_hash(const uint8_t* p, uint32_t K, uint32_t B, bool ci) {

@@ -113,8 +113,17 @@ PatternsHost compile_patterns(const std::vector<PatternSrc>& src, bool case_inse
 // Shared by the set compiler (which picks K) and the filter kernel.
 constexpr uint32_t hash_stride(uint32_t K) { return (K >= 5 && (K & 1u)) ? 2u : 1u; }
 
-// Rolling hash of the first K bytes (mod 2^24), shared by host and kernel.
+// Rolling hash of the first K bytes (mod 2^24): the exact filter's key for
+// K <= 2 (B = 256).
 uint32_t rolling_hash(const uint8_t* p, uint32_t K, uint32_t B, bool ci);
+// Verification-table key of a K-byte window (K <= 16), shared by host and
+// kernel: the window's 3-byte little-endian chunks c_j (bytes past K zero),
+// H = c_0, then H = (H mod 2^24) * kWindowKeyMul + c_j (mod 2^32) for each
+// next chunk, top bit cleared (the table's empty and overflow markers have
+// it set). On the device a chunk is one byte permute and a step one
+// v_mad_u32_u24: 6 operations for 12 bytes.
+constexpr uint32_t kWindowKeyMul = 0x9E3779u;
+uint32_t window_key(const uint8_t* p, uint32_t K, bool ci);
 // The 32-bit bloom hash of a K-byte window at every second position: low
 // half sum b[t] Bl^(K-1-t), high half with Bh, each mod 2^16
 // (BB = Bl | Bh << 16).
