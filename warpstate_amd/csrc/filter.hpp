@@ -315,24 +315,24 @@ __device__ __forceinline__ void slice_put(const SliceOut& so, uint32_t slice, ui
   }
 }
 
-// The 16 bytes from q as four dwords, out of the 32-byte aligned span
-// raw[0..7] that holds them (q - (q & ~15) = sh): two operand selects by
+// The 32 bytes from q as eight dwords, out of the 48-byte aligned span
+// raw[0..11] that holds them (q - (q & ~15) = sh): two operand selects by
 // the dword shift, then one v_alignbyte per output dword.
 // (Bit selects through all-ones / zero masks, v_bfi_b32, so the compiler
 // keeps the span in registers instead of indexing a stack copy.)
-__device__ __forceinline__ void realign16(const uint32_t* raw, uint32_t sh, uint32_t* W) {
+__device__ __forceinline__ void realign32(const uint32_t* raw, uint32_t sh, uint32_t* W) {
   const uint32_t m2 = 0u - ((sh >> 3) & 1u), m1 = 0u - ((sh >> 2) & 1u);
-  uint32_t t[6], u[5];
+  uint32_t t[10], u[9];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) t[i] = (raw[i + 2] & m2) | (raw[i] & ~m2);
+  for (int i = 0; i < 10; ++i) t[i] = (raw[i + 2] & m2) | (raw[i] & ~m2);
 #pragma unroll
-  for (int i = 0; i < 5; ++i) u[i] = (t[i + 1] & m1) | (t[i] & ~m1);
+  for (int i = 0; i < 9; ++i) u[i] = (t[i + 1] & m1) | (t[i] & ~m1);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) W[i] = __builtin_amdgcn_alignbyte(u[i + 1], u[i], sh & 3);
+  for (int i = 0; i < 8; ++i) W[i] = __builtin_amdgcn_alignbyte(u[i + 1], u[i], sh & 3);
 }
 
 // The 36 bytes from base + sh as nine dwords, out of raw[0..12] (the
-// 16-byte aligned span at base): realign16's steps over nine output dwords.
+// 16-byte aligned span at base): realign32 with one more output dword.
 __device__ __forceinline__ void realign36(const uint32_t* raw, uint32_t sh, uint32_t* W) {
   const uint32_t m2 = 0u - ((sh >> 3) & 1u), m1 = 0u - ((sh >> 2) & 1u);
   uint32_t t[11], u[10];
@@ -351,12 +351,9 @@ __device__ __forceinline__ void realign36(const uint32_t* raw, uint32_t sh, uint
 // candidate); 0 otherwise.
 // Every factor has at least K bytes: its first min(K, 16) are compared with
 // masks known at compile time, the rest (a longer factor) by its length.
-// LAZY: W holds bytes 0-15 only and bytes 16-31 are realigned from R (the
-// span's dwords 4-11, shift sh) when a factor is longer than 16 bytes.
-template <bool CI, int K, bool LAZY>
+template <bool CI, int K>
 __device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, uint4 e1, const uint32_t* W,
-                                                 uint64_t q, uint2 seg, uint32_t item, uint4* rec,
-                                                 const uint32_t* R, uint32_t sh) {
+                                                 uint64_t q, uint2 seg, uint32_t item, uint4* rec) {
   const uint32_t len = e0.z & 0x7FFFFFFFu;
   if (q + len > seg.y) return 0;
   const uint32_t pre[4] = {e0.w, e1.x, e1.y, e1.z};
@@ -387,23 +384,12 @@ __device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, 
     // Bytes 16..31 of the factor in one load.
     const uint4 p1 = *reinterpret_cast<const uint4*>(c.fbytes + e0.y + 16);
     const uint32_t pw[4] = {p1.x, p1.y, p1.z, p1.w};
-    uint32_t Wh[4];
-    if constexpr (LAZY) {
-      realign16(R, sh, Wh);
-      if (CI) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Wh[k] = fold_word(Wh[k]);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) Wh[k] = W[4 + k];
-    }
     uint32_t diff = 0;
 #pragma unroll
     for (int k = 4; k < 8; ++k) {
       const uint32_t rem = len > 4u * k ? len - 4 * k : 0u;
       const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
-      diff |= (Wh[k - 4] ^ pw[k - 4]) & msk;
+      diff |= (W[k] ^ pw[k - 4]) & msk;
     }
     if (diff) return 0;
     // Factors longer than 32 bytes: the rest byte by byte.
@@ -444,10 +430,8 @@ struct HitWork {
   uint64_t q;
   uint32_t item;
   uint2 seg;
-  uint32_t W[8];   // windows every position: bytes 0-15 only (W[4..7] from R on demand)
+  uint32_t W[8];
   uint32_t W1[8];
-  uint32_t R[8];   // windows every position: the span's dwords 4-11
-  uint32_t sh;     // and its shift
   uint32_t H;
   uint4 s0, s1, s2, s3;  // its bucket's two slots
   bool live;             // false for a lane without a hit this round
@@ -495,15 +479,10 @@ __device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParam
   hw.item = hr.y;
   hw.seg = seg;
   const uint32_t sh = (uint32_t)(hw.q & 15);
-  realign16(raw, sh, hw.W);
-#pragma unroll
-  for (int k = 4; k < 8; ++k) hw.W[k] = 0;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) hw.R[k] = raw[4 + k];
-  hw.sh = sh;
+  realign32(raw, sh, hw.W);
   if (CI) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) hw.W[k] = fold_word(hw.W[k]);
+    for (int k = 0; k < 8; ++k) hw.W[k] = fold_word(hw.W[k]);
   }
   // The verification table's key of the window (its first 16 bytes).
   const uint32_t H = window_key_dev<K>(hw.W);
@@ -522,7 +501,7 @@ template <bool CI, int S, int K>
 __device__ __forceinline__ uint32_t verify_gram(const ScanCommon& c, uint4 e0, uint4 e1, const HitWork& hw,
                                                 uint4* rec) {
   if constexpr (S == 1) {
-    return verify_entry<CI, K, true>(c, e0, e1, hw.W, hw.q, hw.seg, hw.item, rec, hw.R, hw.sh);
+    return verify_entry<CI, K>(c, e0, e1, hw.W, hw.q, hw.seg, hw.item, rec);
   } else {
     const uint32_t o = e0.y & 15u;
     if (o > hw.q - hw.seg.x) return 0;
@@ -532,7 +511,7 @@ __device__ __forceinline__ uint32_t verify_gram(const ScanCommon& c, uint4 e0, u
 #pragma unroll
     for (int i = 0; i < 8; ++i) W[i] = o ? hw.W1[i] : hw.W[i];
     e0.y &= ~15u;
-    return verify_entry<CI, K, false>(c, e0, e1, W, q, hw.seg, hw.item, rec, nullptr, 0u);
+    return verify_entry<CI, K>(c, e0, e1, W, q, hw.seg, hw.item, rec);
   }
 }
 
