@@ -33,6 +33,8 @@ size_t ac_shmem_bytes(bool lds_table, uint32_t states, uint32_t classes);
 hipError_t launch_ac(bool lds_table, uint32_t grid, hipStream_t st, const ScanCommon& c,
                      const AcParams& a);
 hipError_t launch_walk(uint32_t ncand_host, hipStream_t st, const WalkParams& w);
+hipError_t launch_walk_slices(hipStream_t st, const WalkParams& w, const SliceOut& cand, uint32_t nslices,
+                              uint32_t* cand_total);
 hipError_t launch_rx_scan(hipStream_t st, const WalkParams& w);
 hipError_t launch_order_keys(const Rec* recs, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t pbits,
                              uint64_t* keys, uint32_t* vals, hipStream_t st, const DenseOut& dense = DenseOut{});
@@ -1354,6 +1356,7 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
   caps.max_matches = rs.max_matches;
   caps.lit_ovf_cap = caps.cand_ovf_cap = 0xFFFFFFFFu;
   bool keyed = false;  // ordering pairs already made (and counted) by the compaction
+  SliceOut cand_slices{};  // keyed regex runs: the walk reads the candidates from the filter's slices
   const bool filter = rs.nunits > 0 && !h.fac.empty();  // no factors: no filter pass
   if (part == kRunFilter && !(filter && rs.variant == WS_VARIANT_HASH)) return WS_OK;  // nothing to filter
   if (filter && rs.variant == WS_VARIANT_HASH) {
@@ -1391,12 +1394,21 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
       return WS_OK;
     }
     // The compaction emits the literals' ordering pairs and bucket counts
-    // (into k0/v0, or k2/v2 when a chain sort will use k0..v1) and, for
-    // regex sets, the candidate records.
-    WS_HIP(launch_slice_compact_keys(lso, cso, rs.nvb, cnt + 0, cnt + 1, cnt + kCounterLitSnapshot, c.lit_cap,
-                                     rs.sbits, rs.pbits, (nrx ? ctx->k2 : ctx->k0).as<uint64_t>(),
-                                     (nrx ? ctx->v2 : ctx->v0).as<uint32_t>(), ctx->bcnt.as<uint32_t>(),
-                                     rs.obits - rs.ob, nrx ? c.cand_out : nullptr, c.cand_cap, st, plan.dense));
+    // (into k0/v0, or k2/v2 when a chain sort will use k0..v1) and, for a
+    // batch's regex sets, the candidate records. A single segment's regex
+    // walk reads the candidates from their slices instead (one launch
+    // fewer per run: the 64 MiB regex chunks' step 1.64 -> 1.55 ms; a
+    // batch's slices are too uneven, its step 0.63 -> 0.66 ms that way),
+    // and a set without literals has no pairs to emit (their counters are
+    // zero).
+    const bool from_slices = nrx > 0 && !plan.batch;
+    if (!from_slices || !h.lits.empty())
+      WS_HIP(launch_slice_compact_keys(lso, cso, rs.nvb, cnt + 0, cnt + 1, cnt + kCounterLitSnapshot, c.lit_cap,
+                                       rs.sbits, rs.pbits, (nrx ? ctx->k2 : ctx->k0).as<uint64_t>(),
+                                       (nrx ? ctx->v2 : ctx->v0).as<uint32_t>(), ctx->bcnt.as<uint32_t>(),
+                                       rs.obits - rs.ob, nrx && !from_slices ? c.cand_out : nullptr, c.cand_cap,
+                                       st, plan.dense));
+    if (from_slices) cand_slices = cso;
     keyed = true;
     caps.lit_ovf_cap = lso.ovf_cap;
     caps.cand_ovf_cap = cso.ovf_cap;
@@ -1434,7 +1446,10 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
       w.rbits = rs.rbits;
       w.cbase = plan.batch ? plan.dense.bases : nullptr;
     }
-    WS_HIP(launch_walk(0, st, w));
+    if (cand_slices.buf)
+      WS_HIP(launch_walk_slices(st, w, cand_slices, rs.nvb, cnt + 1));
+    else
+      WS_HIP(launch_walk(0, st, w));
     if (w.nscan) WS_HIP(launch_rx_scan(st, w));
     if (!keyed)
       WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), rs.raw_cap, cnt + 2, ksbits, rs.rbits, ctx->k0.as<uint64_t>(),

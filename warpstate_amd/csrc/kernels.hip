@@ -402,6 +402,39 @@ __global__ __launch_bounds__(kTPB) void k_rx_walk(WalkParams w) {
   }
 }
 
+// The same walk straight from the filter's candidate slices (device-sized
+// runs; no compaction pass): block b < nslices walks slice b, blocks
+// [nslices, nslices + kOvfBlocks) the overflow area, and block nslices also
+// writes the candidate total (the slice counts' sum plus the overflow's).
+constexpr uint32_t kOvfBlocks = 64;
+
+__global__ __launch_bounds__(kTPB) void k_rx_walk_slices(WalkParams w, SliceOut cand, uint32_t nslices,
+                                                         uint32_t* cand_total) {
+  __shared__ uint32_t s_res[2];
+  const bool ovf = blockIdx.x >= nslices;
+  const uint32_t m = ovf ? min(*cand.ovf_count, cand.ovf_cap) : min(cand.counts[blockIdx.x], cand.cap);
+  const Rec* src = ovf ? cand.ovf : cand.buf + (size_t)blockIdx.x * cand.cap;
+  const uint32_t first = ovf ? (blockIdx.x - nslices) * kTPB : 0u;
+  const uint32_t step = ovf ? kOvfBlocks * kTPB : kTPB;
+  if (blockIdx.x == nslices) {
+    const uint32_t t = block_slice_sum(cand, nslices);
+    if (threadIdx.x == 0) *cand_total = t + m;
+  }
+  // Block-uniform loop: every lane reaches the slot reservation.
+  for (uint32_t b0 = first; b0 < m; b0 += step) {
+    const uint32_t i = b0 + threadIdx.x;
+    Rec cd = make_uint4(0, 0, 0, 0);
+    uint32_t last = 0xFFFFFFFFu;
+    if (i < m) {
+      cd = src[i];
+      bool open = false;
+      last = rx_match_at(w, cd.x, w.items[cd.w], cd.y, &open, cd.z);
+      if (open) atomicOr(&w.counters[3], kFlagTruncatedWalk);
+    }
+    emit_raw(w, last != 0xFFFFFFFFu, cd.x, cd.y, last, cd.w, s_res);
+  }
+}
+
 // Every position of every segment, for each regex without a selective
 // prefix (w.scan_rx): position q of the flat position space lies in item i
 // when pos_base[i] <= q < pos_base[i + 1] (a single segment: item 0). An
@@ -827,6 +860,12 @@ static uint32_t grid_for(uint32_t n) {
 // (k_rx_walk reads it), so launch a fixed grid-stride grid.
 hipError_t launch_walk(uint32_t ncand_host, hipStream_t st, const WalkParams& w) {
   hipLaunchKernelGGL(k_rx_walk, dim3(ncand_host ? grid_for(ncand_host) : kDeviceGrid), dim3(kTPB), 0, st, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_walk_slices(hipStream_t st, const WalkParams& w, const SliceOut& cand, uint32_t nslices,
+                              uint32_t* cand_total) {
+  hipLaunchKernelGGL(k_rx_walk_slices, dim3(nslices + kOvfBlocks), dim3(kTPB), 0, st, w, cand, nslices, cand_total);
   return hipGetLastError();
 }
 
