@@ -107,6 +107,29 @@ def test_crowded_buckets_of_equal_grams():
     assert len(exp) > 300
 
 
+@pytest.mark.parametrize("filler,window", [(0, 11), (1200, 12)])
+def test_buckets_of_three_to_nine_equal_grams(filler, window):
+    # Groups of 3..9 literals share their first 12 bytes, so a bucket holds
+    # each group's equal keys: a crowded bucket's slot lists the keys of its
+    # first five further entries (several of them equal here) and reads the
+    # rest one by one. 1,200 more random 12-32 byte literals make the
+    # 12-byte window selective enough for stride 1.
+    r = np.random.default_rng(81)
+    lits = set()
+    for g in range(3, 10):
+        head = W.ACGT[r.integers(0, 4, size=12)].tobytes()
+        tails = set()
+        while len(tails) < g:
+            tails.add(W.ACGT[r.integers(0, 4, size=int(r.integers(1, 6)))].tobytes())
+        lits |= {head + t for t in tails}
+    lits |= set(W.random_literals(filler, 12, 32, W.ACGT, seed=82)) if filler else set()
+    pats = [(0, p) for p in sorted(lits)]
+    assert window_of(pats) == window
+    data = planted(300_000, pats, 83, alpha=W.ACGT, gap=200).tobytes()
+    exp = check_all_variants(pats, data)
+    assert len(exp) > 1000
+
+
 @pytest.mark.parametrize("chunk", [4096, 4097, 65_537])
 def test_stream_chunk_edges(chunk):
     # A factor starting at the last emitted position of a chunk has its

@@ -337,6 +337,9 @@ ws_status upload_patterns(ws_patterns* p) {
         s1[0] = kVtabMore;
         s1[1] = (uint32_t)(vmore.size() / 8);
         s1[2] = n - 1;
+        // Keys of the first five entries in vmore (kVtabEmpty: none; keys
+        // have their top bit clear).
+        for (uint32_t k = 0; k < 5; ++k) s1[3 + k] = k + 1 < n ? h.ent_hash[h.bent[e0 + k + 1]] : kVtabEmpty;
         for (uint32_t k = 1; k < n; ++k) {
           vmore.resize(vmore.size() + 8);
           fill(vmore.data() + vmore.size() - 8, h.bent[e0 + k]);

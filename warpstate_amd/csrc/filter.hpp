@@ -527,14 +527,26 @@ __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams
   } else if (hw.s2.x == hw.H) {
     k1 = verify_gram<CI, S, K>(c, hw.s2, hw.s3, hw, &r1);
   } else if (hw.s2.x == kVtabMore) {
-    // Crowded bucket (rare): per-lane appends.
-    for (uint32_t e = hw.s2.y; e < hw.s2.y + hw.s2.z; ++e) {
-      const uint4 e0 = h.vmore[2 * (size_t)e];
-      if (e0.x != hw.H) continue;
-      uint4 rec;
-      const uint32_t k = verify_gram<CI, S, K>(c, e0, h.vmore[2 * (size_t)e + 1], hw, &rec);
-      if (k == 1) slice_put(lit, blockIdx.x, atomicAdd(&s_n[0], 1u), rec);
-      if (RX && k == 2) slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), rec);
+    // Crowded bucket (rare): its slot holds the keys of its first five
+    // further entries, so of those only the ones whose key is the window's
+    // are read; entries past the fifth are all read. Per-lane appends.
+    const uint32_t keys[5] = {hw.s2.w, hw.s3.x, hw.s3.y, hw.s3.z, hw.s3.w};
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) m |= (uint32_t)(keys[j] == hw.H) << j;
+    const uint32_t n = hw.s2.z;
+    // Visit order: the set bits of m, then indices 5 .. n - 1.
+    for (uint32_t j = m ? __builtin_ctz(m) : 5u; j < n;) {
+      const size_t e = hw.s2.y + (size_t)j;
+      const uint4 e0 = h.vmore[2 * e];
+      if (e0.x == hw.H) {
+        uint4 rec;
+        const uint32_t k = verify_gram<CI, S, K>(c, e0, h.vmore[2 * e + 1], hw, &rec);
+        if (k == 1) slice_put(lit, blockIdx.x, atomicAdd(&s_n[0], 1u), rec);
+        if (RX && k == 2) slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), rec);
+      }
+      m &= m - 1;
+      j = m ? __builtin_ctz(m) : (j < 5 ? 5u : j + 1);
     }
   }
   emit_wave(lit, &s_n[0], k0 == 1, r0);

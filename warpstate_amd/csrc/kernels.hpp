@@ -61,7 +61,9 @@ struct HashParams {
   // length | kind << 31, bytes 0-3},
   // {bytes 4-15, output id} (hash 0xFFFFFFFF = empty); a bucket with more than
   // two factors keeps one in slot 0 and turns slot 1 into
-  // {kVtabMore, first, count, 0} over vmore (same slot format).
+  // {kVtabMore, first, count, key 0}, {keys 1-4} over vmore (same slot
+  // format): the keys of vmore entries first .. first + 4, kVtabEmpty past
+  // count.
   const uint4* vtab;
   const uint4* vmore;
   const uint4* fmeta;   // per factor {byte offset (4-aligned), length, kind << 31 | owner, output id}
