@@ -411,17 +411,22 @@ __device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, 
   return kind;
 }
 
-// Appends rec for every lane with `take` to the block's slice of `so`: one
-// LDS atomic per wave (ballot + prefix count). Call with every active lane.
-__device__ __forceinline__ void emit_wave(const SliceOut& so, uint32_t* s_cnt, bool take, uint4 rec) {
-  const uint64_t m = __ballot(take);
-  if (m == 0) return;
+// Appends r0 for every lane with t0 and r1 for every lane with t1 to the
+// block's slice of `so`: one LDS atomic per wave (two ballots and prefix
+// counts). Call with every active lane.
+__device__ __forceinline__ void emit_wave2(const SliceOut& so, uint32_t* s_cnt, bool t0, uint4 r0, bool t1,
+                                           uint4 r1) {
+  const uint64_t m0 = __ballot(t0), m1 = __ballot(t1);
+  if ((m0 | m1) == 0) return;
   const uint32_t lane = __lane_id();
-  const int leader = __ffsll((unsigned long long)m) - 1;
+  const uint32_t n0 = (uint32_t)__popcll(m0);
+  const int leader = __ffsll((unsigned long long)(m0 | m1)) - 1;
   uint32_t base = 0;
-  if ((int)lane == leader) base = atomicAdd(s_cnt, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  if (take) slice_put(so, blockIdx.x, base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)), rec);
+  if ((int)lane == leader) base = atomicAdd(s_cnt, n0 + (uint32_t)__popcll(m1));
+  base = __builtin_amdgcn_readlane(base, leader);
+  const uint64_t below = (1ull << lane) - 1ull;
+  if (t0) slice_put(so, blockIdx.x, base + (uint32_t)__popcll(m0 & below), r0);
+  if (t1) slice_put(so, blockIdx.x, base + n0 + (uint32_t)__popcll(m1 & below), r1);
 }
 
 // A filter hit being verified: its position, item, segment, the input
@@ -549,10 +554,8 @@ __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams
       j = m ? __builtin_ctz(m) : (j < 5 ? 5u : j + 1);
     }
   }
-  emit_wave(lit, &s_n[0], k0 == 1, r0);
-  if constexpr (RX) emit_wave(cand, &s_n[1], k0 == 2, r0);
-  emit_wave(lit, &s_n[0], k1 == 1, r1);
-  if constexpr (RX) emit_wave(cand, &s_n[1], k1 == 2, r1);
+  emit_wave2(lit, &s_n[0], k0 == 1, r0, k1 == 1, r1);
+  if constexpr (RX) emit_wave2(cand, &s_n[1], k0 == 2, r0, k1 == 2, r1);
 }
 
 // Per-wave LDS ring of filter hits waiting for verification (uint2
