@@ -307,10 +307,10 @@ ws_status upload_patterns(ws_patterns* p) {
     fmeta.push_back(fown[f]);
     fmeta.push_back(fc.kind ? fc.owner : h.lit_pid[fc.owner]);
   }
-  // Verification table (see HashParams): two 32-byte slots per bucket
-  // carrying the gram hash, factor byte offset | gram offset, length and
-  // first 16 (folded) bytes, so a filter hit is checked with one 64-byte
-  // read; crowded buckets continue in `vmore`.
+  // Verification table (see HashParams): one 64-byte entry per bucket
+  // carrying the gram's key, factor byte offset | gram offset, length,
+  // output id and first 32 (folded) bytes, so a filter hit is checked with
+  // one 64-byte read; a bucket's further grams continue in `vmore`.
   std::vector<uint32_t> vtab, vmore;
   if (h.hash_ok) {
     const uint32_t nb = 1u << h.nbits;
@@ -321,29 +321,25 @@ ws_status upload_patterns(ws_patterns* p) {
       e[0] = h.ent_hash[ent];
       e[1] = foff[f] | h.ent_off[ent];
       e[2] = flen[f] | (fc.kind << 31);
-      uint8_t pre[16] = {0};
-      std::memcpy(pre, fb.data() + foff[f], std::min<uint32_t>(16, flen[f]));
-      std::memcpy(&e[3], pre, 16);
-      e[7] = fc.kind ? fc.owner : h.lit_pid[fc.owner];
+      e[3] = fc.kind ? fc.owner : h.lit_pid[fc.owner];
+      uint8_t pre[32] = {0};
+      std::memcpy(pre, fb.data() + foff[f], std::min<uint32_t>(32, flen[f]));
+      std::memcpy(&e[4], pre, 32);
     };
     for (uint32_t b = 0; b < nb; ++b) {
-      uint32_t* s0 = vtab.data() + (size_t)b * 16;
-      uint32_t* s1 = s0 + 8;
-      s0[0] = s1[0] = kVtabEmpty;
+      uint32_t* e = vtab.data() + (size_t)b * 16;
+      e[0] = kVtabEmpty;
       const uint32_t e0 = h.bstart[b], n = h.bstart[b + 1] - e0;
-      if (n >= 1) fill(s0, h.bent[e0]);
-      if (n == 2) fill(s1, h.bent[e0 + 1]);
-      if (n > 2) {
-        s1[0] = kVtabMore;
-        s1[1] = (uint32_t)(vmore.size() / 8);
-        s1[2] = n - 1;
-        // Keys of the first five entries in vmore (kVtabEmpty: none; keys
-        // have their top bit clear).
-        for (uint32_t k = 0; k < 5; ++k) s1[3 + k] = k + 1 < n ? h.ent_hash[h.bent[e0 + k + 1]] : kVtabEmpty;
-        for (uint32_t k = 1; k < n; ++k) {
-          vmore.resize(vmore.size() + 8);
-          fill(vmore.data() + vmore.size() - 8, h.bent[e0 + k]);
-        }
+      if (n >= 1) fill(e, h.bent[e0]);
+      // {first, count, key 0, key 1} of the further entries in vmore
+      // (kVtabEmpty: none; keys have their top bit clear).
+      e[12] = (uint32_t)(vmore.size() / 16);
+      e[13] = n > 1 ? n - 1 : 0;
+      e[14] = n > 1 ? h.ent_hash[h.bent[e0 + 1]] : kVtabEmpty;
+      e[15] = n > 2 ? h.ent_hash[h.bent[e0 + 2]] : kVtabEmpty;
+      for (uint32_t k = 1; k < n; ++k) {
+        vmore.resize(vmore.size() + 16);
+        fill(vmore.data() + vmore.size() - 16, h.bent[e0 + k]);
       }
     }
   }

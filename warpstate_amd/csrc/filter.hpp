@@ -344,54 +344,52 @@ __device__ __forceinline__ void realign36(const uint32_t* raw, uint32_t sh, uint
   for (int i = 0; i < 9; ++i) W[i] = __builtin_amdgcn_alignbyte(u[i + 1], u[i], sh & 3);
 }
 
-// Exact comparison of one table entry ({hash, factor byte offset (16-byte
-// aligned, gram offset cleared), length | kind << 31, bytes 0-3},
-// {bytes 4-15, output id}) at position q. W holds the (folded, when CI) input bytes q..q+31. On a match
-// *rec is the output record and the result is 1 (literal) or 2 (regex
-// candidate); 0 otherwise.
-// Every factor has at least K bytes: its first min(K, 16) are compared with
-// masks known at compile time, the rest (a longer factor) by its length.
+// v_ffbl_b32: the index of the lowest set bit, all ones for 0.
+__device__ __forceinline__ uint32_t lowest_bit(uint32_t x) {
+  uint32_t r;
+  asm("v_ffbl_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// Exact comparison of one table entry (header {key, factor byte offset
+// (16-byte aligned, gram offset cleared), length | kind << 31, output id},
+// the factor's bytes 0-15 and 16-31) at position q. W holds the (folded,
+// when CI) input bytes q..q+31. On a match *rec is the output record and
+// the result is 1 (literal) or 2 (regex candidate); 0 otherwise.
+// Every factor has at least K bytes: its first K are compared with masks
+// known at compile time; in each further dword k of the entry the lowest
+// differing bit must lie at or past the factor's end, 8 len - 32 k
+// (clamped at 0): four operations a dword, no mask arithmetic. Factors
+// longer than 32 bytes compare the rest from the factor bytes.
 template <bool CI, int K>
-__device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, uint4 e1, const uint32_t* W,
-                                                 uint64_t q, uint2 seg, uint32_t item, uint4* rec) {
+__device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, uint4 b0, uint4 b1,
+                                                 const uint32_t* W, uint64_t q, uint2 seg, uint32_t item,
+                                                 uint4* rec) {
   const uint32_t len = e0.z & 0x7FFFFFFFu;
   if (q + len > seg.y) return 0;
-  const uint32_t pre[4] = {e0.w, e1.x, e1.y, e1.z};
-  constexpr int KK = K < 16 ? K : 16;
+  const uint32_t pre[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
   uint32_t d0 = 0;
 #pragma unroll
-  for (int k = 0; 4 * k < KK; ++k) {
+  for (int k = 0; 4 * k < K; ++k) {
     constexpr uint32_t kAll = 0xFFFFFFFFu;
-    const uint32_t msk = 4 * k + 4 <= KK ? kAll : ((1u << (8 * (KK - 4 * k))) - 1u);
+    const uint32_t msk = 4 * k + 4 <= K ? kAll : ((1u << (8 * (K - 4 * k))) - 1u);
     d0 |= (W[k] ^ pre[k]) & msk;
   }
   if (d0) return 0;
-  if (len > (uint32_t)KK) {
+  const uint32_t len8 = 8 * len;
+  bool bad = false;
 #pragma unroll
-    for (int k = KK / 4; k < 4; ++k) {
-      if (4u * k >= len) break;
-      const uint32_t rem = len - 4 * k;
-      const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
-      if ((W[k] ^ pre[k]) & msk) return 0;
-    }
+  for (int k = K / 4; k < 8; ++k) {
+    const uint32_t lim = len8 > 32u * k ? len8 - 32u * k : 0u;
+    bad |= lowest_bit(W[k] ^ pre[k]) < lim;
   }
+  if (bad) return 0;
   const uint32_t start = (uint32_t)(q - seg.x);
   const uint32_t kind = (e0.z >> 31) ? 2u : 1u;
   // A literal match carries its end; a regex candidate its factor's slot
   // (byte offset / 16) + 1, where the walk can resume after the factor.
-  *rec = make_uint4(e1.w, start, kind == 2u ? (e0.y >> 4) + 1u : start + len, item);
-  if (len > 16) {
-    // Bytes 16..31 of the factor in one load.
-    const uint4 p1 = *reinterpret_cast<const uint4*>(c.fbytes + e0.y + 16);
-    const uint32_t pw[4] = {p1.x, p1.y, p1.z, p1.w};
-    uint32_t diff = 0;
-#pragma unroll
-    for (int k = 4; k < 8; ++k) {
-      const uint32_t rem = len > 4u * k ? len - 4 * k : 0u;
-      const uint32_t msk = rem >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
-      diff |= (W[k] ^ pw[k - 4]) & msk;
-    }
-    if (diff) return 0;
+  *rec = make_uint4(e0.w, start, kind == 2u ? (e0.y >> 4) + 1u : start + len, item);
+  if (len > 32) {
     // Factors longer than 32 bytes: the rest byte by byte.
     const uint32_t nw = (len + 3) >> 2;
     const uint32_t* pat = reinterpret_cast<const uint32_t*>(c.fbytes + e0.y);
@@ -411,22 +409,17 @@ __device__ __forceinline__ uint32_t verify_entry(const ScanCommon& c, uint4 e0, 
   return kind;
 }
 
-// Appends r0 for every lane with t0 and r1 for every lane with t1 to the
-// block's slice of `so`: one LDS atomic per wave (two ballots and prefix
-// counts). Call with every active lane.
-__device__ __forceinline__ void emit_wave2(const SliceOut& so, uint32_t* s_cnt, bool t0, uint4 r0, bool t1,
-                                           uint4 r1) {
-  const uint64_t m0 = __ballot(t0), m1 = __ballot(t1);
-  if ((m0 | m1) == 0) return;
+// Appends rec for every lane with `take` to the block's slice of `so`: one
+// LDS atomic per wave (ballot + prefix count). Call with every active lane.
+__device__ __forceinline__ void emit_wave(const SliceOut& so, uint32_t* s_cnt, bool take, uint4 rec) {
+  const uint64_t m = __ballot(take);
+  if (m == 0) return;
   const uint32_t lane = __lane_id();
-  const uint32_t n0 = (uint32_t)__popcll(m0);
-  const int leader = __ffsll((unsigned long long)(m0 | m1)) - 1;
+  const int leader = __ffsll((unsigned long long)m) - 1;
   uint32_t base = 0;
-  if ((int)lane == leader) base = atomicAdd(s_cnt, n0 + (uint32_t)__popcll(m1));
+  if ((int)lane == leader) base = atomicAdd(s_cnt, (uint32_t)__popcll(m));
   base = __builtin_amdgcn_readlane(base, leader);
-  const uint64_t below = (1ull << lane) - 1ull;
-  if (t0) slice_put(so, blockIdx.x, base + (uint32_t)__popcll(m0 & below), r0);
-  if (t1) slice_put(so, blockIdx.x, base + n0 + (uint32_t)__popcll(m1 & below), r1);
+  if (take) slice_put(so, blockIdx.x, base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)), rec);
 }
 
 // A filter hit being verified: its position, item, segment, the input
@@ -438,7 +431,7 @@ struct HitWork {
   uint32_t W[8];
   uint32_t W1[8];
   uint32_t H;
-  uint4 s0, s1, s2, s3;  // its bucket's two slots
+  uint4 s0, s1, s2, s3;  // its bucket's entry: header, bytes 0-15, bytes 16-31, further entries
   bool live;             // false for a lane without a hit this round
 };
 
@@ -503,10 +496,10 @@ __device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParam
 // a factor's gram at offset o (low bits of its byte-offset field) starts the
 // factor at p - o, which must lie in the segment and below emit_end.
 template <bool CI, int S, int K>
-__device__ __forceinline__ uint32_t verify_gram(const ScanCommon& c, uint4 e0, uint4 e1, const HitWork& hw,
+__device__ __forceinline__ uint32_t verify_gram(const ScanCommon& c, uint4 e0, uint4 b0, uint4 b1, const HitWork& hw,
                                                 uint4* rec) {
   if constexpr (S == 1) {
-    return verify_entry<CI, K>(c, e0, e1, hw.W, hw.q, hw.seg, hw.item, rec);
+    return verify_entry<CI, K>(c, e0, b0, b1, hw.W, hw.q, hw.seg, hw.item, rec);
   } else {
     const uint32_t o = e0.y & 15u;
     if (o > hw.q - hw.seg.x) return 0;
@@ -516,7 +509,7 @@ __device__ __forceinline__ uint32_t verify_gram(const ScanCommon& c, uint4 e0, u
 #pragma unroll
     for (int i = 0; i < 8; ++i) W[i] = o ? hw.W1[i] : hw.W[i];
     e0.y &= ~15u;
-    return verify_entry<CI, K>(c, e0, e1, W, q, hw.seg, hw.item, rec);
+    return verify_entry<CI, K>(c, e0, b0, b1, W, q, hw.seg, hw.item, rec);
   }
 }
 
@@ -524,38 +517,32 @@ __device__ __forceinline__ uint32_t verify_gram(const ScanCommon& c, uint4 e0, u
 template <bool CI, bool RX, int K, int S = 1>
 __device__ __forceinline__ void hit_finish(const ScanCommon& c, const HashParams& h, const HitWork& hw,
                                            uint32_t* s_n, const SliceOut& lit, const SliceOut& cand) {
-  uint4 r0 = make_uint4(0, 0, 0, 0), r1 = r0;
-  uint32_t k0 = 0, k1 = 0;
-  if (hw.live && hw.s0.x == hw.H)
-    k0 = verify_gram<CI, S, K>(c, hw.s0, hw.s1, hw, &r0);
-  if (!hw.live) {
-  } else if (hw.s2.x == hw.H) {
-    k1 = verify_gram<CI, S, K>(c, hw.s2, hw.s3, hw, &r1);
-  } else if (hw.s2.x == kVtabMore) {
-    // Crowded bucket (rare): its slot holds the keys of its first five
-    // further entries, so of those only the ones whose key is the window's
-    // are read; entries past the fifth are all read. Per-lane appends.
-    const uint32_t keys[5] = {hw.s2.w, hw.s3.x, hw.s3.y, hw.s3.z, hw.s3.w};
-    uint32_t m = 0;
-#pragma unroll
-    for (int j = 0; j < 5; ++j) m |= (uint32_t)(keys[j] == hw.H) << j;
-    const uint32_t n = hw.s2.z;
-    // Visit order: the set bits of m, then indices 5 .. n - 1.
-    for (uint32_t j = m ? __builtin_ctz(m) : 5u; j < n;) {
-      const size_t e = hw.s2.y + (size_t)j;
-      const uint4 e0 = h.vmore[2 * e];
+  uint4 r0 = make_uint4(0, 0, 0, 0);
+  uint32_t k0 = 0;
+  if (hw.live && hw.s0.x == hw.H) k0 = verify_gram<CI, S, K>(c, hw.s0, hw.s1, hw.s2, hw, &r0);
+  if (hw.live && hw.s3.y != 0) {
+    // A bucket of two or more grams (rare): its entry lists the keys of the
+    // first two further entries, so of those only the ones whose key is
+    // the window's are read; entries past the second are all read.
+    // Per-lane appends.
+    uint32_t m = (uint32_t)(hw.s3.z == hw.H) | (uint32_t)(hw.s3.w == hw.H) << 1;
+    const uint32_t n = hw.s3.y;
+    // Visit order: the set bits of m, then indices 2 .. n - 1.
+    for (uint32_t j = m ? __builtin_ctz(m) : 2u; j < n;) {
+      const uint4* e = h.vmore + 4 * ((size_t)hw.s3.x + j);
+      const uint4 e0 = e[0];
       if (e0.x == hw.H) {
         uint4 rec;
-        const uint32_t k = verify_gram<CI, S, K>(c, e0, h.vmore[2 * e + 1], hw, &rec);
+        const uint32_t k = verify_gram<CI, S, K>(c, e0, e[1], e[2], hw, &rec);
         if (k == 1) slice_put(lit, blockIdx.x, atomicAdd(&s_n[0], 1u), rec);
         if (RX && k == 2) slice_put(cand, blockIdx.x, atomicAdd(&s_n[1], 1u), rec);
       }
       m &= m - 1;
-      j = m ? __builtin_ctz(m) : (j < 5 ? 5u : j + 1);
+      j = m ? __builtin_ctz(m) : (j < 2 ? 2u : j + 1);
     }
   }
-  emit_wave2(lit, &s_n[0], k0 == 1, r0, k1 == 1, r1);
-  if constexpr (RX) emit_wave2(cand, &s_n[1], k0 == 2, r0, k1 == 2, r1);
+  emit_wave(lit, &s_n[0], k0 == 1, r0);
+  if constexpr (RX) emit_wave(cand, &s_n[1], k0 == 2, r0);
 }
 
 // Per-wave LDS ring of filter hits waiting for verification (uint2
@@ -581,7 +568,7 @@ __device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashPar
   HitWork hw;
   hw.live = false;
   hw.s0.x = kVtabEmpty;
-  hw.s2.x = kVtabEmpty;
+  hw.s3.y = 0;
   if (lane < n) {
     const uint2 hr = ring[(head + lane) & (kHitRing - 1)];
     const uint2 seg = c.units ? c.items[hr.y] : seg0;
@@ -639,7 +626,7 @@ __device__ __forceinline__ void verify_queued_s2(const ScanCommon& c, const Hash
   HitWork hw;
   hw.live = false;
   hw.s0.x = kVtabEmpty;
-  hw.s2.x = kVtabEmpty;
+  hw.s3.y = 0;
   if (lane < n) {
     const uint2 hr = ring[(head + lane) & (kHitRing - 1)];
     const uint2 seg = c.units ? c.items[hr.y] : seg0;

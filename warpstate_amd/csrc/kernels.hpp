@@ -56,20 +56,17 @@ struct HashParams {
   // (windows every second position), -multiplier^K, and the bloom
   // byte-address mask ((nwords - 1) << 2).
   uint32_t BB, BB2, NBKK, amask;
-  // Verification table: 2^nbits buckets of two 32-byte slots. A slot is
-  // {gram hash, factor byte offset (16-aligned) | gram offset in the factor,
-  // length | kind << 31, bytes 0-3},
-  // {bytes 4-15, output id} (hash 0xFFFFFFFF = empty); a bucket with more than
-  // two factors keeps one in slot 0 and turns slot 1 into
-  // {kVtabMore, first, count, key 0}, {keys 1-4} over vmore (same slot
-  // format): the keys of vmore entries first .. first + 4, kVtabEmpty past
-  // count.
+  // Verification table: 2^nbits buckets of one 64-byte entry:
+  // {gram key, factor byte offset (16-aligned) | gram offset in the factor,
+  // length | kind << 31, output id}, {bytes 0-15}, {bytes 16-31},
+  // {first, count, key 0, key 1} of the bucket's further entries in vmore
+  // (same entry format; key kVtabEmpty past count). Key kVtabEmpty: an
+  // empty bucket.
   const uint4* vtab;
   const uint4* vmore;
   const uint4* fmeta;   // per factor {byte offset (4-aligned), length, kind << 31 | owner, output id}
 };
 constexpr uint32_t kVtabEmpty = 0xFFFFFFFFu;
-constexpr uint32_t kVtabMore = 0x80000000u;
 
 // Per-block output slices (no shared counter in the hot path) plus a shared
 // overflow area; k_slice_offsets2 + k_slice_gather2 compact them afterwards.
