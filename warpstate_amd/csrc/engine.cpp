@@ -40,15 +40,11 @@ hipError_t launch_order_keys(const Rec* recs, uint32_t n, const uint32_t* nd, ui
                              uint64_t* keys, uint32_t* vals, hipStream_t st, const DenseOut& dense = DenseOut{});
 hipError_t launch_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uint32_t sbits, uint32_t rbits,
                              uint64_t* keys, uint32_t* vals, hipStream_t st, const uint64_t* dbase = nullptr);
-hipError_t launch_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
-                              const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all,
-                              uint8_t* head, hipStream_t st, const uint32_t* carry_src = nullptr,
-                              uint32_t* carry_dst = nullptr, uint32_t ncarry = 0);
 hipError_t launch_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
-                             const uint32_t* nd, uint32_t gshift, const uint8_t* head, const uint32_t* carried,
-                             uint32_t seg_off, const uint32_t* rx_pid, uint32_t ties, Rec* out,
-                             uint32_t out_cap, uint32_t* out_count, uint32_t* carried_out,
-                             hipStream_t st, const OrderOut* oo = nullptr);
+                             const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all,
+                             const uint32_t* carried, uint32_t* carried_out, uint32_t ncarry, uint32_t seg_off,
+                             const uint32_t* rx_pid, uint32_t ties, Rec* out, uint32_t out_cap,
+                             uint32_t* out_count, hipStream_t st, const OrderOut* oo = nullptr);
 hipError_t launch_lit_keys(const Rec* recs, uint32_t n, uint32_t lbits, uint32_t pbits, uint32_t maxlen,
                            uint32_t len_desc, uint64_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t launch_pid_keys(const Rec* recs, uint32_t n, uint32_t sbits, uint64_t* keys, uint32_t* vals,
@@ -190,7 +186,7 @@ struct ws_ctx {
   hipEvent_t ev[8] = {};
   hipEvent_t tev[2] = {};      // around a whole device-sized call (outside any graph)
   uint32_t wall_khz = 0;       // device wall-clock rate (kernel stamps)
-  DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf, head,
+  DevBuf recs, cands, raw, k0, v0, k1, v1, tmp, counters, items, units, input, outbuf,
       scratch, carried, ids, lslices, cslices, scounts, bcnt, bcur, dstate, k2, v2, bcnt2, bases, lkeys, lcnt, lofs,
       posb, lslices2, cslices2, scounts2;
   // Multi-run calls (stream chunks) on the hash variant filter run k + 1 on
@@ -630,12 +626,10 @@ ws_status greedy_pass(ws_ctx* ctx, uint32_t n, uint32_t gshift, uint32_t maxlen,
                       uint32_t* kept) {
   ctx->init_clean = false;
   const hipStream_t st = ctx->stream;
-  WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->scratch.as<Rec>(), n, nullptr,
-                            gshift, nullptr, maxlen, ctx->head.as<uint8_t>(), st));
   WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 1, 0, st));
   WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->scratch.as<Rec>(), n, nullptr,
-                           gshift, ctx->head.as<uint8_t>(), nullptr, 0, nullptr, ties,
-                           ctx->recs.as<Rec>(), n, ctx->counters.as<uint32_t>(), nullptr, st));
+                           gshift, nullptr, maxlen, nullptr, nullptr, 0, 0, nullptr, ties,
+                           ctx->recs.as<Rec>(), n, ctx->counters.as<uint32_t>(), st));
   ws_status s = read_counters(ctx);
   if (s != WS_OK) return s;
   *kept = ctx->hcount[0];
@@ -963,7 +957,6 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
     WS_HIP(ctx->k1.ensure(need * 8));
     WS_HIP(ctx->v0.ensure(need * 4));
     WS_HIP(ctx->v1.ensure(need * 4));
-    WS_HIP(ctx->head.ensure(need));
     WS_HIP(ctx->scratch.ensure(need * sizeof(Rec)));
     uint32_t maxlit = 0;
     for (auto& l : h.lits) maxlit = std::max<uint32_t>(maxlit, (uint32_t)l.size());
@@ -1019,7 +1012,6 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       WS_HIP(ctx->k1.ensure((size_t)nraw * 8));
       WS_HIP(ctx->v0.ensure((size_t)nraw * 4));
       WS_HIP(ctx->v1.ensure((size_t)nraw * 4));
-      WS_HIP(ctx->head.ensure(nraw));
       WS_HIP(launch_chain_keys(ctx->raw.as<Rec>(), nraw, nullptr, res->sbits, rbits, ctx->k0.as<uint64_t>(),
                                ctx->v0.as<uint32_t>(), st));
       ws_status s2 = sort_into(ctx, nraw, cbits, ctx->k0.as<uint64_t>(), ctx->k1.as<uint64_t>(),
@@ -1028,9 +1020,6 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       const uint32_t nrx = (uint32_t)h.rx.size();
       uint32_t* cin = plan.use_carried ? ctx->carried.as<uint32_t>() + (size_t)ctx->carry_phase * nrx : nullptr;
       uint32_t* cout = plan.use_carried ? ctx->carried.as<uint32_t>() + (size_t)(ctx->carry_phase ^ 1u) * nrx : nullptr;
-      WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(),
-                                nraw, nullptr, res->sbits, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st, cin, cout,
-                                nrx));
       // The chain appends kept matches after the literal matches.
       if ((size_t)nlit + nraw > ctx->recs.cap / sizeof(Rec)) {
         // Grow while keeping the literal matches.
@@ -1045,9 +1034,9 @@ ws_status run_plan(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, uint32_t
       }
       WS_HIP(launch_fill_u32(ctx->counters.as<uint32_t>(), 1, nlit, st));
       WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(),
-                               nraw, nullptr, res->sbits, ctx->head.as<uint8_t>(), cin,
+                               nraw, nullptr, res->sbits, p->rx_maxlen, 0, cin, cout, nrx,
                                plan.seg_off, p->rx_pid, 0u, ctx->recs.as<Rec>(),
-                               (uint32_t)(ctx->recs.cap / sizeof(Rec)), ctx->counters.as<uint32_t>(), cout, st));
+                               (uint32_t)(ctx->recs.cap / sizeof(Rec)), ctx->counters.as<uint32_t>(), st));
       if (plan.use_carried) ctx->carry_phase ^= 1u;
       s = read_counters(ctx);
       if (s != WS_OK) return s;
@@ -1257,7 +1246,6 @@ ws_status prepare_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const
   WS_HIP(ctx->k1.ensure((size_t)rs->rec_cap * 8));
   WS_HIP(ctx->v0.ensure((size_t)rs->rec_cap * 4));
   WS_HIP(ctx->v1.ensure((size_t)rs->rec_cap * 4));
-  if (nrx) WS_HIP(ctx->head.ensure(rs->raw_cap));
   const uint32_t per_cu = tuning && tuning->blocks_per_cu ? tuning->blocks_per_cu : 4;
   rs->grid = std::max(1u, std::min(rs->nunits, ctx->cus * per_cu));
   // Filter blocks = output slices (hash variant); the slices keep whatever
@@ -1461,8 +1449,6 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
     // earlier chunks left) and writes the other half.
     uint32_t* cin = plan.use_carried ? ctx->carried.as<uint32_t>() + (rs.items_off & 1) * nrx : nullptr;
     uint32_t* cout = plan.use_carried ? ctx->carried.as<uint32_t>() + ((rs.items_off + 1) & 1) * nrx : nullptr;
-    WS_HIP(launch_chain_heads(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), rs.raw_cap,
-                              cnt + 2, gshift, p->rx_maxlen, 0, ctx->head.as<uint8_t>(), st, cin, cout, nrx));
     OrderOut oo{};
     if (keyed) {
       oo.keys = ctx->k2.as<uint64_t>();
@@ -1474,8 +1460,8 @@ ws_status launch_run(ws_ctx* ctx, const ws_patterns* p, const Plan& plan, const 
       oo.dense = plan.dense;
     }
     WS_HIP(launch_chain_walk(ctx->k1.as<uint64_t>(), ctx->v1.as<uint32_t>(), ctx->raw.as<Rec>(), rs.raw_cap,
-                             cnt + 2, gshift, ctx->head.as<uint8_t>(), cin, plan.seg_off, p->rx_pid, 0u,
-                             ctx->recs.as<Rec>(), rs.rec_cap, cnt + 0, cout, st, keyed ? &oo : nullptr));
+                             cnt + 2, gshift, p->rx_maxlen, 0, cin, cout, nrx, plan.seg_off, p->rx_pid, 0u,
+                             ctx->recs.as<Rec>(), rs.rec_cap, cnt + 0, st, keyed ? &oo : nullptr));
   }
   if (ev) WS_HIP(hipEventRecord(ev[3], st));
 
@@ -2073,7 +2059,7 @@ constexpr uint32_t kNoAllocLitBucketBits = 10;       // literal-only buckets at 
 
 // Byte offsets of the workspaces inside `scratch` (each 256-byte aligned).
 struct NoAllocLayout {
-  size_t recs, cands, raw, k0, k1, k2, v0, v1, v2, head, lslices, cslices, scounts, counters, items, dstate,
+  size_t recs, cands, raw, k0, k1, k2, v0, v1, v2, lslices, cslices, scounts, counters, items, dstate,
       lkeys, counts, lcnt, bcnt2, lofs, bcur, tmp, total;
 };
 constexpr size_t na_align(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -2095,7 +2081,6 @@ constexpr NoAllocLayout no_alloc_layout() {
   L.v0 = take(rec_cap * 4);
   L.v1 = take(rec_cap * 4);
   L.v2 = take(rec_cap * 4);
-  L.head = take(kNoAllocRecords);
   L.lslices = take(kNoAllocSliceRecords * sizeof(Rec));
   L.cslices = take(kNoAllocSliceRecords * sizeof(Rec));
   L.scounts = take(64 * 1024);
@@ -2343,7 +2328,7 @@ ws_status batch_checked(ws_ctx* ctx, const ws_patterns* p, const BatchLayout& L,
 std::vector<DevBuf*> ctx_buffers(ws_ctx* ctx) {
   return {&ctx->recs, &ctx->cands, &ctx->raw, &ctx->k0, &ctx->v0, &ctx->k1, &ctx->v1,
           &ctx->tmp, &ctx->counters, &ctx->items, &ctx->units, &ctx->input, &ctx->outbuf,
-          &ctx->head, &ctx->scratch, &ctx->carried, &ctx->ids,
+          &ctx->scratch, &ctx->carried, &ctx->ids,
           &ctx->lslices, &ctx->cslices, &ctx->scounts, &ctx->bcnt,
           &ctx->bcur, &ctx->dstate, &ctx->k2, &ctx->v2, &ctx->bcnt2, &ctx->bases,
           &ctx->lkeys, &ctx->lcnt, &ctx->lofs, &ctx->posb, &ctx->lslices2, &ctx->cslices2,
@@ -2790,8 +2775,7 @@ ws_status ws_scan_noalloc(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data
     sv.view(ctx->k2, base, L.k2, L.v0 - L.k2);
     sv.view(ctx->v0, base, L.v0, L.v1 - L.v0);
     sv.view(ctx->v1, base, L.v1, L.v2 - L.v1);
-    sv.view(ctx->v2, base, L.v2, L.head - L.v2);
-    sv.view(ctx->head, base, L.head, L.lslices - L.head);
+    sv.view(ctx->v2, base, L.v2, L.lslices - L.v2);
     sv.view(ctx->lslices, base, L.lslices, L.cslices - L.lslices);
     sv.view(ctx->cslices, base, L.cslices, L.scounts - L.cslices);
     sv.view(ctx->scounts, base, L.scounts, L.counters - L.scounts);
@@ -2898,7 +2882,7 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
   if (nrx) WS_HIP(ctx->carried.ensure(2 * (size_t)nrx * sizeof(uint32_t)));
   if (device_path_ok(mode, tuning)) {
     // Every chunk enqueued back to back, one synchronisation at the end.
-    if (nrx) WS_HIP(launch_fill_u32(ctx->carried.as<uint32_t>(), nrx, 0, ctx->stream));
+    if (nrx) WS_HIP(launch_fill_u32(ctx->carried.as<uint32_t>(), 2 * nrx, 0, ctx->stream));
     std::vector<Plan> plans;
     std::vector<uint32_t> offs;
     for (uint64_t a = 0; a < len; a += chunk_size) {
@@ -2938,7 +2922,7 @@ ws_status ws_stream_scan(ws_ctx* ctx, const ws_patterns* p, const uint8_t* data,
       ctx->last = ws_timings{};
     }
   }
-  if (nrx) WS_HIP(launch_fill_u32(ctx->carried.as<uint32_t>(), nrx, 0, ctx->stream));
+  if (nrx) WS_HIP(launch_fill_u32(ctx->carried.as<uint32_t>(), 2 * nrx, 0, ctx->stream));
   ctx->carry_phase = 0;
   uint64_t total = 0;
   for (uint64_t a = 0; a < len; a += chunk_size) {

@@ -503,40 +503,26 @@ __device__ __forceinline__ uint64_t group_at(const uint64_t* keys, const uint32_
   return group_of(keys[j], gshift);
 }
 
-// With carry_dst set, the kernel also copies the ncarry words of carry_src
-// there (whatever the count): the chain walk that follows reads carry_src,
-// the previous chunk's final value, and raises carry_dst, so no lane sees a
-// value another lane of the same launch has already raised.
-__global__ void k_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw,
-                              uint32_t n, const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen,
-                              uint32_t maxlen_all, uint8_t* head, const uint32_t* carry_src, uint32_t* carry_dst,
-                              uint32_t ncarry) {
-  if (carry_dst)
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ncarry; k += gridDim.x * blockDim.x)
-      carry_dst[k] = carry_src[k];
-  n = count_of(n, nd);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint64_t g = group_at(keys, vals, raw, i, gshift);
-    const Rec ri = raw[vals[i]];
-    bool h = (i == 0) || (group_at(keys, vals, raw, i - 1, gshift) != g);
-    if (!h) {
-      // An empty record also depends on an earlier end equal to its start
-      // (find_iter skips an empty match where the previous one ended).
-      const bool empty = ri.y == ri.z;
-      const uint32_t ml = maxlen ? maxlen[ri.x] : maxlen_all;
-      if (ml != 0xFFFFFFFFu) {
-        h = true;
-        for (uint32_t j = i; j-- > 0;) {
-          if (group_at(keys, vals, raw, j, gshift) != g) break;
-          const Rec rj = raw[vals[j]];
-          const uint64_t reach = (uint64_t)rj.y + ml;
-          if (reach < ri.y || (reach == ri.y && !empty)) break;
-          if (rj.z > ri.y || (empty && rj.z == ri.y)) { h = false; break; }
-        }
-      }
-    }
-    head[i] = h;
+// Whether record i, inside its group past the group's first record, starts
+// a chain segment of its own: no earlier record of the group within the
+// regex's maximum length ends after its start (an empty record also
+// depends on an earlier end equal to its start: find_iter skips an empty
+// match where the previous one ended). Unbounded regexes: never.
+__device__ __forceinline__ bool chain_starts_inside(const uint64_t* keys, const uint32_t* vals, const Rec* raw,
+                                                    uint32_t i, uint64_t g, uint32_t gshift,
+                                                    const uint32_t* maxlen, uint32_t maxlen_all) {
+  const Rec ri = raw[vals[i]];
+  const bool empty = ri.y == ri.z;
+  const uint32_t ml = maxlen ? maxlen[ri.x] : maxlen_all;
+  if (ml == 0xFFFFFFFFu) return false;
+  for (uint32_t j = i; j-- > 0;) {
+    if (group_at(keys, vals, raw, j, gshift) != g) break;
+    const Rec rj = raw[vals[j]];
+    const uint64_t reach = (uint64_t)rj.y + ml;
+    if (reach < ri.y || (reach == ri.y && !empty)) break;
+    if (rj.z > ri.y || (empty && rj.z == ri.y)) return false;
   }
+  return true;
 }
 
 // Greedy non-overlapping selection inside each segment (find_iter semantics):
@@ -544,21 +530,29 @@ __global__ void k_chain_heads(const uint64_t* keys, const uint32_t* vals, const 
 // With `ties`, records sharing the last kept (start, end) are kept too
 // (HashScanner reports every id of the shortest length at a position).
 // Output pattern id is rx_pid[rec.x] when rx_pid is set, else rec.x.
-// A head lane walks its chain twice: once to count the records it keeps,
+// Each lane tests whether its record starts a chain segment (the first of
+// its group, or chain_starts_inside); a segment's first lane walks it up
+// to the next segment start, twice: once to count the records it keeps,
 // then, after the block has reserved all its slots at once, to write them.
 // Stream chunks: carried[rx] is 1 + the absolute end of the regex's last
 // kept match in earlier chunks (0: none), read-only in this launch;
-// carried_out (a different buffer, holding a copy of carried) is raised to
-// this launch's kept ends. Both passes of a lane therefore see the same
-// carried value.
+// carried_out, the other half of a double buffer, is raised to carried
+// and to this launch's kept ends (atomicMax in any order: both halves
+// start at zero and only grow, so the half written two chunks ago is at
+// most carried). Both passes of a lane therefore see the same carried
+// value.
 // Launched with kTPB threads per block.
 __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw,
                                                      uint32_t n, const uint32_t* nd, uint32_t gshift,
-                                                     const uint8_t* head, const uint32_t* carried, uint32_t seg_off,
-                                                     const uint32_t* rx_pid, uint32_t ties, Rec* out,
-                                                     uint32_t out_cap, uint32_t* out_count, uint32_t* carried_out,
-                                                     OrderOut oo) {
+                                                     const uint32_t* maxlen, uint32_t maxlen_all,
+                                                     const uint32_t* carried, uint32_t* carried_out,
+                                                     uint32_t ncarry, uint32_t seg_off, const uint32_t* rx_pid,
+                                                     uint32_t ties, Rec* out, uint32_t out_cap,
+                                                     uint32_t* out_count, OrderOut oo) {
   __shared__ uint32_t s_res[2];
+  if (carried_out)
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ncarry; k += gridDim.x * blockDim.x)
+      atomicMax(&carried_out[k], carried[k]);
   n = count_of(n, nd);
   // Writes kept record r to output slot sl (and its ordering pair, and the
   // regex's carried end).
@@ -575,11 +569,10 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
     }
     if (carried_out) atomicMax(&carried_out[r.x], seg_off + r.z + 1u);
   };
-  // The chain from head i: with `emit`, its kept records go to slots
-  // slot, slot + 1, ...; returns how many it keeps, and (counting) the last
-  // kept record in *kept_rec.
-  auto walk = [&](uint32_t i, bool emit, uint32_t slot, Rec* kept_rec) -> uint32_t {
-    const uint64_t g = group_at(keys, vals, raw, i, gshift);
+  // The chain from segment start i (group g): with `emit`, its kept records
+  // go to slots slot, slot + 1, ...; returns how many it keeps, and
+  // (counting) the last kept record in *kept_rec.
+  auto walk = [&](uint32_t i, uint64_t g, bool emit, uint32_t slot, Rec* kept_rec) -> uint32_t {
     uint32_t last_end = 0, last_start = 0xFFFFFFFFu, kept = 0;
     bool have_last = false;
     if (carried) {
@@ -590,7 +583,9 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
       }
     }
     for (uint32_t j = i; j < n; ++j) {
-      if (j > i && (head[j] || group_at(keys, vals, raw, j, gshift) != g)) break;
+      if (j > i && (group_at(keys, vals, raw, j, gshift) != g ||
+                    chain_starts_inside(keys, vals, raw, j, g, gshift, maxlen, maxlen_all)))
+        break;
       const Rec r = raw[vals[j]];
       const bool tie = ties && r.y == last_start && r.z == last_end;
       if (r.y < last_end && !tie) continue;
@@ -610,14 +605,20 @@ __global__ __launch_bounds__(kTPB) void k_chain_walk(const uint64_t* keys, const
   // walked again.
   for (uint32_t b0 = blockIdx.x * kTPB; b0 < n; b0 += gridDim.x * kTPB) {
     const uint32_t i = b0 + threadIdx.x;
-    const bool on = i < n && head[i];
+    uint64_t g = 0;
+    bool on = false;
+    if (i < n) {
+      g = group_at(keys, vals, raw, i, gshift);
+      on = i == 0 || group_at(keys, vals, raw, i - 1, gshift) != g ||
+           chain_starts_inside(keys, vals, raw, i, g, gshift, maxlen, maxlen_all);
+    }
     Rec one = make_uint4(0, 0, 0, 0);
-    const uint32_t kept = on ? walk(i, false, 0, &one) : 0u;
+    const uint32_t kept = on ? walk(i, g, false, 0, &one) : 0u;
     const uint32_t slot = block_reserve(kept, out_count, s_res);
     if (kept == 1)
       emit_rec(one, slot);
     else if (kept)
-      walk(i, true, slot, &one);
+      walk(i, g, true, slot, &one);
   }
 }
 
@@ -896,22 +897,14 @@ hipError_t launch_chain_keys(const Rec* raw, uint32_t n, const uint32_t* nd, uin
   return hipGetLastError();
 }
 
-hipError_t launch_chain_heads(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
-                              const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all,
-                              uint8_t* head, hipStream_t st, const uint32_t* carry_src, uint32_t* carry_dst,
-                              uint32_t ncarry) {
-  hipLaunchKernelGGL(k_chain_heads, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, keys, vals, raw, n, nd, gshift,
-                     maxlen, maxlen_all, head, carry_src, carry_dst, ncarry);
-  return hipGetLastError();
-}
-
 hipError_t launch_chain_walk(const uint64_t* keys, const uint32_t* vals, const Rec* raw, uint32_t n,
-                             const uint32_t* nd, uint32_t gshift, const uint8_t* head, const uint32_t* carried,
-                             uint32_t seg_off, const uint32_t* rx_pid, uint32_t ties, Rec* out,
-                             uint32_t out_cap, uint32_t* out_count, uint32_t* carried_out,
-                             hipStream_t st, const OrderOut* oo) {
-  hipLaunchKernelGGL(k_chain_walk, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, keys, vals, raw, n, nd, gshift, head,
-                     carried, seg_off, rx_pid, ties, out, out_cap, out_count, carried_out, oo ? *oo : OrderOut{});
+                             const uint32_t* nd, uint32_t gshift, const uint32_t* maxlen, uint32_t maxlen_all,
+                             const uint32_t* carried, uint32_t* carried_out, uint32_t ncarry, uint32_t seg_off,
+                             const uint32_t* rx_pid, uint32_t ties, Rec* out, uint32_t out_cap,
+                             uint32_t* out_count, hipStream_t st, const OrderOut* oo) {
+  hipLaunchKernelGGL(k_chain_walk, dim3(grid_of(n, nd)), dim3(kTPB), 0, st, keys, vals, raw, n, nd, gshift, maxlen,
+                     maxlen_all, carried, carried_out, ncarry, seg_off, rx_pid, ties, out, out_cap, out_count,
+                     oo ? *oo : OrderOut{});
   return hipGetLastError();
 }
 
