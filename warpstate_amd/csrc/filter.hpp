@@ -315,35 +315,6 @@ __device__ __forceinline__ void slice_put(const SliceOut& so, uint32_t slice, ui
   }
 }
 
-// The 32 bytes from q as eight dwords, out of the 48-byte aligned span
-// raw[0..11] that holds them (q - (q & ~15) = sh): two operand selects by
-// the dword shift, then one v_alignbyte per output dword.
-// (Bit selects through all-ones / zero masks, v_bfi_b32, so the compiler
-// keeps the span in registers instead of indexing a stack copy.)
-__device__ __forceinline__ void realign32(const uint32_t* raw, uint32_t sh, uint32_t* W) {
-  const uint32_t m2 = 0u - ((sh >> 3) & 1u), m1 = 0u - ((sh >> 2) & 1u);
-  uint32_t t[10], u[9];
-#pragma unroll
-  for (int i = 0; i < 10; ++i) t[i] = (raw[i + 2] & m2) | (raw[i] & ~m2);
-#pragma unroll
-  for (int i = 0; i < 9; ++i) u[i] = (t[i + 1] & m1) | (t[i] & ~m1);
-#pragma unroll
-  for (int i = 0; i < 8; ++i) W[i] = __builtin_amdgcn_alignbyte(u[i + 1], u[i], sh & 3);
-}
-
-// The 36 bytes from base + sh as nine dwords, out of raw[0..12] (the
-// 16-byte aligned span at base): realign32 with one more output dword.
-__device__ __forceinline__ void realign36(const uint32_t* raw, uint32_t sh, uint32_t* W) {
-  const uint32_t m2 = 0u - ((sh >> 3) & 1u), m1 = 0u - ((sh >> 2) & 1u);
-  uint32_t t[11], u[10];
-#pragma unroll
-  for (int i = 0; i < 11; ++i) t[i] = (raw[i + 2] & m2) | (raw[i] & ~m2);
-#pragma unroll
-  for (int i = 0; i < 10; ++i) u[i] = (t[i + 1] & m1) | (t[i] & ~m1);
-#pragma unroll
-  for (int i = 0; i < 9; ++i) W[i] = __builtin_amdgcn_alignbyte(u[i + 1], u[i], sh & 3);
-}
-
 // v_ffbl_b32: the index of the lowest set bit, all ones for 0.
 __device__ __forceinline__ uint32_t lowest_bit(uint32_t x) {
   uint32_t r;
@@ -464,7 +435,8 @@ __device__ __forceinline__ uint32_t window_key_dev(const uint32_t* W) {
   return H & 0x7FFFFFFFu;
 }
 
-// raw: the 48-byte aligned span from q & ~15 (zeros past the loadable end).
+// raw: the nine dwords from q & ~3 (zeros past the loadable end), realigned
+// with one v_alignbyte per dword.
 // The window hash picks the bucket, whose two slots are read at once: one
 // dependent read per hit. (Checking an 8-byte tag pair first and reading the
 // slots only on a tag match was measured 1% slower on 10,000 DNA literals:
@@ -476,8 +448,8 @@ __device__ __forceinline__ void hit_prepare(const ScanCommon& c, const HashParam
   hw.q = hr.x;
   hw.item = hr.y;
   hw.seg = seg;
-  const uint32_t sh = (uint32_t)(hw.q & 15);
-  realign32(raw, sh, hw.W);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) hw.W[i] = __builtin_amdgcn_alignbyte(raw[i + 1], raw[i], hr.x & 3u);
   if (CI) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) hw.W[k] = fold_word(hw.W[k]);
@@ -555,9 +527,12 @@ __host__ __device__ constexpr uint32_t hash_ring_offset(uint32_t nwords) {
 }
 
 // Verifies the n (<= 64) queued hits from ring index `head`, one per lane:
-// the 48-byte aligned input span holding the window (streamed by the filter
-// shortly before), the window hash, its bucket's slots and the exact
-// compare.
+// the input dwords holding the window (streamed by the filter shortly
+// before; nine single-dword loads from the window's dword, each naturally
+// aligned, then one v_alignbyte per dword), the window key, its bucket's
+// entry and the exact compare. (Three 16-byte loads of the 16-byte-aligned
+// span need two bit-select steps more per dword; two unaligned 16-byte
+// loads cost the memory pipeline more than the selects.)
 // Call with the whole wave.
 // seg0: the segment of a single-segment scan (c.units null), wave-uniform.
 template <int K, bool CI, bool RX>
@@ -572,14 +547,10 @@ __device__ __forceinline__ void verify_queued(const ScanCommon& c, const HashPar
   if (lane < n) {
     const uint2 hr = ring[(head + lane) & (kHitRing - 1)];
     const uint2 seg = c.units ? c.items[hr.y] : seg0;
-    const uint32_t a = hr.x & ~15u;
-    uint32_t raw[12];
+    const uint32_t a = hr.x & ~3u;
+    uint32_t raw[9];
 #pragma unroll
-    for (int v = 0; v < 3; ++v) {
-      typedef int i4 __attribute__((ext_vector_type(4)));
-      const i4 x = __builtin_amdgcn_raw_buffer_load_b128(rsrc, a + 16 * v, 0, 0);
-      raw[4 * v] = (uint32_t)x.x; raw[4 * v + 1] = (uint32_t)x.y; raw[4 * v + 2] = (uint32_t)x.z; raw[4 * v + 3] = (uint32_t)x.w;
-    }
+    for (int k = 0; k < 9; ++k) raw[k] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, a + 4 * k, 0, 0);
     hit_prepare<K, CI>(c, h, hr, seg, raw, hw);
   }
   hit_finish<CI, RX, K>(c, h, hw, s_cnt, lit, cand);
@@ -595,9 +566,9 @@ __device__ __forceinline__ void hit_prepare_s2(const ScanCommon& c, const HashPa
   hw.q = hr.x;
   hw.item = hr.y;
   hw.seg = seg;
-  const uint32_t sh = (hr.x - o1) & 15u;
   uint32_t Wb[9];
-  realign36(raw, sh, Wb);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Wb[i] = __builtin_amdgcn_alignbyte(raw[i + 1], raw[i], (hr.x - o1) & 3u);
 #pragma unroll
   for (int i = 0; i < 8; ++i) hw.W[i] = __builtin_amdgcn_alignbyte(Wb[i + 1], Wb[i], o1);
 #pragma unroll
@@ -631,14 +602,10 @@ __device__ __forceinline__ void verify_queued_s2(const ScanCommon& c, const Hash
     const uint2 hr = ring[(head + lane) & (kHitRing - 1)];
     const uint2 seg = c.units ? c.items[hr.y] : seg0;
     const uint32_t o1 = hr.x > 0 ? 1u : 0u;
-    const uint32_t a = (hr.x - o1) & ~15u;
-    uint32_t raw[16];
+    const uint32_t a = (hr.x - o1) & ~3u;
+    uint32_t raw[10];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      typedef int i4 __attribute__((ext_vector_type(4)));
-      const i4 x = __builtin_amdgcn_raw_buffer_load_b128(rsrc, a + 16 * v, 0, 0);
-      raw[4 * v] = (uint32_t)x.x; raw[4 * v + 1] = (uint32_t)x.y; raw[4 * v + 2] = (uint32_t)x.z; raw[4 * v + 3] = (uint32_t)x.w;
-    }
+    for (int k = 0; k < 10; ++k) raw[k] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, a + 4 * k, 0, 0);
     hit_prepare_s2<K, CI>(c, h, hr, seg, raw, o1, hw);
   }
   hit_finish<CI, RX, K, 2>(c, h, hw, s_cnt, lit, cand);
