@@ -44,23 +44,6 @@ __device__ __forceinline__ uint32_t fold_word(uint32_t x) {
   return x | (upper >> 2);
 }
 
-// Exclusive wave prefix of `n` plus one LDS reservation for the wave total.
-// Must be called with the whole wave active.
-__device__ __forceinline__ uint32_t wave_reserve(uint32_t n, uint32_t* counter) {
-  const uint32_t lane = lane_id();
-  uint32_t x = n;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t y = __shfl_up(x, d, 64);
-    if (lane >= (uint32_t)d) x += y;
-  }
-  uint32_t total = __shfl(x, 63, 64);
-  uint32_t base = 0;
-  if (lane == 63 && total) base = atomicAdd(counter, total);
-  base = __shfl(base, 63, 64);
-  return base + x - n;
-}
-
 __device__ __forceinline__ bool verify_factor(const ScanCommon& c, uint64_t q, uint32_t hi,
                                               uint32_t f) {
   const uint32_t L = c.flen[f];

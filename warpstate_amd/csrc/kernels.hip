@@ -1126,11 +1126,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* cnt, uint3
 #pragma unroll
   for (int t = 0; t < T; ++t) {
     own[t] = v[t].x + v[t].y + v[t].z + v[t].w;
-    uint32_t x = own[t];
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(x, d, 64);
-      if (lane >= (uint32_t)d) x += y;
-    }
+    const uint32_t x = wave_incl_sum(own[t], lane);
     inc[t] = x;
     if (lane == 63) s_tot[t * 16 + w] = x;
   }
@@ -1148,11 +1144,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* cnt, uint3
       a[k] = mine ? s_tot[lane * per + k] : 0u;
       sum += a[k];
     }
-    uint32_t x = sum;
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(x, d, 64);
-      if (lane >= (uint32_t)d) x += y;
-    }
+    const uint32_t x = wave_incl_sum(sum, lane);
     uint32_t ex = x - sum;
 #pragma unroll
     for (int k = 0; k < per; ++k) {
@@ -1259,8 +1251,8 @@ __global__ __launch_bounds__(64 * kSortWaves) void k_bucket_rank(uint64_t* keys,
     while (big) {
       const uint32_t l = __builtin_ctzll(big);
       big &= big - 1;
-      const uint32_t cb = __shfl(c, l, 64);
-      const uint32_t bb = (uint32_t)__shfl((uint32_t)b, l, 64);
+      const uint32_t cb = __builtin_amdgcn_readlane(c, l);
+      const uint32_t bb = __builtin_amdgcn_readlane((uint32_t)b, l);
       if (lane == 0) cnt[bb] = 0;
       if (cb > kBucketMax) {
         if (lane == 0) atomicOr(flag, 1u);
@@ -1334,11 +1326,11 @@ __global__ __launch_bounds__(kGatherTPB) void k_gather_buckets(SliceOut lit, uin
     uint64_t todo = __ballot(live);
     while (todo) {
       const int leader = __ffsll((unsigned long long)todo) - 1;
-      const uint32_t lb = __shfl(b, leader);
+      const uint32_t lb = __builtin_amdgcn_readlane(b, leader);
       const uint64_t same = __ballot(live && b == lb) & todo;
       uint32_t base = 0;
       if ((int)lane_id() == leader) base = atomicAdd(&bcnt[lb], (uint32_t)__popcll(same));
-      base = __shfl(base, leader);
+      base = __builtin_amdgcn_readlane(base, leader);
       if ((same >> lane_id()) & 1ull) {
         const uint32_t slot = base + (uint32_t)__popcll(same & ((1ull << lane_id()) - 1ull));
         if (slot < kLitBucketCap)
@@ -1476,12 +1468,7 @@ constexpr uint32_t kSmallPer = kSmallOrder / kSmallOrderTPB;  // records per lan
 // *total gets the sum. s_w: 16 words of scratch.
 __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, uint32_t* total) {
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t x = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t y = __shfl_up(x, d, 64);
-    if (lane >= (uint32_t)d) x += y;
-  }
+  const uint32_t x = wave_incl_sum(v, lane);
   if (lane == 63) s_w[w] = x;
   __syncthreads();
   uint32_t before = 0, all = 0;
