@@ -100,8 +100,7 @@ __device__ __forceinline__ uint32_t block_slice_sum(const SliceOut& so, uint32_t
   __shared__ uint32_t red[kTPB / 64];
   uint32_t v = 0;
   for (uint32_t b = threadIdx.x; b < upto; b += kTPB) v += min(so.counts[b], so.cap);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v = __builtin_amdgcn_readlane(wave_incl_sum(v, threadIdx.x & 63), 63);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   uint32_t t = 0;
@@ -266,13 +265,20 @@ __global__ __launch_bounds__(kTPB) void k_ac_scan(ScanCommon c, AcParams a) {
 // s_res: two LDS words of the caller's, reused call after call (the reset
 // of the total follows the last barrier every thread has passed).
 __device__ __forceinline__ uint32_t block_reserve(uint32_t want, uint32_t* counter, uint32_t* s_res) {
+  // A wave prefix sum (DPP) and one LDS atomic per wave, then one device
+  // atomic per block.
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t incl = wave_incl_sum(want, lane);
+  const uint32_t wtot = __builtin_amdgcn_readlane(incl, 63);
   if (threadIdx.x == 0) s_res[0] = 0;
   __syncthreads();
-  const uint32_t local = want ? atomicAdd(&s_res[0], want) : 0u;
+  uint32_t wbase = 0;
+  if (lane == 63 && wtot) wbase = atomicAdd(&s_res[0], wtot);
+  wbase = __builtin_amdgcn_readlane(wbase, 63);
   __syncthreads();
   if (threadIdx.x == 0) s_res[1] = s_res[0] ? atomicAdd(counter, s_res[0]) : 0u;
   __syncthreads();
-  return s_res[1] + local;
+  return s_res[1] + wbase + incl - want;
 }
 
 // ---------------------------------------------------------------------------
