@@ -1368,7 +1368,7 @@ __global__ __launch_bounds__(kGatherTPB) void k_gather_buckets(SliceOut lit, uin
 // bucket capacity) into output offsets, and the match total. One block.
 __global__ __launch_bounds__(kGatherTPB) void k_scan_buckets(const uint32_t* bcnt, uint32_t* bofs,
                                                              uint32_t* counters) {
-  __shared__ uint32_t s_part[kGatherTPB];
+  __shared__ uint32_t s_wt[kGatherTPB / 64];
   constexpr uint32_t per = kLitBuckets / kGatherTPB;
   const uint32_t b0 = threadIdx.x * per;
   uint32_t v[per];
@@ -1383,15 +1383,20 @@ __global__ __launch_bounds__(kGatherTPB) void k_scan_buckets(const uint32_t* bcn
   uint32_t sum = 0;
 #pragma unroll
   for (uint32_t k = 0; k < per; ++k) sum += v[k];
-  s_part[threadIdx.x] = sum;
+  // Block prefix: a DPP wave prefix, the wave totals through LDS (one
+  // barrier).
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t incl = wave_incl_sum(sum, lane);
+  if (lane == 63) s_wt[w] = incl;
   __syncthreads();
-  for (uint32_t d = 1; d < kGatherTPB; d <<= 1) {
-    const uint32_t x = threadIdx.x >= d ? s_part[threadIdx.x - d] : 0u;
-    __syncthreads();
-    s_part[threadIdx.x] += x;
-    __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kGatherTPB / 64; ++k) {
+    const uint32_t t = s_wt[k];
+    before += k < w ? t : 0u;
+    all += t;
   }
-  uint32_t run = s_part[threadIdx.x] - sum;
+  uint32_t run = before + incl - sum;
 #pragma unroll
   for (uint32_t k = 0; k < per; k += 4) {
     uint4 o;
@@ -1402,8 +1407,8 @@ __global__ __launch_bounds__(kGatherTPB) void k_scan_buckets(const uint32_t* bcn
     *reinterpret_cast<uint4*>(bofs + b0 + k) = o;
   }
   if (threadIdx.x == kGatherTPB - 1) {
-    counters[0] = s_part[threadIdx.x];
-    counters[kCounterLitSnapshot] = s_part[threadIdx.x];
+    counters[0] = all;
+    counters[kCounterLitSnapshot] = all;
   }
 }
 
