@@ -31,30 +31,32 @@ namespace ws {
 __global__ __launch_bounds__(1024) void k_slice_offsets2(SliceOut lit, SliceOut cand, uint32_t n, uint32_t* loffs,
                                                           uint32_t* coffs, uint32_t* lit_total, uint32_t* cand_total,
                                                           uint32_t* lit_snap) {
-  __shared__ uint32_t part[1024];
+  // Each half's prefix: DPP wave prefixes, then the half's eight wave
+  // totals through LDS (one barrier).
+  __shared__ uint32_t s_wt[16];
   const uint32_t half = threadIdx.x >> 9, t = threadIdx.x & 511;
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const SliceOut& so = half ? cand : lit;
   uint32_t* offsets = half ? coffs : loffs;
   const uint32_t per = (n + 511) / 512;
   const uint32_t b0 = t * per;
   uint32_t sum = 0;
   for (uint32_t k = 0; k < per && b0 + k < n; ++k) sum += min(so.counts[b0 + k], so.cap);
-  part[threadIdx.x] = sum;
+  const uint32_t incl = wave_incl_sum(sum, lane);
+  if (lane == 63) s_wt[w] = incl;
   __syncthreads();
-  for (uint32_t d = 1; d < 512; d <<= 1) {
-    uint32_t v = t >= d ? part[threadIdx.x - d] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t run = part[threadIdx.x] - sum;
+  uint32_t before = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) before += k < (w & 7u) ? s_wt[(w & 8u) + k] : 0u;
+  uint32_t run = before + incl - sum;
   for (uint32_t k = 0; k < per && b0 + k < n; ++k) {
     offsets[b0 + k] = run;
     run += min(so.counts[b0 + k], so.cap);
   }
   if (t == 511) {
-    offsets[n] = part[threadIdx.x];
-    const uint32_t tot = part[threadIdx.x] + min(*so.ovf_count, so.ovf_cap);
+    const uint32_t all = before + incl;
+    offsets[n] = all;
+    const uint32_t tot = all + min(*so.ovf_count, so.ovf_cap);
     if (half) {
       *cand_total = tot;
     } else {
